@@ -1,0 +1,193 @@
+/* olcao_gpu.h -- C-ABI of the MI355X (gfx950) integral -> H(k)/S(k) ->
+ * eigenvalue path of OLCAO.
+ *
+ * The reference has no plugin API; its boundary is the set of Fortran module
+ * procedures that src/uolcao/OLCAO.F90 calls (setupSCF :248-279, mainSCF
+ * :482, intgPSCF :645-662, bandPSCF :758).  Each entry point below replaces
+ * one of them; the citation names the procedure.  The Fortran facade
+ * (olcao_amd/fortran/olcao_gpu_facade.f90) binds these with ISO_C_BINDING and
+ * the Python harness binds them with ctypes.
+ *
+ * Conventions
+ *   - Plain C types only; every pointer is host memory owned by the caller.
+ *   - Arrays mirror the reference's Fortran arrays (column-major).  A complex
+ *     matrix valeVale(n,n,nk) is nk*n*n interleaved (re,im) doubles, element
+ *     (row r, column c, k-point k) at offset 2*(r + c*n + k*n*n).  The Gamma
+ *     build (system.gamma = 1) uses real n*n matrices.
+ *   - Integral matrices hold the upper triangle (row <= column) as the
+ *     reference's valeVale does; the strictly lower part of every diagonal
+ *     atom block is zero (intgSaving.F90:484-509).
+ *   - Indices are 0-based.
+ *   - Every call returns olcao_status; olcao_last_error() gives the message.
+ *     The reference stops on the same conditions (integrals.F90:377-384,
+ *     interfaces.F90:107-110).
+ *   - One context is used by one host thread at a time.  All device state
+ *     (system, work lists, S(k), H(k), term matrices) persists in the context
+ *     across SCF iterations.
+ */
+#ifndef OLCAO_GPU_H
+#define OLCAO_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct olcao_ctx olcao_ctx;
+
+typedef enum {
+    OLCAO_OK = 0,
+    OLCAO_ERR_HIP = 1,          /* HIP runtime failure */
+    OLCAO_ERR_ALLOC = 2,        /* device allocation failed */
+    OLCAO_ERR_INPUT = 3,        /* invalid system description */
+    OLCAO_ERR_CELLS = 4,        /* cell list too small (integrals.F90:377) */
+    OLCAO_ERR_NOT_SPD = 5,      /* S(k) not positive definite (zhegv info>n) */
+    OLCAO_ERR_SOLVER = 6,       /* eigensolver failed to converge */
+    OLCAO_ERR_UNSUPPORTED = 7,  /* option not available in this build */
+    OLCAO_ERR_STATE = 8,        /* call order violated (e.g. solve before build) */
+    OLCAO_ERR_NO_DEVICE = 9     /* no GPU visible */
+} olcao_status;
+
+/* Precision of the contraction + Bloch sum (north of the FP64 integrals). */
+enum { OLCAO_PREC_FP64 = 0, OLCAO_PREC_MIXED = 1 };
+
+/* The system as the reference holds it after parseInput
+ * (input.f90:116-244) and before any derived setup. */
+typedef struct {
+    int nAtoms;
+    const double *atomPos;        /* [nAtoms][3] Cartesian bohr */
+    const int *atomType;          /* [nAtoms] 0-based */
+    int nTypes, maxAlphas, maxRadial;
+    const int *typeNumAlphas;     /* [nTypes] */
+    const double *typeAlphas;     /* [nTypes][maxAlphas] strictly ascending */
+    const int *typeNumOrbAlphas;  /* [nTypes][4] alphas used by l=0..3 */
+    const int *typeElement;       /* [nTypes] */
+    const int *typeNumValeRadial; /* [nTypes] */
+    const int *typeNumCoreRadial; /* [nTypes] */
+    const int *typeValeL;         /* [nTypes][maxRadial] */
+    const int *typeCoreL;         /* [nTypes][maxRadial] */
+    const double *typeValeCoef;   /* [nTypes][maxRadial][maxAlphas] */
+    const double *typeCoreCoef;   /* [nTypes][maxRadial][maxAlphas] */
+    int nPotTypes, maxPotAlphas;
+    const int *potNumAlphas;      /* [nPotTypes] */
+    const double *potAlphas;      /* [nPotTypes][maxPotAlphas] ascending */
+    const double *potZ;           /* [nPotTypes] nuclear charge */
+    const double *potNucAlpha;    /* [nPotTypes] */
+    const int *potElement;        /* [nPotTypes] */
+    int nPotSites;
+    const double *potPos;         /* [nPotSites][3], grouped by type */
+    const int *potSiteType;       /* [nPotSites] 0-based */
+    double cellVectors[9];        /* a_i = cellVectors[3*i + xyz], bohr */
+    int nk;
+    const double *kpoints;        /* [nk][3] fractional reciprocal coords */
+    double basisCutoff;           /* BASISFUNCTION cutoff, e.g. 1e-16 */
+    int gamma;                    /* 1: real Gamma-point build */
+} olcao_system;
+
+typedef struct {
+    int precision;                /* OLCAO_PREC_* */
+    int reserved[7];
+} olcao_options;
+
+/* Context lifetime.  Replaces the derived setup of OLCAO.F90:setupSCF
+ * (:196-244): initializeLattice (lattice.f90:310), initializeFindVec (:892),
+ * initializeKPoints (kpoints.f90:937), renormalizeBasis (basis.f90:55),
+ * makeAlphaDist (gaussRelations.f90:64), plus the screened work lists. */
+olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
+                              olcao_ctx **out);
+void olcao_ctx_destroy(olcao_ctx *ctx);
+const char *olcao_last_error(const olcao_ctx *ctx);
+
+/* dims[0..7] = valeDim, coreDim, nk, potDim, nCells, nPairCells,
+ * nPrimitivePairs, nAtomPairs. */
+olcao_status olcao_get_dims(const olcao_ctx *ctx, long long dims[8]);
+
+/* Derived setup for inspection: the sorted cell list (cellDimsReal,
+ * cellSizesReal, lattice.f90:571-695) and the Cartesian k-points
+ * (kpoints.f90:424-454).  Any pointer may be NULL. */
+olcao_status olcao_get_setup(const olcao_ctx *ctx, double *cellDims,
+                             double *cellSizes, double *kCart);
+
+/* Two-centre integral sweeps.  op: 0 overlap (gaussOverlapOL,
+ * integrals.F90:151), 1 kinetic (gaussOverlapKE, :656).  The result stays on
+ * the device (S(k) for op 0) and is copied to vv/cv/cc when non-NULL
+ * (valeVale, coreVale(coreDim,valeDim,nk), coreCore(coreDim,coreDim,nk)). */
+olcao_status olcao_gauss_overlap(olcao_ctx *ctx, int op, double *vv,
+                                 double *cv, double *cc);
+
+/* Nuclear-attraction sweep (gaussOverlapNP, integrals.F90:1531 with
+ * nuclearPE :2533). */
+olcao_status olcao_gauss_overlap_np(olcao_ctx *ctx, double *vv, double *cv,
+                                    double *cc);
+
+/* One electronic-potential term matrix V_j (gaussOverlapEP,
+ * integrals.F90:1946 with electronicPE :2700, called per term by
+ * elecPotGaussOverlap :2357).  term: 0-based over potDim, type-major. */
+olcao_status olcao_elec_pot_gauss_overlap(olcao_ctx *ctx, int term, double *vv,
+                                          double *cv, double *cc);
+
+/* Fused Hamiltonian sweep with the potential coefficients folded in
+ * (gaussOverlapHamPSCF, integrals.F90:2860): KE + sum_j c_j V_j - sum Z N.
+ * potCoeffs: [potDim]. */
+olcao_status olcao_gauss_overlap_ham_pscf(olcao_ctx *ctx, const double *potCoeffs,
+                                          double *vv, double *cv, double *cc);
+
+/* Generalised eigensolve of the device-resident H(k), S(k) from the two
+ * calls above, for all k (secularEqnPSCF, secularEqn.F90:331, with
+ * solveZHEGV interfaces.F90:73 / solveDSYGV :152).  evals: [nk][numStates]
+ * ascending; evecs (may be NULL): [nk][numStates][valeDim] complex
+ * (column s of eigenvector k at 2*(r + s*n + k*n*numStates)), real at Gamma. */
+olcao_status olcao_secular_eqn_pscf(olcao_ctx *ctx, int numStates, double *evals,
+                                    double *evecs);
+
+/* Mode A setup: build and keep S, KE, NP and every V_j on the device
+ * (setupSCF, OLCAO.F90:248-279). */
+olcao_status olcao_setup_scf(olcao_ctx *ctx);
+
+/* Mode A iteration: H = NP + KE + sum_j c_j V_j (a coefficient of exactly 0
+ * adds its term unscaled, matrixSubs.F90:369-373), then the solve for all k
+ * (secularEqnSCF, secularEqn.F90:37). */
+olcao_status olcao_secular_eqn_scf(olcao_ctx *ctx, const double *potCoeffs,
+                                   int numStates, double *evals, double *evecs);
+
+/* Mode B iteration, the unit the benchmark times: overlap sweep, fused
+ * Hamiltonian sweep and the solve for all k (intgPSCF + bandPSCF,
+ * OLCAO.F90:533-770).  evals: [nk][numStates]. */
+olcao_status olcao_scf_iteration(olcao_ctx *ctx, const double *potCoeffs,
+                                 int numStates, double *evals);
+
+/* Stage times (ms, HIP events) of the last olcao_scf_iteration:
+ * t[0] overlap sweep, t[1] Hamiltonian sweep, t[2] contraction + Bloch
+ * (both operators), t[3] solve, t[4] total; t[5] three-centre kernel alone. */
+olcao_status olcao_get_timings(const olcao_ctx *ctx, double t[8]);
+
+/* Work counters of the last three-centre sweep: counts[2*(4*la+lb)] three-
+ * Gaussian overlap evaluations and counts[2*(4*la+lb)+1] nuclear
+ * evaluations of angular class (la, lb). */
+olcao_status olcao_get_work_counts(olcao_ctx *ctx, unsigned long long counts[32]);
+
+/* Drop-in generalised Hermitian solver for host matrices, batched:
+ * replaces solveZHEGV (interfaces.F90:73-120): A (n,n) upper triangle in,
+ * eigenvectors out; B (n,n) upper triangle in.  w: [nbatch][n]. */
+olcao_status olcao_solve_zhegv(int n, int nbatch, double *A, double *B, double *w);
+/* Real twin, replaces solveDSYGV (interfaces.F90:152-200). */
+olcao_status olcao_solve_dsygv(int n, int nbatch, double *A, double *B, double *w);
+
+/* Primitive integral blocks on the device, for parity tests of the kernels'
+ * device functions against the reference subroutines overlap2CIntg
+ * (gaussIntegrals.f90:26, op 0), kinetic2CIntg (:4422, op 1),
+ * electron3CIntg (:11624, op 2), nuclear3CIntg (:16223, op 3).
+ * a: [n][3] exponents, A/B/C: [n][3], sw: [n] l1l2switch, sh: [n][16][16]
+ * column-major blocks (sh(i,j) at i + 16 j); entries outside the class block
+ * are zero. */
+olcao_status olcao_prim_batch(int op, int n, const double *a, const double *A,
+                              const double *B, const double *C, const int *sw,
+                              double *sh);
+
+/* Boys function F_0..F_6 on the device (gaussIntegrals.f90:16080).
+ * T: [n], F: [n][7]. */
+olcao_status olcao_boys_batch(int n, const double *T, double *F);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

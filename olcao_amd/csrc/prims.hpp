@@ -1,0 +1,425 @@
+// Primitive Gaussian integrals over unnormalised real solid harmonics, as
+// device (and host) functions templated on the angular-momentum class.
+//
+// What is computed follows the reference's primitive library
+// (src/uolcao/gaussIntegrals.f90): the Cartesian block pc(a,b) over the
+// components s; x y z; xx yy zz xy xz yz; xyz xxy xxz yyx yyz zzx zzy xxx yyy
+// zzz (:36-38) up to the class's l, transformed to the solid harmonics
+// s; x y z; xy xz yz xx-yy 2zz-xx-yy; xyz xxz-yyz xxx-3yyx 3xxy-yyy
+// 2zzz-3xxz-3yyz 4zzx-xxx-yyx 4zzy-xxy-yyy (:32-34).
+//
+// How differs from the reference: overlap, kinetic and three-Gaussian
+// overlap integrals factorise into x, y and z; each axis runs the 1-D
+// Obara-Saika recursion (Obara & Saika, J. Chem. Phys. 84, 3963 (1986))
+//   E(i+1,j) = XA E(i,j) + h (i E(i-1,j) + j E(i,j-1)),  E(0,0) = 1
+// and the Cartesian block is pref * Ex * Ey * Ez.  The nuclear term keeps the
+// two-index 3-D recursion with the auxiliary index m, because the Boys
+// function couples the axes.  Every table index is a template constant, so
+// the tables live in registers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#define OHD __host__ __device__ __forceinline__
+
+namespace olcao {
+
+constexpr double kPi = 3.141592653589793;
+
+OHD constexpr int ncart(int l) { return (l + 1) * (l + 2) * (l + 3) / 6; }
+OHD constexpr int nsh(int l) { return (l + 1) * (l + 1); }
+
+// Cartesian component c -> lx + 4 ly + 16 lz (reference pc order).
+OHD constexpr int cart_code(int c)
+{
+    return c == 0 ? 0 : c == 1 ? 1 : c == 2 ? 4 : c == 3 ? 16 : c == 4 ? 2 :
+           c == 5 ? 8 : c == 6 ? 32 : c == 7 ? 5 : c == 8 ? 17 : c == 9 ? 20 :
+           c == 10 ? 21 : c == 11 ? 6 : c == 12 ? 18 : c == 13 ? 9 :
+           c == 14 ? 24 : c == 15 ? 33 : c == 16 ? 36 : c == 17 ? 3 :
+           c == 18 ? 12 : 48;
+}
+OHD constexpr int cart_from_code(int code)
+{
+    return code == 0 ? 0 : code == 1 ? 1 : code == 4 ? 2 : code == 16 ? 3 :
+           code == 2 ? 4 : code == 8 ? 5 : code == 32 ? 6 : code == 5 ? 7 :
+           code == 17 ? 8 : code == 20 ? 9 : code == 21 ? 10 : code == 6 ? 11 :
+           code == 18 ? 12 : code == 9 ? 13 : code == 24 ? 14 : code == 33 ? 15 :
+           code == 36 ? 16 : code == 3 ? 17 : code == 12 ? 18 : code == 48 ? 19 : -1;
+}
+OHD constexpr int cexp(int c, int axis) { return (cart_code(c) >> (2 * axis)) & 3; }
+OHD constexpr int ctot(int c) { return cexp(c, 0) + cexp(c, 1) + cexp(c, 2); }
+OHD constexpr int cfirst(int c)
+{
+    return cexp(c, 0) > 0 ? 0 : cexp(c, 1) > 0 ? 1 : cexp(c, 2) > 0 ? 2 : -1;
+}
+// Component with one (k = 1) or two (k = 2) fewer units along axis.
+OHD constexpr int cdec(int c, int axis, int k)
+{
+    return cexp(c, axis) >= k ? cart_from_code(cart_code(c) - k * (1 << (2 * axis))) : 0;
+}
+
+// Solid harmonic i as up to three weighted Cartesian components.
+OHD constexpr int sh_nterm(int i)
+{
+    return (i == 7 || i == 10 || i == 11 || i == 12) ? 2 :
+           (i == 8 || i == 13 || i == 14 || i == 15) ? 3 : 1;
+}
+OHD constexpr int sh_cart(int i, int t)
+{
+    return i == 0 ? 0 : i == 1 ? 1 : i == 2 ? 2 : i == 3 ? 3 : i == 4 ? 7 :
+           i == 5 ? 8 : i == 6 ? 9 :
+           i == 7 ? (t == 0 ? 4 : 5) :
+           i == 8 ? (t == 0 ? 6 : t == 1 ? 4 : 5) :
+           i == 9 ? 10 :
+           i == 10 ? (t == 0 ? 12 : 14) :
+           i == 11 ? (t == 0 ? 17 : 13) :
+           i == 12 ? (t == 0 ? 11 : 18) :
+           i == 13 ? (t == 0 ? 19 : t == 1 ? 12 : 14) :
+           i == 14 ? (t == 0 ? 15 : t == 1 ? 17 : 13) :
+                     (t == 0 ? 16 : t == 1 ? 11 : 18);
+}
+OHD constexpr double sh_weight(int i, int t)
+{
+    return i == 7 ? (t == 0 ? 1.0 : -1.0) :
+           i == 8 ? (t == 0 ? 2.0 : -1.0) :
+           i == 10 ? (t == 0 ? 1.0 : -1.0) :
+           i == 11 ? (t == 0 ? 1.0 : -3.0) :
+           i == 12 ? (t == 0 ? 3.0 : -1.0) :
+           i == 13 ? (t == 0 ? 2.0 : -3.0) :
+           i == 14 ? (t == 0 ? 4.0 : -1.0) :
+           i == 15 ? (t == 0 ? 4.0 : -1.0) : 1.0;
+}
+
+// sh(i,j) = sum_ab T(i,a) pc(a,b) T(j,b) for the class block.
+template <int LA, int LB>
+OHD void cart_to_sh(const double (&pc)[ncart(LA)][ncart(LB)],
+                    double (&sh)[nsh(LA)][nsh(LB)])
+{
+    constexpr int NB = ncart(LB), SA = nsh(LA), SB = nsh(LB);
+    double tmp[SA][NB];
+#pragma unroll
+    for (int i = 0; i < SA; ++i)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            double s = 0.0;
+#pragma unroll
+            for (int t = 0; t < sh_nterm(i); ++t) s += sh_weight(i, t) * pc[sh_cart(i, t)][b];
+            tmp[i][b] = s;
+        }
+#pragma unroll
+    for (int i = 0; i < SA; ++i)
+#pragma unroll
+        for (int j = 0; j < SB; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int t = 0; t < sh_nterm(j); ++t) s += tmp[i][sh_cart(j, t)] * sh_weight(j, t);
+            sh[i][j] = s;
+        }
+}
+
+// 1-D overlap-type Obara-Saika table, E(0,0) = 1.
+template <int LA, int LB>
+OHD void os1d(double XA, double XB, double h, double (&E)[LA + 1][LB + 1])
+{
+    E[0][0] = 1.0;
+#pragma unroll
+    for (int i = 1; i <= LA; ++i) {
+        double v = XA * E[i - 1][0];
+        if (i > 1) v += h * (i - 1) * E[i > 1 ? i - 2 : 0][0];
+        E[i][0] = v;
+    }
+#pragma unroll
+    for (int j = 1; j <= LB; ++j)
+#pragma unroll
+        for (int i = 0; i <= LA; ++i) {
+            double v = XB * E[i][j - 1];
+            double t = 0.0;
+            if (i > 0) t += i * E[i > 0 ? i - 1 : 0][j - 1];
+            if (j > 1) t += (j - 1) * E[i][j > 1 ? j - 2 : 0];
+            E[i][j] = v + h * t;
+        }
+}
+
+// 1-D kinetic table from the Obara-Saika kinetic recursion,
+// T(0,0) = xi (1 - 2 xi X^2) with X = A - B along the axis.
+template <int LA, int LB>
+OHD void ke1d(double XA, double XB, double h, double xi, double X, double a1,
+              double a2, const double (&S)[LA + 1][LB + 1],
+              double (&T)[LA + 1][LB + 1])
+{
+    T[0][0] = xi * (1.0 - 2.0 * xi * X * X);
+#pragma unroll
+    for (int i = 1; i <= LA; ++i) {
+        double v = XA * T[i - 1][0];
+        double s = S[i][0];
+        if (i > 1) {
+            v += h * (i - 1) * T[i > 1 ? i - 2 : 0][0];
+            s -= (i - 1) / (2.0 * a1) * S[i > 1 ? i - 2 : 0][0];
+        }
+        T[i][0] = v + 2.0 * xi * s;
+    }
+#pragma unroll
+    for (int j = 1; j <= LB; ++j)
+#pragma unroll
+        for (int i = 0; i <= LA; ++i) {
+            double v = XB * T[i][j - 1];
+            double t = 0.0;
+            if (i > 0) t += i * T[i > 0 ? i - 1 : 0][j - 1];
+            double s = S[i][j];
+            if (j > 1) {
+                t += (j - 1) * T[i][j > 1 ? j - 2 : 0];
+                s -= (j - 1) / (2.0 * a2) * S[i][j > 1 ? j - 2 : 0];
+            }
+            T[i][j] = v + h * t + 2.0 * xi * s;
+        }
+}
+
+// ---------------------------------------------------------------------
+// Boys function F_0..F_M with the reference's numerical character
+// (gaussIntegrals.f90:16080-16221): branch point float(0.1); series branch
+// with Gamma(N+1/2+k) rounded to float; closed form
+//   F_m = c_m [ sqrt(pi)/(4^m T^(m+1/2)) erf(sqrt T)
+//               - exp(-T) sum_p (m-p+1)!/(4^(p-1) (2(m-p+1))! T^p) ],
+//   c_m = (2m)!/(2 m!)  (c_1 = 1),  F_0 = sqrt(pi)/(2 sqrt T) erf(sqrt T).
+OHD constexpr double gamma_half(int k)   // Gamma(k + 1/2)
+{
+    double v = 1.7724538509055160273;     // sqrt(pi)
+    for (int i = 0; i < k; ++i) v *= (i + 0.5);
+    return v;
+}
+OHD constexpr double gamma_half_f32(int k) { return (double)(float)gamma_half(k); }
+OHD constexpr double factd(int k)
+{
+    double v = 1.0;
+    for (int i = 2; i <= k; ++i) v *= i;
+    return v;
+}
+OHD constexpr double pow4(int m)
+{
+    double v = 1.0;
+    for (int i = 0; i < m; ++i) v *= 4.0;
+    return v;
+}
+// x^p by binary powering.
+OHD double ipow_sq(double x, int p)
+{
+    double r = 1.0, b = x;
+    bool have = false;
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        if (p & 1) { r = have ? r * b : b; have = true; }
+        p >>= 1;
+        if (p) b = b * b;
+    }
+    return r;
+}
+
+template <int M>
+OHD void boys(double T, double (&F)[M + 1])
+{
+    const double exp_T = exp(-T);
+    const double sqrt_pi = sqrt(kPi);
+    const double sqT = sqrt(T);
+    const double erf_T = erf(sqT);
+    if (T <= (double)(float)0.1) {
+        double Tp[7];
+        Tp[0] = 1.0;
+        Tp[1] = T;
+#pragma unroll
+        for (int p = 2; p <= 6; ++p) Tp[p] = ipow_sq(T, p);
+#pragma unroll
+        for (int N = 0; N <= M; ++N) {
+            double s = (1.0 / gamma_half_f32(N + 1)) + (T / gamma_half_f32(N + 2));
+#pragma unroll
+            for (int p = 2; p <= 6; ++p) s += Tp[p] / gamma_half_f32(N + p + 1);
+            F[N] = 0.5 * exp_T * gamma_half_f32(N) * s;
+        }
+    } else {
+        F[0] = 0.5 * ((sqrt_pi / sqT) * erf_T);
+        double Tm = 1.0;
+#pragma unroll
+        for (int m = 1; m <= M; ++m) {
+            Tm *= T;
+            double lead = (sqrt_pi / (pow4(m) * (ipow_sq(T, m) * sqT))) * erf_T;
+            double sum = 0.0;
+#pragma unroll
+            for (int p = 1; p <= m; ++p) {
+                double den = pow4(p - 1) * factd(2 * (m - p + 1));
+                sum += factd(m - p + 1) / (den * ipow_sq(T, p));
+            }
+            double v = lead - exp_T * sum;
+            F[m] = (m == 1) ? v : (factd(2 * m) / (2.0 * factd(m))) * v;
+        }
+        (void)Tm;
+    }
+}
+
+// ---------------------------------------------------------------------
+// Values that depend only on the primitive pair (a1 at A, a2 at B).
+struct PairConst {
+    double a1, a2, zeta, xi, xid2;
+    double A[3], B[3], P[3], S[3];   // S = a1 A + a2 B
+};
+
+OHD PairConst make_pair(double a1, double a2, const double *A, const double *B)
+{
+    PairConst q;
+    q.a1 = a1;
+    q.a2 = a2;
+    q.zeta = a1 + a2;
+    q.xi = a1 * a2 / q.zeta;
+    double d2 = 0.0;
+    for (int x = 0; x < 3; ++x) {
+        q.A[x] = A[x];
+        q.B[x] = B[x];
+        q.S[x] = a1 * A[x] + a2 * B[x];
+        q.P[x] = q.S[x] / q.zeta;
+        double d = A[x] - B[x];
+        d2 += d * d;
+    }
+    q.xid2 = q.xi * d2;
+    return q;
+}
+
+// Cartesian overlap (KIN = false) or kinetic (KIN = true) block.
+template <int LA, int LB, bool KIN>
+OHD void two_center_cart(const PairConst &q, double (&pc)[ncart(LA)][ncart(LB)])
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB);
+    const double t = kPi / q.zeta;
+    const double pref = t * sqrt(t) * exp(-q.xid2);
+    const double h = 1.0 / (2.0 * q.zeta);
+    double E[3][LA + 1][LB + 1];
+    double Tk[3][LA + 1][LB + 1];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        os1d<LA, LB>(q.P[x] - q.A[x], q.P[x] - q.B[x], h, E[x]);
+        if (KIN)
+            ke1d<LA, LB>(q.P[x] - q.A[x], q.P[x] - q.B[x], h, q.xi, q.A[x] - q.B[x],
+                         q.a1, q.a2, E[x], Tk[x]);
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int ax = cexp(a, 0), ay = cexp(a, 1), az = cexp(a, 2);
+            const int bx = cexp(b, 0), by = cexp(b, 1), bz = cexp(b, 2);
+            double v;
+            if (KIN)
+                v = Tk[0][ax][bx] * E[1][ay][by] * E[2][az][bz] +
+                    E[0][ax][bx] * Tk[1][ay][by] * E[2][az][bz] +
+                    E[0][ax][bx] * E[1][ay][by] * Tk[2][az][bz];
+            else
+                v = E[0][ax][bx] * E[1][ay][by] * E[2][az][bz];
+            pc[a][b] = pref * v;
+        }
+}
+
+// acc += w * three-Gaussian overlap block <g1 | exp(-a3 |r-C|^2) | g2>
+// (electron3CIntg): prefactor (pi/zeta3)^(3/2) exp(-xi d^2 - zeta a3/zeta3
+// |P-C|^2), recursion centre G = (a1 A + a2 B + a3 C)/zeta3, h = 1/(2 zeta3).
+template <int LA, int LB>
+OHD void acc_e3c(const PairConst &q, double a3, const double *C, double w,
+                 double (&acc)[ncart(LA)][ncart(LB)])
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB);
+    const double z3 = q.zeta + a3;
+    const double inv = 1.0 / z3;
+    const double h = 0.5 * inv;
+    double GA[3], GB[3], pc2 = 0.0;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        double G = (q.S[x] + a3 * C[x]) * inv;
+        GA[x] = G - q.A[x];
+        GB[x] = G - q.B[x];
+        double d = q.P[x] - C[x];
+        pc2 += d * d;
+    }
+    const double t = kPi * inv;
+    const double pref = w * t * sqrt(t) * exp(-q.xid2 - (q.zeta * a3 * inv) * pc2);
+    double E[3][LA + 1][LB + 1];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) os1d<LA, LB>(GA[x], GB[x], h, E[x]);
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const double exy = E[0][cexp(a, 0)][cexp(b, 0)] * E[1][cexp(a, 1)][cexp(b, 1)];
+            acc[a][b] += (pref * exy) * E[2][cexp(a, 2)][cexp(b, 2)];
+        }
+}
+
+// acc += w * Gaussian-damped nuclear attraction <g1| exp(-a3 r_C^2)/r_C |g2>
+// (nuclear3CIntg), two-index Obara-Saika with auxiliary index m.
+template <int LA, int LB>
+OHD void acc_nuc(const PairConst &q, double a3, const double *C, double w,
+                 double (&acc)[ncart(LA)][ncart(LB)])
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB), L = LA + LB;
+    const double z3 = q.zeta + a3;
+    const double inv = 1.0 / z3;
+    const double h = 0.5 * inv;
+    double GA[3], GB[3], GC[3], gc2 = 0.0, e = 0.0;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        double G = (q.S[x] + a3 * C[x]) * inv;
+        GA[x] = G - q.A[x];
+        GB[x] = G - q.B[x];
+        GC[x] = G - C[x];
+        gc2 += GC[x] * GC[x];
+        double d1 = q.A[x] - q.B[x], d2 = q.A[x] - C[x], d3 = q.B[x] - C[x];
+        e += q.a1 * q.a2 * (d1 * d1) + q.a1 * a3 * (d2 * d2) + q.a2 * a3 * (d3 * d3);
+    }
+    double F[L + 1];
+    boys<L>(z3 * gc2, F);
+    const double pfac = 2.0 * (kPi * inv) * exp(-e * inv);
+    double th[L + 1][NA][NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            constexpr int dummy = 0;
+            (void)dummy;
+            const int mmax = L - ctot(a) - ctot(b);
+#pragma unroll
+            for (int m = 0; m <= L; ++m) {
+                if (m > mmax) continue;
+                double v;
+                if (a == 0 && b == 0) {
+                    v = F[m] * pfac;
+                } else if (b != 0) {
+                    const int i = cfirst(b);
+                    const int b1 = cdec(b, i, 1);
+                    const int m1 = m < L ? m + 1 : L;
+                    v = GB[i] * th[m][a][b1] - GC[i] * th[m1][a][b1];
+                    double t = 0.0;
+                    if (cexp(a, i) > 0) {
+                        const int a1 = cdec(a, i, 1);
+                        t += cexp(a, i) * (th[m][a1][b1] - th[m1][a1][b1]);
+                    }
+                    if (cexp(b, i) > 1) {
+                        const int b2 = cdec(b, i, 2);
+                        t += (cexp(b, i) - 1) * (th[m][a][b2] - th[m1][a][b2]);
+                    }
+                    v += h * t;
+                } else {
+                    const int i = cfirst(a);
+                    const int a1 = cdec(a, i, 1);
+                    const int m1 = m < L ? m + 1 : L;
+                    v = GA[i] * th[m][a1][0] - GC[i] * th[m1][a1][0];
+                    if (cexp(a, i) > 1) {
+                        const int a2 = cdec(a, i, 2);
+                        v += h * (cexp(a, i) - 1) * (th[m][a2][0] - th[m1][a2][0]);
+                    }
+                }
+                th[m][a][b] = v;
+            }
+        }
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[a][b] += w * th[0][a][b];
+}
+
+}  // namespace olcao

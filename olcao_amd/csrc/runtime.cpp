@@ -1,0 +1,1003 @@
+// Host runtime of the OLCAO MI355X path: derived setup, screened work
+// lists, device residency, kernel orchestration and the batched FP64
+// generalised eigensolve.  Exposes the C-ABI declared in include/olcao_gpu.h.
+//
+// Derived setup restated from the reference:
+//   basis / lmIndex / lmAlphaIndex   src/uolcao/basis.f90:55-243
+//   reciprocal vectors               src/uolcao/lattice.f90:174-237
+//   cell list (radius, border cells, sort by |L|^2)
+//                                    src/uolcao/lattice.f90:310-443, 469-781
+//   Cartesian k-points, phases       src/uolcao/kpoints.f90:424-489
+//   screening radii                  src/uolcao/gaussRelations.f90:64-135
+//   pair / cell / alpha-pair walk    src/uolcao/integrals.F90:323-570
+// With ascending exponents the reference's alpha walk visits exactly the
+// pairs with alphaDist(a1,a2) >= s^2, which is the list built here.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/olcao_gpu.h"
+#include "olcao_internal.h"
+
+using namespace olcao;
+
+namespace {
+
+constexpr double kPiHost = 3.141592653589793;
+
+struct Fail {
+    olcao_status st;
+    std::string msg;
+};
+
+[[noreturn]] void fail(olcao_status st, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    throw Fail{st, buf};
+}
+
+#define HIPCHK(x)                                                                     \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) fail(OLCAO_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            fail(OLCAO_ERR_ALLOC, "hipMalloc of %zu bytes failed", count * sizeof(T));
+        }
+        n = count;
+    }
+    void upload(const std::vector<T> &v, hipStream_t s)
+    {
+        alloc(v.size());
+        if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T),
+                                              hipMemcpyHostToDevice, s));
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { release(); }
+};
+
+double angnorm(int lm)   // lm = 1..16, basis.f90:86-101
+{
+    const double pi = kPiHost;
+    switch (lm) {
+    case 1: return std::sqrt(1.0 / (4 * pi));
+    case 2: case 3: case 4: return std::sqrt(3.0 / (4 * pi));
+    case 5: case 6: case 7: return std::sqrt(15.0 / (4 * pi));
+    case 8: return std::sqrt(15.0 / (16 * pi));
+    case 9: return std::sqrt(5.0 / (16 * pi));
+    case 10: return std::sqrt(105.0 / (4 * pi));
+    case 11: return std::sqrt(105.0 / (16 * pi));
+    case 12: case 13: return std::sqrt(35.0 / (32 * pi));
+    case 14: return std::sqrt(7.0 / (16 * pi));
+    default: return std::sqrt(21.0 / (32 * pi));
+    }
+}
+
+int lclass(int lmAlphaIndex) { return lmAlphaIndex == 1 ? 0 : lmAlphaIndex == 4 ? 1 : lmAlphaIndex == 9 ? 2 : 3; }
+int nshh(int l) { return (l + 1) * (l + 1); }
+
+}  // namespace
+
+struct olcao_ctx {
+    std::string err;
+    int precision = 0;
+    bool gamma = false;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    hipEvent_t ev[8] = {};
+
+    // host-side derived setup
+    int nAtoms = 0, nT = 0, maxA = 0, maxS = 0, nPT = 0, maxPA = 0, nPS = 0;
+    int valeDim = 0, coreDim = 0, potDim = 0, nk = 0;
+    double tau = 0.0, vec[9] = {}, inv[9] = {}, recip[9] = {};
+    std::vector<double> hPos, hAlpha, hElemAlpha, hBasis, hPotAlpha, hPotZ, hPotNuc, hPotPos;
+    std::vector<int> hAtomType, hLmIndex, hLmAlpha, hNVale, hNCore, hCumV, hCumC, hPotNA,
+        hPotCumul, hPotSiteType, hNAlpha;
+    std::vector<double> hCellDims, hCellSizes, hKcart, hPhase, hCorr;
+    // work lists
+    std::vector<int> hPairI, hPairJ, hPairPc, hPcPair, hPcCell, hPcPp, hPpPc, hPpAB;
+    std::vector<double> hPairLv, hPcB, hPcS2;
+    std::vector<long long> hPpOff;
+    long long blockDoubles = 0;
+    std::vector<std::vector<int>> classIdx;
+
+    // device
+    DBuf<double> dPos, dAlpha, dElemAlpha, dBasis, dPotAlpha, dPotZ, dPotNuc, dPotPos, dCellDims,
+        dCellSizes, dKcart, dPhase, dPairLv, dPcB, dPcS2;
+    DBuf<int> dAtomType, dLmIndex, dNVale, dNCore, dCumV, dCumC, dPotNA, dPotCumul, dPotSiteType,
+        dPairI, dPairJ, dPairPc, dPcPair, dPcCell, dPcPp, dPpPc, dPpAB;
+    DBuf<long long> dPpOff;
+    DBuf<int> dClass[16];
+    DBuf<double> dBlocks, dX, dCoef;
+    DBuf<double> S, H, Aw, Bw, D, E, CV_S, CC_S, CV_H, CC_H;
+    DBuf<int> dInfo;
+    DBuf<unsigned long long> dCounters;
+    // Mode A term matrices
+    DBuf<double> KEm, NPm;
+    std::vector<DBuf<double>> Vm;
+    DBuf<double *> dVptr;
+    bool haveS = false, haveH = false, haveScf = false;
+    double timings[8] = {};
+
+    DevCtx dev{};
+    ClassLists cls{};
+
+    size_t matDoubles() const { return (size_t)(gamma ? 1 : 2) * valeDim * valeDim * (gamma ? 1 : nk); }
+    size_t cvDoubles() const { return (size_t)(gamma ? 1 : 2) * coreDim * valeDim * (gamma ? 1 : nk); }
+    size_t ccDoubles() const { return (size_t)(gamma ? 1 : 2) * coreDim * coreDim * (gamma ? 1 : nk); }
+};
+
+namespace {
+
+void inverse3(const double *v, double *inv)
+{
+    const double *a = v, *b = v + 3, *c = v + 6;
+    double bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+    double ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+    double ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    double vol = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+    for (int x = 0; x < 3; ++x) {
+        inv[x] = bc[x] / vol;
+        inv[3 + x] = ca[x] / vol;
+        inv[6 + x] = ab[x] / vol;
+    }
+}
+
+double sq3(const double *v) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
+
+void nearest_host(const olcao_ctx &c, const double *v, double *out)
+{
+    double base[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i) {
+        double f = v[0] * c.inv[3 * i] + v[1] * c.inv[3 * i + 1] + v[2] * c.inv[3 * i + 2];
+        double r = std::trunc(std::fabs(f) + 0.5);
+        if (f < 0.0) r = -r;
+        for (int x = 0; x < 3; ++x) base[x] += c.vec[3 * i + x] * r;
+    }
+    double rem[3] = {v[0] - base[0], v[1] - base[1], v[2] - base[2]};
+    double best = sq3(rem);
+    int bi = -1;
+    int nc = (int)c.hCorr.size() / 3;
+    for (int q = 0; q < nc; ++q) {
+        double r[3] = {rem[0] - c.hCorr[3 * q], rem[1] - c.hCorr[3 * q + 1], rem[2] - c.hCorr[3 * q + 2]};
+        double s = sq3(r);
+        if (s < best * (1.0 - 1e-14)) {
+            best = s;
+            bi = q;
+        }
+    }
+    for (int x = 0; x < 3; ++x) out[x] = base[x] + (bi >= 0 ? c.hCorr[3 * bi + x] : 0.0);
+}
+
+// ---------------------------------------------------------------------------
+void setup_system(olcao_ctx &c, const olcao_system &s)
+{
+    if (s.nAtoms <= 0 || s.nTypes <= 0 || s.nPotTypes <= 0 || s.maxAlphas <= 0 ||
+        s.maxAlphas > 255)
+        fail(OLCAO_ERR_INPUT, "invalid counts (nAtoms=%d nTypes=%d nPotTypes=%d maxAlphas=%d)",
+             s.nAtoms, s.nTypes, s.nPotTypes, s.maxAlphas);
+    if (!(s.basisCutoff > 0.0 && s.basisCutoff < 1.0))
+        fail(OLCAO_ERR_INPUT, "basisCutoff must lie in (0,1)");
+    c.gamma = s.gamma != 0;
+    c.nAtoms = s.nAtoms;
+    c.nT = s.nTypes;
+    c.maxA = s.maxAlphas;
+    c.tau = -std::log(s.basisCutoff);   // setCutoffThresh, lattice.f90:293
+    std::memcpy(c.vec, s.cellVectors, sizeof(c.vec));
+    inverse3(c.vec, c.inv);
+    // reciprocal vectors b_i = 2 pi inv_i
+    for (int i = 0; i < 9; ++i) c.recip[i] = 2.0 * kPiHost * c.inv[i];
+    c.hPos.assign(s.atomPos, s.atomPos + 3 * s.nAtoms);
+    c.hAtomType.assign(s.atomType, s.atomType + s.nAtoms);
+    for (int a : c.hAtomType)
+        if (a < 0 || a >= s.nTypes) fail(OLCAO_ERR_INPUT, "atom type out of range");
+
+    // --- basis (basis.f90:55-243) ---
+    int maxS = 1;
+    for (int t = 0; t < s.nTypes; ++t) {
+        int ns = 0;
+        for (int r = 0; r < s.typeNumValeRadial[t]; ++r) ns += 2 * s.typeValeL[t * s.maxRadial + r] + 1;
+        for (int r = 0; r < s.typeNumCoreRadial[t]; ++r) ns += 2 * s.typeCoreL[t * s.maxRadial + r] + 1;
+        maxS = std::max(maxS, ns);
+    }
+    c.maxS = maxS;
+    const int mA = c.maxA;
+    c.hAlpha.assign((size_t)s.nTypes * mA, 0.0);
+    c.hElemAlpha.assign((size_t)s.nTypes * mA, 0.0);
+    c.hLmAlpha.assign((size_t)s.nTypes * mA, 0);
+    c.hLmIndex.assign((size_t)s.nTypes * maxS, 0);
+    c.hBasis.assign((size_t)s.nTypes * maxS * mA, 0.0);
+    c.hNVale.assign(s.nTypes, 0);
+    c.hNCore.assign(s.nTypes, 0);
+    c.hNAlpha.assign(s.typeNumAlphas, s.typeNumAlphas + s.nTypes);
+    for (int t = 0; t < s.nTypes; ++t) {
+        const int na = s.typeNumAlphas[t];
+        if (na <= 0 || na > mA) fail(OLCAO_ERR_INPUT, "type %d: bad alpha count", t);
+        const double *al = s.typeAlphas + (size_t)t * mA;
+        for (int a = 0; a < na; ++a) {
+            if (!(al[a] > 0.0) || (a > 0 && !(al[a] > al[a - 1])))
+                fail(OLCAO_ERR_INPUT, "type %d: alphas must be positive and strictly ascending", t);
+            c.hAlpha[t * mA + a] = al[a];
+        }
+        const int *no = s.typeNumOrbAlphas + 4 * t;
+        for (int a = 0; a < na; ++a)
+            c.hLmAlpha[t * mA + a] = a < no[3] ? 16 : a < no[2] ? 9 : a < no[1] ? 4 : 1;
+        int st = 0;
+        auto add = [&](int l, const double *coef) {
+            if (l < 0 || l > 3) fail(OLCAO_ERR_INPUT, "type %d: l=%d unsupported", t, l);
+            const int nal = no[l];
+            if (nal > na) fail(OLCAO_ERR_INPUT, "type %d: numOrbAlphas exceeds alphas", t);
+            for (int k = 1; k <= 2 * l + 1; ++k) {
+                const int lm = l * l + k;
+                c.hLmIndex[t * maxS + st] = lm;
+                for (int a = 0; a < nal; ++a) c.hBasis[((size_t)t * maxS + st) * mA + a] = coef[a] * angnorm(lm);
+                ++st;
+            }
+        };
+        for (int r = 0; r < s.typeNumValeRadial[t]; ++r)
+            add(s.typeValeL[t * s.maxRadial + r], s.typeValeCoef + ((size_t)t * s.maxRadial + r) * mA);
+        c.hNVale[t] = st;
+        for (int r = 0; r < s.typeNumCoreRadial[t]; ++r)
+            add(s.typeCoreL[t * s.maxRadial + r], s.typeCoreCoef + ((size_t)t * s.maxRadial + r) * mA);
+        c.hNCore[t] = st - c.hNVale[t];
+    }
+    // screening exponents: those of the first type of each element
+    // (makeAlphaDist, gaussRelations.f90:89-99)
+    for (int t = 0; t < s.nTypes; ++t) {
+        int first = t;
+        for (int u = 0; u < t; ++u)
+            if (s.typeElement[u] == s.typeElement[t]) { first = u; break; }
+        for (int a = 0; a < mA; ++a) c.hElemAlpha[t * mA + a] = c.hAlpha[first * mA + a];
+        if (first != t && s.typeNumAlphas[first] < s.typeNumAlphas[t])
+            fail(OLCAO_ERR_INPUT, "types of one element must share their alphas");
+    }
+    c.hCumV.assign(s.nAtoms, 0);
+    c.hCumC.assign(s.nAtoms, 0);
+    for (int a = 0; a < s.nAtoms; ++a) {
+        c.hCumV[a] = c.valeDim;
+        c.hCumC[a] = c.coreDim;
+        c.valeDim += c.hNVale[c.hAtomType[a]];
+        c.coreDim += c.hNCore[c.hAtomType[a]];
+    }
+
+    // --- potential ---
+    c.nPT = s.nPotTypes;
+    c.maxPA = s.maxPotAlphas;
+    c.nPS = s.nPotSites;
+    c.hPotNA.assign(s.potNumAlphas, s.potNumAlphas + s.nPotTypes);
+    c.hPotAlpha.assign(s.potAlphas, s.potAlphas + (size_t)s.nPotTypes * s.maxPotAlphas);
+    c.hPotZ.assign(s.potZ, s.potZ + s.nPotTypes);
+    c.hPotNuc.assign(s.potNucAlpha, s.potNucAlpha + s.nPotTypes);
+    c.hPotCumul.assign(s.nPotTypes, 0);
+    for (int t = 0; t < s.nPotTypes; ++t) {
+        c.hPotCumul[t] = c.potDim;
+        c.potDim += s.potNumAlphas[t];
+        for (int a = 0; a < s.potNumAlphas[t]; ++a) {
+            double v = s.potAlphas[(size_t)t * s.maxPotAlphas + a];
+            if (!(v > 0.0) || (a > 0 && !(v > s.potAlphas[(size_t)t * s.maxPotAlphas + a - 1])))
+                fail(OLCAO_ERR_INPUT, "pot type %d: alphas must be positive, ascending", t);
+        }
+        if (!(s.potNucAlpha[t] > 0.0)) fail(OLCAO_ERR_INPUT, "pot type %d: nuclear alpha", t);
+    }
+    c.hPotPos.assign(s.potPos, s.potPos + 3 * (size_t)s.nPotSites);
+    c.hPotSiteType.assign(s.potSiteType, s.potSiteType + s.nPotSites);
+    for (int i = 0; i < s.nPotSites; ++i) {
+        int t = c.hPotSiteType[i];
+        if (t < 0 || t >= s.nPotTypes) fail(OLCAO_ERR_INPUT, "pot site type out of range");
+        if (i > 0 && t < c.hPotSiteType[i - 1]) fail(OLCAO_ERR_INPUT, "pot sites must be grouped by type");
+    }
+
+    // --- k-points (kpoints.f90:424-454) ---
+    c.nk = c.gamma ? 1 : s.nk;
+    if (!c.gamma && s.nk <= 0) fail(OLCAO_ERR_INPUT, "nk must be positive");
+    c.hKcart.assign(3 * (size_t)c.nk, 0.0);
+    if (!c.gamma)
+        for (int k = 0; k < s.nk; ++k)
+            for (int x = 0; x < 3; ++x)
+                c.hKcart[3 * k + x] = s.kpoints[3 * k] * c.recip[x] + s.kpoints[3 * k + 1] * c.recip[3 + x] +
+                                      s.kpoints[3 * k + 2] * c.recip[6 + x];
+
+    // --- cell list (lattice.f90:310-443, 469-781) ---
+    double minA = 1e300, minP = 1e300;
+    for (int t = 0; t < s.nTypes; ++t)
+        for (int a = 0; a < s.typeNumAlphas[t]; ++a) minA = std::min(minA, c.hAlpha[t * mA + a]);
+    for (int t = 0; t < s.nPotTypes; ++t)
+        for (int a = 0; a < s.potNumAlphas[t]; ++a)
+            minP = std::min(minP, s.potAlphas[(size_t)t * s.maxPotAlphas + a]);
+    double nl = c.tau * std::max(2.0 / minA, 1.0 / minP + 0.5 / minA);
+    double maxSq = 0.0;
+    for (int i = 0; i < 3; ++i) maxSq = std::max(maxSq, sq3(c.vec + 3 * i));
+    nl = nl + 0.75 * maxSq + std::sqrt(3.0 * maxSq * nl);
+    nl = std::max(nl, 2.0 * c.tau / minA);
+    int reps[3];
+    for (int i = 0; i < 3; ++i) reps[i] = 2 * (int)std::ceil(std::sqrt(nl) / std::sqrt(sq3(c.vec + 3 * i))) + 1;
+    double diag[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i)
+        for (int x = 0; x < 3; ++x) diag[x] += c.vec[3 * i + x];
+    struct Cell { double size; int idx; double L[3]; };
+    std::vector<Cell> cells;
+    int gen = 0;
+    for (int i = 1; i <= reps[0]; ++i)
+        for (int j = 1; j <= reps[1]; ++j)
+            for (int k = 1; k <= reps[2]; ++k) {
+                int rep[3] = {i - (1 + reps[0]) / 2, j - (1 + reps[1]) / 2, k - (1 + reps[2]) / 2};
+                double L[3] = {0, 0, 0};
+                for (int l = 0; l < 3; ++l)
+                    for (int m = 0; m < 3; ++m) L[m] += c.vec[3 * l + m] * rep[l];
+                double ss = sq3(L);
+                if (ss > nl) {
+                    // seven box corners reached through the x, y, z parts of a1+a2+a3
+                    double t[3] = {L[0], L[1], L[2]};
+                    bool in = false;
+                    const int seq[7][2] = {{0, 1}, {1, 1}, {0, -1}, {2, 1}, {0, 1}, {1, -1}, {0, -1}};
+                    for (int q = 0; q < 7 && !in; ++q) {
+                        t[seq[q][0]] += seq[q][1] * diag[seq[q][0]];
+                        if (sq3(t) < nl) in = true;
+                    }
+                    if (!in) continue;
+                }
+                Cell cc{ss, gen++, {L[0], L[1], L[2]}};
+                cells.push_back(cc);
+            }
+    std::stable_sort(cells.begin(), cells.end(), [](const Cell &a, const Cell &b) { return a.size < b.size; });
+    c.hCellDims.resize(3 * cells.size());
+    c.hCellSizes.resize(cells.size());
+    for (size_t q = 0; q < cells.size(); ++q) {
+        c.hCellSizes[q] = cells[q].size;
+        for (int x = 0; x < 3; ++x) c.hCellDims[3 * q + x] = cells[q].L[x];
+    }
+    const int nCells = (int)cells.size();
+
+    // --- phases exp(i k.L) (kpoints.f90:459-489) ---
+    if (!c.gamma) {
+        c.hPhase.resize(2 * (size_t)nCells * c.nk);
+        for (int q = 0; q < nCells; ++q)
+            for (int k = 0; k < c.nk; ++k) {
+                const double *kp = &c.hKcart[3 * k];
+                const double *L = &c.hCellDims[3 * q];
+                double dot = kp[0] * L[0] + kp[1] * L[1] + kp[2] * L[2];
+                c.hPhase[2 * ((size_t)q * c.nk + k)] = std::cos(dot);
+                c.hPhase[2 * ((size_t)q * c.nk + k) + 1] = std::sin(dot);
+            }
+    } else {
+        c.hPhase.assign(2, 0.0);
+    }
+
+    // --- nearest-image corrections: lattice vectors v (|n_i| <= 1) that can be
+    // nearer than the rounded point for some residual in the cell:
+    // sum_i |a_i . v| > |v|^2 ---
+    c.hCorr.clear();
+    for (int i = -1; i <= 1; ++i)
+        for (int j = -1; j <= 1; ++j)
+            for (int k = -1; k <= 1; ++k) {
+                if (!i && !j && !k) continue;
+                double v[3];
+                for (int x = 0; x < 3; ++x) v[x] = i * c.vec[x] + j * c.vec[3 + x] + k * c.vec[6 + x];
+                double s = 0.0;
+                for (int a = 0; a < 3; ++a)
+                    s += std::fabs(c.vec[3 * a] * v[0] + c.vec[3 * a + 1] * v[1] + c.vec[3 * a + 2] * v[2]);
+                if (s > sq3(v) * (1.0 + 1e-12))
+                    for (int x = 0; x < 3; ++x) c.hCorr.push_back(v[x]);
+            }
+}
+
+// Screened work lists (integrals.F90:323-570).
+void build_lists(olcao_ctx &c)
+{
+    const int mA = c.maxA;
+    const int nCells = (int)c.hCellSizes.size();
+    const double maxCell = c.hCellSizes.empty() ? 0.0 : c.hCellSizes.back();
+    c.hPairI.clear(); c.hPairJ.clear(); c.hPairLv.clear(); c.hPairPc.assign(1, 0);
+    c.hPcPair.clear(); c.hPcCell.clear(); c.hPcB.clear(); c.hPcS2.clear(); c.hPcPp.assign(1, 0);
+    c.hPpPc.clear(); c.hPpAB.clear(); c.hPpOff.clear();
+    c.classIdx.assign(16, {});
+    long long off = 0;
+    for (int i = 0; i < c.nAtoms; ++i) {
+        const int ti = c.hAtomType[i];
+        const double *A = &c.hPos[3 * i];
+        for (int j = i; j < c.nAtoms; ++j) {
+            const int tj = c.hAtomType[j];
+            const double *Bj = &c.hPos[3 * j];
+            double dv[3] = {A[0] - Bj[0], A[1] - Bj[1], A[2] - Bj[2]};
+            double lv[3];
+            nearest_host(c, dv, lv);
+            double r0[3] = {A[0] - Bj[0] - lv[0], A[1] - Bj[1] - lv[1], A[2] - Bj[2] - lv[2]};
+            const double sep = sq3(r0);
+            const double *ea1 = &c.hElemAlpha[ti * mA], *ea2 = &c.hElemAlpha[tj * mA];
+            const double lim = c.tau / ea2[0] / (ea1[0] / (ea1[0] + ea2[0]));
+            if (sep > lim) continue;
+            const double maxLR = sep + lim + 2.0 * std::sqrt(sep * lim);
+            if (maxLR > maxCell)
+                fail(OLCAO_ERR_CELLS, "more lattice points needed for atom pair (%d,%d): "
+                     "maxLatticeRadius=%g available=%g", i, j, maxLR, maxCell);
+            const int pairIdx = (int)c.hPairI.size();
+            bool anyPc = false;
+            for (int k = 0; k < nCells; ++k) {
+                if (c.hCellSizes[k] > maxLR) break;
+                const double *L = &c.hCellDims[3 * k];
+                double B[3] = {Bj[0] + lv[0] + L[0], Bj[1] + lv[1] + L[1], Bj[2] + lv[2] + L[2]};
+                double d[3] = {A[0] - B[0], A[1] - B[1], A[2] - B[2]};
+                const double s2 = sq3(d);
+                if (s2 > lim) continue;
+                const int pc = (int)c.hPcPair.size();
+                c.hPcPair.push_back(pairIdx);
+                c.hPcCell.push_back(k);
+                c.hPcB.insert(c.hPcB.end(), B, B + 3);
+                c.hPcS2.push_back(s2);
+                const int na1 = c.hNAlpha[ti], na2 = c.hNAlpha[tj];
+                for (int a1 = 0; a1 < na1; ++a1) {
+                    if (c.tau / ea2[0] / (ea1[a1] / (ea1[a1] + ea2[0])) < s2) break;
+                    for (int a2 = 0; a2 < na2; ++a2) {
+                        const double ad = c.tau / ea2[a2] / (ea1[a1] / (ea1[a1] + ea2[a2]));
+                        if (ad < s2) break;
+                        const int la = lclass(c.hLmAlpha[ti * mA + a1]);
+                        const int lb = lclass(c.hLmAlpha[tj * mA + a2]);
+                        const int pp = (int)c.hPpPc.size();
+                        c.hPpPc.push_back(pc);
+                        c.hPpAB.push_back(a1 | (a2 << 8) | (la << 16) | (lb << 20));
+                        c.hPpOff.push_back(off);
+                        off += (long long)nshh(la) * nshh(lb);
+                        c.classIdx[la * 4 + lb].push_back(pp);
+                    }
+                }
+                c.hPcPp.push_back((int)c.hPpPc.size());
+                anyPc = true;
+            }
+            if (anyPc) {
+                c.hPairI.push_back(i);
+                c.hPairJ.push_back(j);
+                c.hPairLv.insert(c.hPairLv.end(), lv, lv + 3);
+                c.hPairPc.push_back((int)c.hPcPair.size());
+            }
+        }
+    }
+    c.blockDoubles = off;
+}
+
+void upload(olcao_ctx &c)
+{
+    hipStream_t s = c.stream;
+    c.dPos.upload(c.hPos, s); c.dAtomType.upload(c.hAtomType, s);
+    c.dCumV.upload(c.hCumV, s); c.dCumC.upload(c.hCumC, s);
+    c.dAlpha.upload(c.hAlpha, s); c.dElemAlpha.upload(c.hElemAlpha, s);
+    c.dBasis.upload(c.hBasis, s); c.dLmIndex.upload(c.hLmIndex, s);
+    c.dNVale.upload(c.hNVale, s); c.dNCore.upload(c.hNCore, s);
+    c.dPotNA.upload(c.hPotNA, s); c.dPotAlpha.upload(c.hPotAlpha, s);
+    c.dPotZ.upload(c.hPotZ, s); c.dPotNuc.upload(c.hPotNuc, s);
+    c.dPotCumul.upload(c.hPotCumul, s); c.dPotPos.upload(c.hPotPos, s);
+    c.dPotSiteType.upload(c.hPotSiteType, s);
+    c.dCellDims.upload(c.hCellDims, s); c.dCellSizes.upload(c.hCellSizes, s);
+    c.dKcart.upload(c.hKcart, s); c.dPhase.upload(c.hPhase, s);
+    c.dPairI.upload(c.hPairI, s); c.dPairJ.upload(c.hPairJ, s);
+    c.dPairLv.upload(c.hPairLv, s); c.dPairPc.upload(c.hPairPc, s);
+    c.dPcPair.upload(c.hPcPair, s); c.dPcCell.upload(c.hPcCell, s);
+    c.dPcB.upload(c.hPcB, s); c.dPcS2.upload(c.hPcS2, s); c.dPcPp.upload(c.hPcPp, s);
+    c.dPpPc.upload(c.hPpPc, s); c.dPpAB.upload(c.hPpAB, s); c.dPpOff.upload(c.hPpOff, s);
+    for (int k = 0; k < 16; ++k) {
+        c.dClass[k].upload(c.classIdx[k], s);
+        c.cls.idx[k] = c.dClass[k].p;
+        c.cls.count[k] = (int)c.classIdx[k].size();
+    }
+    c.dBlocks.alloc(std::max<long long>(c.blockDoubles, 1));
+    c.dX.alloc(std::max<size_t>(c.hPcPair.size() * (size_t)c.maxS * c.maxS, 1));
+    c.dCoef.alloc(std::max(c.potDim, 1));
+    c.dCounters.alloc(32);
+
+    DevCtx &d = c.dev;
+    d.pos = c.dPos.p; d.atomType = c.dAtomType.p; d.cumulVale = c.dCumV.p; d.cumulCore = c.dCumC.p;
+    d.valeDim = c.valeDim; d.coreDim = c.coreDim; d.maxA = c.maxA; d.maxS = c.maxS;
+    d.alpha = c.dAlpha.p; d.elemAlpha = c.dElemAlpha.p; d.basis = c.dBasis.p;
+    d.lmIndex = c.dLmIndex.p; d.nVale = c.dNVale.p; d.nCore = c.dNCore.p;
+    d.nPotTypes = c.nPT; d.maxPA = c.maxPA; d.nPotSites = c.nPS;
+    d.potNA = c.dPotNA.p; d.potAlpha = c.dPotAlpha.p; d.potZ = c.dPotZ.p; d.potNuc = c.dPotNuc.p;
+    d.potCumul = c.dPotCumul.p; d.potPos = c.dPotPos.p; d.potSiteType = c.dPotSiteType.p;
+    std::memcpy(d.vec, c.vec, sizeof(d.vec));
+    std::memcpy(d.inv, c.inv, sizeof(d.inv));
+    d.nCorr = (int)c.hCorr.size() / 3;
+    for (size_t q = 0; q < c.hCorr.size(); ++q) d.corr[q] = c.hCorr[q];
+    d.nCells = (int)c.hCellSizes.size();
+    d.cellDims = c.dCellDims.p; d.cellSizes = c.dCellSizes.p;
+    d.nk = c.nk; d.kcart = c.dKcart.p; d.phase = c.dPhase.p; d.tau = c.tau;
+    d.nPairs = (int)c.hPairI.size();
+    d.pairI = c.dPairI.p; d.pairJ = c.dPairJ.p; d.pairLv = c.dPairLv.p; d.pairPc = c.dPairPc.p;
+    d.nPc = (int)c.hPcPair.size();
+    d.pcPair = c.dPcPair.p; d.pcCell = c.dPcCell.p; d.pcB = c.dPcB.p; d.pcS2 = c.dPcS2.p;
+    d.pcPp = c.dPcPp.p;
+    d.nPp = (long long)c.hPpPc.size();
+    d.ppPc = c.dPpPc.p; d.ppAB = c.dPpAB.p; d.ppOff = c.dPpOff.p;
+    d.counters = c.dCounters.p;
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+// Three-centre sweep with its work counters cleared first (a fill kernel,
+// so the sequence stays capturable in a graph).
+void three_center(olcao_ctx &c, int kind, int tt, int ti, const double *coef, int accumulate)
+{
+    launch_fill(reinterpret_cast<double *>(c.dCounters.p), 32, 0.0, c.stream);
+    launch_three_center(c.dev, c.cls, kind, tt, ti, coef, accumulate, c.dBlocks.p, c.stream);
+}
+
+// ---------------------------------------------------------------------------
+// One integral sweep into (vv, cv, cc) device matrices.
+enum Sweep { SW_OL, SW_KE, SW_NP, SW_EP, SW_HAM };
+
+void sweep(olcao_ctx &c, Sweep kind, int term, double *vv, double *cv, double *cc)
+{
+    hipStream_t s = c.stream;
+    const DevCtx &d = c.dev;
+    switch (kind) {
+    case SW_OL: launch_two_center(d, c.cls, OP_OL, c.dBlocks.p, s); break;
+    case SW_KE: launch_two_center(d, c.cls, OP_KE, c.dBlocks.p, s); break;
+    case SW_NP:
+        three_center(c, K3_NP, 0, 0, nullptr, 0);
+        break;
+    case SW_EP: {
+        int t = 0;
+        while (t + 1 < c.nPT && c.hPotCumul[t + 1] <= term) ++t;
+        three_center(c, K3_EP, t, term - c.hPotCumul[t], nullptr, 0);
+        break;
+    }
+    case SW_HAM:
+        launch_two_center(d, c.cls, OP_KE, c.dBlocks.p, s);
+        three_center(c, K3_HAM, 0, 0, c.dCoef.p, 1);
+        break;
+    }
+    launch_contract(d, c.dBlocks.p, c.dX.p, s);
+    launch_fill(vv, (long long)c.matDoubles(), 0.0, s);
+    if (c.coreDim) {
+        launch_fill(cv, (long long)c.cvDoubles(), 0.0, s);
+        launch_fill(cc, (long long)c.ccDoubles(), 0.0, s);
+    }
+    launch_bloch_scatter(d, c.dX.p, vv, cv, cc, c.gamma ? 1 : 0, s);
+    HIPCHK(hipGetLastError());
+}
+
+void copy_out(olcao_ctx &c, const DBuf<double> &src, double *dst, size_t n)
+{
+    if (!dst || n == 0) return;
+    HIPCHK(hipMemcpyAsync(dst, src.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+}
+
+void ensure_mats(olcao_ctx &c)
+{
+    c.S.alloc(c.matDoubles());
+    c.H.alloc(c.matDoubles());
+    if (c.coreDim) {
+        c.CV_S.alloc(c.cvDoubles()); c.CC_S.alloc(c.ccDoubles());
+        c.CV_H.alloc(c.cvDoubles()); c.CC_H.alloc(c.ccDoubles());
+    }
+}
+
+// Batched generalised eigensolve of H, S (upper) on the device.
+void solve_device(olcao_ctx &c, int numStates, const double *Hsrc, const double *Ssrc,
+                  double *evals, double *evecs)
+{
+    hipStream_t s = c.stream;
+    const int n = c.valeDim, nk = c.nk;
+    if (numStates <= 0 || numStates > n) fail(OLCAO_ERR_INPUT, "numStates must lie in 1..%d", n);
+    if (c.coreDim) fail(OLCAO_ERR_UNSUPPORTED, "core orthogonalisation is not in this build; "
+                        "use coreDim = 0 systems for the solve");
+    c.Aw.alloc(c.matDoubles());
+    c.Bw.alloc(c.matDoubles());
+    c.D.alloc((size_t)n * nk);
+    c.E.alloc((size_t)n * nk);
+    c.dInfo.alloc(nk);
+    HIPCHK(hipMemcpyAsync(c.Aw.p, Hsrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
+    rocblas_status st;
+    if (c.gamma) {
+        st = rocsolver_dsygvd(c.blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n,
+                              c.Aw.p, n, c.Bw.p, n, c.D.p, c.E.p, c.dInfo.p);
+    } else {
+        st = rocsolver_zhegvd_strided_batched(
+            c.blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n,
+            (rocblas_double_complex *)c.Aw.p, n, (rocblas_stride)n * n,
+            (rocblas_double_complex *)c.Bw.p, n, (rocblas_stride)n * n, c.D.p, n, c.E.p, n,
+            c.dInfo.p, nk);
+    }
+    if (st != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "rocsolver hegvd status %d", (int)st);
+    std::vector<int> info(nk);
+    HIPCHK(hipMemcpyAsync(info.data(), c.dInfo.p, nk * sizeof(int), hipMemcpyDeviceToHost, s));
+    std::vector<double> w((size_t)n * nk);
+    HIPCHK(hipMemcpyAsync(w.data(), c.D.p, w.size() * 8, hipMemcpyDeviceToHost, s));
+    if (evecs) {
+        DBuf<double> tmp;
+        const int cp = c.gamma ? 1 : 2;
+        tmp.alloc((size_t)nk * numStates * n * cp);
+        launch_extract_evecs(c.Aw.p, n, nk, numStates, cp, tmp.p, s);
+        HIPCHK(hipMemcpyAsync(evecs, tmp.p, tmp.n * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    for (int k = 0; k < nk; ++k) {
+        if (info[k] > n) fail(OLCAO_ERR_NOT_SPD, "S(k) not positive definite at k=%d (info=%d)", k, info[k]);
+        if (info[k] != 0) fail(OLCAO_ERR_SOLVER, "eigensolver did not converge at k=%d (info=%d)", k, info[k]);
+    }
+    if (evals)
+        for (int k = 0; k < nk; ++k)
+            std::memcpy(evals + (size_t)k * numStates, w.data() + (size_t)k * n, numStates * sizeof(double));
+}
+
+thread_local std::string g_noctx_err;   // message of context-free calls
+
+template <class F>
+olcao_status guard(olcao_ctx *c, F &&f)
+{
+    std::string &msg = c ? c->err : g_noctx_err;
+    try {
+        f();
+        msg.clear();
+        return OLCAO_OK;
+    } catch (const Fail &e) {
+        msg = e.msg;
+        return e.st;
+    } catch (const std::exception &e) {
+        msg = e.what();
+        return OLCAO_ERR_ALLOC;
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt, olcao_ctx **out)
+{
+    if (!sys || !out) return OLCAO_ERR_INPUT;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_noctx_err = "no GPU visible";
+        return OLCAO_ERR_NO_DEVICE;
+    }
+    olcao_ctx *c = new olcao_ctx();
+    olcao_status st = guard(c, [&] {
+        c->precision = opt ? opt->precision : 0;
+        if (c->precision != OLCAO_PREC_FP64)
+            fail(OLCAO_ERR_UNSUPPORTED, "only the FP64 precision form is available");
+        HIPCHK(hipStreamCreate(&c->stream));
+        for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+        if (rocblas_create_handle(&c->blas) != rocblas_status_success)
+            fail(OLCAO_ERR_HIP, "rocblas_create_handle failed");
+        rocblas_set_stream(c->blas, c->stream);
+        setup_system(*c, *sys);
+        build_lists(*c);
+        upload(*c);
+    });
+    if (st != OLCAO_OK) {
+        g_noctx_err = c->err;
+        olcao_ctx_destroy(c);
+        return st;
+    }
+    *out = c;
+    return OLCAO_OK;
+}
+
+void olcao_ctx_destroy(olcao_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->blas) rocblas_destroy_handle(c->blas);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    hipStream_t s = c->stream;
+    delete c;
+    if (s) (void)hipStreamDestroy(s);
+}
+
+const char *olcao_last_error(const olcao_ctx *c)
+{
+    return c ? c->err.c_str() : g_noctx_err.c_str();
+}
+
+olcao_status olcao_get_dims(const olcao_ctx *c, long long dims[8])
+{
+    if (!c || !dims) return OLCAO_ERR_INPUT;
+    dims[0] = c->valeDim; dims[1] = c->coreDim; dims[2] = c->nk; dims[3] = c->potDim;
+    dims[4] = (long long)c->hCellSizes.size(); dims[5] = (long long)c->hPcPair.size();
+    dims[6] = (long long)c->hPpPc.size(); dims[7] = (long long)c->hPairI.size();
+    return OLCAO_OK;
+}
+
+olcao_status olcao_get_setup(const olcao_ctx *c, double *cellDims, double *cellSizes, double *kCart)
+{
+    if (!c) return OLCAO_ERR_INPUT;
+    if (cellDims) std::memcpy(cellDims, c->hCellDims.data(), c->hCellDims.size() * 8);
+    if (cellSizes) std::memcpy(cellSizes, c->hCellSizes.data(), c->hCellSizes.size() * 8);
+    if (kCart) std::memcpy(kCart, c->hKcart.data(), c->hKcart.size() * 8);
+    return OLCAO_OK;
+}
+
+olcao_status olcao_gauss_overlap(olcao_ctx *c, int op, double *vv, double *cv, double *cc)
+{
+    if (!c) return OLCAO_ERR_INPUT;
+    return guard(c, [&] {
+        if (op != 0 && op != 1) fail(OLCAO_ERR_INPUT, "op must be 0 (overlap) or 1 (kinetic)");
+        ensure_mats(*c);
+        if (op == 0) {
+            sweep(*c, SW_OL, 0, c->S.p, c->CV_S.p, c->CC_S.p);
+            c->haveS = true;
+            copy_out(*c, c->S, vv, c->matDoubles());
+            if (c->coreDim) { copy_out(*c, c->CV_S, cv, c->cvDoubles()); copy_out(*c, c->CC_S, cc, c->ccDoubles()); }
+        } else {
+            sweep(*c, SW_KE, 0, c->H.p, c->CV_H.p, c->CC_H.p);
+            c->haveH = false;
+            copy_out(*c, c->H, vv, c->matDoubles());
+            if (c->coreDim) { copy_out(*c, c->CV_H, cv, c->cvDoubles()); copy_out(*c, c->CC_H, cc, c->ccDoubles()); }
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+olcao_status olcao_gauss_overlap_np(olcao_ctx *c, double *vv, double *cv, double *cc)
+{
+    if (!c) return OLCAO_ERR_INPUT;
+    return guard(c, [&] {
+        ensure_mats(*c);
+        sweep(*c, SW_NP, 0, c->H.p, c->CV_H.p, c->CC_H.p);
+        c->haveH = false;
+        copy_out(*c, c->H, vv, c->matDoubles());
+        if (c->coreDim) { copy_out(*c, c->CV_H, cv, c->cvDoubles()); copy_out(*c, c->CC_H, cc, c->ccDoubles()); }
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+olcao_status olcao_elec_pot_gauss_overlap(olcao_ctx *c, int term, double *vv, double *cv, double *cc)
+{
+    if (!c) return OLCAO_ERR_INPUT;
+    return guard(c, [&] {
+        if (term < 0 || term >= c->potDim) fail(OLCAO_ERR_INPUT, "term out of range");
+        ensure_mats(*c);
+        sweep(*c, SW_EP, term, c->H.p, c->CV_H.p, c->CC_H.p);
+        c->haveH = false;
+        copy_out(*c, c->H, vv, c->matDoubles());
+        if (c->coreDim) { copy_out(*c, c->CV_H, cv, c->cvDoubles()); copy_out(*c, c->CC_H, cc, c->ccDoubles()); }
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+olcao_status olcao_gauss_overlap_ham_pscf(olcao_ctx *c, const double *coef, double *vv, double *cv, double *cc)
+{
+    if (!c || !coef) return OLCAO_ERR_INPUT;
+    return guard(c, [&] {
+        ensure_mats(*c);
+        HIPCHK(hipMemcpyAsync(c->dCoef.p, coef, c->potDim * 8, hipMemcpyHostToDevice, c->stream));
+        sweep(*c, SW_HAM, 0, c->H.p, c->CV_H.p, c->CC_H.p);
+        c->haveH = true;
+        copy_out(*c, c->H, vv, c->matDoubles());
+        if (c->coreDim) { copy_out(*c, c->CV_H, cv, c->cvDoubles()); copy_out(*c, c->CC_H, cc, c->ccDoubles()); }
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+olcao_status olcao_secular_eqn_pscf(olcao_ctx *c, int numStates, double *evals, double *evecs)
+{
+    if (!c) return OLCAO_ERR_INPUT;
+    return guard(c, [&] {
+        if (!c->haveS || !c->haveH)
+            fail(OLCAO_ERR_STATE, "build S (olcao_gauss_overlap op 0) and H (olcao_gauss_overlap_ham_pscf) first");
+        solve_device(*c, numStates, c->H.p, c->S.p, evals, evecs);
+    });
+}
+
+olcao_status olcao_setup_scf(olcao_ctx *c)
+{
+    if (!c) return OLCAO_ERR_INPUT;
+    return guard(c, [&] {
+        if (c->coreDim) fail(OLCAO_ERR_UNSUPPORTED, "core orthogonalisation is not in this build");
+        ensure_mats(*c);
+        const size_t m = c->matDoubles();
+        sweep(*c, SW_OL, 0, c->S.p, c->CV_S.p, c->CC_S.p);
+        c->haveS = true;
+        c->KEm.alloc(m);
+        sweep(*c, SW_KE, 0, c->KEm.p, nullptr, nullptr);
+        c->NPm.alloc(m);
+        sweep(*c, SW_NP, 0, c->NPm.p, nullptr, nullptr);
+        c->Vm.resize(c->potDim);
+        std::vector<double *> ptrs(c->potDim);
+        for (int j = 0; j < c->potDim; ++j) {
+            c->Vm[j].alloc(m);
+            sweep(*c, SW_EP, j, c->Vm[j].p, nullptr, nullptr);
+            ptrs[j] = c->Vm[j].p;
+        }
+        c->dVptr.alloc(std::max(c->potDim, 1));
+        HIPCHK(hipMemcpyAsync(c->dVptr.p, ptrs.data(), ptrs.size() * sizeof(double *),
+                              hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->haveScf = true;
+    });
+}
+
+olcao_status olcao_secular_eqn_scf(olcao_ctx *c, const double *coef, int numStates, double *evals, double *evecs)
+{
+    if (!c || !coef) return OLCAO_ERR_INPUT;
+    return guard(c, [&] {
+        if (!c->haveScf) fail(OLCAO_ERR_STATE, "call olcao_setup_scf first");
+        ensure_mats(*c);
+        HIPCHK(hipMemcpyAsync(c->dCoef.p, coef, c->potDim * 8, hipMemcpyHostToDevice, c->stream));
+        launch_assemble((long long)c->matDoubles(), c->valeDim, c->NPm.p, c->KEm.p, c->dVptr.p,
+                        c->dCoef.p, c->potDim, c->H.p, c->gamma ? 1 : 2, c->stream);
+        HIPCHK(hipGetLastError());
+        c->haveH = true;
+        solve_device(*c, numStates, c->H.p, c->S.p, evals, evecs);
+    });
+}
+
+olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates, double *evals)
+{
+    if (!c || !coef) return OLCAO_ERR_INPUT;
+    return guard(c, [&] {
+        ensure_mats(*c);
+        hipStream_t s = c->stream;
+        HIPCHK(hipMemcpyAsync(c->dCoef.p, coef, c->potDim * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipEventRecord(c->ev[0], s));
+        sweep(*c, SW_OL, 0, c->S.p, c->CV_S.p, c->CC_S.p);
+        HIPCHK(hipEventRecord(c->ev[1], s));
+        launch_two_center(c->dev, c->cls, OP_KE, c->dBlocks.p, s);
+        launch_fill(reinterpret_cast<double *>(c->dCounters.p), 32, 0.0, s);
+        HIPCHK(hipEventRecord(c->ev[2], s));
+        launch_three_center(c->dev, c->cls, K3_HAM, 0, 0, c->dCoef.p, 1, c->dBlocks.p, s);
+        HIPCHK(hipEventRecord(c->ev[3], s));
+        launch_contract(c->dev, c->dBlocks.p, c->dX.p, s);
+        launch_fill(c->H.p, (long long)c->matDoubles(), 0.0, s);
+        if (c->coreDim) {
+            launch_fill(c->CV_H.p, (long long)c->cvDoubles(), 0.0, s);
+            launch_fill(c->CC_H.p, (long long)c->ccDoubles(), 0.0, s);
+        }
+        launch_bloch_scatter(c->dev, c->dX.p, c->H.p, c->CV_H.p, c->CC_H.p, c->gamma ? 1 : 0, s);
+        HIPCHK(hipEventRecord(c->ev[4], s));
+        c->haveS = c->haveH = true;
+        solve_device(*c, numStates, c->H.p, c->S.p, evals, nullptr);
+        HIPCHK(hipEventRecord(c->ev[5], s));
+        HIPCHK(hipEventSynchronize(c->ev[5]));
+        float ms[5];
+        for (int q = 0; q < 5; ++q) HIPCHK(hipEventElapsedTime(&ms[q], c->ev[q], c->ev[q + 1]));
+        c->timings[0] = ms[0];
+        c->timings[1] = ms[1] + ms[2];
+        c->timings[2] = ms[3];
+        c->timings[3] = ms[4];
+        float tot;
+        HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[5]));
+        c->timings[4] = tot;
+        c->timings[5] = ms[2];
+    });
+}
+
+olcao_status olcao_get_work_counts(olcao_ctx *c, unsigned long long counts[32])
+{
+    if (!c || !counts) return OLCAO_ERR_INPUT;
+    return guard(c, [&] {
+        HIPCHK(hipMemcpyAsync(counts, c->dCounters.p, 32 * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+olcao_status olcao_get_timings(const olcao_ctx *c, double t[8])
+{
+    if (!c || !t) return OLCAO_ERR_INPUT;
+    std::memcpy(t, c->timings, sizeof(c->timings));
+    return OLCAO_OK;
+}
+
+static olcao_status solve_host(int n, int nb, double *A, double *B, double *w, bool cplx)
+{
+    return guard(nullptr, [&] {
+        if (n <= 0 || nb <= 0) fail(OLCAO_ERR_INPUT, "n and nbatch must be positive");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) fail(OLCAO_ERR_NO_DEVICE, "no GPU");
+        hipStream_t s;
+        HIPCHK(hipStreamCreate(&s));
+        rocblas_handle h;
+        rocblas_create_handle(&h);
+        rocblas_set_stream(h, s);
+        const size_t per = (size_t)n * n * (cplx ? 2 : 1);
+        DBuf<double> dA, dB, dD, dE;
+        DBuf<int> dI;
+        dA.alloc(per * nb); dB.alloc(per * nb); dD.alloc((size_t)n * nb); dE.alloc((size_t)n * nb);
+        dI.alloc(nb);
+        HIPCHK(hipMemcpyAsync(dA.p, A, per * nb * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(dB.p, B, per * nb * 8, hipMemcpyHostToDevice, s));
+        rocblas_status st;
+        if (cplx)
+            st = rocsolver_zhegvd_strided_batched(h, rocblas_eform_ax, rocblas_evect_original,
+                                                  rocblas_fill_upper, n, (rocblas_double_complex *)dA.p, n,
+                                                  (rocblas_stride)n * n, (rocblas_double_complex *)dB.p, n,
+                                                  (rocblas_stride)n * n, dD.p, n, dE.p, n, dI.p, nb);
+        else
+            st = rocsolver_dsygvd_strided_batched(h, rocblas_eform_ax, rocblas_evect_original,
+                                                  rocblas_fill_upper, n, dA.p, n, (rocblas_stride)n * n,
+                                                  dB.p, n, (rocblas_stride)n * n, dD.p, n, dE.p, n, dI.p, nb);
+        std::vector<int> info(nb);
+        HIPCHK(hipMemcpyAsync(info.data(), dI.p, nb * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(A, dA.p, per * nb * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(w, dD.p, (size_t)n * nb * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        rocblas_destroy_handle(h);
+        (void)hipStreamDestroy(s);
+        if (st != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "rocsolver status %d", (int)st);
+        for (int k = 0; k < nb; ++k) {
+            if (info[k] > n) fail(OLCAO_ERR_NOT_SPD, "B not positive definite in batch %d", k);
+            if (info[k] != 0) fail(OLCAO_ERR_SOLVER, "no convergence in batch %d", k);
+        }
+    });
+}
+
+olcao_status olcao_solve_zhegv(int n, int nb, double *A, double *B, double *w)
+{
+    return solve_host(n, nb, A, B, w, true);
+}
+
+olcao_status olcao_solve_dsygv(int n, int nb, double *A, double *B, double *w)
+{
+    return solve_host(n, nb, A, B, w, false);
+}
+
+olcao_status olcao_prim_batch(int op, int n, const double *a, const double *A, const double *B,
+                              const double *C, const int *sw, double *sh)
+{
+    return guard(nullptr, [&] {
+        if (op < 0 || op > 3 || n < 0) fail(OLCAO_ERR_INPUT, "bad op or n");
+        if (n == 0) return;
+        for (int i = 0; i < n; ++i) {
+            int lo = sw[i] & 15, hi = (sw[i] >> 4) & 15;
+            auto ok = [](int v) { return v == 1 || v == 2 || v == 4 || v == 8; };
+            if (!ok(lo) || !ok(hi) || (sw[i] >> 8)) fail(OLCAO_ERR_INPUT, "bad l1l2switch %d", sw[i]);
+        }
+        DBuf<double> da, dA, dB, dC, dsh;
+        DBuf<int> dsw;
+        std::vector<double> va(a, a + 3 * (size_t)n), vA(A, A + 3 * (size_t)n), vB(B, B + 3 * (size_t)n),
+            vC(C, C + 3 * (size_t)n);
+        std::vector<int> vs(sw, sw + n);
+        da.upload(va, 0); dA.upload(vA, 0); dB.upload(vB, 0); dC.upload(vC, 0); dsw.upload(vs, 0);
+        dsh.alloc(256 * (size_t)n);
+        launch_prim_batch(op, n, da.p, dA.p, dB.p, dC.p, dsw.p, dsh.p, 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(sh, dsh.p, 256 * (size_t)n * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+olcao_status olcao_boys_batch(int n, const double *T, double *F)
+{
+    return guard(nullptr, [&] {
+        if (n <= 0) return;
+        DBuf<double> dT, dF;
+        std::vector<double> vT(T, T + n);
+        dT.upload(vT, 0);
+        dF.alloc(7 * (size_t)n);
+        launch_boys_batch(n, dT.p, dF.p, 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(F, dF.p, 7 * (size_t)n * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,39 @@
+! C-callable driver around the reference's primitive-integral subroutines
+! (src/uolcao/gaussIntegrals.f90).  Test infrastructure only: it is linked
+! into oracle/_ref/libolcao_ref_prims.so, which exists solely to pin the
+! oracle's primitive integrals and Boys function against the reference's own
+! compiled code.  Nothing in the product loads it.
+module ref_prims_shim
+   use iso_c_binding
+   use O_GaussianIntegrals
+   implicit none
+contains
+   ! op: 0 overlap2CIntg, 1 kinetic2CIntg, 2 electron3CIntg,
+   !     3 nuclear3CIntg, 4 massvel2CIntg.  sh is column-major (16,16); only
+   !     the block selected by sw is defined, the rest is left at zero.
+   subroutine ref_prim(op, a1, a2, a3, A, B, C, sw, sh) bind(C, name="ref_prim")
+      integer(c_int), value :: op, sw
+      real(c_double), value :: a1, a2, a3
+      real(c_double), intent(in) :: A(3), B(3), C(3)
+      real(c_double), intent(inout) :: sh(16,16)
+      sh(:,:) = 0.0d0
+      select case (op)
+      case (0)
+         call overlap2CIntg(a1, a2, A, B, sw, sh)
+      case (1)
+         call kinetic2CIntg(a1, a2, A, B, sw, sh)
+      case (2)
+         call electron3CIntg(a1, a2, a3, A, B, C, sw, sh)
+      case (3)
+         call nuclear3CIntg(a1, a2, a3, A, B, C, sw, sh)
+      case (4)
+         call massvel2CIntg(a1, a2, A, B, sw, sh)
+      end select
+   end subroutine ref_prim
+
+   subroutine ref_boys(T, F) bind(C, name="ref_boys")
+      real(c_double), value :: T
+      real(c_double), intent(out) :: F(7)
+      call boys(T, F)
+   end subroutine ref_boys
+end module ref_prims_shim

@@ -1,0 +1,51 @@
+"""The C-ABI library loads without a GPU and exports every entry point that
+include/olcao_gpu.h declares; host-side validation paths answer with status
+codes (no compute runs here)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from olcao_amd import configs, gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "olcao_gpu.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(olcao_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = gpu.lib()
+    names = declared_functions()
+    assert len(names) >= 18
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(gpu.FUNCTIONS)
+
+
+def test_context_creation_reports_missing_gpu_or_succeeds():
+    try:
+        g = gpu.OlcaoGPU(configs.small_test())
+    except gpu.OlcaoError as e:
+        assert e.code == 9          # OLCAO_ERR_NO_DEVICE on a host without GPU
+        return
+    g.close()
+
+
+def test_null_arguments_are_rejected():
+    lib = gpu.lib()
+    assert lib.olcao_ctx_create(None, None, None) == 3
+    assert lib.olcao_get_dims(None, None) == 3
+    assert lib.olcao_scf_iteration(None, None, 1, None) == 3
+
+
+def test_fortran_facade_source_names_reference_procedures():
+    src = open(os.path.join(ROOT, "olcao_amd", "fortran", "olcao_gpu_facade.f90")).read()
+    for proc in ("gaussOverlapOL", "gaussOverlapKE", "gaussOverlapNP",
+                 "elecPotGaussOverlap", "gaussOverlapHamPSCF", "secularEqnPSCF",
+                 "secularEqnSCF", "solveZHEGV", "solveDSYGV"):
+        assert re.search(r"subroutine\s+" + proc + r"\b", src, re.I), proc

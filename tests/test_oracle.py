@@ -1,0 +1,419 @@
+"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+
+Each function of the oracle is checked against something that does not
+share its code: golden vectors from the reference's own compiled primitive
+subroutines (tests/golden/prims_golden.npz, written by make_golden.py),
+closed forms, exact Gauss-Hermite quadrature of the solid-harmonic
+functions, symmetry invariants, brute force on tiny systems and LAPACK/SciPy.
+"""
+import math
+
+import numpy as np
+import pytest
+from numpy.polynomial.hermite import hermgauss
+from scipy import integrate, special
+
+import oracle
+from olcao_amd import configs
+from tests.conftest import block_rel_err
+
+OPS = {"overlap": 0, "kinetic": 1, "electron3c": 2, "nuclear3c": 3}
+NSH = [1, 4, 9, 16]
+
+
+def _golden_case(golden, name):
+    cls, inp, sh = golden[name + "_cls"], golden[name + "_in"], golden[name + "_sh"]
+    return cls, inp[:, :3], inp[:, 3:6], inp[:, 6:9], inp[:, 9:12], sh
+
+
+# --------------------------------------------------------------------------
+# primitives against the reference's compiled subroutines
+@pytest.mark.parametrize("name", list(OPS))
+def test_primitives_match_reference_golden(golden, name):
+    cls, a, A, B, C, ref = _golden_case(golden, name)
+    out = oracle.prim_batch(OPS[name], a, A, B, C, cls[:, 2])
+    for i in range(len(cls)):
+        na, nb = NSH[cls[i, 0]], NSH[cls[i, 1]]
+        assert block_rel_err(out[i][:na, :nb], ref[i][:na, :nb]) < 1e-13, (name, i, cls[i])
+
+
+def test_boys_bit_exact_against_reference(golden):
+    F = np.array([oracle.boys(t) for t in golden["boys_T"]])
+    assert np.array_equal(F, golden["boys_F"])
+
+
+def _boys_exact(m, T):
+    return integrate.quad(lambda t: t ** (2 * m) * math.exp(-T * t * t), 0.0, 1.0,
+                          epsabs=1e-300, epsrel=1e-13, limit=200)[0]
+
+
+def test_boys_accuracy_character():
+    """Above T = 1 the closed form is accurate to rounding; in the series
+    branch (T <= float(0.1)) the float-rounded Gamma constants limit
+    F_1..F_6 to ~1e-7 relative, as in the reference."""
+    for T in [1.0, 3.7, 12.0, 40.0]:
+        F = oracle.boys(T)
+        for m in range(7):
+            assert abs(F[m] / _boys_exact(m, T) - 1) < 1e-12
+    F = oracle.boys(0.05)
+    errs = [abs(F[m] / _boys_exact(m, 0.05) - 1) for m in range(7)]
+    assert max(errs) < 2e-7 and max(errs[1:]) > 1e-9
+
+
+# --------------------------------------------------------------------------
+# closed forms for s-s blocks
+def _rand_geom(rng):
+    a1, a2, a3 = np.exp(rng.uniform(np.log(0.1), np.log(50.0), 3))
+    A, B, C = rng.uniform(-1.5, 1.5, (3, 3))
+    return a1, a2, a3, A, B, C
+
+
+def test_ss_closed_forms():
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        a1, a2, a3, A, B, C = _rand_geom(rng)
+        z, xi = a1 + a2, a1 * a2 / (a1 + a2)
+        d2 = np.sum((A - B) ** 2)
+        P = (a1 * A + a2 * B) / z
+        S = (np.pi / z) ** 1.5 * np.exp(-xi * d2)
+        args = (np.array([[a1, a2, a3]]), A[None], B[None], C[None], [17])
+        assert abs(oracle.prim_batch(0, *args)[0, 0, 0] / S - 1) < 1e-13
+        T = xi * (3 - 2 * xi * d2) * S
+        assert abs(oracle.prim_batch(1, *args)[0, 0, 0] - T) < 1e-13 * abs(S) * (1 + xi * d2) * xi
+        z3 = z + a3
+        E3 = (np.pi / z3) ** 1.5 * np.exp(-xi * d2 - z * a3 / z3 * np.sum((P - C) ** 2))
+        assert abs(oracle.prim_batch(2, *args)[0, 0, 0] / E3 - 1) < 1e-13
+        G = (a1 * A + a2 * B + a3 * C) / z3
+        U = z3 * np.sum((G - C) ** 2)
+        if U > 0.2:
+            e = a1 * a2 * d2 + a1 * a3 * np.sum((A - C) ** 2) + a2 * a3 * np.sum((B - C) ** 2)
+            F0 = 0.5 * math.sqrt(math.pi / U) * math.erf(math.sqrt(U))
+            N = 2 * np.pi / z3 * np.exp(-e / z3) * F0
+            assert abs(oracle.prim_batch(3, *args)[0, 0, 0] / N - 1) < 1e-12
+
+
+# --------------------------------------------------------------------------
+# Gauss-Hermite quadrature of the labelled solid harmonics (exact for the
+# polynomial degrees involved); independent of the recursion and transform
+SH_POLY = [
+    lambda x, y, z: np.ones_like(x), lambda x, y, z: x, lambda x, y, z: y,
+    lambda x, y, z: z, lambda x, y, z: x * y, lambda x, y, z: x * z,
+    lambda x, y, z: y * z, lambda x, y, z: x * x - y * y,
+    lambda x, y, z: 2 * z * z - x * x - y * y, lambda x, y, z: x * y * z,
+    lambda x, y, z: x * x * z - y * y * z, lambda x, y, z: x ** 3 - 3 * y * y * x,
+    lambda x, y, z: 3 * x * x * y - y ** 3,
+    lambda x, y, z: 2 * z ** 3 - 3 * x * x * z - 3 * y * y * z,
+    lambda x, y, z: 4 * z * z * x - x ** 3 - y * y * x,
+    lambda x, y, z: 4 * z * z * y - x * x * y - y ** 3]
+
+
+def _gh_grid(center, zeta, n=8):
+    t, w = hermgauss(n)
+    s = t / math.sqrt(zeta)
+    X, Y, Z = np.meshgrid(center[0] + s, center[1] + s, center[2] + s, indexing="ij")
+    W = np.einsum("i,j,k->ijk", w, w, w) / zeta ** 1.5
+    return X, Y, Z, W
+
+
+@pytest.mark.parametrize("op", [0, 2])
+def test_overlap_quadrature_all_classes(op):
+    rng = np.random.default_rng(11 + op)
+    for la in range(4):
+        for lb in range(4):
+            a1, a2, a3, A, B, C = _rand_geom(rng)
+            sw = (1 << la) + 16 * (1 << lb)
+            sh = oracle.prim_batch(op, np.array([[a1, a2, a3]]), A[None], B[None], C[None],
+                                   [sw])[0]
+            z = a1 + a2 + (a3 if op == 2 else 0.0)
+            ctr = (a1 * A + a2 * B + (a3 * C if op == 2 else 0.0)) / z
+            X, Y, Z, W = _gh_grid(ctr, z)
+            # weight exp(-z|r-ctr|^2) is in W; remaining Gaussian factor constant
+            rA = (X - A[0]) ** 2 + (Y - A[1]) ** 2 + (Z - A[2]) ** 2
+            rB = (X - B[0]) ** 2 + (Y - B[1]) ** 2 + (Z - B[2]) ** 2
+            rC = (X - C[0]) ** 2 + (Y - C[1]) ** 2 + (Z - C[2]) ** 2
+            rG = (X - ctr[0]) ** 2 + (Y - ctr[1]) ** 2 + (Z - ctr[2]) ** 2
+            g = np.exp(-a1 * rA - a2 * rB - (a3 * rC if op == 2 else 0.0) + z * rG)
+            ref = np.zeros((NSH[la], NSH[lb]))
+            for i in range(NSH[la]):
+                fa = SH_POLY[i](X - A[0], Y - A[1], Z - A[2])
+                for j in range(NSH[lb]):
+                    fb = SH_POLY[j](X - B[0], Y - B[1], Z - B[2])
+                    ref[i, j] = np.sum(W * g * fa * fb)
+            assert block_rel_err(sh[:NSH[la], :NSH[lb]], ref) < 1e-11, (op, la, lb)
+
+
+def test_kinetic_quadrature_as_gradient_overlap():
+    """<a|-1/2 lap|b> = 1/2 <grad a|grad b>, with analytic gradients."""
+    rng = np.random.default_rng(3)
+    for la in range(4):
+        for lb in range(4):
+            a1, a2, _, A, B, _ = _rand_geom(rng)
+            sw = (1 << la) + 16 * (1 << lb)
+            sh = oracle.prim_batch(1, np.array([[a1, a2, 0]]), A[None], B[None], A[None], [sw])[0]
+            z = a1 + a2
+            ctr = (a1 * A + a2 * B) / z
+            X, Y, Z, W = _gh_grid(ctr, z, n=10)
+            rA = (X - A[0]) ** 2 + (Y - A[1]) ** 2 + (Z - A[2]) ** 2
+            rB = (X - B[0]) ** 2 + (Y - B[1]) ** 2 + (Z - B[2]) ** 2
+            rG = (X - ctr[0]) ** 2 + (Y - ctr[1]) ** 2 + (Z - ctr[2]) ** 2
+            g = np.exp(-a1 * rA - a2 * rB + z * rG)
+            h = 1e-6
+
+            def grad(f, c, a):
+                xs = (X - c[0], Y - c[1], Z - c[2])
+                val = f(*xs)
+                out = []
+                for ax in range(3):
+                    e = [0.0, 0.0, 0.0]
+                    e[ax] = h
+                    dp = f(xs[0] + e[0], xs[1] + e[1], xs[2] + e[2])
+                    dm = f(xs[0] - e[0], xs[1] - e[1], xs[2] - e[2])
+                    out.append((dp - dm) / (2 * h) - 2 * a * xs[ax] * val)
+                return out
+            ref = np.zeros((NSH[la], NSH[lb]))
+            for i in range(NSH[la]):
+                ga = grad(SH_POLY[i], A, a1)
+                for j in range(NSH[lb]):
+                    gb = grad(SH_POLY[j], B, a2)
+                    ref[i, j] = 0.5 * np.sum(W * g * sum(p * q for p, q in zip(ga, gb)))
+            assert block_rel_err(sh[:NSH[la], :NSH[lb]], ref) < 1e-7, (la, lb)
+
+
+@pytest.mark.parametrize("op", [0, 1, 2, 3])
+def test_swap_and_translation_invariance(op):
+    rng = np.random.default_rng(21 + op)
+    for la in range(4):
+        for lb in range(4):
+            a1, a2, a3, A, B, C = _rand_geom(rng)
+            sw = (1 << la) + 16 * (1 << lb)
+            swr = (1 << lb) + 16 * (1 << la)
+            s1 = oracle.prim_batch(op, np.array([[a1, a2, a3]]), A[None], B[None], C[None], [sw])[0]
+            s2 = oracle.prim_batch(op, np.array([[a2, a1, a3]]), B[None], A[None], C[None], [swr])[0]
+            blk = s1[:NSH[la], :NSH[lb]]
+            assert block_rel_err(s2[:NSH[lb], :NSH[la]].T, blk) < 1e-12
+            t = rng.uniform(-3, 3, 3)
+            s3 = oracle.prim_batch(op, np.array([[a1, a2, a3]]), (A + t)[None], (B + t)[None],
+                                   (C + t)[None], [sw])[0]
+            assert block_rel_err(s3[:NSH[la], :NSH[lb]], blk) < 1e-11
+
+
+# --------------------------------------------------------------------------
+# lattice
+@pytest.mark.parametrize("cell", [
+    configs.cubic(6.74),
+    configs.hexagonal(9.28, 10.21),
+    np.array([[7.0, 0.3, -0.2], [1.1, 6.4, 0.5], [-0.8, 0.9, 8.2]])])
+def test_cell_list_and_nearest_image(cell):
+    s = configs.small_test()
+    s.cell = cell
+    dims, sizes, nl = oracle.make_lattice(s)
+    assert np.all(np.diff(sizes) >= 0)
+    assert np.allclose(np.sum(dims ** 2, axis=1), sizes)
+    # every lattice point inside the negligibility sphere is present
+    r = int(math.ceil(math.sqrt(nl) / min(np.linalg.norm(cell, axis=1)))) + 2
+    rng_ = np.arange(-r, r + 1)
+    n = np.array(np.meshgrid(rng_, rng_, rng_, indexing="ij")).reshape(3, -1).T
+    pts = n @ cell
+    inside = pts[np.sum(pts ** 2, axis=1) <= nl * (1 - 1e-12)]
+    have = {tuple(np.round(x, 8)) for x in dims}
+    assert all(tuple(np.round(x, 8)) in have for x in inside)
+    # nearest lattice vector against brute force
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        v = rng.uniform(-30, 30, 3)
+        lv = oracle.nearest_lattice_vector(cell, v)
+        d = np.sum((v - pts) ** 2, axis=1).min()
+        assert abs(np.sum((v - lv) ** 2) - d) < 1e-9 * max(d, 1.0)
+
+
+# --------------------------------------------------------------------------
+# integral sweeps
+@pytest.fixture(scope="module")
+def small():
+    s = configs.small_test(kmesh=(2, 2, 1))
+    o = oracle.Oracle(s)
+    return s, o
+
+
+def test_sweeps_kpoint_symmetry(small):
+    """H(-k) = conj H(k); the 2x2x1 shifted mesh holds both."""
+    s, o = small
+    for op in (oracle.OL, oracle.KE, oracle.NP, oracle.HAM):
+        M, _, _ = o.integral(op, coef=s.pot_coeffs)
+        k = s.kpoints
+        for a in range(len(k)):
+            b = np.where(np.all(np.abs(k + k[a]) < 1e-12, axis=1))[0][0]
+            assert block_rel_err(M[b], np.conj(M[a])) < 1e-12
+
+
+def test_reciprocal_vector_shift_invariance():
+    s = configs.small_test(kmesh=(1, 1, 1))
+    k0 = np.array([0.17, -0.31, 0.05])
+    s.kpoints = np.array([k0, k0 + [1, 0, 0], k0 + [0, -1, 2]])
+    s.kweights = np.ones(3)
+    M, _, _ = oracle.Oracle(s).integral(oracle.HAM, coef=s.pot_coeffs)
+    assert block_rel_err(M[1], M[0]) < 1e-11 and block_rel_err(M[2], M[0]) < 1e-11
+
+
+def test_gamma_build_equals_k0():
+    sg = configs.small_test(gamma=True, lmax=2)
+    sk = configs.small_test(gamma=True, lmax=2)
+    sk.gamma = False
+    Mg, _, _ = oracle.Oracle(sg).integral(oracle.HAM, coef=sg.pot_coeffs)
+    Mk, _, _ = oracle.Oracle(sk).integral(oracle.HAM, coef=sk.pot_coeffs)
+    assert np.abs(Mk[0].imag).max() < 1e-12 * np.abs(Mk).max()
+    assert block_rel_err(Mg, Mk[0].real) < 1e-12
+
+
+def test_hamiltonian_is_sum_of_terms():
+    """The fused sweep equals KE + NP + sum_j c_j V_j from the separate
+    sweeps (integrals.F90:2860 against :656, :1531, :1946)."""
+    s = configs.small_test(two_types=True)
+    o = oracle.Oracle(s)
+    H, _, _ = o.integral(oracle.HAM, coef=s.pot_coeffs)
+    acc = o.integral(oracle.KE)[0] + o.integral(oracle.NP)[0]
+    for j, c in enumerate(s.pot_coeffs):
+        acc = acc + c * o.integral(oracle.EP, term=j)[0]
+    assert block_rel_err(H, acc) < 1e-12
+
+
+def _full(M):
+    return np.triu(M) + np.conj(np.triu(M, 1)).swapaxes(-1, -2)
+
+
+def test_overlap_brute_force_lattice_sum():
+    """S(k) from the screened sweep against an unscreened direct sum over a
+    large image set, contracted explicitly (contraction, Bloch sum, origin
+    shift and scatter all checked)."""
+    s = configs.small_test(kmesh=(2, 1, 1), n_alpha=4, lmax=1)
+    o = oracle.Oracle(s)
+    S, _, _ = o.integral(oracle.OL)
+    lmai, lmi, basis = o.lmai, o.lmi, o.basis
+    kc = o.kcart
+    n = o.n
+    ref = np.zeros((len(kc), n, n), complex)
+    off = np.cumsum([0] + [s.types[t].num_vale_states for t in s.atom_type])
+    R = 4
+    rr = np.arange(-R, R + 1)
+    Ls = np.array(np.meshgrid(rr, rr, rr, indexing="ij")).reshape(3, -1).T @ s.cell
+    for i in range(s.n_atoms):
+        for j in range(s.n_atoms):
+            ti, tj = s.atom_type[i], s.atom_type[j]
+            t1, t2 = s.types[ti], s.types[tj]
+            for L in Ls:
+                B = s.atom_pos[j] + L
+                X = np.zeros((t1.num_vale_states, t2.num_vale_states))
+                for a1 in range(len(t1.alphas)):
+                    for a2 in range(len(t2.alphas)):
+                        la = {1: 0, 4: 1, 9: 2, 16: 3}[lmai[ti, a1]]
+                        lb = {1: 0, 4: 1, 9: 2, 16: 3}[lmai[tj, a2]]
+                        sh = oracle.prim_batch(0, np.array([[t1.alphas[a1], t2.alphas[a2], 0]]),
+                                               s.atom_pos[i][None], B[None], B[None],
+                                               [(1 << la) + 16 * (1 << lb)])[0]
+                        for p in range(t1.num_vale_states):
+                            for q in range(t2.num_vale_states):
+                                X[p, q] += basis[ti, p, a1] * basis[tj, q, a2] * \
+                                    sh[lmi[ti, p] - 1, lmi[tj, q] - 1]
+                for k in range(len(kc)):
+                    ref[k, off[i]:off[i + 1], off[j]:off[j + 1]] += np.exp(1j * kc[k] @ L) * X
+    assert block_rel_err(_full(S), ref) < 1e-11
+
+
+def test_contracted_orbitals_are_normalised():
+    """Loewdin-normalised radial functions give unit on-site norms: the
+    diagonal of S at a single-atom Gamma build in a huge box is 1."""
+    s = configs.small_test(gamma=True, lmax=2, a_ang=40.0)
+    s.atom_pos = s.atom_pos[:1]
+    s.atom_type = s.atom_type[:1]
+    s.pot_pos = s.pot_pos[:1]
+    s.pot_type = s.pot_type[:1]
+    S, _, _ = oracle.Oracle(s).integral(oracle.OL)
+    assert np.allclose(np.diag(S), 1.0, atol=1e-12)
+
+
+def test_isolated_atom_bands_are_flat():
+    s = configs.small_test(kmesh=(2, 2, 2), lmax=1, a_ang=30.0)
+    s.atom_pos = s.atom_pos[:1]
+    s.atom_type = s.atom_type[:1]
+    s.pot_pos = s.pot_pos[:1]
+    s.pot_type = s.pot_type[:1]
+    o = oracle.Oracle(s)
+    H, _, _ = o.integral(oracle.HAM, coef=s.pot_coeffs)
+    S, _, _ = o.integral(oracle.OL)
+    w, _ = oracle.solve(H, S)
+    assert np.abs(w - w[0]).max() < 1e-9
+
+
+def test_diamond_gamma_triplets():
+    """At Gamma the cubic diamond crystal has triply degenerate levels."""
+    s = configs.diamond_c8(n_alpha=10, n_pot=6)
+    s.gamma = True
+    o = oracle.Oracle(s)
+    H, _, _ = o.integral(oracle.HAM, coef=s.pot_coeffs)
+    S, _, _ = o.integral(oracle.OL)
+    w, _ = oracle.solve(H, S)
+    gaps = np.diff(w)
+    # some consecutive triples are degenerate to rounding
+    trip = [i for i in range(len(w) - 2) if gaps[i] < 1e-9 and gaps[i + 1] < 1e-9]
+    assert len(trip) >= 2
+
+
+# --------------------------------------------------------------------------
+# packing, assembly, orthogonalisation, solve
+def test_pack_unpack_and_index():
+    rng = np.random.default_rng(0)
+    n = 7
+    M = np.triu(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    p = oracle.pack_upper(M)
+    for i in range(n):
+        for j in range(i + 1):
+            assert p[i * (i + 1) // 2 + j] == M[j, i]
+    assert np.array_equal(oracle.unpack_upper(p, n), M)
+    small = M.copy()
+    small[0, 3] = 3e-11 + 1j
+    q = oracle.pack_upper(small, zero_small=True)
+    assert q[3 * 4 // 2] == 1j
+
+
+def test_assembly_zero_coefficient_rule():
+    a, b = np.array([1.0, 2.0]), np.array([10.0, 20.0])
+    V = [np.array([1.0, 1.0]), np.array([5.0, 7.0])]
+    H = oracle.assemble_packed(a, b, V, [0.0, 2.0])
+    assert np.array_equal(H, a + b + V[0] + 2 * V[1])
+
+
+def test_core_orthogonalisation_identity():
+    """S' and O' of intgOrtho.F90 equal P^H S P and P^H O P with the
+    projector P = [1; -S_cv] that removes the core components."""
+    rng = np.random.default_rng(4)
+    nv, nc = 6, 2
+    Z = rng.normal(size=(nv + nc, nv + nc)) + 1j * rng.normal(size=(nv + nc, nv + nc))
+    S = Z @ Z.conj().T + (nv + nc) * np.eye(nv + nc)
+    O = rng.normal(size=S.shape) + 1j * rng.normal(size=S.shape)
+    O = O + O.conj().T
+    Svv, Scv, Scc = S[:nv, :nv], S[nv:, :nv], S[nv:, nv:]
+    Ovv, Ocv, Occ = O[:nv, :nv], O[nv:, :nv], O[nv:, nv:]
+    Pm = np.vstack([np.eye(nv), -Scv])
+    assert np.allclose(oracle.ortho_overlap(Svv, Scv, Scc), np.triu(Pm.conj().T @ S @ Pm))
+    assert np.allclose(oracle.ortho_operator(Ovv, Ocv, Occ, Scv), np.triu(Pm.conj().T @ O @ Pm))
+
+
+def test_solve_matches_scipy_eigh():
+    from scipy.linalg import eigh
+    s = configs.small_test(lmax=2, kmesh=(2, 1, 1))
+    o = oracle.Oracle(s)
+    H, _, _ = o.integral(oracle.HAM, coef=s.pot_coeffs)
+    S, _, _ = o.integral(oracle.OL)
+    w, v = oracle.solve(H, S)
+    for k in range(len(w)):
+        Hf, Sf = _full(H[k]), _full(S[k])
+        assert np.allclose(w[k], eigh(Hf, Sf, eigvals_only=True), atol=1e-10)
+        r = Hf @ v[k] - Sf @ v[k] * w[k]
+        assert np.abs(r).max() < 1e-9
+        assert np.allclose(v[k].conj().T @ Sf @ v[k], np.eye(len(w[k])), atol=1e-9)
+
+
+def test_solve_raises_on_indefinite_overlap():
+    S = np.diag([1.0, -1.0]).astype(complex)
+    with pytest.raises(RuntimeError):
+        oracle.solve(np.eye(2, dtype=complex), S)
