@@ -222,22 +222,22 @@ void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, double *bl
     two_one<3, 2>(c, cl, op, blocks, s); two_one<3, 3>(c, cl, op, blocks, s);
 }
 
-void launch_3c_la0(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
-                   double *, hipStream_t);
-void launch_3c_la1(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
-                   double *, hipStream_t);
-void launch_3c_la2(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
-                   double *, hipStream_t);
-void launch_3c_la3(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
-                   double *, hipStream_t);
+void launch_3c_la0(const DevCtx &, const ClassLists &, int *, int, int, int, const double *,
+                   int, double *, hipStream_t);
+void launch_3c_la1(const DevCtx &, const ClassLists &, int *, int, int, int, const double *,
+                   int, double *, hipStream_t);
+void launch_3c_la2(const DevCtx &, const ClassLists &, int *, int, int, int, const double *,
+                   int, double *, hipStream_t);
+void launch_3c_la3(const DevCtx &, const ClassLists &, int *, int, int, int, const double *,
+                   int, double *, hipStream_t);
 
-void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind, int tt, int ti,
-                         const double *coef, int accumulate, double *blocks, hipStream_t s)
+void launch_three_center(const DevCtx &c, const ClassLists &cl, int *heads, int kind, int tt,
+                         int ti, const double *coef, int accumulate, double *blocks, hipStream_t s)
 {
-    launch_3c_la0(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
-    launch_3c_la1(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
-    launch_3c_la2(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
-    launch_3c_la3(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
+    launch_3c_la0(c, cl, heads, kind, tt, ti, coef, accumulate, blocks, s);
+    launch_3c_la1(c, cl, heads, kind, tt, ti, coef, accumulate, blocks, s);
+    launch_3c_la2(c, cl, heads, kind, tt, ti, coef, accumulate, blocks, s);
+    launch_3c_la3(c, cl, heads, kind, tt, ti, coef, accumulate, blocks, s);
 }
 
 void launch_contract(const DevCtx &c, const double *blocks, double *X, hipStream_t s)

@@ -83,7 +83,8 @@ struct ClassLists {
 
 void launch_two_center(const DevCtx &c, const ClassLists &cl, int op,
                        double *blocks, hipStream_t s);
-void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind,
+// heads: 32 zeroed ints, the work counters of the persistent kernels.
+void launch_three_center(const DevCtx &c, const ClassLists &cl, int *heads, int kind,
                          int termType, int termIdx, const double *coef,
                          int accumulate, double *blocks, hipStream_t s);
 void launch_contract(const DevCtx &c, const double *blocks, double *X,
@@ -94,9 +95,10 @@ void launch_fill(double *p, long long n, double v, hipStream_t s);
 void launch_assemble(long long n2nk, int n, const double *NP, const double *KE,
                      const double *const *V, const double *coef, int nTerms,
                      double *H, int cplx, hipStream_t s);
+// classIdx[k]: device list of input indices of class k = 4 la + lb.
 void launch_prim_batch(int op, int n, const double *a, const double *A,
-                       const double *B, const double *C, const int *sw,
-                       double *sh, hipStream_t s);
+                       const double *B, const double *C, const int *const *classIdx,
+                       const int *classCount, double *sh, hipStream_t s);
 void launch_boys_batch(int n, const double *T, double *F, hipStream_t s);
 void launch_extract_evecs(const double *A, int n, int nk, int numStates,
                           int cplx, double *out, hipStream_t s);

@@ -8,13 +8,14 @@
 // s; x y z; xy xz yz xx-yy 2zz-xx-yy; xyz xxz-yyz xxx-3yyx 3xxy-yyy
 // 2zzz-3xxz-3yyz 4zzx-xxx-yyx 4zzy-xxy-yyy (:32-34).
 //
-// How differs from the reference: overlap, kinetic and three-Gaussian
-// overlap integrals factorise into x, y and z; each axis runs the 1-D
-// Obara-Saika recursion (Obara & Saika, J. Chem. Phys. 84, 3963 (1986))
+// How differs from the reference: overlap and kinetic integrals factorise
+// into x, y and z; each axis runs the 1-D Obara-Saika recursion (Obara &
+// Saika, J. Chem. Phys. 84, 3963 (1986))
 //   E(i+1,j) = XA E(i,j) + h (i E(i-1,j) + j E(i,j-1)),  E(0,0) = 1
-// and the Cartesian block is pref * Ex * Ey * Ez.  The nuclear term keeps the
-// two-index 3-D recursion with the auxiliary index m, because the Boys
-// function couples the axes.  Every table index is a template constant, so
+// and the Cartesian block is pref * Ex * Ey * Ez.  The three-centre
+// integrals use the McMurchie-Davidson form about the pair centre (below),
+// which lets a lattice sum over many potential images accumulate a few
+// Hermite quantities per pair.  Every table index is a template constant, so
 // the tables live in registers.
 #pragma once
 
@@ -237,10 +238,8 @@ OHD void boys(double T, double (&F)[M + 1])
         }
     } else {
         F[0] = 0.5 * ((sqrt_pi / sqT) * erf_T);
-        double Tm = 1.0;
 #pragma unroll
         for (int m = 1; m <= M; ++m) {
-            Tm *= T;
             double lead = (sqrt_pi / (pow4(m) * (ipow_sq(T, m) * sqT))) * erf_T;
             double sum = 0.0;
 #pragma unroll
@@ -251,7 +250,6 @@ OHD void boys(double T, double (&F)[M + 1])
             double v = lead - exp_T * sum;
             F[m] = (m == 1) ? v : (factd(2 * m) / (2.0 * factd(m))) * v;
         }
-        (void)Tm;
     }
 }
 
@@ -316,110 +314,188 @@ OHD void two_center_cart(const PairConst &q, double (&pc)[ncart(LA)][ncart(LB)])
         }
 }
 
-// acc += w * three-Gaussian overlap block <g1 | exp(-a3 |r-C|^2) | g2>
-// (electron3CIntg): prefactor (pi/zeta3)^(3/2) exp(-xi d^2 - zeta a3/zeta3
-// |P-C|^2), recursion centre G = (a1 A + a2 B + a3 C)/zeta3, h = 1/(2 zeta3).
-template <int LA, int LB>
-OHD void acc_e3c(const PairConst &q, double a3, const double *C, double w,
-                 double (&acc)[ncart(LA)][ncart(LB)])
+// ---------------------------------------------------------------------
+// Three-centre integrals in McMurchie-Davidson form about the pair centre P.
+//
+// The pair product is expanded in Hermite Gaussians about P,
+//   g1 g2 = K_AB sum_tuv Ex^{ax bx}_t Ey^{ay by}_u Ez^{az bz}_v Lambda_tuv(r; P, zeta),
+// K_AB = exp(-xi |A-B|^2), so every three-centre integral of the pair is
+//   <a|O_C|b> = K_AB sum_tuv E_t E_u E_v R_tuv,   R_tuv = d^t/dPx d^u/dPy d^v/dPz f(P - C)
+// with f the integral of exp(-zeta|r-P|^2) against the operator at C.  R
+// depends on the image C, E only on the pair; the kernels therefore sum
+// W_tuv = sum_images,terms w R_tuv (35 numbers for a d-d pair instead of
+// 100 Cartesian products) and contract with E once per pair.
+//   three-Gaussian overlap (operator exp(-a3|r-C|^2)):
+//     f = (pi/zeta3)^(3/2) exp(-mu X^2),  mu = zeta a3/zeta3,  X = P - C;
+//     R_tuv = f h_t(X_x) h_u(X_y) h_v(X_z),
+//     h_0 = 1, h_1 = -2 mu x, h_{t+1} = -2 mu x h_t - 2 mu t h_{t-1}.
+//   Gaussian-damped nuclear attraction (operator exp(-a3 r_C^2)/r_C):
+//     f = (2 pi/zeta3) exp(-mu X^2) F_0(U),  U = kappa X^2 = zeta3 |G-C|^2,
+//     kappa = zeta^2/zeta3; with G_n = exp(-mu X^2) sum_k C(n,k) mu^(n-k)
+//     kappa^k F_k(U) the derivatives follow the usual recursion
+//     R^(n)_000 = (2 pi/zeta3) (-2)^n G_n,
+//     R^(n)_{t+1,u,v} = t R^(n+1)_{t-1,u,v} + X_x R^(n+1)_{t,u,v}
+//     (same along y, z), and R_tuv = R^(0)_tuv.  The Boys values F_k(U) are
+//     the reference's (boys() above), at the reference's argument U.
+
+OHD constexpr int nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+// Index of (t,u,v), t+u+v <= L, in a packed array (grouped by total order).
+OHD constexpr int herm_index(int t, int u, int v)
 {
-    constexpr int NA = ncart(LA), NB = ncart(LB);
+    const int n = t + u + v;
+    // entries of lower total order, then position within order n
+    const int lower = n * (n + 1) * (n + 2) / 6;
+    const int rest = u + v;          // (n - t)
+    return lower + rest * (rest + 1) / 2 + v;
+}
+
+// acc += w * R_tuv of the three-Gaussian overlap for one image/term.
+template <int L>
+OHD void acc_e3c_herm(const PairConst &q, double a3, const double *C, double w,
+                      double (&W)[nherm(L)])
+{
     const double z3 = q.zeta + a3;
     const double inv = 1.0 / z3;
-    const double h = 0.5 * inv;
-    double GA[3], GB[3], pc2 = 0.0;
+    const double mu = q.zeta * a3 * inv;
+    double X[3], X2 = 0.0;
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-        double G = (q.S[x] + a3 * C[x]) * inv;
-        GA[x] = G - q.A[x];
-        GB[x] = G - q.B[x];
-        double d = q.P[x] - C[x];
-        pc2 += d * d;
+        X[x] = q.P[x] - C[x];
+        X2 += X[x] * X[x];
     }
     const double t = kPi * inv;
-    const double pref = w * t * sqrt(t) * exp(-q.xid2 - (q.zeta * a3 * inv) * pc2);
-    double E[3][LA + 1][LB + 1];
+    const double pref = w * t * sqrt(t) * exp(-mu * X2);
+    double h[3][L + 1];
 #pragma unroll
-    for (int x = 0; x < 3; ++x) os1d<LA, LB>(GA[x], GB[x], h, E[x]);
+    for (int x = 0; x < 3; ++x) {
+        h[x][0] = 1.0;
+        if (L >= 1) h[x][L >= 1 ? 1 : 0] = -2.0 * mu * X[x];
 #pragma unroll
-    for (int a = 0; a < NA; ++a)
+        for (int k = 1; k < L; ++k)
+            h[x][k + 1] = -2.0 * mu * (X[x] * h[x][k] + k * h[x][k - 1]);
+    }
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const double exy = E[0][cexp(a, 0)][cexp(b, 0)] * E[1][cexp(a, 1)][cexp(b, 1)];
-            acc[a][b] += (pref * exy) * E[2][cexp(a, 2)][cexp(b, 2)];
+    for (int tt = 0; tt <= L; ++tt)
+#pragma unroll
+        for (int u = 0; u <= L - tt; ++u) {
+            const double hxy = pref * h[0][tt] * h[1][u];
+#pragma unroll
+            for (int v = 0; v <= L - tt - u; ++v) W[herm_index(tt, u, v)] += hxy * h[2][v];
         }
 }
 
-// acc += w * Gaussian-damped nuclear attraction <g1| exp(-a3 r_C^2)/r_C |g2>
-// (nuclear3CIntg), two-index Obara-Saika with auxiliary index m.
-template <int LA, int LB>
-OHD void acc_nuc(const PairConst &q, double a3, const double *C, double w,
-                 double (&acc)[ncart(LA)][ncart(LB)])
+// acc += w * R_tuv of the Gaussian-damped nuclear attraction.
+template <int L>
+OHD void acc_nuc_herm(const PairConst &q, double a3, const double *C, double w,
+                      double (&W)[nherm(L)])
 {
-    constexpr int NA = ncart(LA), NB = ncart(LB), L = LA + LB;
     const double z3 = q.zeta + a3;
     const double inv = 1.0 / z3;
-    const double h = 0.5 * inv;
-    double GA[3], GB[3], GC[3], gc2 = 0.0, e = 0.0;
+    const double mu = q.zeta * a3 * inv;
+    const double kap = q.zeta * q.zeta * inv;
+    double X[3], X2 = 0.0;
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-        double G = (q.S[x] + a3 * C[x]) * inv;
-        GA[x] = G - q.A[x];
-        GB[x] = G - q.B[x];
-        GC[x] = G - C[x];
-        gc2 += GC[x] * GC[x];
-        double d1 = q.A[x] - q.B[x], d2 = q.A[x] - C[x], d3 = q.B[x] - C[x];
-        e += q.a1 * q.a2 * (d1 * d1) + q.a1 * a3 * (d2 * d2) + q.a2 * a3 * (d3 * d3);
+        X[x] = q.P[x] - C[x];
+        X2 += X[x] * X[x];
     }
     double F[L + 1];
-    boys<L>(z3 * gc2, F);
-    const double pfac = 2.0 * (kPi * inv) * exp(-e * inv);
-    double th[L + 1][NA][NB];
+    boys<L>(kap * X2, F);
+    const double pre = w * 2.0 * (kPi * inv) * exp(-mu * X2);
+    // R^(n)_000 = pre (-2)^n G_n
+    double Rn[L + 1][nherm(L)];
+    {
+        double mup[L + 1], kp[L + 1];
+        mup[0] = 1.0;
+        kp[0] = 1.0;
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
+        for (int k = 1; k <= L; ++k) {
+            mup[k] = mup[k - 1] * mu;
+            kp[k] = kp[k - 1] * kap;
+        }
+        double sgn = 1.0;
 #pragma unroll
-        for (int a = 0; a < NA; ++a) {
-            constexpr int dummy = 0;
-            (void)dummy;
-            const int mmax = L - ctot(a) - ctot(b);
+        for (int n = 0; n <= L; ++n) {
+            double g = 0.0;
+            double binom = 1.0;
 #pragma unroll
-            for (int m = 0; m <= L; ++m) {
-                if (m > mmax) continue;
-                double v;
-                if (a == 0 && b == 0) {
-                    v = F[m] * pfac;
-                } else if (b != 0) {
-                    const int i = cfirst(b);
-                    const int b1 = cdec(b, i, 1);
-                    const int m1 = m < L ? m + 1 : L;
-                    v = GB[i] * th[m][a][b1] - GC[i] * th[m1][a][b1];
-                    double t = 0.0;
-                    if (cexp(a, i) > 0) {
-                        const int a1 = cdec(a, i, 1);
-                        t += cexp(a, i) * (th[m][a1][b1] - th[m1][a1][b1]);
+            for (int k = 0; k <= n; ++k) {
+                g += binom * mup[n - k] * kp[k] * F[k];
+                binom = binom * (n - k) / (k + 1);
+            }
+            Rn[n][0] = pre * sgn * g;
+            sgn = -2.0 * sgn;
+        }
+    }
+#pragma unroll
+    for (int n = L - 1; n >= 0; --n)
+#pragma unroll
+        for (int tt = 0; tt <= L - n; ++tt)
+#pragma unroll
+            for (int u = 0; u <= L - n - tt; ++u)
+#pragma unroll
+                for (int v = 0; v <= L - n - tt - u; ++v) {
+                    if (tt + u + v == 0) continue;
+                    double val;
+                    if (tt > 0) {
+                        val = X[0] * Rn[n + 1][herm_index(tt - 1, u, v)];
+                        if (tt > 1) val += (tt - 1) * Rn[n + 1][herm_index(tt > 1 ? tt - 2 : 0, u, v)];
+                    } else if (u > 0) {
+                        val = X[1] * Rn[n + 1][herm_index(tt, u - 1, v)];
+                        if (u > 1) val += (u - 1) * Rn[n + 1][herm_index(tt, u > 1 ? u - 2 : 0, v)];
+                    } else {
+                        val = X[2] * Rn[n + 1][herm_index(tt, u, v - 1)];
+                        if (v > 1) val += (v - 1) * Rn[n + 1][herm_index(tt, u, v > 1 ? v - 2 : 0)];
                     }
-                    if (cexp(b, i) > 1) {
-                        const int b2 = cdec(b, i, 2);
-                        t += (cexp(b, i) - 1) * (th[m][a][b2] - th[m1][a][b2]);
-                    }
-                    v += h * t;
-                } else {
-                    const int i = cfirst(a);
-                    const int a1 = cdec(a, i, 1);
-                    const int m1 = m < L ? m + 1 : L;
-                    v = GA[i] * th[m][a1][0] - GC[i] * th[m1][a1][0];
-                    if (cexp(a, i) > 1) {
-                        const int a2 = cdec(a, i, 2);
-                        v += h * (cexp(a, i) - 1) * (th[m][a2][0] - th[m1][a2][0]);
-                    }
+                    Rn[n][herm_index(tt, u, v)] = val;
                 }
-                th[m][a][b] = v;
+#pragma unroll
+    for (int e = 0; e < nherm(L); ++e) W[e] += Rn[0][e];
+}
+
+// Hermite expansion coefficients of one axis about P:
+// E^{i+1,j}_t = h E^{ij}_{t-1} + XPA E^{ij}_t + (t+1) E^{ij}_{t+1}, and the
+// analogue along j with XPB; E^{00}_0 = 1, h = 1/(2 zeta).
+template <int LA, int LB>
+OHD void herm_e(double XPA, double XPB, double h, double (&E)[LA + 1][LB + 1][LA + LB + 1])
+{
+    constexpr int L = LA + LB;
+#pragma unroll
+    for (int i = 0; i <= LA; ++i)
+#pragma unroll
+        for (int j = 0; j <= LB; ++j)
+#pragma unroll
+            for (int t = 0; t <= L; ++t) E[i][j][t] = 0.0;
+    E[0][0][0] = 1.0;
+#pragma unroll
+    for (int i = 0; i <= LA; ++i)
+#pragma unroll
+        for (int j = 0; j <= LB; ++j) {
+            if (i == 0 && j == 0) continue;
+            const int pi = j > 0 ? i : i - 1, pj = j > 0 ? j - 1 : j;
+            const double X = j > 0 ? XPB : XPA;
+#pragma unroll
+            for (int t = 0; t <= i + j; ++t) {
+                double v = X * E[pi][pj][t];
+                if (t > 0) v += h * E[pi][pj][t > 0 ? t - 1 : 0];
+                if (t + 1 <= L) v += (t + 1) * E[pi][pj][t + 1 <= L ? t + 1 : L];
+                E[i][j][t] = v;
             }
         }
-#pragma unroll
-    for (int a = 0; a < NA; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] += w * th[0][a][b];
+}
+
+// Cartesian component (a, b) of K_AB sum_tuv E E E W.
+template <int LA, int LB>
+OHD double herm_contract(const double (&E)[3][LA + 1][LB + 1][LA + LB + 1], const double *W,
+                         int ax, int ay, int az, int bx, int by, int bz)
+{
+    double s = 0.0;
+    for (int t = 0; t <= ax + bx; ++t)
+        for (int u = 0; u <= ay + by; ++u) {
+            const double exy = E[0][ax][bx][t] * E[1][ay][by][u];
+            for (int v = 0; v <= az + bz; ++v) s += exy * E[2][az][bz][v] * W[herm_index(t, u, v)];
+        }
+    return s;
 }
 
 }  // namespace olcao

@@ -140,6 +140,7 @@ struct olcao_ctx {
     DBuf<double> S, H, Aw, Bw, D, E, CV_S, CC_S, CV_H, CC_H;
     DBuf<int> dInfo;
     DBuf<unsigned long long> dCounters;
+    DBuf<int> dHeads;
     // Mode A term matrices
     DBuf<double> KEm, NPm;
     std::vector<DBuf<double>> Vm;
@@ -480,6 +481,24 @@ void build_lists(olcao_ctx &c)
         }
     }
     c.blockDoubles = off;
+    // heaviest primitive pairs first: estimated cost ~ volume of the widest
+    // potential screening sphere, (budget (1/zeta + 1/a3min))^(3/2)
+    double a3min = 1e300;
+    for (int t = 0; t < c.nPT; ++t)
+        if (c.hPotNA[t] > 0) a3min = std::min(a3min, c.hPotAlpha[(size_t)t * c.maxPA]);
+        else a3min = std::min(a3min, c.hPotNuc[t]);
+    std::vector<double> cost(c.hPpPc.size());
+    for (size_t pp = 0; pp < c.hPpPc.size(); ++pp) {
+        const int pc = c.hPpPc[pp], ab = c.hPpAB[pp];
+        const int pr = c.hPcPair[pc];
+        const int ti = c.hAtomType[c.hPairI[pr]], tj = c.hAtomType[c.hPairJ[pr]];
+        const double e1 = c.hElemAlpha[ti * mA + pp_a1(ab)], e2 = c.hElemAlpha[tj * mA + pp_a2(ab)];
+        const double budget = std::max(0.0, c.tau * (1.0 - c.hPcS2[pc] / (c.tau / e2 / (e1 / (e1 + e2)))));
+        const double z = c.hAlpha[ti * mA + pp_a1(ab)] + c.hAlpha[tj * mA + pp_a2(ab)];
+        cost[pp] = std::pow(budget * (1.0 / z + 1.0 / a3min), 1.5);
+    }
+    for (auto &lst : c.classIdx)
+        std::stable_sort(lst.begin(), lst.end(), [&](int x, int y) { return cost[x] > cost[y]; });
 }
 
 void upload(olcao_ctx &c)
@@ -510,6 +529,7 @@ void upload(olcao_ctx &c)
     c.dX.alloc(std::max<size_t>(c.hPcPair.size() * (size_t)c.maxS * c.maxS, 1));
     c.dCoef.alloc(std::max(c.potDim, 1));
     c.dCounters.alloc(32);
+    c.dHeads.alloc(32);
 
     DevCtx &d = c.dev;
     d.pos = c.dPos.p; d.atomType = c.dAtomType.p; d.cumulVale = c.dCumV.p; d.cumulCore = c.dCumC.p;
@@ -542,7 +562,9 @@ void upload(olcao_ctx &c)
 void three_center(olcao_ctx &c, int kind, int tt, int ti, const double *coef, int accumulate)
 {
     launch_fill(reinterpret_cast<double *>(c.dCounters.p), 32, 0.0, c.stream);
-    launch_three_center(c.dev, c.cls, kind, tt, ti, coef, accumulate, c.dBlocks.p, c.stream);
+    launch_fill(reinterpret_cast<double *>(c.dHeads.p), 16, 0.0, c.stream);
+    launch_three_center(c.dev, c.cls, c.dHeads.p, kind, tt, ti, coef, accumulate, c.dBlocks.p,
+                        c.stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -864,8 +886,10 @@ olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates
         HIPCHK(hipEventRecord(c->ev[1], s));
         launch_two_center(c->dev, c->cls, OP_KE, c->dBlocks.p, s);
         launch_fill(reinterpret_cast<double *>(c->dCounters.p), 32, 0.0, s);
+        launch_fill(reinterpret_cast<double *>(c->dHeads.p), 16, 0.0, s);
         HIPCHK(hipEventRecord(c->ev[2], s));
-        launch_three_center(c->dev, c->cls, K3_HAM, 0, 0, c->dCoef.p, 1, c->dBlocks.p, s);
+        launch_three_center(c->dev, c->cls, c->dHeads.p, K3_HAM, 0, 0, c->dCoef.p, 1,
+                            c->dBlocks.p, s);
         HIPCHK(hipEventRecord(c->ev[3], s));
         launch_contract(c->dev, c->dBlocks.p, c->dX.p, s);
         launch_fill(c->H.p, (long long)c->matDoubles(), 0.0, s);
@@ -974,13 +998,22 @@ olcao_status olcao_prim_batch(int op, int n, const double *a, const double *A, c
             if (!ok(lo) || !ok(hi) || (sw[i] >> 8)) fail(OLCAO_ERR_INPUT, "bad l1l2switch %d", sw[i]);
         }
         DBuf<double> da, dA, dB, dC, dsh;
-        DBuf<int> dsw;
         std::vector<double> va(a, a + 3 * (size_t)n), vA(A, A + 3 * (size_t)n), vB(B, B + 3 * (size_t)n),
             vC(C, C + 3 * (size_t)n);
-        std::vector<int> vs(sw, sw + n);
-        da.upload(va, 0); dA.upload(vA, 0); dB.upload(vB, 0); dC.upload(vC, 0); dsw.upload(vs, 0);
+        da.upload(va, 0); dA.upload(vA, 0); dB.upload(vB, 0); dC.upload(vC, 0);
+        std::vector<std::vector<int>> byClass(16);
+        auto lof = [](int nib) { return nib == 1 ? 0 : nib == 2 ? 1 : nib == 4 ? 2 : 3; };
+        for (int i = 0; i < n; ++i) byClass[lof(sw[i] & 15) * 4 + lof((sw[i] >> 4) & 15)].push_back(i);
+        DBuf<int> dIdx[16];
+        const int *idx[16];
+        int cnt[16];
+        for (int k = 0; k < 16; ++k) {
+            dIdx[k].upload(byClass[k], 0);
+            idx[k] = dIdx[k].p;
+            cnt[k] = (int)byClass[k].size();
+        }
         dsh.alloc(256 * (size_t)n);
-        launch_prim_batch(op, n, da.p, dA.p, dB.p, dC.p, dsw.p, dsh.p, 0);
+        launch_prim_batch(op, n, da.p, dA.p, dB.p, dC.p, idx, cnt, dsh.p, 0);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(sh, dsh.p, 256 * (size_t)n * 8, hipMemcpyDeviceToHost));
     });
