@@ -184,6 +184,25 @@ __global__ void k_assemble(long long total, int n, const double *NP, const doubl
     }
 }
 
+// Lower triangle from the upper one: A(r,c) = conj(A(c,r)) for r > c.
+__global__ void k_herm_fill(double *A, int n, int nb, int cplx)
+{
+    const long long per = (long long)n * n;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < per * nb;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long k = t / per, e = t - k * per;
+        const int r = (int)(e % n), col = (int)(e / n);
+        if (r <= col) continue;
+        const long long src = k * per + (long long)col + (long long)r * n;
+        if (cplx == 2) {
+            A[2 * t] = A[2 * src];
+            A[2 * t + 1] = -A[2 * src + 1];
+        } else {
+            A[t] = A[src];
+        }
+    }
+}
+
 // first numStates eigenvector columns of each k, compacted
 __global__ void k_extract(const double *A, int n, int nk, int ns, int cplx, double *out)
 {
@@ -222,22 +241,160 @@ void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, double *bl
     two_one<3, 2>(c, cl, op, blocks, s); two_one<3, 3>(c, cl, op, blocks, s);
 }
 
-void launch_3c_la0(const DevCtx &, const ClassLists &, int *, int, int, int, const double *,
-                   int, double *, hipStream_t);
-void launch_3c_la1(const DevCtx &, const ClassLists &, int *, int, int, int, const double *,
-                   int, double *, hipStream_t);
-void launch_3c_la2(const DevCtx &, const ClassLists &, int *, int, int, int, const double *,
-                   int, double *, hipStream_t);
-void launch_3c_la3(const DevCtx &, const ClassLists &, int *, int, int, int, const double *,
-                   int, double *, hipStream_t);
+void launch_3c_la0(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
+                   double *, hipStream_t);
+void launch_3c_la1(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
+                   double *, hipStream_t);
+void launch_3c_la2(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
+                   double *, hipStream_t);
+void launch_3c_la3(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
+                   double *, hipStream_t);
 
-void launch_three_center(const DevCtx &c, const ClassLists &cl, int *heads, int kind, int tt,
-                         int ti, const double *coef, int accumulate, double *blocks, hipStream_t s)
+void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind, int tt, int ti,
+                         const double *coef, int accumulate, double *blocks, hipStream_t s)
 {
-    launch_3c_la0(c, cl, heads, kind, tt, ti, coef, accumulate, blocks, s);
-    launch_3c_la1(c, cl, heads, kind, tt, ti, coef, accumulate, blocks, s);
-    launch_3c_la2(c, cl, heads, kind, tt, ti, coef, accumulate, blocks, s);
-    launch_3c_la3(c, cl, heads, kind, tt, ti, coef, accumulate, blocks, s);
+    launch_3c_la0(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
+    launch_3c_la1(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
+    launch_3c_la2(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
+    launch_3c_la3(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
+}
+
+// ---------------------------------------------------------------------------
+// Per-pair image lists of the three-centre sweeps, built once per geometry.
+//
+// For primitive pair i (one thread each) every image C = site + L of every
+// potential site inside the widest range any sweep kind can use is visited
+// (a box in fractional coordinates about the screening centre P, then the
+// exact distance), and the reference's range tests are applied term by term
+// exactly as the sweeps apply them (integrals.F90:2625-2663 electronic and
+// nuclear sweeps, :3211-3430 the fused sweep; the term loop stops at the
+// first term out of range because the exponents ascend).  Pass 1 counts the
+// entries and the fused-sweep items, pass 2 writes the entries (one kernel
+// with a run-time switch, so both passes run the same arithmetic; pass 2
+// still never writes past its record's segment and flags any mismatch).
+__global__ __launch_bounds__(64) void k_image_list(DevCtx c, const PPRec *recs, int n, int fill,
+                                                   int *nEnt, int *nItems, const long long *off,
+                                                   unsigned long long *ent, int *overflow)
+{
+    __shared__ double tAlpha[kMaxPotTypes * kMaxPotAlphas];
+    __shared__ double tZ[kMaxPotTypes], tNuc[kMaxPotTypes];
+    __shared__ int tNA[kMaxPotTypes];
+    extern __shared__ double dyn[];
+    double *sitePos = dyn;
+    int *siteType = reinterpret_cast<int *>(dyn + 3 * c.nPotSites);
+    const int lane = threadIdx.x;
+    for (int q = lane; q < c.nPotSites; q += 64) {
+        sitePos[3 * q] = c.potPos[3 * q];
+        sitePos[3 * q + 1] = c.potPos[3 * q + 1];
+        sitePos[3 * q + 2] = c.potPos[3 * q + 2];
+        siteType[q] = c.potSiteType[q];
+    }
+    for (int q = lane; q < c.nPotTypes * kMaxPotAlphas; q += 64) {
+        const int t = q / kMaxPotAlphas, a = q - t * kMaxPotAlphas;
+        tAlpha[q] = a < c.potNA[t] ? c.potAlpha[t * c.maxPA + a] : 0.0;
+    }
+    for (int t = lane; t < c.nPotTypes; t += 64) {
+        tZ[t] = c.potZ[t];
+        tNuc[t] = c.potNuc[t];
+        tNA[t] = c.potNA[t];
+    }
+    __syncthreads();
+    const int i = blockIdx.x * 64 + lane;
+    if (i >= n) return;
+    const PPRec rec = recs[i];
+    // screening centre and ranges (element-level exponents, gaussRelations.f90)
+    const double acen = rec.ea1 / (rec.ea1 + rec.ea2);
+    const double adist = c.tau / rec.ea2 / acen;
+    double P[3];
+    for (int x = 0; x < 3; ++x) P[x] = acen * (rec.A[x] - rec.B[x]) + rec.B[x];
+    const double shrink = 1.0 - rec.s2 / adist;
+    const double budget = c.tau * shrink;
+    const double a12 = rec.a1 + rec.a2, e12 = rec.ea1 + rec.ea2, tau = c.tau;
+    // fused sweep: tau (1 - s^2/AD) (a1 + a2 + a3) / (a1 + a2) / a3 (integral exponents);
+    // separate sweeps: (1 - s^2/AD) tau (e1 + e2 + a3) / (e1 + e2) / a3
+    auto tadHam = [&](double a3) { return budget * (a12 + a3) / a12 / a3; };
+    auto tadEp = [&](double a3) { return shrink * (tau * (e12 + a3) / e12 / a3); };
+    double invn[3];
+    for (int q = 0; q < 3; ++q)
+        invn[q] = sqrt(c.inv[3 * q] * c.inv[3 * q] + c.inv[3 * q + 1] * c.inv[3 * q + 1] +
+                       c.inv[3 * q + 2] * c.inv[3 * q + 2]);
+    long long w = fill ? off[i] : 0;
+    const long long wend = fill ? off[i + 1] : 0;
+    int cnt = 0, items = 0, bad = 0;
+    for (int st = 0; st < c.nPotSites; ++st) {
+        const int t = siteType[st];
+        const int na = tNA[t];
+        const bool zok = tZ[t] >= kSmallThresh;
+        double R2 = -1.0;
+        if (na > 0) R2 = fmax(tadHam(tAlpha[t * kMaxPotAlphas]), tadEp(tAlpha[t * kMaxPotAlphas]));
+        if (zok) R2 = fmax(R2, fmax(tadHam(tNuc[t]), tadEp(tNuc[t])));
+        if (!(R2 >= 0.0)) continue;
+        const double r = sqrt(R2) * (1.0 + 1e-12) + 1e-12;
+        double cov[3];
+        for (int x = 0; x < 3; ++x) cov[x] = P[x] - sitePos[3 * st + x];
+        int lo[3], hi[3];
+        for (int q = 0; q < 3; ++q) {
+            const double f = c.inv[3 * q] * cov[0] + c.inv[3 * q + 1] * cov[1] +
+                             c.inv[3 * q + 2] * cov[2];
+            lo[q] = (int)ceil(f - r * invn[q]);
+            hi[q] = (int)floor(f + r * invn[q]);
+        }
+        for (int nz = lo[2]; nz <= hi[2]; ++nz)
+            for (int ny = lo[1]; ny <= hi[1]; ++ny)
+                for (int nx = lo[0]; nx <= hi[0]; ++nx) {
+                    double Lv[3];
+                    for (int x = 0; x < 3; ++x)
+                        Lv[x] = nx * c.vec[x] + ny * c.vec[3 + x] + nz * c.vec[6 + x];
+                    const double r0 = cov[0] - Lv[0], r1 = cov[1] - Lv[1], r2 = cov[2] - Lv[2];
+                    const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+                    if (d2 > R2) continue;
+                    int nHam = 0, nEp = 0;
+                    while (nHam < na && d2 <= tadHam(tAlpha[t * kMaxPotAlphas + nHam])) ++nHam;
+                    while (nEp < na && d2 <= tadEp(tAlpha[t * kMaxPotAlphas + nEp])) ++nEp;
+                    const int nucHam = zok && d2 <= tadHam(tNuc[t]) ? 1 : 0;
+                    const int nucNp = zok && d2 <= tadEp(tNuc[t]) ? 1 : 0;
+                    if (nHam + nEp + nucHam + nucNp == 0) continue;
+                    ++cnt;
+                    items += nHam;
+                    if (fill) {
+                        if (nx < -kImgLatOff || nx >= kImgLatOff || ny < -kImgLatOff ||
+                            ny >= kImgLatOff || nz < -kImgLatOff || nz >= kImgLatOff)
+                            bad |= 1;
+                        if (w < wend)
+                            ent[w++] = img_pack(st, nx, ny, nz, nHam, nEp, nucHam, nucNp);
+                        else
+                            bad |= 2;
+                    }
+                }
+    }
+    if (!fill) {
+        nEnt[i] = cnt;
+        nItems[i] = items;
+    } else {
+        if (w != wend) bad |= 2;
+        if (bad) atomicOr(overflow, bad);
+    }
+}
+
+static size_t site_lds(const DevCtx &c)
+{
+    return (size_t)c.nPotSites * (3 * sizeof(double) + sizeof(int)) + 16;
+}
+
+void launch_image_count(const DevCtx &c, const PPRec *recs, int n, int *nEnt, int *nItems,
+                        hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_image_list, dim3((n + 63) / 64), dim3(64), site_lds(c), s, c, recs, n,
+                       0, nEnt, nItems, nullptr, nullptr, nullptr);
+}
+
+void launch_image_fill(const DevCtx &c, const PPRec *recs, int n, const long long *off,
+                       unsigned long long *ent, int *overflow, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_image_list, dim3((n + 63) / 64), dim3(64), site_lds(c), s, c, recs, n,
+                       1, nullptr, nullptr, off, ent, overflow);
 }
 
 void launch_contract(const DevCtx &c, const double *blocks, double *X, hipStream_t s)
@@ -269,6 +426,14 @@ void launch_assemble(long long total, int n, const double *NP, const double *KE,
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_assemble, dim3((unsigned)blocks), dim3(256), 0, s, total, n, NP, KE,
                        V, coef, nTerms, H, cplx);
+}
+
+void launch_herm_fill(double *A, int n, int nb, int cplx, hipStream_t s)
+{
+    long long total = (long long)n * n * nb;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_herm_fill, dim3((unsigned)blocks), dim3(256), 0, s, A, n, nb, cplx);
 }
 
 void launch_extract_evecs(const double *A, int n, int nk, int ns, int cplx, double *out,

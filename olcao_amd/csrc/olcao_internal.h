@@ -7,6 +7,9 @@
 namespace olcao {
 
 constexpr double kSmallThresh = 1.0e-8;   // src/constants.f90 smallThresh
+constexpr int kMaxPotTypes = 8;           // potential types staged in LDS
+constexpr int kMaxPotAlphas = 24;         // potential exponents per type
+constexpr int kMaxPotSites = 1024;        // sites staged in LDS (28 B each)
 
 // Three-centre sweep kinds.
 enum ThreeKind { K3_HAM = 0, K3_EP = 1, K3_NP = 2 };
@@ -68,6 +71,7 @@ struct DevCtx {
     // optional work counters [16 classes][2]: three-Gaussian overlap and
     // nuclear evaluations of the last three-centre sweep
     unsigned long long *counters;
+    int anyNuc;                 // some potential type has Z >= smallThresh
 };
 
 inline __host__ __device__ int pp_a1(int ab) { return ab & 255; }
@@ -75,18 +79,65 @@ inline __host__ __device__ int pp_a2(int ab) { return (ab >> 8) & 255; }
 inline __host__ __device__ int pp_la(int ab) { return (ab >> 16) & 15; }
 inline __host__ __device__ int pp_lb(int ab) { return (ab >> 20) & 15; }
 
+// One primitive pair as the three-centre kernels read it (one load).
+struct PPRec {
+    double A[3], B[3];          // atom-1 position, shifted atom-2 position
+    double a1, a2;              // integral exponents
+    double ea1, ea2;            // screening exponents
+    double s2;                  // |A - B|^2
+    long long off;              // block offset (doubles)
+};
+
+// Image entry of the per-pair three-centre lists (64 bits): the potential
+// image C = site + nx a1 + ny a2 + nz a3 and how many terms each sweep kind
+// takes from it, i.e. the reference's range tests evaluated once per
+// geometry:
+//   site [0,12) | nx+512 [12,22) | ny+512 [22,32) | nz+512 [32,42) |
+//   nHam [42,48) | nEp [48,54) | nucHam [54] | nucNp [55]
+// nHam / nEp: leading potential terms of the site's type inside range with
+// the fused-sweep / separate-sweep expression; nucHam / nucNp: the nucleus.
+constexpr int kImgLatOff = 512;
+inline __host__ __device__ unsigned long long img_pack(int site, int nx, int ny, int nz, int nHam,
+                                                       int nEp, int nucHam, int nucNp)
+{
+    return (unsigned long long)site | ((unsigned long long)(nx + kImgLatOff) << 12) |
+           ((unsigned long long)(ny + kImgLatOff) << 22) |
+           ((unsigned long long)(nz + kImgLatOff) << 32) | ((unsigned long long)nHam << 42) |
+           ((unsigned long long)nEp << 48) | ((unsigned long long)nucHam << 54) |
+           ((unsigned long long)nucNp << 55);
+}
+inline __host__ __device__ int img_site(unsigned long long e) { return (int)(e & 4095); }
+inline __host__ __device__ int img_n(unsigned long long e, int k)   // k = 0, 1, 2 -> nx, ny, nz
+{
+    return (int)((e >> (12 + 10 * k)) & 1023) - kImgLatOff;
+}
+inline __host__ __device__ int img_nham(unsigned long long e) { return (int)((e >> 42) & 63); }
+inline __host__ __device__ int img_nep(unsigned long long e) { return (int)((e >> 48) & 63); }
+inline __host__ __device__ int img_nucham(unsigned long long e) { return (int)((e >> 54) & 1); }
+inline __host__ __device__ int img_nucnp(unsigned long long e) { return (int)((e >> 55) & 1); }
+
 // Kernel launchers (kernels.hip).  All asynchronous on `s`.
 struct ClassLists {
     const int *idx[16];         // device list of primitive pairs per class
+    const PPRec *rec[16];       // the class's pairs as packed records, heaviest first
+    const long long *ioff[16];  // [count + 1] image-list offsets per record
+    const unsigned long long *img[16];   // image entries
+    double *W[16];              // Hermite sums [nherm(la+lb)][count] per class
     int count[16];
 };
 
 void launch_two_center(const DevCtx &c, const ClassLists &cl, int op,
                        double *blocks, hipStream_t s);
-// heads: 32 zeroed ints, the work counters of the persistent kernels.
-void launch_three_center(const DevCtx &c, const ClassLists &cl, int *heads, int kind,
-                         int termType, int termIdx, const double *coef,
-                         int accumulate, double *blocks, hipStream_t s);
+void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind, int termType,
+                         int termIdx, const double *coef, int accumulate, double *blocks,
+                         hipStream_t s);
+// Image lists: pass 1 counts entries and fused-sweep items per record, pass 2
+// writes the entries at off[i]; *overflow is set if a lattice index does not
+// fit the entry.
+void launch_image_count(const DevCtx &c, const PPRec *recs, int n, int *nEnt, int *nItems,
+                        hipStream_t s);
+void launch_image_fill(const DevCtx &c, const PPRec *recs, int n, const long long *off,
+                       unsigned long long *ent, int *overflow, hipStream_t s);
 void launch_contract(const DevCtx &c, const double *blocks, double *X,
                      hipStream_t s);
 void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv,
@@ -100,6 +151,7 @@ void launch_prim_batch(int op, int n, const double *a, const double *A,
                        const double *B, const double *C, const int *const *classIdx,
                        const int *classCount, double *sh, hipStream_t s);
 void launch_boys_batch(int n, const double *T, double *F, hipStream_t s);
+void launch_herm_fill(double *A, int n, int nb, int cplx, hipStream_t s);
 void launch_extract_evecs(const double *A, int n, int nk, int numStates,
                           int cplx, double *out, hipStream_t s);
 

@@ -17,6 +17,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -128,6 +129,8 @@ struct olcao_ctx {
     std::vector<long long> hPpOff;
     long long blockDoubles = 0;
     std::vector<std::vector<int>> classIdx;
+    // three-centre records per class: primitive pairs, heaviest first
+    std::vector<std::vector<int>> recOrder;
 
     // device
     DBuf<double> dPos, dAlpha, dElemAlpha, dBasis, dPotAlpha, dPotZ, dPotNuc, dPotPos, dCellDims,
@@ -136,11 +139,16 @@ struct olcao_ctx {
         dPairI, dPairJ, dPairPc, dPcPair, dPcCell, dPcPp, dPpPc, dPpAB;
     DBuf<long long> dPpOff;
     DBuf<int> dClass[16];
+    DBuf<PPRec> dRec[16];
+    DBuf<long long> dIoff[16];
+    DBuf<unsigned long long> dImg[16];
+    DBuf<double> dW;
+    long long imageEntries = 0;
+    double listMs = 0.0;
     DBuf<double> dBlocks, dX, dCoef;
     DBuf<double> S, H, Aw, Bw, D, E, CV_S, CC_S, CV_H, CC_H;
     DBuf<int> dInfo;
     DBuf<unsigned long long> dCounters;
-    DBuf<int> dHeads;
     // Mode A term matrices
     DBuf<double> KEm, NPm;
     std::vector<DBuf<double>> Vm;
@@ -290,6 +298,11 @@ void setup_system(olcao_ctx &c, const olcao_system &s)
     }
 
     // --- potential ---
+    if (s.nPotTypes > kMaxPotTypes || s.maxPotAlphas > kMaxPotAlphas)
+        fail(OLCAO_ERR_UNSUPPORTED, "at most %d potential types of at most %d exponents are supported",
+             kMaxPotTypes, kMaxPotAlphas);
+    if (s.nPotSites > kMaxPotSites)
+        fail(OLCAO_ERR_UNSUPPORTED, "at most %d potential sites are supported", kMaxPotSites);
     c.nPT = s.nPotTypes;
     c.maxPA = s.maxPotAlphas;
     c.nPS = s.nPotSites;
@@ -501,6 +514,26 @@ void build_lists(olcao_ctx &c)
         std::stable_sort(lst.begin(), lst.end(), [&](int x, int y) { return cost[x] > cost[y]; });
 }
 
+PPRec make_rec(const olcao_ctx &c, int pp)
+{
+    const int pc = c.hPpPc[pp], ab = c.hPpAB[pp];
+    const int pr = c.hPcPair[pc];
+    const int i = c.hPairI[pr], j = c.hPairJ[pr];
+    const int ti = c.hAtomType[i], tj = c.hAtomType[j];
+    PPRec r;
+    for (int x = 0; x < 3; ++x) {
+        r.A[x] = c.hPos[3 * i + x];
+        r.B[x] = c.hPcB[3 * pc + x];
+    }
+    r.a1 = c.hAlpha[ti * c.maxA + pp_a1(ab)];
+    r.a2 = c.hAlpha[tj * c.maxA + pp_a2(ab)];
+    r.ea1 = c.hElemAlpha[ti * c.maxA + pp_a1(ab)];
+    r.ea2 = c.hElemAlpha[tj * c.maxA + pp_a2(ab)];
+    r.s2 = c.hPcS2[pc];
+    r.off = c.hPpOff[pp];
+    return r;
+}
+
 void upload(olcao_ctx &c)
 {
     hipStream_t s = c.stream;
@@ -524,12 +557,25 @@ void upload(olcao_ctx &c)
         c.dClass[k].upload(c.classIdx[k], s);
         c.cls.idx[k] = c.dClass[k].p;
         c.cls.count[k] = (int)c.classIdx[k].size();
+        std::vector<PPRec> recs(c.classIdx[k].size());
+        for (size_t q = 0; q < recs.size(); ++q) recs[q] = make_rec(c, c.classIdx[k][q]);
+        c.dRec[k].upload(recs, s);
+        c.cls.rec[k] = c.dRec[k].p;
     }
+    // Hermite-sum scratch of the three-centre sweeps, one slice per class
+    size_t wTotal = 0;
+    size_t wOff[16];
+    for (int k = 0; k < 16; ++k) {
+        const int L = k / 4 + k % 4;
+        wOff[k] = wTotal;
+        wTotal += (size_t)(L + 1) * (L + 2) * (L + 3) / 6 * c.classIdx[k].size();
+    }
+    c.dW.alloc(std::max<size_t>(wTotal, 1));
+    for (int k = 0; k < 16; ++k) c.cls.W[k] = c.dW.p + wOff[k];
     c.dBlocks.alloc(std::max<long long>(c.blockDoubles, 1));
     c.dX.alloc(std::max<size_t>(c.hPcPair.size() * (size_t)c.maxS * c.maxS, 1));
     c.dCoef.alloc(std::max(c.potDim, 1));
     c.dCounters.alloc(32);
-    c.dHeads.alloc(32);
 
     DevCtx &d = c.dev;
     d.pos = c.dPos.p; d.atomType = c.dAtomType.p; d.cumulVale = c.dCumV.p; d.cumulCore = c.dCumC.p;
@@ -554,7 +600,65 @@ void upload(olcao_ctx &c)
     d.nPp = (long long)c.hPpPc.size();
     d.ppPc = c.dPpPc.p; d.ppAB = c.dPpAB.p; d.ppOff = c.dPpOff.p;
     d.counters = c.dCounters.p;
+    d.anyNuc = 0;
+    for (int t = 0; t < c.nPT; ++t)
+        if (c.hPotZ[t] >= kSmallThresh) d.anyNuc = 1;
     HIPCHK(hipStreamSynchronize(s));
+}
+
+// Per-pair image lists of the three-centre sweeps.  The screening depends on
+// the geometry and the exponents only, so it runs once per context: count
+// the entries and the fused-sweep items of every primitive pair, order each
+// class's records by item count (heaviest first, so that the 64 lanes of a
+// wave carry nearly equal work), then write the entries in that order.
+void build_image_lists(olcao_ctx &c)
+{
+    hipStream_t s = c.stream;
+    const auto t0 = std::chrono::steady_clock::now();
+    DBuf<int> dCnt, dItems, dOverflow;
+    dOverflow.alloc(1);
+    HIPCHK(hipMemsetAsync(dOverflow.p, 0, sizeof(int), s));
+    c.recOrder.assign(16, {});
+    c.imageEntries = 0;
+    for (int k = 0; k < 16; ++k) {
+        const int n = c.cls.count[k];
+        if (n == 0) continue;
+        dCnt.alloc(n);
+        dItems.alloc(n);
+        launch_image_count(c.dev, c.cls.rec[k], n, dCnt.p, dItems.p, s);
+        std::vector<int> cnt(n), items(n);
+        HIPCHK(hipMemcpyAsync(cnt.data(), dCnt.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(items.data(), dItems.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<int> order(n);
+        for (int q = 0; q < n; ++q) order[q] = q;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return items[x] > items[y]; });
+        std::vector<PPRec> recs(n);
+        std::vector<long long> off(n + 1, 0);
+        c.recOrder[k].resize(n);
+        for (int q = 0; q < n; ++q) {
+            c.recOrder[k][q] = c.classIdx[k][order[q]];
+            recs[q] = make_rec(c, c.recOrder[k][q]);
+            off[q + 1] = off[q] + cnt[order[q]];
+        }
+        c.dRec[k].upload(recs, s);
+        c.cls.rec[k] = c.dRec[k].p;
+        c.dIoff[k].upload(off, s);
+        c.cls.ioff[k] = c.dIoff[k].p;
+        c.dImg[k].alloc(std::max<long long>(off[n], 1));
+        c.cls.img[k] = c.dImg[k].p;
+        c.imageEntries += off[n];
+        launch_image_fill(c.dev, c.cls.rec[k], n, c.cls.ioff[k], c.dImg[k].p, dOverflow.p, s);
+    }
+    int overflow = 0;
+    HIPCHK(hipMemcpyAsync(&overflow, dOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    if (overflow & 1)
+        fail(OLCAO_ERR_UNSUPPORTED, "a potential image lies more than %d cells away", kImgLatOff);
+    if (overflow & 2) fail(OLCAO_ERR_HIP, "image-list passes disagree");
+    c.listMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // Three-centre sweep with its work counters cleared first (a fill kernel,
@@ -562,9 +666,7 @@ void upload(olcao_ctx &c)
 void three_center(olcao_ctx &c, int kind, int tt, int ti, const double *coef, int accumulate)
 {
     launch_fill(reinterpret_cast<double *>(c.dCounters.p), 32, 0.0, c.stream);
-    launch_fill(reinterpret_cast<double *>(c.dHeads.p), 16, 0.0, c.stream);
-    launch_three_center(c.dev, c.cls, c.dHeads.p, kind, tt, ti, coef, accumulate, c.dBlocks.p,
-                        c.stream);
+    launch_three_center(c.dev, c.cls, kind, tt, ti, coef, accumulate, c.dBlocks.p, c.stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -618,6 +720,73 @@ void ensure_mats(olcao_ctx &c)
     }
 }
 
+// Batched generalised eigensolve A x = w B x from the upper triangles
+// (what zhegv/dsygv with itype 1, 'V', 'U' compute, interfaces.F90:73-200):
+// B = U^H U (potrf), C = U^-H A U^-1 (two BLAS-3 triangular solves on the
+// Hermitian-filled A), C = Y diag(w) Y^H (heevd), X = U^-1 Y.  A is replaced
+// by the eigenvectors (all n columns) and B by U.  info[k] > n flags a B(k)
+// that is not positive definite, as zhegv does.
+void gen_eig(rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *A, double *B,
+             double *D, double *E, int *dinfo, std::vector<int> &info)
+{
+    const rocblas_stride st = (rocblas_stride)n * n;
+    info.assign(nb, 0);
+    rocblas_status r;
+    if (cplx)
+        r = rocsolver_zpotrf_strided_batched(h, rocblas_fill_upper, n, (rocblas_double_complex *)B, n,
+                                             st, dinfo, nb);
+    else
+        r = rocsolver_dpotrf_strided_batched(h, rocblas_fill_upper, n, B, n, st, dinfo, nb);
+    if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "potrf status %d", (int)r);
+    HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int k = 0; k < nb; ++k)
+        if (info[k] != 0) {
+            info[k] += n;   // zhegv's convention: B not positive definite
+            return;
+        }
+    launch_herm_fill(A, n, nb, cplx ? 2 : 1, s);
+    if (cplx) {
+        const rocblas_double_complex one = {1.0, 0.0};
+        auto *Az = (rocblas_double_complex *)A;
+        auto *Bz = (const rocblas_double_complex *)B;
+        r = rocblas_ztrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper,
+                                          rocblas_operation_conjugate_transpose,
+                                          rocblas_diagonal_non_unit, n, n, &one, Bz, n, st, Az, n,
+                                          st, nb);
+        if (r == rocblas_status_success)
+            r = rocblas_ztrsm_strided_batched(h, rocblas_side_right, rocblas_fill_upper,
+                                              rocblas_operation_none, rocblas_diagonal_non_unit, n,
+                                              n, &one, Bz, n, st, Az, n, st, nb);
+        if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "trsm status %d", (int)r);
+        r = rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n, Az,
+                                             n, st, D, n, E, n, dinfo, nb);
+        if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "heevd status %d", (int)r);
+        r = rocblas_ztrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper,
+                                          rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
+                                          &one, Bz, n, st, Az, n, st, nb);
+    } else {
+        const double one = 1.0;
+        r = rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper,
+                                          rocblas_operation_transpose, rocblas_diagonal_non_unit, n,
+                                          n, &one, B, n, st, A, n, st, nb);
+        if (r == rocblas_status_success)
+            r = rocblas_dtrsm_strided_batched(h, rocblas_side_right, rocblas_fill_upper,
+                                              rocblas_operation_none, rocblas_diagonal_non_unit, n,
+                                              n, &one, B, n, st, A, n, st, nb);
+        if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "trsm status %d", (int)r);
+        r = rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n, A, n,
+                                             st, D, n, E, n, dinfo, nb);
+        if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "syevd status %d", (int)r);
+        r = rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper,
+                                          rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
+                                          &one, B, n, st, A, n, st, nb);
+    }
+    if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "trsm status %d", (int)r);
+    HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
 // Batched generalised eigensolve of H, S (upper) on the device.
 void solve_device(olcao_ctx &c, int numStates, const double *Hsrc, const double *Ssrc,
                   double *evals, double *evecs)
@@ -634,20 +803,8 @@ void solve_device(olcao_ctx &c, int numStates, const double *Hsrc, const double 
     c.dInfo.alloc(nk);
     HIPCHK(hipMemcpyAsync(c.Aw.p, Hsrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
-    rocblas_status st;
-    if (c.gamma) {
-        st = rocsolver_dsygvd(c.blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n,
-                              c.Aw.p, n, c.Bw.p, n, c.D.p, c.E.p, c.dInfo.p);
-    } else {
-        st = rocsolver_zhegvd_strided_batched(
-            c.blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n,
-            (rocblas_double_complex *)c.Aw.p, n, (rocblas_stride)n * n,
-            (rocblas_double_complex *)c.Bw.p, n, (rocblas_stride)n * n, c.D.p, n, c.E.p, n,
-            c.dInfo.p, nk);
-    }
-    if (st != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "rocsolver hegvd status %d", (int)st);
-    std::vector<int> info(nk);
-    HIPCHK(hipMemcpyAsync(info.data(), c.dInfo.p, nk * sizeof(int), hipMemcpyDeviceToHost, s));
+    std::vector<int> info;
+    gen_eig(c.blas, s, !c.gamma, n, nk, c.Aw.p, c.Bw.p, c.D.p, c.E.p, c.dInfo.p, info);
     std::vector<double> w((size_t)n * nk);
     HIPCHK(hipMemcpyAsync(w.data(), c.D.p, w.size() * 8, hipMemcpyDeviceToHost, s));
     if (evecs) {
@@ -714,6 +871,7 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
         setup_system(*c, *sys);
         build_lists(*c);
         upload(*c);
+        build_image_lists(*c);
     });
     if (st != OLCAO_OK) {
         g_noctx_err = c->err;
@@ -886,9 +1044,8 @@ olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates
         HIPCHK(hipEventRecord(c->ev[1], s));
         launch_two_center(c->dev, c->cls, OP_KE, c->dBlocks.p, s);
         launch_fill(reinterpret_cast<double *>(c->dCounters.p), 32, 0.0, s);
-        launch_fill(reinterpret_cast<double *>(c->dHeads.p), 16, 0.0, s);
         HIPCHK(hipEventRecord(c->ev[2], s));
-        launch_three_center(c->dev, c->cls, c->dHeads.p, K3_HAM, 0, 0, c->dCoef.p, 1,
+        launch_three_center(c->dev, c->cls, K3_HAM, 0, 0, c->dCoef.p, 1,
                             c->dBlocks.p, s);
         HIPCHK(hipEventRecord(c->ev[3], s));
         launch_contract(c->dev, c->dBlocks.p, c->dX.p, s);
@@ -951,24 +1108,13 @@ static olcao_status solve_host(int n, int nb, double *A, double *B, double *w, b
         dI.alloc(nb);
         HIPCHK(hipMemcpyAsync(dA.p, A, per * nb * 8, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(dB.p, B, per * nb * 8, hipMemcpyHostToDevice, s));
-        rocblas_status st;
-        if (cplx)
-            st = rocsolver_zhegvd_strided_batched(h, rocblas_eform_ax, rocblas_evect_original,
-                                                  rocblas_fill_upper, n, (rocblas_double_complex *)dA.p, n,
-                                                  (rocblas_stride)n * n, (rocblas_double_complex *)dB.p, n,
-                                                  (rocblas_stride)n * n, dD.p, n, dE.p, n, dI.p, nb);
-        else
-            st = rocsolver_dsygvd_strided_batched(h, rocblas_eform_ax, rocblas_evect_original,
-                                                  rocblas_fill_upper, n, dA.p, n, (rocblas_stride)n * n,
-                                                  dB.p, n, (rocblas_stride)n * n, dD.p, n, dE.p, n, dI.p, nb);
-        std::vector<int> info(nb);
-        HIPCHK(hipMemcpyAsync(info.data(), dI.p, nb * 4, hipMemcpyDeviceToHost, s));
+        std::vector<int> info;
+        gen_eig(h, s, cplx, n, nb, dA.p, dB.p, dD.p, dE.p, dI.p, info);
         HIPCHK(hipMemcpyAsync(A, dA.p, per * nb * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(w, dD.p, (size_t)n * nb * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         rocblas_destroy_handle(h);
         (void)hipStreamDestroy(s);
-        if (st != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "rocsolver status %d", (int)st);
         for (int k = 0; k < nb; ++k) {
             if (info[k] > n) fail(OLCAO_ERR_NOT_SPD, "B not positive definite in batch %d", k);
             if (info[k] != 0) fail(OLCAO_ERR_SOLVER, "no convergence in batch %d", k);
