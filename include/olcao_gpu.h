@@ -81,6 +81,9 @@ typedef struct {
     const double *kpoints;        /* [nk][3] fractional reciprocal coords */
     double basisCutoff;           /* BASISFUNCTION cutoff, e.g. 1e-16 */
     int gamma;                    /* 1: real Gamma-point build */
+    int rel;                      /* 1: scalar relativistic, adds the mass-velocity
+                                     term (O_Potential rel, potential.f90) */
+    int spin;                     /* 1 or 2 potential-coefficient sets */
 } olcao_system;
 
 typedef struct {
@@ -108,9 +111,15 @@ olcao_status olcao_get_setup(const olcao_ctx *ctx, double *cellDims,
                              double *cellSizes, double *kCart);
 
 /* Two-centre integral sweeps.  op: 0 overlap (gaussOverlapOL,
- * integrals.F90:151), 1 kinetic (gaussOverlapKE, :656).  The result stays on
- * the device (S(k) for op 0) and is copied to vv/cv/cc when non-NULL
- * (valeVale, coreVale(coreDim,valeDim,nk), coreCore(coreDim,coreDim,nk)). */
+ * integrals.F90:151), 1 kinetic (gaussOverlapKE, :656), 2 mass velocity
+ * (gaussOverlapMV, :1091; needs rel = 1; stored negated, as the term enters
+ * H).  The result stays on the device (S(k) for op 0) and is copied to
+ * vv/cv/cc when non-NULL.  vv is valeVale as the reference writes it: when
+ * coreDim > 0 it is orthogonalised against the core (orthoOL / ortho,
+ * integrals.F90:3608-3910) with components below 1e-10 zeroed
+ * (intgOrtho.F90:142-190); cv and cc are the raw coreVale(coreDim,valeDim,nk)
+ * and coreCore(coreDim,coreDim,nk).  Operator sweeps run the overlap sweep
+ * first when coreDim > 0 and it has not run. */
 olcao_status olcao_gauss_overlap(olcao_ctx *ctx, int op, double *vv,
                                  double *cv, double *cc);
 
@@ -126,38 +135,46 @@ olcao_status olcao_elec_pot_gauss_overlap(olcao_ctx *ctx, int term, double *vv,
                                           double *cv, double *cc);
 
 /* Fused Hamiltonian sweep with the potential coefficients folded in
- * (gaussOverlapHamPSCF, integrals.F90:2860): KE + sum_j c_j V_j - sum Z N.
- * potCoeffs: [potDim]. */
+ * (gaussOverlapHamPSCF, integrals.F90:2860): KE (- MV when rel = 1)
+ * + sum_j c_j V_j - sum Z N, orthogonalised when coreDim > 0
+ * (orthoHamPSCF :3913).  potCoeffs: [spin][potDim]; vv, cv, cc hold one
+ * block per spin, spin-major. */
 olcao_status olcao_gauss_overlap_ham_pscf(olcao_ctx *ctx, const double *potCoeffs,
                                           double *vv, double *cv, double *cc);
 
 /* Generalised eigensolve of the device-resident H(k), S(k) from the two
- * calls above, for all k (secularEqnPSCF, secularEqn.F90:331, with
- * solveZHEGV interfaces.F90:73 / solveDSYGV :152).  evals: [nk][numStates]
- * ascending; evecs (may be NULL): [nk][numStates][valeDim] complex
- * (column s of eigenvector k at 2*(r + s*n + k*n*numStates)), real at Gamma. */
+ * calls above, for all k and spins (secularEqnPSCF, secularEqn.F90:331, with
+ * solveZHEGV interfaces.F90:73 / solveDSYGV :152).  evals:
+ * [spin][nk][numStates] ascending; evecs (may be NULL):
+ * [spin][nk][numStates][valeDim] complex (column s of eigenvector k at
+ * 2*(r + s*n + k*n*numStates) within a spin), real at Gamma. */
 olcao_status olcao_secular_eqn_pscf(olcao_ctx *ctx, int numStates, double *evals,
                                     double *evecs);
 
-/* Mode A setup: build and keep S, KE, NP and every V_j on the device
- * (setupSCF, OLCAO.F90:248-279). */
+/* Mode A setup: build and keep S, KE, NP, MV (rel = 1) and every V_j on
+ * the device, each orthogonalised when coreDim > 0 (setupSCF,
+ * OLCAO.F90:248-279). */
 olcao_status olcao_setup_scf(olcao_ctx *ctx);
 
-/* Mode A iteration: H = NP + KE + sum_j c_j V_j (a coefficient of exactly 0
- * adds its term unscaled, matrixSubs.F90:369-373), then the solve for all k
- * (secularEqnSCF, secularEqn.F90:37). */
+/* Mode A iteration: H = NP + KE (+ MV) + sum_j c_j V_j per spin (MV and a
+ * coefficient of exactly 0 add their term unscaled, matrixSubs.F90:369-373),
+ * then the solve for all k (secularEqnSCF, secularEqn.F90:37).  potCoeffs:
+ * [spin][potDim]; evals, evecs as olcao_secular_eqn_pscf. */
 olcao_status olcao_secular_eqn_scf(olcao_ctx *ctx, const double *potCoeffs,
                                    int numStates, double *evals, double *evecs);
 
 /* Mode B iteration, the unit the benchmark times: overlap sweep, fused
- * Hamiltonian sweep and the solve for all k (intgPSCF + bandPSCF,
- * OLCAO.F90:533-770).  evals: [nk][numStates]. */
+ * Hamiltonian sweep per spin and the solve for all k (intgPSCF + bandPSCF,
+ * OLCAO.F90:533-770).  potCoeffs: [spin][potDim]; evals:
+ * [spin][nk][numStates]. */
 olcao_status olcao_scf_iteration(olcao_ctx *ctx, const double *potCoeffs,
                                  int numStates, double *evals);
 
 /* Stage times (ms, HIP events) of the last olcao_scf_iteration:
- * t[0] overlap sweep, t[1] Hamiltonian sweep, t[2] contraction + Bloch
- * (both operators), t[3] solve, t[4] total; t[5] three-centre kernel alone. */
+ * t[0] overlap sweep (incl. its contraction and Bloch sum), t[1] Hamiltonian
+ * primitive blocks, t[2] Hamiltonian contraction + Bloch sum (+ ortho),
+ * t[3] solve, t[4] total; t[5] three-centre kernels alone (all summed over
+ * spins). */
 olcao_status olcao_get_timings(const olcao_ctx *ctx, double t[8]);
 
 /* Work counters of the last three-centre sweep: counts[2*(4*la+lb)] three-
@@ -175,7 +192,8 @@ olcao_status olcao_solve_dsygv(int n, int nbatch, double *A, double *B, double *
 /* Primitive integral blocks on the device, for parity tests of the kernels'
  * device functions against the reference subroutines overlap2CIntg
  * (gaussIntegrals.f90:26, op 0), kinetic2CIntg (:4422, op 1),
- * electron3CIntg (:11624, op 2), nuclear3CIntg (:16223, op 3).
+ * electron3CIntg (:11624, op 2), nuclear3CIntg (:16223, op 3),
+ * massvel2CIntg (:39070, op 4).
  * a: [n][3] exponents, A/B/C: [n][3], sw: [n] l1l2switch, sh: [n][16][16]
  * column-major blocks (sh(i,j) at i + 16 j); entries outside the class block
  * are zero. */

@@ -105,15 +105,17 @@ def _pot_sites_from_atoms(pos, atom_type, type_to_pot):
 
 
 def _assemble(name, cell, pos, atom_type, types, pot_types, type_to_pot, kpts,
-              kw, rng, gamma=False, cutoff=1e-16):
+              kw, rng, gamma=False, cutoff=1e-16, rel=0, spin=1):
     ppos, ptype = _pot_sites_from_atoms(pos, atom_type, type_to_pot)
     pot_dim = sum(len(p.alphas) for p in pot_types)
     coeffs = rng.uniform(-1.0, 1.0, size=pot_dim)
+    if spin == 2:   # a second, spin-split coefficient set
+        coeffs = np.stack([coeffs, coeffs + rng.uniform(-0.1, 0.1, size=pot_dim)])
     return SystemInput(name=name, cell=cell, atom_pos=pos,
                        atom_type=np.asarray(atom_type, np.int32), types=types,
                        pot_types=pot_types, pot_pos=ppos, pot_type=ptype,
                        kpoints=kpts, kweights=kw, pot_coeffs=coeffs,
-                       basis_cutoff=cutoff, gamma=gamma)
+                       basis_cutoff=cutoff, gamma=gamma, rel=rel, spin=spin)
 
 
 def diamond_c8(seed=1, kmesh=(4, 4, 4), n_alpha=20, n_pot=16):
@@ -182,7 +184,8 @@ def quartz(seed=3, kmesh=(8, 8, 8), n_alpha_si=24, n_alpha_o=20, n_pot=16):
 
 
 def small_test(seed=7, kmesh=(2, 2, 1), n_alpha=8, lmax=1, hexa=False,
-               two_types=False, core=False, gamma=False, a_ang=3.2, n_pot=5):
+               two_types=False, core=False, gamma=False, a_ang=3.2, n_pot=5, rel=0,
+               spin=1):
     """Few-second oracle workloads for the parity tests."""
     rng = np.random.default_rng(seed)
     if hexa:
@@ -212,7 +215,7 @@ def small_test(seed=7, kmesh=(2, 2, 1), n_alpha=8, lmax=1, hexa=False,
     if gamma:
         k, w = np.zeros((1, 3)), np.array([2.0])
     return _assemble("small_test", cell, pos, atom_types, types, pots, t2p, k, w, rng,
-                     gamma=gamma)
+                     gamma=gamma, rel=rel, spin=spin)
 
 
 CONFIGS = {"diamond_c8": diamond_c8, "si64": si64, "quartz": quartz}

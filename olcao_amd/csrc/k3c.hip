@@ -135,48 +135,76 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
 
 // W -> Cartesian block -> solid harmonics, one thread per primitive pair:
 //   <a|O|b> = K_AB sum_tuv Ex^{ax bx}_t Ey^{ay by}_u Ez^{az bz}_v W_tuv,
-// then the reference's Cartesian -> solid-harmonic map, column by column of
-// the ket so that only one Cartesian column is live at a time.
+// then the reference's Cartesian -> solid-harmonic map, one ket harmonic
+// (column) at a time so that only one Cartesian column is live.  All loops
+// are compile-time (static_for), so W and the E tables stay in registers.
+// Blocks of up to kFinishStage doubles are staged in LDS and written (or
+// accumulated) block by block with the whole wave, so that the stores to
+// the primitive-pair store are contiguous runs instead of one 8-byte word
+// per lane per element.
+constexpr int kFinishStage = 81;
+
 template <int LA, int LB>
 __global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const double *W, int nrec,
                                                     int accumulate, double *blocks)
 {
     constexpr int L = LA + LB, NH = nherm(L);
     constexpr int NA = ncart(LA), SA = nsh(LA), SB = nsh(LB);
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    if (i >= nrec) return;
-    const PPRec rec = recs[i];
+    constexpr bool STAGE = SA * SB <= kFinishStage;
+    __shared__ double sOut[STAGE ? 64 * SA * SB + 1 : 1];
+    __shared__ long long sOff[64];
+    const int lane = threadIdx.x;
+    const int i = blockIdx.x * 64 + lane;
+    const bool active = i < nrec;
+    const PPRec rec = recs[active ? i : nrec - 1];
     const PairConst q = make_pair(rec.a1, rec.a2, rec.A, rec.B);
     double Wl[NH];
 #pragma unroll
-    for (int e = 0; e < NH; ++e) Wl[e] = W[(long long)e * nrec + i];
+    for (int e = 0; e < NH; ++e) Wl[e] = active ? W[(long long)e * nrec + i] : 0.0;
     double E[3][LA + 1][LB + 1][L + 1];
 #pragma unroll
     for (int x = 0; x < 3; ++x) herm_e<LA, LB>(q.P[x] - q.A[x], q.P[x] - q.B[x], 0.5 / q.zeta, E[x]);
     const double kab = exp(-q.xid2);
     double *o = blocks + rec.off;
-#pragma unroll
-    for (int j = 0; j < SB; ++j) {
+    static_for<SB>([&](auto j_) __attribute__((always_inline)) {
+        constexpr int j = decltype(j_)::value;
         double col[NA];
-#pragma unroll
-        for (int a = 0; a < NA; ++a) col[a] = 0.0;
-#pragma unroll
-        for (int tj = 0; tj < sh_nterm(j); ++tj) {
-            const int b = sh_cart(j, tj);
-            const double wj = sh_weight(j, tj);
-#pragma unroll
-            for (int a = 0; a < NA; ++a)
-                col[a] += wj * herm_contract<LA, LB>(E, Wl, cexp(a, 0), cexp(a, 1), cexp(a, 2),
-                                                     cexp(b, 0), cexp(b, 1), cexp(b, 2));
-        }
+        static_for<NA>([&](auto a_) __attribute__((always_inline)) {
+            constexpr int a = decltype(a_)::value;
+            double v = 0.0;
+            static_for<sh_nterm(j)>([&](auto t_) __attribute__((always_inline)) {
+                constexpr int tj = decltype(t_)::value;
+                constexpr int b = sh_cart(j, tj);
+                v += sh_weight(j, tj) *
+                     herm_contract_ct<LA, LB, cexp(a, 0), cexp(a, 1), cexp(a, 2), cexp(b, 0),
+                                      cexp(b, 1), cexp(b, 2)>(E, Wl);
+            });
+            col[a] = v;
+        });
 #pragma unroll
         for (int ii = 0; ii < SA; ++ii) {
             double s = 0.0;
 #pragma unroll
             for (int ti = 0; ti < sh_nterm(ii); ++ti) s += sh_weight(ii, ti) * col[sh_cart(ii, ti)];
             s *= kab;
-            double *p = o + ii * SB + j;
-            *p = accumulate ? *p + s : s;
+            if (STAGE) {
+                sOut[lane * SA * SB + ii * SB + j] = s;
+            } else if (active) {
+                double *p = o + ii * SB + j;
+                *p = accumulate ? *p + s : s;
+            }
+        }
+    });
+    if (STAGE) {
+        sOff[lane] = active ? rec.off : -1;
+        __syncthreads();
+        const int nr = min(64, nrec - blockIdx.x * 64);
+        for (int r = 0; r < nr; ++r) {
+            double *p = blocks + sOff[r];
+            for (int e = lane; e < SA * SB; e += 64) {
+                const double v = sOut[r * SA * SB + e];
+                p[e] = accumulate ? p[e] + v : v;
+            }
         }
     }
 }

@@ -22,58 +22,149 @@
 namespace olcao {
 
 // ---------------------------------------------------------------------------
-// two-centre blocks
+// two-centre blocks, one lane per primitive pair; blocks of up to 81 doubles
+// are staged in LDS and stored block by block with the whole wave, so the
+// stores are contiguous runs.
+// op OP_MV stores the negated mass-velocity block (the sign with which the
+// term enters H); accumulate adds to the stored block instead of writing it.
 template <int LA, int LB>
-__global__ __launch_bounds__(256) void k_two_center(DevCtx c, const int *list, int n,
-                                                    int op, double *blocks)
+__global__ __launch_bounds__(64) void k_two_center(DevCtx c, const int *list, int n,
+                                                   int op, int accumulate, double *blocks)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int pp = list[t];
+    constexpr int SA = nsh(LA), SB = nsh(LB);
+    constexpr bool STAGE = SA * SB <= 81;
+    __shared__ double sOut[STAGE ? 64 * SA * SB + 1 : 1];
+    __shared__ long long sOff[64];
+    const int lane = threadIdx.x;
+    const int t = blockIdx.x * 64 + lane;
+    const bool active = t < n;
+    const int pp = list[active ? t : n - 1];
     const PPInfo p = load_pp(c, pp);
     const PairConst q = make_pair(c.alpha[p.ti * c.maxA + p.ia],
                                   c.alpha[p.tj * c.maxA + p.ib], p.A, p.B);
     double pc[ncart(LA)][ncart(LB)];
     if (op == OP_KE)
         two_center_cart<LA, LB, true>(q, pc);
+    else if (op == OP_MV)
+        massvel_cart<LA, LB>(q, pc);
     else
         two_center_cart<LA, LB, false>(q, pc);
-    double sh[nsh(LA)][nsh(LB)];
+    double sh[SA][SB];
     cart_to_sh<LA, LB>(pc, sh);
-    double *o = blocks + c.ppOff[pp];
+    const double sg = op == OP_MV ? -1.0 : 1.0;
+    const long long off = c.ppOff[pp];
+    if (STAGE) {
 #pragma unroll
-    for (int i = 0; i < nsh(LA); ++i)
+        for (int i = 0; i < SA; ++i)
 #pragma unroll
-        for (int j = 0; j < nsh(LB); ++j) o[i * nsh(LB) + j] = sh[i][j];
+            for (int j = 0; j < SB; ++j) sOut[lane * SA * SB + i * SB + j] = sg * sh[i][j];
+        sOff[lane] = off;
+        __syncthreads();
+        const int nr = min(64, n - (int)blockIdx.x * 64);
+        for (int r = 0; r < nr; ++r) {
+            double *o = blocks + sOff[r];
+            for (int e = lane; e < SA * SB; e += 64) {
+                const double v = sOut[r * SA * SB + e];
+                o[e] = accumulate ? o[e] + v : v;
+            }
+        }
+    } else if (active) {
+        double *o = blocks + off;
+#pragma unroll
+        for (int i = 0; i < SA; ++i)
+#pragma unroll
+            for (int j = 0; j < SB; ++j)
+                o[i * SB + j] = accumulate ? o[i * SB + j] + sg * sh[i][j] : sg * sh[i][j];
+    }
 }
 
 // ---------------------------------------------------------------------------
 // contraction: X(s1,s2) = sum over the pair-cell's primitive pairs of
 //   C1(a1,s1) * block(lm(s1), lm(s2)) * C2(a2,s2)
-__global__ __launch_bounds__(256) void k_contract(DevCtx c, const double *blocks, double *X)
+// The primitive pairs of a pair-cell are cut into segments of at most
+// kSegPP (an on-site pair-cell has several hundred, a distant one a few), one
+// wavefront per segment, so that no pair-cell serialises the launch.  Lane q
+// holds the record of the segment's q-th primitive pair and the wave walks
+// them in order, reading each record from that lane; each lane sums up to
+// kContractOut outputs at a time.  A single-segment pair-cell writes X
+// directly; otherwise each segment writes a partial block and k_contract_sum
+// adds the partials in segment order (deterministic).
+constexpr int kContractOut = 4;
+
+static __device__ __forceinline__ long long readlane64(long long v, int l)
+{
+    const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+    return ((long long)hi << 32) | (unsigned int)lo;
+}
+
+__global__ __launch_bounds__(256) void k_contract_seg(DevCtx c, const int4 *segs, int nSeg,
+                                                      const double *blocks, double *X,
+                                                      double *partial)
 {
     const int lane = threadIdx.x & 63;
-    const int pc = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (pc >= c.nPc) return;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= nSeg) return;
+    const int4 sg = segs[w];
+    const int pc = sg.x, p0 = sg.y, np = sg.z, part = sg.w;
     const int pr = c.pcPair[pc];
     const int ti = c.atomType[c.pairI[pr]], tj = c.atomType[c.pairJ[pr]];
     const int nt1 = c.nVale[ti] + c.nCore[ti], nt2 = c.nVale[tj] + c.nCore[tj];
-    const int p0 = c.pcPp[pc], p1 = c.pcPp[pc + 1];
     const int ms = c.maxS, ma = c.maxA;
-    double *xo = X + (long long)pc * ms * ms;
-    for (int o = lane; o < nt1 * nt2; o += 64) {
-        const int s1 = o / nt2, s2 = o - s1 * nt2;
-        const int lm1 = c.lmIndex[ti * ms + s1] - 1, lm2 = c.lmIndex[tj * ms + s2] - 1;
-        const double *c1 = c.basis + (long long)(ti * ms + s1) * ma;
-        const double *c2 = c.basis + (long long)(tj * ms + s2) * ma;
-        double sum = 0.0;
-        for (int pp = p0; pp < p1; ++pp) {
-            const int ab = c.ppAB[pp];
-            const int n1 = nsh(pp_la(ab)), n2 = nsh(pp_lb(ab));
-            if (lm1 < n1 && lm2 < n2)
-                sum += c1[pp_a1(ab)] * blocks[c.ppOff[pp] + lm1 * n2 + lm2] * c2[pp_a2(ab)];
+    const int myAB = lane < np ? c.ppAB[p0 + lane] : 0;
+    const long long myOff = lane < np ? c.ppOff[p0 + lane] : 0;
+    double *dst = (part < 0 ? X + (long long)pc * ms * ms : partial + (long long)part * ms * ms);
+    const int nout = nt1 * nt2;
+    for (int g = 0; g < nout; g += 64 * kContractOut) {
+        double acc[kContractOut];
+        int lm1v[kContractOut], lm2v[kContractOut], dv[kContractOut];
+        const double *c1v[kContractOut], *c2v[kContractOut];
+#pragma unroll
+        for (int r = 0; r < kContractOut; ++r) {
+            acc[r] = 0.0;
+            const int o = g + r * 64 + lane;
+            const bool ok = o < nout;
+            const int s1 = ok ? o / nt2 : 0, s2 = ok ? o - s1 * nt2 : 0;
+            lm1v[r] = ok ? c.lmIndex[ti * ms + s1] - 1 : 99;
+            lm2v[r] = ok ? c.lmIndex[tj * ms + s2] - 1 : 99;
+            dv[r] = ok ? s1 * ms + s2 : -1;
+            c1v[r] = c.basis + (long long)(ti * ms + s1) * ma;
+            c2v[r] = c.basis + (long long)(tj * ms + s2) * ma;
         }
-        xo[s1 * ms + s2] = sum;
+        for (int q = 0; q < np; ++q) {
+            const int ab = __builtin_amdgcn_readlane(myAB, q);
+            const long long off = readlane64(myOff, q);
+            const int n1 = nsh(pp_la(ab)), n2 = nsh(pp_lb(ab));
+            const int a1 = pp_a1(ab), a2 = pp_a2(ab);
+#pragma unroll
+            for (int r = 0; r < kContractOut; ++r)
+                if (lm1v[r] < n1 && lm2v[r] < n2)
+                    acc[r] += c1v[r][a1] * blocks[off + lm1v[r] * n2 + lm2v[r]] * c2v[r][a2];
+        }
+#pragma unroll
+        for (int r = 0; r < kContractOut; ++r)
+            if (dv[r] >= 0) dst[dv[r]] = acc[r];
+    }
+}
+
+// X of the multi-segment pair-cells: sum of the partials in segment order.
+__global__ __launch_bounds__(64) void k_contract_sum(DevCtx c, const int4 *multi, int nMulti,
+                                                     const double *partial, double *X)
+{
+    const int m = blockIdx.x;
+    if (m >= nMulti) return;
+    const int4 e = multi[m];   // pc, first partial, segment count
+    const int pc = e.x;
+    const int pr = c.pcPair[pc];
+    const int ti = c.atomType[c.pairI[pr]], tj = c.atomType[c.pairJ[pr]];
+    const int nt1 = c.nVale[ti] + c.nCore[ti], nt2 = c.nVale[tj] + c.nCore[tj];
+    const int ms = c.maxS;
+    for (int o = threadIdx.x; o < nt1 * nt2; o += 64) {
+        const int s1 = o / nt2, s2 = o - s1 * nt2;
+        const long long d = (long long)s1 * ms + s2;
+        double v = 0.0;
+        for (int q = 0; q < e.z; ++q) v += partial[(long long)(e.y + q) * ms * ms + d];
+        X[(long long)pc * ms * ms + d] = v;
     }
 }
 
@@ -160,11 +251,13 @@ __global__ void k_fill(double *p, long long n, double v)
         p[t] = v;
 }
 
-// H = NP + KE + sum_j c_j V_j on the upper triangle of every k matrix;
-// c_j == 0 adds V_j unscaled (readPackedMatrixAccum, matrixSubs.F90:369-373).
+// H = NP + KE (+ MV) + sum_j c_j V_j on the upper triangle of every k
+// matrix, in the reference's order; MV (the negated mass-velocity term,
+// present when rel = 1) and a c_j == 0 are added unscaled
+// (readPackedMatrixAccum, matrixSubs.F90:369-373; secularEqn.F90:147-170).
 __global__ void k_assemble(long long total, int n, const double *NP, const double *KE,
-                           const double *const *V, const double *coef, int nTerms,
-                           double *H, int cplx)
+                           const double *MV, const double *const *V, const double *coef,
+                           int nTerms, double *H, int cplx)
 {
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
@@ -176,6 +269,7 @@ __global__ void k_assemble(long long total, int n, const double *NP, const doubl
             continue;
         }
         double v = NP[t] + KE[t];
+        if (MV) v = v + MV[t];
         for (int j = 0; j < nTerms; ++j) {
             const double cj = coef[j];
             v = v + (cj == 0.0 ? V[j][t] : V[j][t] * cj);
@@ -218,27 +312,27 @@ __global__ void k_extract(const double *A, int n, int nk, int ns, int cplx, doub
 // ---------------------------------------------------------------------------
 // launchers
 template <int LA, int LB>
-static void two_one(const DevCtx &c, const ClassLists &cl, int op, double *blocks,
+static void two_one(const DevCtx &c, const ClassLists &cl, int op, int acc, double *blocks,
                     hipStream_t s)
 {
     const int k = LA * 4 + LB;
     const int n = cl.count[k];
     if (n == 0) return;
-    hipLaunchKernelGGL((k_two_center<LA, LB>), dim3((n + 255) / 256), dim3(256), 0, s, c,
-                       cl.idx[k], n, op, blocks);
+    hipLaunchKernelGGL((k_two_center<LA, LB>), dim3((n + 63) / 64), dim3(64), 0, s, c,
+                       cl.idx[k], n, op, acc, blocks);
 }
 
-void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, double *blocks,
+void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int acc, double *blocks,
                        hipStream_t s)
 {
-    two_one<0, 0>(c, cl, op, blocks, s); two_one<0, 1>(c, cl, op, blocks, s);
-    two_one<0, 2>(c, cl, op, blocks, s); two_one<0, 3>(c, cl, op, blocks, s);
-    two_one<1, 0>(c, cl, op, blocks, s); two_one<1, 1>(c, cl, op, blocks, s);
-    two_one<1, 2>(c, cl, op, blocks, s); two_one<1, 3>(c, cl, op, blocks, s);
-    two_one<2, 0>(c, cl, op, blocks, s); two_one<2, 1>(c, cl, op, blocks, s);
-    two_one<2, 2>(c, cl, op, blocks, s); two_one<2, 3>(c, cl, op, blocks, s);
-    two_one<3, 0>(c, cl, op, blocks, s); two_one<3, 1>(c, cl, op, blocks, s);
-    two_one<3, 2>(c, cl, op, blocks, s); two_one<3, 3>(c, cl, op, blocks, s);
+    two_one<0, 0>(c, cl, op, acc, blocks, s); two_one<0, 1>(c, cl, op, acc, blocks, s);
+    two_one<0, 2>(c, cl, op, acc, blocks, s); two_one<0, 3>(c, cl, op, acc, blocks, s);
+    two_one<1, 0>(c, cl, op, acc, blocks, s); two_one<1, 1>(c, cl, op, acc, blocks, s);
+    two_one<1, 2>(c, cl, op, acc, blocks, s); two_one<1, 3>(c, cl, op, acc, blocks, s);
+    two_one<2, 0>(c, cl, op, acc, blocks, s); two_one<2, 1>(c, cl, op, acc, blocks, s);
+    two_one<2, 2>(c, cl, op, acc, blocks, s); two_one<2, 3>(c, cl, op, acc, blocks, s);
+    two_one<3, 0>(c, cl, op, acc, blocks, s); two_one<3, 1>(c, cl, op, acc, blocks, s);
+    two_one<3, 2>(c, cl, op, acc, blocks, s); two_one<3, 3>(c, cl, op, acc, blocks, s);
 }
 
 void launch_3c_la0(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
@@ -397,10 +491,15 @@ void launch_image_fill(const DevCtx &c, const PPRec *recs, int n, const long lon
                        1, nullptr, nullptr, off, ent, overflow);
 }
 
-void launch_contract(const DevCtx &c, const double *blocks, double *X, hipStream_t s)
+void launch_contract(const DevCtx &c, const ContractPlan &cp, const double *blocks, double *X,
+                     hipStream_t s)
 {
-    if (c.nPc == 0) return;
-    hipLaunchKernelGGL(k_contract, dim3((c.nPc + 3) / 4), dim3(256), 0, s, c, blocks, X);
+    if (cp.nSeg == 0) return;
+    hipLaunchKernelGGL(k_contract_seg, dim3((cp.nSeg + 3) / 4), dim3(256), 0, s, c, cp.segs,
+                       cp.nSeg, blocks, X, cp.partial);
+    if (cp.nMulti > 0)
+        hipLaunchKernelGGL(k_contract_sum, dim3(cp.nMulti), dim3(64), 0, s, c, cp.multi,
+                           cp.nMulti, cp.partial, X);
 }
 
 void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv, double *cv,
@@ -419,13 +518,35 @@ void launch_fill(double *p, long long n, double v, hipStream_t s)
 }
 
 void launch_assemble(long long total, int n, const double *NP, const double *KE,
-                     const double *const *V, const double *coef, int nTerms, double *H,
-                     int cplx, hipStream_t s)
+                     const double *MV, const double *const *V, const double *coef, int nTerms,
+                     double *H, int cplx, hipStream_t s)
 {
     long long blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_assemble, dim3((unsigned)blocks), dim3(256), 0, s, total, n, NP, KE,
-                       V, coef, nTerms, H, cplx);
+                       MV, V, coef, nTerms, H, cplx);
+}
+
+// After orthogonalisation: keep the upper triangle, zero the rest, and zero
+// components below 1e-10 in magnitude as makeValeVale packs them
+// (intgOrtho.F90:142-190).
+__global__ void k_ortho_finish(double *M, int n, int nb, int cplx)
+{
+    const long long per = (long long)n * n;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < per * nb * cplx;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long e = (t / cplx) % per;
+        const int r = (int)(e % n), col = (int)(e / n);
+        if (r > col || fabs(M[t]) < 1.0e-10) M[t] = 0.0;
+    }
+}
+
+void launch_ortho_finish(double *M, int n, int nb, int cplx, hipStream_t s)
+{
+    long long total = (long long)n * n * nb * cplx;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_ortho_finish, dim3((unsigned)blocks), dim3(256), 0, s, M, n, nb, cplx);
 }
 
 void launch_herm_fill(double *A, int n, int nb, int cplx, hipStream_t s)

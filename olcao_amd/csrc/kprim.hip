@@ -16,6 +16,8 @@ __device__ void prim_one(int op, const double *a, const double *A, const double 
         two_center_cart<LA, LB, false>(q, pc);
     } else if (op == 1) {
         two_center_cart<LA, LB, true>(q, pc);
+    } else if (op == 4) {
+        massvel_cart<LA, LB>(q, pc);
     } else {
         constexpr int L = LA + LB;
         double W[nherm(L)];

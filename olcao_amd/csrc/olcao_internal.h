@@ -14,7 +14,7 @@ constexpr int kMaxPotSites = 1024;        // sites staged in LDS (28 B each)
 // Three-centre sweep kinds.
 enum ThreeKind { K3_HAM = 0, K3_EP = 1, K3_NP = 2 };
 // Two-centre operators.
-enum TwoOp { OP_OL = 0, OP_KE = 1 };
+enum TwoOp { OP_OL = 0, OP_KE = 1, OP_MV = 2 };
 
 // Device view of the system and the screened work lists.  Passed by value
 // to every kernel; all pointers are device memory.
@@ -67,7 +67,7 @@ struct DevCtx {
     long long nPp;
     const int *ppPc;
     const int *ppAB;            // a1 | a2 << 8 | la << 16 | lb << 20
-    const long long *ppOff;     // block offset (doubles)
+    const long long *ppOff;     // [nPp + 1] block offsets (doubles), store end appended
     // optional work counters [16 classes][2]: three-Gaussian overlap and
     // nuclear evaluations of the last three-centre sweep
     unsigned long long *counters;
@@ -126,7 +126,7 @@ struct ClassLists {
     int count[16];
 };
 
-void launch_two_center(const DevCtx &c, const ClassLists &cl, int op,
+void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int accumulate,
                        double *blocks, hipStream_t s);
 void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind, int termType,
                          int termIdx, const double *coef, int accumulate, double *blocks,
@@ -138,14 +138,26 @@ void launch_image_count(const DevCtx &c, const PPRec *recs, int n, int *nEnt, in
                         hipStream_t s);
 void launch_image_fill(const DevCtx &c, const PPRec *recs, int n, const long long *off,
                        unsigned long long *ent, int *overflow, hipStream_t s);
-void launch_contract(const DevCtx &c, const double *blocks, double *X,
+// Contraction work: segments {pair-cell, first primitive pair, count,
+// partial index or -1} of at most kSegPP primitive pairs, and the pair-cells
+// that have several segments {pair-cell, first partial, segment count}.
+constexpr int kSegPP = 32;
+struct ContractPlan {
+    const int4 *segs;
+    int nSeg;
+    const int4 *multi;
+    int nMulti;
+    double *partial;            // [parts][maxS][maxS]
+};
+void launch_contract(const DevCtx &c, const ContractPlan &cp, const double *blocks, double *X,
                      hipStream_t s);
 void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv,
                           double *cv, double *cc, int gamma, hipStream_t s);
 void launch_fill(double *p, long long n, double v, hipStream_t s);
 void launch_assemble(long long n2nk, int n, const double *NP, const double *KE,
-                     const double *const *V, const double *coef, int nTerms,
+                     const double *MV, const double *const *V, const double *coef, int nTerms,
                      double *H, int cplx, hipStream_t s);
+void launch_ortho_finish(double *M, int n, int nb, int cplx, hipStream_t s);
 // classIdx[k]: device list of input indices of class k = 4 la + lb.
 void launch_prim_batch(int op, int n, const double *a, const double *A,
                        const double *B, const double *C, const int *const *classIdx,

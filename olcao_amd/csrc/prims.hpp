@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <utility>
+
 #define OHD __host__ __device__ __forceinline__
 
 namespace olcao {
@@ -314,6 +316,97 @@ OHD void two_center_cart(const PairConst &q, double (&pc)[ncart(LA)][ncart(LB)])
         }
 }
 
+// Mass-velocity block (massvel2CIntg, gaussIntegrals.f90:39070):
+//   (alpha_fs^2 / 8) <a| nabla^4 |b>,  alpha_fs = fineStructure * 1e-3,
+// with nabla^4 = sum_i d_i^4 + 2 sum_{i<j} d_i^2 d_j^2 applied to the more
+// diffuse primitive (<nabla^4 a|b> = <a|nabla^4 b>): on the tight one its
+// terms are ~(a_tight/a_diffuse)^2 larger than the result and cancel.
+// Along one axis, with g_j = (x - B)^j exp(-b (x - B)^2),
+//   d^2 g_j = j(j-1) g_{j-2} - 2 b (2j+1) g_j + 4 b^2 g_{j+2},
+// so <i|d^2|j> and <i|d^4|j> are combinations of the 1-D overlap table
+// E(i, j') for j' <= j + 4.
+constexpr double kFineStructure = 7.2973525693;   // src/constants.f90 (x 1e-3)
+
+template <int N>
+OHD double d2_1d(const double (&Ei)[N], int j, double b)
+{
+    double v = -2.0 * b * (2 * j + 1) * Ei[j] + 4.0 * b * b * Ei[j + 2];
+    if (j >= 2) v += j * (j - 1) * Ei[j >= 2 ? j - 2 : 0];
+    return v;
+}
+
+template <int N>
+OHD double d4_1d(const double (&Ei)[N], int j, double b)
+{
+    double v = -2.0 * b * (2 * j + 1) * d2_1d(Ei, j, b) + 4.0 * b * b * d2_1d(Ei, j + 2, b);
+    if (j >= 2) v += j * (j - 1) * d2_1d(Ei, j >= 2 ? j - 2 : 0, b);
+    return v;
+}
+
+// S, d^2 and d^4 tables [i][j] (bra power i, ket power j) of one axis, the
+// derivatives taken on the ket (KET) or on the bra.
+template <int LA, int LB, bool KET>
+OHD void mv_axis(double XPA, double XPB, double h, double e, double (&S)[LA + 1][LB + 1],
+                 double (&D2)[LA + 1][LB + 1], double (&D4)[LA + 1][LB + 1])
+{
+    if (KET) {
+        double E[LA + 1][LB + 5];
+        os1d<LA, LB + 4>(XPA, XPB, h, E);
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+            for (int j = 0; j <= LB; ++j) {
+                S[i][j] = E[i][j];
+                D2[i][j] = d2_1d(E[i], j, e);
+                D4[i][j] = d4_1d(E[i], j, e);
+            }
+    } else {
+        double E[LB + 1][LA + 5];   // roles swapped: rows ket power, columns bra power
+        os1d<LB, LA + 4>(XPB, XPA, h, E);
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+            for (int j = 0; j <= LB; ++j) {
+                S[i][j] = E[j][i];
+                D2[i][j] = d2_1d(E[j], i, e);
+                D4[i][j] = d4_1d(E[j], i, e);
+            }
+    }
+}
+
+template <int LA, int LB>
+OHD void massvel_cart(const PairConst &q, double (&pc)[ncart(LA)][ncart(LB)])
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB);
+    const double t = kPi / q.zeta;
+    const double fs = kFineStructure * 0.001;
+    const double pref = (fs * fs / 8.0) * (t * sqrt(t) * exp(-q.xid2));
+    const double h = 1.0 / (2.0 * q.zeta);
+    double S[3][LA + 1][LB + 1], D2[3][LA + 1][LB + 1], D4[3][LA + 1][LB + 1];
+    if (q.a1 < q.a2) {
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+            mv_axis<LA, LB, false>(q.P[x] - q.A[x], q.P[x] - q.B[x], h, q.a1, S[x], D2[x], D4[x]);
+    } else {
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+            mv_axis<LA, LB, true>(q.P[x] - q.A[x], q.P[x] - q.B[x], h, q.a2, S[x], D2[x], D4[x]);
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int bb = 0; bb < NB; ++bb) {
+            const int ax = cexp(a, 0), ay = cexp(a, 1), az = cexp(a, 2);
+            const int bx = cexp(bb, 0), by = cexp(bb, 1), bz = cexp(bb, 2);
+            const double sx = S[0][ax][bx], sy = S[1][ay][by], sz = S[2][az][bz];
+            const double dx = D2[0][ax][bx], dy = D2[1][ay][by], dz = D2[2][az][bz];
+            const double v = D4[0][ax][bx] * sy * sz + sx * D4[1][ay][by] * sz +
+                             sx * sy * D4[2][az][bz] +
+                             2.0 * (dx * dy * sz + dx * sy * dz + sx * dy * dz);
+            pc[a][bb] = pref * v;
+        }
+}
+
 // ---------------------------------------------------------------------
 // Three-centre integrals in McMurchie-Davidson form about the pair centre P.
 //
@@ -495,6 +588,42 @@ OHD double herm_contract(const double (&E)[3][LA + 1][LB + 1][LA + LB + 1], cons
             const double exy = E[0][ax][bx][t] * E[1][ay][by][u];
             for (int v = 0; v <= az + bz; ++v) s += exy * E[2][az][bz][v] * W[herm_index(t, u, v)];
         }
+    return s;
+}
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1, so that
+// every table index inside f is a constant and the tables stay in registers
+// whatever the unroller decides.
+template <class F, int... I>
+OHD void static_for_impl(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+OHD void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// herm_contract with the Cartesian exponents as template constants.
+template <int LA, int LB, int AX, int AY, int AZ, int BX, int BY, int BZ>
+OHD double herm_contract_ct(const double (&E)[3][LA + 1][LB + 1][LA + LB + 1],
+                            const double (&W)[nherm(LA + LB)])
+{
+    double s = 0.0;
+    static_for<AX + BX + 1>([&](auto t_) __attribute__((always_inline)) {
+        constexpr int t = decltype(t_)::value;
+        static_for<AY + BY + 1>([&](auto u_) __attribute__((always_inline)) {
+            constexpr int u = decltype(u_)::value;
+            const double exy = E[0][AX][BX][t] * E[1][AY][BY][u];
+            double z = 0.0;
+            static_for<AZ + BZ + 1>([&](auto v_) __attribute__((always_inline)) {
+                constexpr int v = decltype(v_)::value;
+                z += E[2][AZ][BZ][v] * W[herm_index(t, u, v)];
+            });
+            s += exy * z;
+        });
+    });
     return s;
 }
 

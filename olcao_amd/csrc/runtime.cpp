@@ -111,9 +111,11 @@ struct olcao_ctx {
     std::string err;
     int precision = 0;
     bool gamma = false;
+    int rel = 0;                 // 1: scalar-relativistic (mass-velocity term)
+    int spin = 1;                // 1 or 2 potential-coefficient sets
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[12] = {};
 
     // host-side derived setup
     int nAtoms = 0, nT = 0, maxA = 0, maxS = 0, nPT = 0, maxPA = 0, nPS = 0;
@@ -146,11 +148,12 @@ struct olcao_ctx {
     long long imageEntries = 0;
     double listMs = 0.0;
     DBuf<double> dBlocks, dX, dCoef;
-    DBuf<double> S, H, Aw, Bw, D, E, CV_S, CC_S, CV_H, CC_H;
+    DBuf<double> S, Aw, Bw, D, E, CV_S, CC_S, Tmp;
+    DBuf<double> H[2], CV_H[2], CC_H[2];   // per spin
     DBuf<int> dInfo;
     DBuf<unsigned long long> dCounters;
-    // Mode A term matrices
-    DBuf<double> KEm, NPm;
+    // Mode A term matrices (orthogonalised when coreDim > 0)
+    DBuf<double> KEm, NPm, MVm;
     std::vector<DBuf<double>> Vm;
     DBuf<double *> dVptr;
     bool haveS = false, haveH = false, haveScf = false;
@@ -158,6 +161,9 @@ struct olcao_ctx {
 
     DevCtx dev{};
     ClassLists cls{};
+    ContractPlan cplan{};
+    DBuf<int4> dSegs, dMulti;
+    DBuf<double> dPartial;
 
     size_t matDoubles() const { return (size_t)(gamma ? 1 : 2) * valeDim * valeDim * (gamma ? 1 : nk); }
     size_t cvDoubles() const { return (size_t)(gamma ? 1 : 2) * coreDim * valeDim * (gamma ? 1 : nk); }
@@ -216,6 +222,10 @@ void setup_system(olcao_ctx &c, const olcao_system &s)
     if (!(s.basisCutoff > 0.0 && s.basisCutoff < 1.0))
         fail(OLCAO_ERR_INPUT, "basisCutoff must lie in (0,1)");
     c.gamma = s.gamma != 0;
+    if (s.rel != 0 && s.rel != 1) fail(OLCAO_ERR_INPUT, "rel must be 0 or 1");
+    if (s.spin != 1 && s.spin != 2) fail(OLCAO_ERR_INPUT, "spin must be 1 or 2");
+    c.rel = s.rel;
+    c.spin = s.spin;
     c.nAtoms = s.nAtoms;
     c.nT = s.nTypes;
     c.maxA = s.maxAlphas;
@@ -552,7 +562,13 @@ void upload(olcao_ctx &c)
     c.dPairLv.upload(c.hPairLv, s); c.dPairPc.upload(c.hPairPc, s);
     c.dPcPair.upload(c.hPcPair, s); c.dPcCell.upload(c.hPcCell, s);
     c.dPcB.upload(c.hPcB, s); c.dPcS2.upload(c.hPcS2, s); c.dPcPp.upload(c.hPcPp, s);
-    c.dPpPc.upload(c.hPpPc, s); c.dPpAB.upload(c.hPpAB, s); c.dPpOff.upload(c.hPpOff, s);
+    c.dPpPc.upload(c.hPpPc, s); c.dPpAB.upload(c.hPpAB, s);
+    {   // block offsets with the store's end appended (ppOff[nPp] = total)
+        std::vector<long long> off(c.hPpOff);
+        off.push_back(c.blockDoubles);
+        c.dPpOff.upload(off, s);
+        HIPCHK(hipStreamSynchronize(s));
+    }
     for (int k = 0; k < 16; ++k) {
         c.dClass[k].upload(c.classIdx[k], s);
         c.cls.idx[k] = c.dClass[k].p;
@@ -574,7 +590,33 @@ void upload(olcao_ctx &c)
     for (int k = 0; k < 16; ++k) c.cls.W[k] = c.dW.p + wOff[k];
     c.dBlocks.alloc(std::max<long long>(c.blockDoubles, 1));
     c.dX.alloc(std::max<size_t>(c.hPcPair.size() * (size_t)c.maxS * c.maxS, 1));
-    c.dCoef.alloc(std::max(c.potDim, 1));
+    {   // contraction segments (at most kSegPP primitive pairs each)
+        std::vector<int4> segs, multi;
+        int parts = 0;
+        for (size_t pc = 0; pc < c.hPcPair.size(); ++pc) {
+            const int p0 = c.hPcPp[pc], np = c.hPcPp[pc + 1] - p0;
+            const int nseg = std::max(1, (np + kSegPP - 1) / kSegPP);
+            if (nseg == 1) {
+                segs.push_back(make_int4((int)pc, p0, np, -1));
+                continue;
+            }
+            multi.push_back(make_int4((int)pc, parts, nseg, 0));
+            for (int q = 0; q < nseg; ++q)
+                segs.push_back(make_int4((int)pc, p0 + q * kSegPP, std::min(kSegPP, np - q * kSegPP),
+                                         parts + q));
+            parts += nseg;
+        }
+        c.dSegs.upload(segs, s);
+        c.dMulti.upload(multi, s);
+        c.dPartial.alloc(std::max<size_t>((size_t)parts * c.maxS * c.maxS, 1));
+        HIPCHK(hipStreamSynchronize(s));
+        c.cplan.segs = c.dSegs.p;
+        c.cplan.nSeg = (int)segs.size();
+        c.cplan.multi = c.dMulti.p;
+        c.cplan.nMulti = (int)multi.size();
+        c.cplan.partial = c.dPartial.p;
+    }
+    c.dCoef.alloc(std::max(c.potDim * c.spin, 1));
     c.dCounters.alloc(32);
 
     DevCtx &d = c.dev;
@@ -671,18 +713,23 @@ void three_center(olcao_ctx &c, int kind, int tt, int ti, const double *coef, in
 
 // ---------------------------------------------------------------------------
 // One integral sweep into (vv, cv, cc) device matrices.
-enum Sweep { SW_OL, SW_KE, SW_NP, SW_EP, SW_HAM };
+enum Sweep { SW_OL, SW_KE, SW_NP, SW_EP, SW_HAM, SW_MV };
 
-void sweep(olcao_ctx &c, Sweep kind, int term, double *vv, double *cv, double *cc)
+// Primitive-pair blocks of one sweep.  SW_MV stores the negated
+// mass-velocity blocks, the term as it enters H (gaussOverlapMV accumulates
+// with a minus sign, integrals.F90:1434-1447).  SW_HAM is the fused PSCF
+// block of one spin: kinetic, minus mass-velocity when rel = 1, plus the
+// potential terms with that spin's coefficients and the nuclear attraction
+// (integrals.F90:3211-3490).
+void sweep_blocks(olcao_ctx &c, Sweep kind, int term, int spinIdx)
 {
     hipStream_t s = c.stream;
     const DevCtx &d = c.dev;
     switch (kind) {
-    case SW_OL: launch_two_center(d, c.cls, OP_OL, c.dBlocks.p, s); break;
-    case SW_KE: launch_two_center(d, c.cls, OP_KE, c.dBlocks.p, s); break;
-    case SW_NP:
-        three_center(c, K3_NP, 0, 0, nullptr, 0);
-        break;
+    case SW_OL: launch_two_center(d, c.cls, OP_OL, 0, c.dBlocks.p, s); break;
+    case SW_KE: launch_two_center(d, c.cls, OP_KE, 0, c.dBlocks.p, s); break;
+    case SW_MV: launch_two_center(d, c.cls, OP_MV, 0, c.dBlocks.p, s); break;
+    case SW_NP: three_center(c, K3_NP, 0, 0, nullptr, 0); break;
     case SW_EP: {
         int t = 0;
         while (t + 1 < c.nPT && c.hPotCumul[t + 1] <= term) ++t;
@@ -690,33 +737,127 @@ void sweep(olcao_ctx &c, Sweep kind, int term, double *vv, double *cv, double *c
         break;
     }
     case SW_HAM:
-        launch_two_center(d, c.cls, OP_KE, c.dBlocks.p, s);
-        three_center(c, K3_HAM, 0, 0, c.dCoef.p, 1);
+        launch_two_center(d, c.cls, OP_KE, 0, c.dBlocks.p, s);
+        if (c.rel) launch_two_center(d, c.cls, OP_MV, 1, c.dBlocks.p, s);
+        three_center(c, K3_HAM, 0, 0, c.dCoef.p + (size_t)spinIdx * c.potDim, 1);
         break;
     }
-    launch_contract(d, c.dBlocks.p, c.dX.p, s);
+}
+
+// Contraction, Bloch sum and scatter of the current blocks.
+void contract_scatter(olcao_ctx &c, double *vv, double *cv, double *cc)
+{
+    hipStream_t s = c.stream;
+    launch_contract(c.dev, c.cplan, c.dBlocks.p, c.dX.p, s);
     launch_fill(vv, (long long)c.matDoubles(), 0.0, s);
     if (c.coreDim) {
         launch_fill(cv, (long long)c.cvDoubles(), 0.0, s);
         launch_fill(cc, (long long)c.ccDoubles(), 0.0, s);
     }
-    launch_bloch_scatter(d, c.dX.p, vv, cv, cc, c.gamma ? 1 : 0, s);
+    launch_bloch_scatter(c.dev, c.dX.p, vv, cv, cc, c.gamma ? 1 : 0, s);
     HIPCHK(hipGetLastError());
 }
 
-void copy_out(olcao_ctx &c, const DBuf<double> &src, double *dst, size_t n)
+void sweep(olcao_ctx &c, Sweep kind, int term, int spinIdx, double *vv, double *cv, double *cc)
+{
+    sweep_blocks(c, kind, term, spinIdx);
+    contract_scatter(c, vv, cv, cc);
+}
+
+// Orthogonalisation of valence against core (intgOrtho.F90, integrals.F90
+// orthoOL :3608, ortho :3747, orthoHamPSCF :3913), per k with batched GEMMs:
+//   overlap   S' = S_vv - 2 S_cv^H S_cv + S_cv^H S_cc S_cv
+//   operator  O' = O_vv - S_cv^H O_cv - O_cv^H S_cv + S_cv^H O_cc S_cv
+// then the upper triangle is kept and components below 1e-10 are zeroed, as
+// makeValeVale packs them (intgOrtho.F90:142-190).  Nothing happens when
+// coreDim = 0 (the reference packs without zeroing then).
+void gemm_b(olcao_ctx &c, bool conjA, int m, int n, int k, double alpha, const double *A, int lda,
+            long long sA, const double *B, int ldb, long long sB, double beta, double *Cm,
+            int ldc, long long sC)
+{
+    const int nb = c.gamma ? 1 : c.nk;
+    rocblas_status r;
+    if (c.gamma) {
+        r = rocblas_dgemm_strided_batched(c.blas, conjA ? rocblas_operation_transpose
+                                                        : rocblas_operation_none,
+                                          rocblas_operation_none, m, n, k, &alpha, A, lda, sA, B,
+                                          ldb, sB, &beta, Cm, ldc, sC, nb);
+    } else {
+        const rocblas_double_complex al = {alpha, 0.0}, be = {beta, 0.0};
+        r = rocblas_zgemm_strided_batched(
+            c.blas, conjA ? rocblas_operation_conjugate_transpose : rocblas_operation_none,
+            rocblas_operation_none, m, n, k, &al, (const rocblas_double_complex *)A, lda, sA,
+            (const rocblas_double_complex *)B, ldb, sB, &be, (rocblas_double_complex *)Cm, ldc,
+            sC, nb);
+    }
+    if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "gemm status %d", (int)r);
+}
+
+void ortho(olcao_ctx &c, double *Ovv, const double *Ocv, const double *Occ, bool overlap)
+{
+    if (!c.coreDim) return;
+    const int n = c.valeDim, nc = c.coreDim;
+    const long long svv = (long long)n * n, scv = (long long)nc * n, scc = (long long)nc * nc;
+    const double *Scv = c.CV_S.p;
+    c.Tmp.alloc(c.cvDoubles());
+    if (overlap) {
+        gemm_b(c, true, n, n, nc, -2.0, Scv, nc, scv, Scv, nc, scv, 1.0, Ovv, n, svv);
+    } else {
+        gemm_b(c, true, n, n, nc, -1.0, Scv, nc, scv, Ocv, nc, scv, 1.0, Ovv, n, svv);
+        gemm_b(c, true, n, n, nc, -1.0, Ocv, nc, scv, Scv, nc, scv, 1.0, Ovv, n, svv);
+    }
+    // T = O_cc S_cv, then O_vv += S_cv^H T
+    gemm_b(c, false, nc, n, nc, 1.0, Occ, nc, scc, Scv, nc, scv, 0.0, c.Tmp.p, nc, scv);
+    gemm_b(c, true, n, n, nc, 1.0, Scv, nc, scv, c.Tmp.p, nc, scv, 1.0, Ovv, n, svv);
+    launch_ortho_finish(Ovv, n, c.gamma ? 1 : c.nk, c.gamma ? 1 : 2, c.stream);
+    HIPCHK(hipGetLastError());
+}
+
+// The overlap sweep with S_cv and S_cc kept for the later orthogonalisations.
+void overlap_sweep(olcao_ctx &c)
+{
+    sweep(c, SW_OL, 0, 0, c.S.p, c.CV_S.p, c.CC_S.p);
+    ortho(c, c.S.p, c.CV_S.p, c.CC_S.p, true);
+    c.haveS = true;
+}
+
+// An operator sweep into vv, orthogonalised with its own core blocks (which
+// are left in cv, cc).  The overlap sweep runs first if it has not.
+void operator_sweep(olcao_ctx &c, Sweep kind, int term, int spinIdx, double *vv, double *cv,
+                    double *cc)
+{
+    if (c.coreDim && !c.haveS) overlap_sweep(c);
+    sweep(c, kind, term, spinIdx, vv, cv, cc);
+    ortho(c, vv, cv, cc, false);
+}
+
+void copy_out(olcao_ctx &c, const double *src, double *dst, size_t n)
 {
     if (!dst || n == 0) return;
-    HIPCHK(hipMemcpyAsync(dst, src.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+}
+
+// vv (and, when coreDim > 0, the raw cv, cc) of spin slot q to host buffers
+// laid out spin-major.
+void copy_out_spin(olcao_ctx &c, int q, double *vv, double *cv, double *cc)
+{
+    copy_out(c, c.H[q].p, vv ? vv + (size_t)q * c.matDoubles() : nullptr, c.matDoubles());
+    if (c.coreDim) {
+        copy_out(c, c.CV_H[q].p, cv ? cv + (size_t)q * c.cvDoubles() : nullptr, c.cvDoubles());
+        copy_out(c, c.CC_H[q].p, cc ? cc + (size_t)q * c.ccDoubles() : nullptr, c.ccDoubles());
+    }
 }
 
 void ensure_mats(olcao_ctx &c)
 {
     c.S.alloc(c.matDoubles());
-    c.H.alloc(c.matDoubles());
+    for (int q = 0; q < c.spin; ++q) c.H[q].alloc(c.matDoubles());
     if (c.coreDim) {
         c.CV_S.alloc(c.cvDoubles()); c.CC_S.alloc(c.ccDoubles());
-        c.CV_H.alloc(c.cvDoubles()); c.CC_H.alloc(c.ccDoubles());
+        for (int q = 0; q < c.spin; ++q) {
+            c.CV_H[q].alloc(c.cvDoubles());
+            c.CC_H[q].alloc(c.ccDoubles());
+        }
     }
 }
 
@@ -794,8 +935,6 @@ void solve_device(olcao_ctx &c, int numStates, const double *Hsrc, const double 
     hipStream_t s = c.stream;
     const int n = c.valeDim, nk = c.nk;
     if (numStates <= 0 || numStates > n) fail(OLCAO_ERR_INPUT, "numStates must lie in 1..%d", n);
-    if (c.coreDim) fail(OLCAO_ERR_UNSUPPORTED, "core orthogonalisation is not in this build; "
-                        "use coreDim = 0 systems for the solve");
     c.Aw.alloc(c.matDoubles());
     c.Bw.alloc(c.matDoubles());
     c.D.alloc((size_t)n * nk);
@@ -921,18 +1060,20 @@ olcao_status olcao_gauss_overlap(olcao_ctx *c, int op, double *vv, double *cv, d
 {
     if (!c) return OLCAO_ERR_INPUT;
     return guard(c, [&] {
-        if (op != 0 && op != 1) fail(OLCAO_ERR_INPUT, "op must be 0 (overlap) or 1 (kinetic)");
+        if (op < 0 || op > 2) fail(OLCAO_ERR_INPUT, "op must be 0 (overlap), 1 (kinetic) or 2 (mass velocity)");
+        if (op == 2 && !c->rel) fail(OLCAO_ERR_STATE, "the mass-velocity term needs rel = 1");
         ensure_mats(*c);
         if (op == 0) {
-            sweep(*c, SW_OL, 0, c->S.p, c->CV_S.p, c->CC_S.p);
-            c->haveS = true;
-            copy_out(*c, c->S, vv, c->matDoubles());
-            if (c->coreDim) { copy_out(*c, c->CV_S, cv, c->cvDoubles()); copy_out(*c, c->CC_S, cc, c->ccDoubles()); }
+            overlap_sweep(*c);
+            copy_out(*c, c->S.p, vv, c->matDoubles());
+            if (c->coreDim) {
+                copy_out(*c, c->CV_S.p, cv, c->cvDoubles());
+                copy_out(*c, c->CC_S.p, cc, c->ccDoubles());
+            }
         } else {
-            sweep(*c, SW_KE, 0, c->H.p, c->CV_H.p, c->CC_H.p);
+            operator_sweep(*c, op == 1 ? SW_KE : SW_MV, 0, 0, c->H[0].p, c->CV_H[0].p, c->CC_H[0].p);
             c->haveH = false;
-            copy_out(*c, c->H, vv, c->matDoubles());
-            if (c->coreDim) { copy_out(*c, c->CV_H, cv, c->cvDoubles()); copy_out(*c, c->CC_H, cc, c->ccDoubles()); }
+            copy_out_spin(*c, 0, vv, cv, cc);
         }
         HIPCHK(hipStreamSynchronize(c->stream));
     });
@@ -943,10 +1084,9 @@ olcao_status olcao_gauss_overlap_np(olcao_ctx *c, double *vv, double *cv, double
     if (!c) return OLCAO_ERR_INPUT;
     return guard(c, [&] {
         ensure_mats(*c);
-        sweep(*c, SW_NP, 0, c->H.p, c->CV_H.p, c->CC_H.p);
+        operator_sweep(*c, SW_NP, 0, 0, c->H[0].p, c->CV_H[0].p, c->CC_H[0].p);
         c->haveH = false;
-        copy_out(*c, c->H, vv, c->matDoubles());
-        if (c->coreDim) { copy_out(*c, c->CV_H, cv, c->cvDoubles()); copy_out(*c, c->CC_H, cc, c->ccDoubles()); }
+        copy_out_spin(*c, 0, vv, cv, cc);
         HIPCHK(hipStreamSynchronize(c->stream));
     });
 }
@@ -957,10 +1097,9 @@ olcao_status olcao_elec_pot_gauss_overlap(olcao_ctx *c, int term, double *vv, do
     return guard(c, [&] {
         if (term < 0 || term >= c->potDim) fail(OLCAO_ERR_INPUT, "term out of range");
         ensure_mats(*c);
-        sweep(*c, SW_EP, term, c->H.p, c->CV_H.p, c->CC_H.p);
+        operator_sweep(*c, SW_EP, term, 0, c->H[0].p, c->CV_H[0].p, c->CC_H[0].p);
         c->haveH = false;
-        copy_out(*c, c->H, vv, c->matDoubles());
-        if (c->coreDim) { copy_out(*c, c->CV_H, cv, c->cvDoubles()); copy_out(*c, c->CC_H, cc, c->ccDoubles()); }
+        copy_out_spin(*c, 0, vv, cv, cc);
         HIPCHK(hipStreamSynchronize(c->stream));
     });
 }
@@ -970,13 +1109,26 @@ olcao_status olcao_gauss_overlap_ham_pscf(olcao_ctx *c, const double *coef, doub
     if (!c || !coef) return OLCAO_ERR_INPUT;
     return guard(c, [&] {
         ensure_mats(*c);
-        HIPCHK(hipMemcpyAsync(c->dCoef.p, coef, c->potDim * 8, hipMemcpyHostToDevice, c->stream));
-        sweep(*c, SW_HAM, 0, c->H.p, c->CV_H.p, c->CC_H.p);
+        HIPCHK(hipMemcpyAsync(c->dCoef.p, coef, (size_t)c->potDim * c->spin * 8,
+                              hipMemcpyHostToDevice, c->stream));
+        for (int q = 0; q < c->spin; ++q) {
+            operator_sweep(*c, SW_HAM, 0, q, c->H[q].p, c->CV_H[q].p, c->CC_H[q].p);
+            copy_out_spin(*c, q, vv, cv, cc);
+        }
         c->haveH = true;
-        copy_out(*c, c->H, vv, c->matDoubles());
-        if (c->coreDim) { copy_out(*c, c->CV_H, cv, c->cvDoubles()); copy_out(*c, c->CC_H, cc, c->ccDoubles()); }
         HIPCHK(hipStreamSynchronize(c->stream));
     });
+}
+
+// Solve every spin's H against S: evals [spin][nk][numStates], evecs
+// [spin][nk][numStates][valeDim].
+static void solve_spins(olcao_ctx *c, int numStates, double *evals, double *evecs)
+{
+    const size_t ev = (size_t)c->nk * numStates;
+    const size_t vec = ev * c->valeDim * (c->gamma ? 1 : 2);
+    for (int q = 0; q < c->spin; ++q)
+        solve_device(*c, numStates, c->H[q].p, c->S.p, evals ? evals + q * ev : nullptr,
+                     evecs ? evecs + q * vec : nullptr);
 }
 
 olcao_status olcao_secular_eqn_pscf(olcao_ctx *c, int numStates, double *evals, double *evecs)
@@ -985,7 +1137,7 @@ olcao_status olcao_secular_eqn_pscf(olcao_ctx *c, int numStates, double *evals, 
     return guard(c, [&] {
         if (!c->haveS || !c->haveH)
             fail(OLCAO_ERR_STATE, "build S (olcao_gauss_overlap op 0) and H (olcao_gauss_overlap_ham_pscf) first");
-        solve_device(*c, numStates, c->H.p, c->S.p, evals, evecs);
+        solve_spins(c, numStates, evals, evecs);
     });
 }
 
@@ -993,26 +1145,30 @@ olcao_status olcao_setup_scf(olcao_ctx *c)
 {
     if (!c) return OLCAO_ERR_INPUT;
     return guard(c, [&] {
-        if (c->coreDim) fail(OLCAO_ERR_UNSUPPORTED, "core orthogonalisation is not in this build");
         ensure_mats(*c);
         const size_t m = c->matDoubles();
-        sweep(*c, SW_OL, 0, c->S.p, c->CV_S.p, c->CC_S.p);
-        c->haveS = true;
+        overlap_sweep(*c);
+        double *cv = c->CV_H[0].p, *cc = c->CC_H[0].p;   // scratch for the core blocks
         c->KEm.alloc(m);
-        sweep(*c, SW_KE, 0, c->KEm.p, nullptr, nullptr);
+        operator_sweep(*c, SW_KE, 0, 0, c->KEm.p, cv, cc);
         c->NPm.alloc(m);
-        sweep(*c, SW_NP, 0, c->NPm.p, nullptr, nullptr);
+        operator_sweep(*c, SW_NP, 0, 0, c->NPm.p, cv, cc);
+        if (c->rel) {
+            c->MVm.alloc(m);
+            operator_sweep(*c, SW_MV, 0, 0, c->MVm.p, cv, cc);
+        }
         c->Vm.resize(c->potDim);
         std::vector<double *> ptrs(c->potDim);
         for (int j = 0; j < c->potDim; ++j) {
             c->Vm[j].alloc(m);
-            sweep(*c, SW_EP, j, c->Vm[j].p, nullptr, nullptr);
+            operator_sweep(*c, SW_EP, j, 0, c->Vm[j].p, cv, cc);
             ptrs[j] = c->Vm[j].p;
         }
         c->dVptr.alloc(std::max(c->potDim, 1));
         HIPCHK(hipMemcpyAsync(c->dVptr.p, ptrs.data(), ptrs.size() * sizeof(double *),
                               hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        c->haveH = false;
         c->haveScf = true;
     });
 }
@@ -1023,12 +1179,16 @@ olcao_status olcao_secular_eqn_scf(olcao_ctx *c, const double *coef, int numStat
     return guard(c, [&] {
         if (!c->haveScf) fail(OLCAO_ERR_STATE, "call olcao_setup_scf first");
         ensure_mats(*c);
-        HIPCHK(hipMemcpyAsync(c->dCoef.p, coef, c->potDim * 8, hipMemcpyHostToDevice, c->stream));
-        launch_assemble((long long)c->matDoubles(), c->valeDim, c->NPm.p, c->KEm.p, c->dVptr.p,
-                        c->dCoef.p, c->potDim, c->H.p, c->gamma ? 1 : 2, c->stream);
+        HIPCHK(hipMemcpyAsync(c->dCoef.p, coef, (size_t)c->potDim * c->spin * 8,
+                              hipMemcpyHostToDevice, c->stream));
+        for (int q = 0; q < c->spin; ++q)
+            launch_assemble((long long)c->matDoubles(), c->valeDim, c->NPm.p, c->KEm.p,
+                            c->rel ? c->MVm.p : nullptr, c->dVptr.p,
+                            c->dCoef.p + (size_t)q * c->potDim, c->potDim, c->H[q].p,
+                            c->gamma ? 1 : 2, c->stream);
         HIPCHK(hipGetLastError());
         c->haveH = true;
-        solve_device(*c, numStates, c->H.p, c->S.p, evals, evecs);
+        solve_spins(c, numStates, evals, evecs);
     });
 }
 
@@ -1038,38 +1198,47 @@ olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates
     return guard(c, [&] {
         ensure_mats(*c);
         hipStream_t s = c->stream;
-        HIPCHK(hipMemcpyAsync(c->dCoef.p, coef, c->potDim * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->dCoef.p, coef, (size_t)c->potDim * c->spin * 8,
+                              hipMemcpyHostToDevice, s));
         HIPCHK(hipEventRecord(c->ev[0], s));
-        sweep(*c, SW_OL, 0, c->S.p, c->CV_S.p, c->CC_S.p);
+        overlap_sweep(*c);
         HIPCHK(hipEventRecord(c->ev[1], s));
-        launch_two_center(c->dev, c->cls, OP_KE, c->dBlocks.p, s);
-        launch_fill(reinterpret_cast<double *>(c->dCounters.p), 32, 0.0, s);
-        HIPCHK(hipEventRecord(c->ev[2], s));
-        launch_three_center(c->dev, c->cls, K3_HAM, 0, 0, c->dCoef.p, 1,
-                            c->dBlocks.p, s);
-        HIPCHK(hipEventRecord(c->ev[3], s));
-        launch_contract(c->dev, c->dBlocks.p, c->dX.p, s);
-        launch_fill(c->H.p, (long long)c->matDoubles(), 0.0, s);
-        if (c->coreDim) {
-            launch_fill(c->CV_H.p, (long long)c->cvDoubles(), 0.0, s);
-            launch_fill(c->CC_H.p, (long long)c->ccDoubles(), 0.0, s);
+        // events: 0 start, 1 overlap done; per spin q: 2+3q before the
+        // three-centre sweep, 3+3q after it, 4+3q after contraction, Bloch
+        // sum and orthogonalisation; 8 after the solve
+        for (int q = 0; q < c->spin; ++q) {
+            launch_two_center(c->dev, c->cls, OP_KE, 0, c->dBlocks.p, s);
+            if (c->rel) launch_two_center(c->dev, c->cls, OP_MV, 1, c->dBlocks.p, s);
+            launch_fill(reinterpret_cast<double *>(c->dCounters.p), 32, 0.0, s);
+            HIPCHK(hipEventRecord(c->ev[2 + 3 * q], s));
+            launch_three_center(c->dev, c->cls, K3_HAM, 0, 0, c->dCoef.p + (size_t)q * c->potDim,
+                                1, c->dBlocks.p, s);
+            HIPCHK(hipEventRecord(c->ev[3 + 3 * q], s));
+            contract_scatter(*c, c->H[q].p, c->CV_H[q].p, c->CC_H[q].p);
+            ortho(*c, c->H[q].p, c->CV_H[q].p, c->CC_H[q].p, false);
+            HIPCHK(hipEventRecord(c->ev[4 + 3 * q], s));
         }
-        launch_bloch_scatter(c->dev, c->dX.p, c->H.p, c->CV_H.p, c->CC_H.p, c->gamma ? 1 : 0, s);
-        HIPCHK(hipEventRecord(c->ev[4], s));
-        c->haveS = c->haveH = true;
-        solve_device(*c, numStates, c->H.p, c->S.p, evals, nullptr);
-        HIPCHK(hipEventRecord(c->ev[5], s));
-        HIPCHK(hipEventSynchronize(c->ev[5]));
-        float ms[5];
-        for (int q = 0; q < 5; ++q) HIPCHK(hipEventElapsedTime(&ms[q], c->ev[q], c->ev[q + 1]));
-        c->timings[0] = ms[0];
-        c->timings[1] = ms[1] + ms[2];
-        c->timings[2] = ms[3];
-        c->timings[3] = ms[4];
-        float tot;
-        HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[5]));
-        c->timings[4] = tot;
-        c->timings[5] = ms[2];
+        c->haveH = true;
+        solve_spins(c, numStates, evals, nullptr);
+        HIPCHK(hipEventRecord(c->ev[8], s));
+        HIPCHK(hipEventSynchronize(c->ev[8]));
+        auto el = [&](int a, int b) {
+            float v = 0.f;
+            HIPCHK(hipEventElapsedTime(&v, c->ev[a], c->ev[b]));
+            return (double)v;
+        };
+        double ham = 0.0, t3c = 0.0, cb = 0.0;
+        for (int q = 0; q < c->spin; ++q) {
+            ham += el(q == 0 ? 1 : 1 + 3 * q, 3 + 3 * q);
+            t3c += el(2 + 3 * q, 3 + 3 * q);
+            cb += el(3 + 3 * q, 4 + 3 * q);
+        }
+        c->timings[0] = el(0, 1);
+        c->timings[1] = ham;
+        c->timings[2] = cb;
+        c->timings[3] = el(1 + 3 * c->spin, 8);
+        c->timings[4] = el(0, 8);
+        c->timings[5] = t3c;
     });
 }
 
@@ -1136,7 +1305,7 @@ olcao_status olcao_prim_batch(int op, int n, const double *a, const double *A, c
                               const double *C, const int *sw, double *sh)
 {
     return guard(nullptr, [&] {
-        if (op < 0 || op > 3 || n < 0) fail(OLCAO_ERR_INPUT, "bad op or n");
+        if (op < 0 || op > 4 || n < 0) fail(OLCAO_ERR_INPUT, "bad op or n");
         if (n == 0) return;
         for (int i = 0; i < n; ++i) {
             int lo = sw[i] & 15, hi = (sw[i] >> 4) & 15;

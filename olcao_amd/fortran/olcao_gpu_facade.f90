@@ -21,7 +21,7 @@ module O_GPUFacade
    integer, parameter :: double = kind(0.0d0)
 
    public :: gpuInit, gpuFinalize, gpuDims
-   public :: gaussOverlapOL, gaussOverlapKE, gaussOverlapNP, elecPotGaussOverlap
+   public :: gaussOverlapOL, gaussOverlapKE, gaussOverlapMV, gaussOverlapNP, elecPotGaussOverlap
    public :: gaussOverlapHamPSCF, secularEqnPSCF, setupSCF, secularEqnSCF
    public :: solveZHEGV, solveDSYGV
 
@@ -41,6 +41,8 @@ module O_GPUFacade
       type(c_ptr) :: kpoints
       real(c_double) :: basisCutoff
       integer(c_int) :: gamma
+      integer(c_int) :: rel
+      integer(c_int) :: spin
    end type olcao_system_t
 
    type, bind(C) :: olcao_options_t
@@ -142,7 +144,7 @@ contains
          & typeElement, typeNumValeRadial, typeNumCoreRadial, typeValeL, typeCoreL, &
          & typeValeCoef, typeCoreCoef, potAlphas, potNumAlphas, potZ, potNucAlpha, &
          & potElement, potPos, potSiteType, realVectors, kPointsFract, basisCutoff, &
-         & gammaBuild)
+         & gammaBuild, rel, spin)
       real(double), intent(in), target :: atomPos(:,:)          ! (3, nAtoms)
       integer, intent(in) :: atomType(:)
       real(double), intent(in), target :: typeAlphas(:,:)       ! (maxAlphas, nTypes)
@@ -158,6 +160,8 @@ contains
       real(double), intent(in), target :: kPointsFract(:,:)     ! (3, nk)
       real(double), intent(in) :: basisCutoff
       integer, intent(in) :: gammaBuild
+      integer, intent(in), optional :: rel    ! O_Potential rel (default 0)
+      integer, intent(in), optional :: spin   ! O_Potential spin (default 1)
       integer(c_int), allocatable, target :: at0(:), pt0(:)
       type(olcao_system_t) :: s
       type(olcao_options_t) :: o
@@ -198,6 +202,10 @@ contains
       s%kpoints = c_loc(kPointsFract)
       s%basisCutoff = basisCutoff
       s%gamma = gammaBuild
+      s%rel = 0
+      s%spin = 1
+      if (present(rel)) s%rel = rel
+      if (present(spin)) s%spin = spin
       o%precision = 0
       o%reserved = 0
       st = c_ctx_create(s, o, ctx)
@@ -234,6 +242,14 @@ contains
       if (st /= 0) call gpuStop(st, 'gaussOverlapKE')
    end subroutine gaussOverlapKE
 
+   ! integrals.F90:1091 (rel = 1; the negated mass-velocity matrices)
+   subroutine gaussOverlapMV(valeVale, coreVale, coreCore)
+      complex(double), intent(out), target :: valeVale(:,:,:), coreVale(:,:,:), coreCore(:,:,:)
+      integer(c_int) :: st
+      st = c_overlap(ctx, 2_c_int, c_loc(valeVale), opt_loc(coreVale), opt_loc(coreCore))
+      if (st /= 0) call gpuStop(st, 'gaussOverlapMV')
+   end subroutine gaussOverlapMV
+
    ! integrals.F90:1531
    subroutine gaussOverlapNP(valeVale, coreVale, coreCore)
       complex(double), intent(out), target :: valeVale(:,:,:), coreVale(:,:,:), coreCore(:,:,:)
@@ -251,19 +267,20 @@ contains
       if (st /= 0) call gpuStop(st, 'elecPotGaussOverlap')
    end subroutine elecPotGaussOverlap
 
-   ! integrals.F90:2860
+   ! integrals.F90:2860; potCoeffs(potDim, spin); the last dimension of the
+   ! matrices runs over k within spin (numKPoints * spin)
    subroutine gaussOverlapHamPSCF(potCoeffs, valeVale, coreVale, coreCore)
-      real(double), intent(in), target :: potCoeffs(:)
+      real(double), intent(in), target :: potCoeffs(:,:)
       complex(double), intent(out), target :: valeVale(:,:,:), coreVale(:,:,:), coreCore(:,:,:)
       integer(c_int) :: st
       st = c_ham(ctx, c_loc(potCoeffs), c_loc(valeVale), opt_loc(coreVale), opt_loc(coreCore))
       if (st /= 0) call gpuStop(st, 'gaussOverlapHamPSCF')
    end subroutine gaussOverlapHamPSCF
 
-   ! secularEqn.F90:331; energyEigenValues(numStates, numKPoints)
+   ! secularEqn.F90:331; energyEigenValues(numStates, numKPoints, spin)
    subroutine secularEqnPSCF(numStates, energyEigenValues)
       integer, intent(in) :: numStates
-      real(double), intent(out), target :: energyEigenValues(:,:)
+      real(double), intent(out), target :: energyEigenValues(:,:,:)
       integer(c_int) :: st
       st = c_sec_pscf(ctx, int(numStates, c_int), c_loc(energyEigenValues), c_null_ptr)
       if (st /= 0) call gpuStop(st, 'secularEqnPSCF')
@@ -278,9 +295,9 @@ contains
 
    ! secularEqn.F90:37
    subroutine secularEqnSCF(potCoeffs, numStates, energyEigenValues)
-      real(double), intent(in), target :: potCoeffs(:)
+      real(double), intent(in), target :: potCoeffs(:,:)       ! (potDim, spin)
       integer, intent(in) :: numStates
-      real(double), intent(out), target :: energyEigenValues(:,:)
+      real(double), intent(out), target :: energyEigenValues(:,:,:) ! (numStates, nk, spin)
       integer(c_int) :: st
       st = c_sec_scf(ctx, c_loc(potCoeffs), int(numStates, c_int), &
             & c_loc(energyEigenValues), c_null_ptr)

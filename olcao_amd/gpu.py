@@ -53,7 +53,7 @@ class _System(C.Structure):
         ("potNucAlpha", C.c_void_p), ("potElement", C.c_void_p),
         ("nPotSites", C.c_int), ("potPos", C.c_void_p), ("potSiteType", C.c_void_p),
         ("cellVectors", C.c_double * 9), ("nk", C.c_int), ("kpoints", C.c_void_p),
-        ("basisCutoff", C.c_double), ("gamma", C.c_int)]
+        ("basisCutoff", C.c_double), ("gamma", C.c_int), ("rel", C.c_int), ("spin", C.c_int)]
 
 
 class _Options(C.Structure):
@@ -186,6 +186,7 @@ class OlcaoGPU:
         for i, v in enumerate(f["cellVectors"]):
             s.cellVectors[i] = v
         s.nk, s.basisCutoff, s.gamma = f["nk"], f["basisCutoff"], f["gamma"]
+        s.rel, s.spin = f["rel"], f["spin"]
         self._sys = s
         opt = _Options()
         opt.precision = precision
@@ -198,6 +199,7 @@ class OlcaoGPU:
         (self.vale_dim, self.core_dim, self.nk, self.pot_dim, self.n_cells,
          self.n_pair_cells, self.n_prim_pairs, self.n_pairs) = [int(x) for x in dims]
         self.gamma = bool(sysin.gamma)
+        self.spin = int(sysin.spin)
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -225,26 +227,38 @@ class OlcaoGPU:
         return cells, sizes, k
 
     # ---- output buffers ----------------------------------------------------
-    def _bufs(self):
+    def _bufs(self, nspin=1):
         n, nc, nk = self.vale_dim, self.core_dim, self.nk
         cp = 1 if self.gamma else 2
-        vv = np.zeros(cp * n * n * nk)
-        cv = np.zeros(max(cp * nc * n * nk, 1))
-        cc = np.zeros(max(cp * nc * nc * nk, 1))
+        vv = np.zeros(nspin * cp * n * n * nk)
+        cv = np.zeros(max(nspin * cp * nc * n * nk, 1))
+        cc = np.zeros(max(nspin * cp * nc * nc * nk, 1))
         return vv, cv, cc
 
-    def _shape(self, vv, cv, cc):
+    def _shape(self, vv, cv, cc, nspin=1):
         n, nc, nk = self.vale_dim, self.core_dim, self.nk
+        cp = 1 if self.gamma else 2
 
-        def sh(a, r, c):
+        def sh(a, r, c, q):
+            a = a[q * cp * r * c * nk:(q + 1) * cp * r * c * nk]
             if not self.gamma:
-                a = a[:2 * r * c * nk].view(np.complex128).reshape(nk, c, r)
+                a = a.view(np.complex128).reshape(nk, c, r)
                 return a.transpose(0, 2, 1).copy()
-            return a[:r * c].reshape(c, r).T.copy()
-        return sh(vv, n, n), (sh(cv, nc, n) if nc else None), (sh(cc, nc, nc) if nc else None)
+            return a.reshape(c, r).T.copy()
+        out = [(sh(vv, n, n, q), (sh(cv, nc, n, q) if nc else None),
+                (sh(cc, nc, nc, q) if nc else None)) for q in range(nspin)]
+        return out[0] if nspin == 1 else tuple(out)
+
+    def _coeffs(self, pot_coeffs):
+        c = np.ascontiguousarray(pot_coeffs, np.float64).reshape(-1)
+        if c.size != self.spin * self.pot_dim:
+            raise ValueError(f"pot_coeffs must hold spin x potDim = {self.spin} x "
+                             f"{self.pot_dim} values")
+        return c
 
     # ---- integral procedures -----------------------------------------------
     def gaussOverlapOL(self):
+        """valeVale (orthogonalised when coreDim > 0), raw coreVale, coreCore."""
         vv, cv, cc = self._bufs()
         _check(lib().olcao_gauss_overlap(self.ctx, 0, _p(vv), _p(cv), _p(cc)), self.ctx)
         return self._shape(vv, cv, cc)
@@ -252,6 +266,12 @@ class OlcaoGPU:
     def gaussOverlapKE(self):
         vv, cv, cc = self._bufs()
         _check(lib().olcao_gauss_overlap(self.ctx, 1, _p(vv), _p(cv), _p(cc)), self.ctx)
+        return self._shape(vv, cv, cc)
+
+    def gaussOverlapMV(self):
+        """Negated mass-velocity matrices (rel = 1), as gaussOverlapMV stores."""
+        vv, cv, cc = self._bufs()
+        _check(lib().olcao_gauss_overlap(self.ctx, 2, _p(vv), _p(cv), _p(cc)), self.ctx)
         return self._shape(vv, cv, cc)
 
     def gaussOverlapNP(self):
@@ -266,44 +286,57 @@ class OlcaoGPU:
         return self._shape(vv, cv, cc)
 
     def gaussOverlapHamPSCF(self, pot_coeffs):
-        c = np.ascontiguousarray(pot_coeffs, np.float64)
-        vv, cv, cc = self._bufs()
+        """One (vv, cv, cc) per spin; a single tuple when spin = 1."""
+        c = self._coeffs(pot_coeffs)
+        vv, cv, cc = self._bufs(self.spin)
         _check(lib().olcao_gauss_overlap_ham_pscf(self.ctx, _p(c), _p(vv), _p(cv), _p(cc)),
                self.ctx)
-        return self._shape(vv, cv, cc)
+        return self._shape(vv, cv, cc, self.spin)
 
     # ---- solve ---------------------------------------------------------------
     def _evecs_shape(self, ev, ns):
         n, nk = self.vale_dim, self.nk
-        if self.gamma:
-            return ev.reshape(ns, n).T.copy()
-        return ev.view(np.complex128).reshape(nk, ns, n).transpose(0, 2, 1).copy()
+        out = []
+        per = (1 if self.gamma else 2) * n * ns * nk
+        for q in range(self.spin):
+            e = ev[q * per:(q + 1) * per]
+            if self.gamma:
+                out.append(e.reshape(ns, n).T.copy())
+            else:
+                out.append(e.view(np.complex128).reshape(nk, ns, n).transpose(0, 2, 1).copy())
+        return out[0] if self.spin == 1 else np.array(out)
+
+    def _evals(self, w):
+        return w[0] if self.spin == 1 else w
 
     def secularEqnPSCF(self, num_states, vectors=False):
+        """Eigenvalues [nk, numStates] ([spin, nk, numStates] when spin = 2)."""
         n, nk = self.vale_dim, self.nk
-        w = np.zeros((nk, num_states))
-        ev = np.zeros((1 if self.gamma else 2) * n * num_states * nk) if vectors else None
+        w = np.zeros((self.spin, nk, num_states))
+        ev = (np.zeros(self.spin * (1 if self.gamma else 2) * n * num_states * nk)
+              if vectors else None)
         _check(lib().olcao_secular_eqn_pscf(self.ctx, num_states, _p(w), _p(ev)), self.ctx)
-        return (w, self._evecs_shape(ev, num_states)) if vectors else w
+        return (self._evals(w), self._evecs_shape(ev, num_states)) if vectors else self._evals(w)
 
     def setupSCF(self):
         _check(lib().olcao_setup_scf(self.ctx), self.ctx)
 
     def secularEqnSCF(self, pot_coeffs, num_states, vectors=False):
-        c = np.ascontiguousarray(pot_coeffs, np.float64)
+        c = self._coeffs(pot_coeffs)
         n, nk = self.vale_dim, self.nk
-        w = np.zeros((nk, num_states))
-        ev = np.zeros((1 if self.gamma else 2) * n * num_states * nk) if vectors else None
+        w = np.zeros((self.spin, nk, num_states))
+        ev = (np.zeros(self.spin * (1 if self.gamma else 2) * n * num_states * nk)
+              if vectors else None)
         _check(lib().olcao_secular_eqn_scf(self.ctx, _p(c), num_states, _p(w), _p(ev)),
                self.ctx)
-        return (w, self._evecs_shape(ev, num_states)) if vectors else w
+        return (self._evals(w), self._evecs_shape(ev, num_states)) if vectors else self._evals(w)
 
     def scf_iteration(self, pot_coeffs, num_states):
         """Mode B iteration: overlap sweep, fused Hamiltonian sweep, solve."""
-        c = np.ascontiguousarray(pot_coeffs, np.float64)
-        w = np.zeros((self.nk, num_states))
+        c = self._coeffs(pot_coeffs)
+        w = np.zeros((self.spin, self.nk, num_states))
         _check(lib().olcao_scf_iteration(self.ctx, _p(c), num_states, _p(w)), self.ctx)
-        return w
+        return self._evals(w)
 
     def work_counts(self):
         """Three-centre evaluations of the last sweep per class (la, lb):

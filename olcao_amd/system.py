@@ -64,9 +64,11 @@ class SystemInput:
     pot_type: np.ndarray         # (nPotSites,) 0-based, sites grouped by type
     kpoints: np.ndarray          # (nk, 3) fractional reciprocal coordinates
     kweights: np.ndarray         # (nk,)
-    pot_coeffs: np.ndarray       # (potDim,) one spin
+    pot_coeffs: np.ndarray       # (potDim,) or (spin, potDim)
     basis_cutoff: float = 1.0e-16
     gamma: bool = False          # real Gamma-point build (-DGAMMA)
+    rel: int = 0                 # 1: scalar relativistic (mass-velocity term)
+    spin: int = 1                # potential-coefficient sets (potCoeffs(:,spin))
 
     # ---- sizes -----------------------------------------------------------
     @property
@@ -101,8 +103,10 @@ class SystemInput:
                 raise ValueError("potential alphas must be strictly ascending")
         if np.any(np.diff(self.pot_type) < 0):
             raise ValueError("potential sites must be grouped by type")
-        if len(self.pot_coeffs) != self.pot_dim:
-            raise ValueError("pot_coeffs length must equal potDim")
+        if self.spin not in (1, 2) or self.rel not in (0, 1):
+            raise ValueError("spin must be 1 or 2 and rel 0 or 1")
+        if np.asarray(self.pot_coeffs).size != self.pot_dim * self.spin:
+            raise ValueError("pot_coeffs must hold spin x potDim values")
 
     # ---- flat arrays shared by the C interfaces -----------------------------
     def flat(self) -> dict:
@@ -165,4 +169,5 @@ class SystemInput:
             nk=int(self.kpoints.shape[0]),
             kpoints=c(self.kpoints, dtype=np.float64),
             basisCutoff=float(self.basis_cutoff), gamma=int(self.gamma),
+            rel=int(self.rel), spin=int(self.spin),
         )

@@ -33,7 +33,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libolcao_ref_prims.so")
 
-OL, KE, NP, EP, HAM = 0, 1, 2, 3, 4
+OL, KE, NP, EP, HAM, MV = 0, 1, 2, 3, 4, 5
 SMALL_THRESH = 1.0e-8
 
 _lib = None
@@ -186,7 +186,8 @@ class _Sys(C.Structure):
         ("potNucAlpha", C.c_void_p), ("nPotSites", C.c_int), ("potPos", C.c_void_p),
         ("potSiteType", C.c_void_p), ("vec", C.c_double * 9), ("nCells", C.c_int),
         ("cellDims", C.c_void_p), ("cellSizes", C.c_void_p), ("nk", C.c_int),
-        ("kpt", C.c_void_p), ("logThresh", C.c_double), ("gamma", C.c_int)]
+        ("kpt", C.c_void_p), ("logThresh", C.c_double), ("gamma", C.c_int),
+        ("rel", C.c_int)]
 
 
 class Oracle:
@@ -234,15 +235,17 @@ class Oracle:
         s.kpt = k["kpt"].ctypes.data
         s.logThresh = -math.log(sysin.basis_cutoff)
         s.gamma = int(sysin.gamma)
+        s.rel = int(getattr(sysin, "rel", 0))
         self._s = s
         self.n = sysin.vale_dim
         self.nc = sysin.core_dim
         self.nk = sysin.nk
 
     def integral(self, op, term=0, coef=None, nthreads=0):
-        """(valeVale, coreVale, coreCore) of one integral sweep.  Complex
-        builds return arrays [k, row, col]; the Gamma build returns real
-        [row, col] (integrals.F90:151-3605)."""
+        """(valeVale, coreVale, coreCore) of one integral sweep, before any
+        core orthogonalisation.  Complex builds return arrays [k, row, col];
+        the Gamma build returns real [row, col] (integrals.F90:151-3605).
+        HAM takes one spin's coefficients [potDim]."""
         n, nc, nk = self.n, self.nc, self.nk
         cplx = 1 if self.sysin.gamma else 2
         vv = np.zeros(cplx * n * n * nk)
@@ -289,6 +292,22 @@ def ortho_operator(O_vv, O_cv, O_cc, S_cv):
     H = lambda x: np.conj(x).swapaxes(-1, -2)
     out = O_vv - H(S_cv) @ O_cv - H(O_cv) @ S_cv + H(S_cv) @ O_cc @ S_cv
     return np.triu(out) if out.ndim == 2 else np.stack([np.triu(o) for o in out])
+
+
+def ortho_dataset(vv, cv, cc, S_cv, overlap=False):
+    """valeVale as the reference writes it after orthogonalisation against
+    the core (orthoOL / ortho, integrals.F90:3608-3910): the projected upper
+    triangle with components below 1e-10 zeroed (makeValeVale packing,
+    intgOrtho.F90:142-190); unchanged when there is no core."""
+    if cv is None or cv.shape[-2] == 0:
+        return vv
+    out = ortho_overlap(vv, cv, cc) if overlap else ortho_operator(vv, cv, cc, S_cv)
+    if np.iscomplexobj(out):
+        re, im = out.real.copy(), out.imag.copy()
+        re[np.abs(re) < 1e-10] = 0.0
+        im[np.abs(im) < 1e-10] = 0.0
+        return re + 1j * im
+    return np.where(np.abs(out) < 1e-10, 0.0, out)
 
 
 def pack_upper(M, zero_small=False):

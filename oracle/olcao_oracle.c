@@ -318,6 +318,80 @@ static void os_nuclear(int la, int lb, const double pf[7], const double GA[3],
         }
 }
 
+/* Mass-velocity block (massvel2CIntg, gaussIntegrals.f90:39070):
+ * (alpha_fs^2/8) <a|nabla^4|b>, alpha_fs = fineStructure * 1e-3
+ * (src/constants.f90:47), with nabla^4 = sum_i d_i^4 + 2 sum_{i<j} d_i^2 d_j^2.
+ * The operator is applied to the more diffuse primitive (<nabla^4 a|b> when
+ * a1 < a2, else <a|nabla^4 b>; equal by integration by parts): on the tight
+ * side its terms are ~(a_tight/a_diffuse)^2 larger than the result and
+ * cancel.  One 1-D Cartesian Gaussian g_i = (x-A)^i exp(-a (x-A)^2) has
+ *   d^2 g_i = i(i-1) g_{i-2} - 2a(2i+1) g_i + 4a^2 g_{i+2},
+ * so the 1-D derivative integrals are combinations of the 1-D overlap table
+ * extended by four along the differentiated side. */
+#define MVMAX 8
+static void os1d_table(int la, int lb, double XA, double XB, double h,
+                       double E[MVMAX][MVMAX])
+{
+    E[0][0] = 1.0;
+    for (int i = 1; i <= la; i++)
+        E[i][0] = XA * E[i - 1][0] + (i > 1 ? h * (i - 1) * E[i - 2][0] : 0.0);
+    for (int j = 1; j <= lb; j++)
+        for (int i = 0; i <= la; i++) {
+            double t = 0.0;
+            if (i > 0) t += i * E[i - 1][j - 1];
+            if (j > 1) t += (j - 1) * E[i][j - 2];
+            E[i][j] = XB * E[i][j - 1] + h * t;
+        }
+}
+
+/* <g_i | d^2 | g_j> with the derivative on the ket (exponent b), from E */
+static double d2_ket(double E[MVMAX][MVMAX], int i, int j, double b)
+{
+    double v = -2.0 * b * (2 * j + 1) * E[i][j] + 4.0 * b * b * E[i][j + 2];
+    if (j >= 2) v += j * (j - 1) * E[i][j - 2];
+    return v;
+}
+
+static double d4_ket(double E[MVMAX][MVMAX], int i, int j, double b)
+{
+    double v = -2.0 * b * (2 * j + 1) * d2_ket(E, i, j, b) +
+               4.0 * b * b * d2_ket(E, i, j + 2, b);
+    if (j >= 2) v += j * (j - 1) * d2_ket(E, i, j - 2, b);
+    return v;
+}
+
+static void os_massvel(int la, int lb, double a1, double a2, double pref,
+                       const double PA[3], const double PB[3], double h,
+                       double pc[20][20])
+{
+    static const double fs = 7.2973525693 * 0.001;
+    const int bra = a1 < a2;   /* differentiate the bra (transposed tables) */
+    double E[3][MVMAX][MVMAX];
+    /* table T[p][q]: p the undifferentiated side's power, q the other's */
+    for (int x = 0; x < 3; x++) {
+        if (bra)
+            os1d_table(lb, la + 4, PB[x], PA[x], h, E[x]);
+        else
+            os1d_table(la, lb + 4, PA[x], PB[x], h, E[x]);
+    }
+    const double e = bra ? a1 : a2;
+    int na = NCART[la], nb = NCART[lb];
+    for (int a = 0; a < na; a++)
+        for (int b = 0; b < nb; b++) {
+            double S[3], D2[3], D4[3];
+            for (int x = 0; x < 3; x++) {
+                int p = bra ? CART[b][x] : CART[a][x];
+                int q = bra ? CART[a][x] : CART[b][x];
+                S[x] = E[x][p][q];
+                D2[x] = d2_ket(E[x], p, q, e);
+                D4[x] = d4_ket(E[x], p, q, e);
+            }
+            double v = D4[0] * S[1] * S[2] + S[0] * D4[1] * S[2] + S[0] * S[1] * D4[2] +
+                       2.0 * (D2[0] * D2[1] * S[2] + D2[0] * S[1] * D2[2] + S[0] * D2[1] * D2[2]);
+            pc[a][b] = (fs * fs / 8.0) * pref * v;
+        }
+}
+
 void orc_prim(int op, double a1, double a2, double a3, const double *A,
               const double *B, const double *C, int sw, double *sh)
 {
@@ -336,6 +410,12 @@ void orc_prim(int op, double a1, double a2, double a3, const double *A,
         d2 += d[x] * d[x];
     }
     static __thread double pc[20][20], pk[20][20], th[7][20][20];
+    if (op == 4) {
+        double pref = pow(PI / zeta, 1.5) * exp(-xi * d2);
+        os_massvel(la, lb, a1, a2, pref, PA, PB, 1.0 / (2.0 * zeta), pc);
+        cart_to_sh(la, lb, pc, sh);
+        return;
+    }
     if (op == 0 || op == 1) {
         double pref = pow(PI / zeta, 1.5) * exp(-xi * d2);
         double h = 1.0 / (2.0 * zeta);
@@ -616,6 +696,11 @@ static void pair_block(const drv_ctx *c, const op_args *o, int ti, int tj,
         orc_prim(o->op == ORC_OL ? 0 : 1, a1, a2, 0.0, A, B, A, sw, blk);
         return;
     }
+    if (o->op == ORC_MV) {   /* accumulated with a minus sign, integrals.F90:1434-1447 */
+        orc_prim(4, a1, a2, 0.0, A, B, A, sw, blk);
+        for (int q = 0; q < 256; q++) blk[q] = -blk[q];
+        return;
+    }
     for (int q = 0; q < 256; q++) blk[q] = 0.0;
     double acen = alpha_center(ea1, ea2);
     double P[3];
@@ -737,6 +822,11 @@ static void pair_block(const drv_ctx *c, const op_args *o, int ti, int tj,
         orc_prim(1, a1, a2, 0.0, A, B, A, sw, one);
         for (int jj = 0; jj < n2; jj++)
             for (int ii = 0; ii < n1; ii++) blk[ii + 16 * jj] += one[ii + 16 * jj];
+        if (s->rel) {   /* integrals.F90:3463-3490 */
+            orc_prim(4, a1, a2, 0.0, A, B, A, sw, one);
+            for (int jj = 0; jj < n2; jj++)
+                for (int ii = 0; ii < n1; ii++) blk[ii + 16 * jj] -= one[ii + 16 * jj];
+        }
     }
 #undef SITE_LOOP
     __atomic_fetch_add(&((op_args *)o)->ntr, ntr, __ATOMIC_RELAXED);

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-enum { ORC_OL = 0, ORC_KE = 1, ORC_NP = 2, ORC_EP = 3, ORC_HAM = 4 };
+enum { ORC_OL = 0, ORC_KE = 1, ORC_NP = 2, ORC_EP = 3, ORC_HAM = 4, ORC_MV = 5 };
 enum { ORC_OK = 0, ORC_ERR_CELLS = 1, ORC_ERR_INPUT = 2 };
 
 typedef struct {
@@ -56,10 +56,12 @@ typedef struct {
     const double *kpt;        /* [nk][3] */
     double logThresh;         /* tau = -ln(cutoff) */
     int gamma;                /* 1: real Gamma-only build (no phases) */
+    int rel;                  /* 1: HAM subtracts the mass-velocity term */
 } orc_sys;
 
 /* Primitive integrals (gaussIntegrals.f90).  op: 0 overlap, 1 kinetic,
- * 2 three-centre overlap (electron3CIntg), 3 nuclear (nuclear3CIntg). */
+ * 2 three-centre overlap (electron3CIntg), 3 nuclear (nuclear3CIntg),
+ * 4 mass velocity (massvel2CIntg). */
 void orc_boys(double T, double F[7]);
 void orc_prim(int op, double a1, double a2, double a3, const double *A,
               const double *B, const double *C, int l1l2switch, double *sh);
@@ -78,7 +80,8 @@ void orc_nearest_lattice_vector(const double *vec, const double *v,
 /* Integral drivers (integrals.F90 + intgSaving.F90).  vv is valeVale
  * (complex, or real when s->gamma), cv coreVale, cc coreCore; all zeroed
  * by the call.  term: global potential term index for ORC_EP.  coef:
- * potential coefficients [potDim] for ORC_HAM.  nthreads <= 0: default. */
+ * potential coefficients [potDim] of one spin for ORC_HAM.  ORC_MV is the
+ * negated mass-velocity sweep (gaussOverlapMV).  nthreads <= 0: default. */
 int orc_integral(const orc_sys *s, int op, int term, const double *coef,
                  double *vv, double *cv, double *cc, int nthreads);
 

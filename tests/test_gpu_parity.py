@@ -18,7 +18,7 @@ from tests.conftest import block_rel_err
 pytestmark = pytest.mark.gpu
 HARTREE_EV = 27.211386245988
 NSH = [1, 4, 9, 16]
-OPS = {"overlap": 0, "kinetic": 1, "electron3c": 2, "nuclear3c": 3}
+OPS = {"overlap": 0, "kinetic": 1, "electron3c": 2, "nuclear3c": 3, "massvel": 4}
 
 
 def _log(msg):
@@ -43,6 +43,8 @@ def test_device_primitives_match_reference_golden(gpu, golden, name):
         e = block_rel_err(out[i][:na, :nb], ref[i][:na, :nb])
         worst = max(worst, e)
         tol = 1e-10 if (name != "nuclear3c" or cls[i, 0] + cls[i, 1] <= 4) else 5e-6
+        if name == "massvel":   # the reference's own cancellation (see test_oracle)
+            tol = 3e-13 + 1e-15 * (inp[i, :2].max() / inp[i, :2].min()) ** 2
         assert e < tol, (name, i, cls[i], e)
     _log(f"  worst {worst:.2e}")
 
@@ -65,7 +67,7 @@ def test_device_primitives_random_vs_oracle(gpu):
     la = rng.integers(0, 3, n)
     lb = rng.integers(0, 3, n)
     sw = (1 << la) + 16 * (1 << lb)
-    for op in range(4):
+    for op in range(5):
         g = gpu.prim_batch(op, a, A, B, C, sw)
         o = oracle.prim_batch(op, a, A, B, C, sw)
         errs = [block_rel_err(g[i][:NSH[la[i]], :NSH[lb[i]]], o[i][:NSH[la[i]], :NSH[lb[i]]])
@@ -84,7 +86,19 @@ SMALL = {
     "spdf": dict(lmax=3, n_alpha=6),
     "core": dict(core=True),
     "gamma_spd": dict(gamma=True, lmax=2),
+    "rel_spin2": dict(lmax=2, rel=1, spin=2),
+    "core_gamma_rel": dict(core=True, gamma=True, rel=1, two_types=True),
 }
+
+
+def _ortho_close(a, b, tol):
+    """block_rel_err allowing the 1e-10 zeroing of the orthogonalised
+    packing to fall differently on components within rounding of 1e-10."""
+    a, b = np.asarray(a), np.asarray(b)
+    d = np.abs(a - b)
+    edge = (np.minimum(np.abs(a), np.abs(b)) == 0) & (np.maximum(np.abs(a), np.abs(b)) < 2e-10)
+    d = np.where(edge, 0.0, d)
+    return float(d.max() / max(np.abs(b).max(), 1e-300)) < tol
 
 
 @pytest.mark.parametrize("case", list(SMALL))
@@ -102,25 +116,66 @@ def test_setup_matches_oracle(gpu, case):
 
 @pytest.mark.parametrize("case", list(SMALL))
 def test_integral_sweeps_match_oracle(gpu, case):
+    """Every sweep against the oracle: raw coreVale/coreCore, and valeVale as
+    the reference writes it (orthogonalised against the core when there is
+    one); per spin for the fused Hamiltonian."""
     _log(f"sweeps {case}")
     s = configs.small_test(**SMALL[case])
     g, o = _pair(gpu, s)
     tol = 1e-10 if case != "spdf" else 1e-8
-    checks = [("OL", g.gaussOverlapOL, lambda: o.integral(oracle.OL)),
-              ("KE", g.gaussOverlapKE, lambda: o.integral(oracle.KE)),
-              ("NP", g.gaussOverlapNP, lambda: o.integral(oracle.NP)),
-              ("HAM", lambda: g.gaussOverlapHamPSCF(s.pot_coeffs),
-               lambda: o.integral(oracle.HAM, coef=s.pot_coeffs))]
+    So = o.integral(oracle.OL)
+    coefs = np.asarray(s.pot_coeffs).reshape(s.spin, -1)
+    checks = [("OL", g.gaussOverlapOL, lambda: So, True),
+              ("KE", g.gaussOverlapKE, lambda: o.integral(oracle.KE), False),
+              ("NP", g.gaussOverlapNP, lambda: o.integral(oracle.NP), False)]
+    if s.rel:
+        checks.append(("MV", g.gaussOverlapMV, lambda: o.integral(oracle.MV), False))
     for j in range(s.pot_dim):
         checks.append((f"EP{j}", lambda j=j: g.elecPotGaussOverlap(j),
-                       lambda j=j: o.integral(oracle.EP, term=j)))
-    for name, fg, fo in checks:
+                       lambda j=j: o.integral(oracle.EP, term=j), False))
+    hams = g.gaussOverlapHamPSCF(s.pot_coeffs)
+    hams = [hams] if s.spin == 1 else list(hams)
+    for q in range(s.spin):
+        checks.append((f"HAM{q}", lambda q=q: hams[q],
+                       lambda q=q: o.integral(oracle.HAM, coef=coefs[q]), False))
+    for name, fg, fo, isol in checks:
         mg, mo = fg(), fo()
-        e = block_rel_err(mg[0], mo[0])
-        assert e < tol, (case, name, e)
         if s.core_dim:
+            ref = oracle.ortho_dataset(mo[0], mo[1], mo[2], So[1], overlap=isol)
+            assert _ortho_close(mg[0], ref, tol), (case, name)
             assert block_rel_err(mg[1], mo[1]) < tol, (case, name, "coreVale")
             assert block_rel_err(mg[2], mo[2]) < tol, (case, name, "coreCore")
+        else:
+            e = block_rel_err(mg[0], mo[0])
+            assert e < tol, (case, name, e)
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["core", "rel_spin2", "core_gamma_rel"])
+def test_solve_core_rel_spin_matches_oracle(gpu, case):
+    """Eigenvalues of every spin (Mode B) and of Mode A after the core
+    orthogonalisation and with the mass-velocity term, against LAPACK on the
+    oracle's matrices."""
+    s = configs.small_test(**SMALL[case])
+    g, o = _pair(gpu, s)
+    coefs = np.asarray(s.pot_coeffs).reshape(s.spin, -1)
+    So = o.integral(oracle.OL)
+    S = oracle.ortho_dataset(*So, So[1], overlap=True)
+    wo = []
+    for q in range(s.spin):
+        Ho = o.integral(oracle.HAM, coef=coefs[q])
+        wo.append(oracle.solve(oracle.ortho_dataset(*Ho, So[1]), S)[0])
+    wo = np.array(wo)
+    n = g.vale_dim
+    wb = np.asarray(g.scf_iteration(s.pot_coeffs, n)).reshape(s.spin, -1, n)
+    err = np.abs(wb - wo.reshape(wb.shape)).max() * HARTREE_EV
+    _log(f"  {case} Mode B eigenvalue error {err:.2e} eV")
+    assert err < 1e-6
+    g.setupSCF()
+    wa = np.asarray(g.secularEqnSCF(s.pot_coeffs, n)).reshape(wb.shape)
+    err_a = np.abs(wa - wo.reshape(wb.shape)).max() * HARTREE_EV
+    _log(f"  {case} Mode A eigenvalue error {err_a:.2e} eV")
+    assert err_a < 1e-6
     g.close()
 
 

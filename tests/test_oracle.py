@@ -17,7 +17,7 @@ import oracle
 from olcao_amd import configs
 from tests.conftest import block_rel_err
 
-OPS = {"overlap": 0, "kinetic": 1, "electron3c": 2, "nuclear3c": 3}
+OPS = {"overlap": 0, "kinetic": 1, "electron3c": 2, "nuclear3c": 3, "massvel": 4}
 NSH = [1, 4, 9, 16]
 
 
@@ -34,7 +34,82 @@ def test_primitives_match_reference_golden(golden, name):
     out = oracle.prim_batch(OPS[name], a, A, B, C, cls[:, 2])
     for i in range(len(cls)):
         na, nb = NSH[cls[i, 0]], NSH[cls[i, 1]]
-        assert block_rel_err(out[i][:na, :nb], ref[i][:na, :nb]) < 1e-13, (name, i, cls[i])
+        assert block_rel_err(out[i][:na, :nb], ref[i][:na, :nb]) < _golden_tol(name, inp_a=a[i]), \
+            (name, i, cls[i])
+
+
+def _golden_tol(name, inp_a):
+    """1e-13 of the block scale.  The reference's mass-velocity recursion
+    applies nabla^4 to the ket; when the ket is the tight primitive its
+    terms are ~(a_tight/a_diffuse)^2 larger than the result and cancel, so
+    the reference's own values carry ~1e-16 (a_max/a_min)^2 relative error
+    (the oracle differentiates the diffuse side and is pinned to 1e-13 by the
+    exact s-s and quadrature tests below); the bound allows 10x that."""
+    if name != "massvel":
+        return 1e-13
+    ratio = max(inp_a[0], inp_a[1]) / min(inp_a[0], inp_a[1])
+    return 3e-13 + 1e-15 * ratio * ratio
+
+
+LVAL = [0, 1, 1, 1, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3]
+
+
+def test_massvel_ss_exact(golden):
+    """s-s mass-velocity against the closed form evaluated in 50-digit
+    arithmetic (mpmath): (alpha^2/8) S [16 b^4 <r_B^4> - 80 b^3 <r_B^2> + 60 b^2]
+    over the product Gaussian, at the reference's golden geometries
+    (including exponent ratios up to 1e7)."""
+    import mpmath as mp
+    mp.mp.dps = 50
+    cls, a, A, B, C, ref = _golden_case(golden, "massvel")
+    sel = np.where(cls[:, 2] == 17)[0]
+    out = oracle.prim_batch(4, a[sel], A[sel], B[sel], C[sel], cls[sel, 2])
+    fs = mp.mpf("7.2973525693") / 1000
+    for q, i in enumerate(sel):
+        a1, a2 = mp.mpf(a[i, 0]), mp.mpf(a[i, 1])
+        Am = [mp.mpf(x) for x in A[i]]
+        Bm = [mp.mpf(x) for x in B[i]]
+        z = a1 + a2
+        d2 = sum((x - y) ** 2 for x, y in zip(Am, Bm))
+        S = (mp.pi / z) ** mp.mpf(1.5) * mp.exp(-a1 * a2 / z * d2)
+        PB = [(a1 * x + a2 * y) / z - y for x, y in zip(Am, Bm)]
+        s2 = 1 / (2 * z)
+        r2 = sum(p * p for p in PB) + 3 * s2
+        r4 = sum(p ** 4 + 6 * p * p * s2 + 3 * s2 * s2 for p in PB) + 2 * sum(
+            (PB[u] ** 2 + s2) * (PB[v] ** 2 + s2) for u, v in [(0, 1), (0, 2), (1, 2)])
+        exact = fs * fs / 8 * S * (16 * a2 ** 4 * r4 - 80 * a2 ** 3 * r2 + 60 * a2 ** 2)
+        assert abs(out[q][0, 0] / float(exact) - 1) < 1e-12, (i, out[q][0, 0], float(exact))
+
+
+def test_massvel_quadrature_all_classes():
+    """(alpha^2/8) <a|nabla^4|b> = (alpha^2/8) <nabla^2 a|nabla^2 b>, and for a
+    harmonic polynomial Y_l, nabla^2 (Y_l e^{-a r^2}) = (4a^2 r^2 - 2a(2l+3))
+    Y_l e^{-a r^2}: an exact Gauss-Hermite quadrature of that product checks
+    every class without the recursion."""
+    fs = 7.2973525693e-3
+    rng = np.random.default_rng(21)
+    for la in range(4):
+        for lb in range(4):
+            a1, a2, _, A, B, C = _rand_geom(rng)
+            sw = (1 << la) + 16 * (1 << lb)
+            sh = oracle.prim_batch(4, np.array([[a1, a2, 1.0]]), A[None], B[None], C[None],
+                                   [sw])[0]
+            z = a1 + a2
+            ctr = (a1 * A + a2 * B) / z
+            X, Y, Z, W = _gh_grid(ctr, z, n=10)
+            rA = (X - A[0]) ** 2 + (Y - A[1]) ** 2 + (Z - A[2]) ** 2
+            rB = (X - B[0]) ** 2 + (Y - B[1]) ** 2 + (Z - B[2]) ** 2
+            rG = (X - ctr[0]) ** 2 + (Y - ctr[1]) ** 2 + (Z - ctr[2]) ** 2
+            g = np.exp(-a1 * rA - a2 * rB + z * rG)
+            ref = np.zeros((NSH[la], NSH[lb]))
+            for i in range(NSH[la]):
+                fa = SH_POLY[i](X - A[0], Y - A[1], Z - A[2]) * \
+                    (4 * a1 * a1 * rA - 2 * a1 * (2 * LVAL[i] + 3))
+                for j in range(NSH[lb]):
+                    fb = SH_POLY[j](X - B[0], Y - B[1], Z - B[2]) * \
+                        (4 * a2 * a2 * rB - 2 * a2 * (2 * LVAL[j] + 3))
+                    ref[i, j] = fs * fs / 8.0 * np.sum(W * g * fa * fb)
+            assert block_rel_err(sh[:NSH[la], :NSH[lb]], ref) < 1e-11, (la, lb)
 
 
 def test_boys_bit_exact_against_reference(golden):
@@ -380,6 +455,24 @@ def test_assembly_zero_coefficient_rule():
     V = [np.array([1.0, 1.0]), np.array([5.0, 7.0])]
     H = oracle.assemble_packed(a, b, V, [0.0, 2.0])
     assert np.array_equal(H, a + b + V[0] + 2 * V[1])
+
+
+def test_rel_hamiltonian_subtracts_mass_velocity():
+    """The fused Hamiltonian with rel = 1 equals the rel = 0 one plus the
+    separately swept (negated) mass-velocity matrix, and that matrix is
+    Hermitian in k (M(-k) = conj M(k))."""
+    s0 = configs.small_test(lmax=2, kmesh=(2, 2, 2))
+    s1 = configs.small_test(lmax=2, kmesh=(2, 2, 2), rel=1)
+    o0, o1 = oracle.Oracle(s0), oracle.Oracle(s1)
+    H0 = o0.integral(oracle.HAM, coef=s0.pot_coeffs)[0]
+    H1 = o1.integral(oracle.HAM, coef=s1.pot_coeffs)[0]
+    M = o1.integral(oracle.MV)[0]
+    assert block_rel_err(H1, H0 + M) < 1e-12
+    assert np.abs(M).max() > 1e-6 * np.abs(H0).max()
+    k = s1.kpoints
+    for a in range(len(k)):
+        b = np.where(np.all(np.abs(k + k[a]) < 1e-12, axis=1))[0][0]
+        assert block_rel_err(M[b], np.conj(M[a])) < 1e-12
 
 
 def test_core_orthogonalisation_identity():
