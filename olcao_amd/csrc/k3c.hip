@@ -29,7 +29,11 @@
 
 namespace olcao {
 
-template <int LA, int LB, bool NUC>
+// TAB: the per-(pair, term) constants mu and pref0 of the three-Gaussian
+// items are computed once per lane into an LDS table (they depend on the
+// pair and the potential exponent only), which takes the division and the
+// square root out of the item loop.  MIX: mixed-precision exponential.
+template <int LA, int LB, bool NUC, bool TAB, bool MIX>
 __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs, int n,
                                                      const long long *ioff,
                                                      const unsigned long long *img, int kind,
@@ -40,6 +44,7 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
     __shared__ double tAlpha[kMaxPotTypes * kMaxPotAlphas];
     __shared__ double tCoef[kMaxPotTypes * kMaxPotAlphas];
     __shared__ double tZ[kMaxPotTypes], tNuc[kMaxPotTypes];
+    __shared__ double sExp2[256];   // 2^(j/256) for the table exponential
     // potential sites: dynamic LDS, positions then types
     extern __shared__ double dyn[];
     double *sitePos = dyn;
@@ -62,6 +67,7 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
         tZ[t] = c.potZ[t];
         tNuc[t] = c.potNuc[t];
     }
+    for (int j = lane; j < 256; j += 64) sExp2[j] = exp2(j * (1.0 / 256.0));
     __syncthreads();
 
     const int i = blockIdx.x * 64 + lane;
@@ -69,6 +75,27 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
     const int ri = active ? i : 0;
     const PPRec rec = recs[ri];
     const PairConst q = make_pair(rec.a1, rec.a2, rec.A, rec.B);
+    // term tables [type * maxPA + alpha][lane] of mu and pref0, after the sites
+    const int nTab = c.nPotTypes * c.maxPA * 64;
+    double *tabMu = dyn + ((3 * c.nPotSites + (c.nPotSites + 1) / 2 + 1) & ~1);
+    double *tabPref = tabMu + nTab;
+    if (TAB && !NUC) {
+        for (int t = 0; t < c.nPotTypes; ++t)
+            for (int a = 0; a < c.maxPA; ++a) {
+                const double a3 = tAlpha[t * kMaxPotAlphas + a];
+                const double wgt = kind == K3_HAM ? tCoef[t * kMaxPotAlphas + a] : 1.0;
+                double mu = 0.0, pref0 = 0.0;
+                if (a3 > 0.0) {
+                    const double z3 = q.zeta + a3;
+                    const double inv = 1.0 / z3;
+                    const double tt = kPi * inv;
+                    mu = q.zeta * a3 * inv;
+                    pref0 = wgt * tt * sqrt(tt);
+                }
+                tabMu[(t * c.maxPA + a) * 64 + lane] = mu;
+                tabPref[(t * c.maxPA + a) * 64 + lane] = pref0;
+            }
+    }
     double acc[NH];
 #pragma unroll
     for (int e = 0; e < NH; ++e) acc[e] = 0.0;
@@ -112,8 +139,13 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
                 acc_nuc_herm<L>(q, tNuc[t], C, -tZ[t], acc);
             } else {
                 const int term = kind == K3_HAM ? nn : termIdx;
-                const double wgt = kind == K3_HAM ? tCoef[t * kMaxPotAlphas + term] : 1.0;
-                acc_e3c_herm<L>(q, tAlpha[t * kMaxPotAlphas + term], C, wgt, acc);
+                if (TAB) {
+                    const int ix = (t * c.maxPA + term) * 64 + lane;
+                    acc_e3c_herm_tab<L, MIX>(q, tabMu[ix], tabPref[ix], C, sExp2, acc);
+                } else {
+                    const double wgt = kind == K3_HAM ? tCoef[t * kMaxPotAlphas + term] : 1.0;
+                    acc_e3c_herm<L>(q, tAlpha[t * kMaxPotAlphas + term], C, wgt, acc);
+                }
             }
             ++nn;
             ++items;
@@ -209,9 +241,22 @@ __global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const dou
     }
 }
 
+// Dynamic LDS of the three-centre kernels: site positions and types, then
+// (TAB) the per-lane term table.
+static size_t site_bytes(const DevCtx &c)
+{
+    return (size_t)(((3 * c.nPotSites + (c.nPotSites + 1) / 2 + 1) & ~1)) * sizeof(double);
+}
+static size_t table_bytes(const DevCtx &c)
+{
+    return (size_t)c.nPotTypes * c.maxPA * 64 * 2 * sizeof(double);
+}
+constexpr size_t kTableLdsMax = 48 * 1024;
+
 template <int LA, int LB>
 static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, int ti,
-                      const double *coef, int accumulate, double *blocks, hipStream_t s)
+                      const double *coef, int accumulate, double *blocks, int mixed,
+                      hipStream_t s)
 {
     const int k = LA * 4 + LB;
     const int n = cl.count[k];
@@ -219,15 +264,25 @@ static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, i
     constexpr int NH = nherm(LA + LB);
     double *W = cl.W[k];
     const dim3 grid((n + 63) / 64);
-    const size_t dyn = (size_t)c.nPotSites * (3 * sizeof(double) + sizeof(int)) + 16;
+    const size_t dyn = site_bytes(c) + 16;
+    const bool tab = table_bytes(c) <= kTableLdsMax;
+    const size_t dynTab = site_bytes(c) + table_bytes(c) + 16;
     const bool e3c = kind != K3_NP;
     const bool nuc = kind != K3_EP && c.anyNuc;
-    if (e3c)
-        hipLaunchKernelGGL((k_three_center<LA, LB, false>), grid, dim3(64), dyn, s, c, cl.rec[k],
-                           n, cl.ioff[k], cl.img[k], kind, tt, ti, coef, 0, W);
+    if (e3c) {
+        if (tab && mixed)
+            hipLaunchKernelGGL((k_three_center<LA, LB, false, true, true>), grid, dim3(64), dynTab,
+                               s, c, cl.rec[k], n, cl.ioff[k], cl.img[k], kind, tt, ti, coef, 0, W);
+        else if (tab)
+            hipLaunchKernelGGL((k_three_center<LA, LB, false, true, false>), grid, dim3(64), dynTab,
+                               s, c, cl.rec[k], n, cl.ioff[k], cl.img[k], kind, tt, ti, coef, 0, W);
+        else
+            hipLaunchKernelGGL((k_three_center<LA, LB, false, false, false>), grid, dim3(64), dyn,
+                               s, c, cl.rec[k], n, cl.ioff[k], cl.img[k], kind, tt, ti, coef, 0, W);
+    }
     if (nuc)
-        hipLaunchKernelGGL((k_three_center<LA, LB, true>), grid, dim3(64), dyn, s, c, cl.rec[k],
-                           n, cl.ioff[k], cl.img[k], kind, tt, ti, coef, e3c ? 1 : 0, W);
+        hipLaunchKernelGGL((k_three_center<LA, LB, true, false, false>), grid, dim3(64), dyn, s, c,
+                           cl.rec[k], n, cl.ioff[k], cl.img[k], kind, tt, ti, coef, e3c ? 1 : 0, W);
     if (!e3c && !nuc) launch_fill(W, (long long)NH * n, 0.0, s);
     hipLaunchKernelGGL((k_herm_finish<LA, LB>), grid, dim3(64), 0, s, cl.rec[k], W, n,
                        accumulate, blocks);
@@ -237,12 +292,12 @@ static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, i
 #define OLCAO_CAT(a, b) OLCAO_CAT2(a, b)
 void OLCAO_CAT(launch_3c_la, OLCAO_LA)(const DevCtx &c, const ClassLists &cl, int kind,
                                         int tt, int ti, const double *coef, int accumulate,
-                                        double *blocks, hipStream_t s)
+                                        double *blocks, int mixed, hipStream_t s)
 {
-    three_one<OLCAO_LA, 0>(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
-    three_one<OLCAO_LA, 1>(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
-    three_one<OLCAO_LA, 2>(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
-    three_one<OLCAO_LA, 3>(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
+    three_one<OLCAO_LA, 0>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+    three_one<OLCAO_LA, 1>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+    three_one<OLCAO_LA, 2>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+    three_one<OLCAO_LA, 3>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
 }
 
 }  // namespace olcao

@@ -336,21 +336,22 @@ void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int acc, d
 }
 
 void launch_3c_la0(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
-                   double *, hipStream_t);
+                   double *, int, hipStream_t);
 void launch_3c_la1(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
-                   double *, hipStream_t);
+                   double *, int, hipStream_t);
 void launch_3c_la2(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
-                   double *, hipStream_t);
+                   double *, int, hipStream_t);
 void launch_3c_la3(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
-                   double *, hipStream_t);
+                   double *, int, hipStream_t);
 
 void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind, int tt, int ti,
-                         const double *coef, int accumulate, double *blocks, hipStream_t s)
+                         const double *coef, int accumulate, double *blocks, int mixed,
+                         hipStream_t s)
 {
-    launch_3c_la0(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
-    launch_3c_la1(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
-    launch_3c_la2(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
-    launch_3c_la3(c, cl, kind, tt, ti, coef, accumulate, blocks, s);
+    launch_3c_la0(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+    launch_3c_la1(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+    launch_3c_la2(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+    launch_3c_la3(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
 }
 
 // ---------------------------------------------------------------------------

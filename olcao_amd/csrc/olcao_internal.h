@@ -128,9 +128,10 @@ struct ClassLists {
 
 void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int accumulate,
                        double *blocks, hipStream_t s);
+// mixed: three-Gaussian exponentials by the mixed-precision evaluation.
 void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind, int termType,
                          int termIdx, const double *coef, int accumulate, double *blocks,
-                         hipStream_t s);
+                         int mixed, hipStream_t s);
 // Image lists: pass 1 counts entries and fused-sweep items per record, pass 2
 // writes the entries at off[i]; *overflow is set if a lattice index does not
 // fit the entry.

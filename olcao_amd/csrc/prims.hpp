@@ -477,6 +477,78 @@ OHD void acc_e3c_herm(const PairConst &q, double a3, const double *C, double w,
         }
 }
 
+// As acc_e3c_herm with the term's pair constants precomputed:
+// mu = zeta a3 / (zeta + a3), pref0 = w (pi/(zeta + a3))^(3/2) (the same
+// operations in the same order, so the sums are identical).  MIX evaluates
+// exp(-mu X^2) from the FP64 argument with a base-2 range reduction in FP64
+// and the hardware FP32 exponential of the fractional part (relative error
+// ~1e-7), the polynomial part and the accumulation staying in FP64.
+OHD double exp_neg_mixed(double arg)   // exp(-arg), arg >= 0
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = -arg * 1.4426950408889634;   // log2(e)
+    const double n = floor(y);
+    const float f = (float)(y - n);               // in [0, 1)
+    return ldexp((double)__builtin_amdgcn_exp2f(f), (int)n);
+#else
+    return exp(-arg);
+#endif
+}
+
+// exp(-arg), arg >= 0, in FP64 from a 256-entry table T[j] = 2^(j/256):
+// N = rint(-arg 256/ln2), r = -arg - N ln2/256 (two-part constant, |r| <=
+// ln2/512), exp(-arg) = 2^(N div 256) T[N mod 256] (1 + r + ... + r^5/120);
+// the truncation error r^6/720 is below 1e-20 and the result is within a
+// few ulp.  Arguments beyond the FP64 range give 0.
+OHD double exp_neg_tab(double arg, const double *T)
+{
+    if (arg > 700.0) return 0.0;
+    const double x = -arg;
+    const double Nd = rint(x * 369.3299304675746);              // 256 / ln 2
+    // ln2/256 = hi + lo, hi with 22 trailing zero bits so that N hi is exact
+    const double r = fma(Nd, -1.6548911585192516e-12, fma(Nd, -0.002707606172407395, x));
+    const int N = (int)Nd;
+    const int j = N & 255;
+    const int m = (N - j) / 256;
+    double p = 1.0 / 120.0;
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(T[j] * p, m);
+}
+
+template <int L, bool MIX>
+OHD void acc_e3c_herm_tab(const PairConst &q, double mu, double pref0, const double *C,
+                          const double *expTab, double (&W)[nherm(L)])
+{
+    double X[3], X2 = 0.0;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        X[x] = q.P[x] - C[x];
+        X2 += X[x] * X[x];
+    }
+    const double pref = pref0 * (MIX ? exp_neg_mixed(mu * X2) : exp_neg_tab(mu * X2, expTab));
+    double h[3][L + 1];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        h[x][0] = 1.0;
+        if (L >= 1) h[x][L >= 1 ? 1 : 0] = -2.0 * mu * X[x];
+#pragma unroll
+        for (int k = 1; k < L; ++k)
+            h[x][k + 1] = -2.0 * mu * (X[x] * h[x][k] + k * h[x][k - 1]);
+    }
+#pragma unroll
+    for (int tt = 0; tt <= L; ++tt)
+#pragma unroll
+        for (int u = 0; u <= L - tt; ++u) {
+            const double hxy = pref * h[0][tt] * h[1][u];
+#pragma unroll
+            for (int v = 0; v <= L - tt - u; ++v) W[herm_index(tt, u, v)] += hxy * h[2][v];
+        }
+}
+
 // acc += w * R_tuv of the Gaussian-damped nuclear attraction.
 template <int L>
 OHD void acc_nuc_herm(const PairConst &q, double a3, const double *C, double w,

@@ -708,7 +708,8 @@ void build_image_lists(olcao_ctx &c)
 void three_center(olcao_ctx &c, int kind, int tt, int ti, const double *coef, int accumulate)
 {
     launch_fill(reinterpret_cast<double *>(c.dCounters.p), 32, 0.0, c.stream);
-    launch_three_center(c.dev, c.cls, kind, tt, ti, coef, accumulate, c.dBlocks.p, c.stream);
+    launch_three_center(c.dev, c.cls, kind, tt, ti, coef, accumulate, c.dBlocks.p,
+                        c.precision == OLCAO_PREC_MIXED, c.stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -1000,8 +1001,8 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
     olcao_ctx *c = new olcao_ctx();
     olcao_status st = guard(c, [&] {
         c->precision = opt ? opt->precision : 0;
-        if (c->precision != OLCAO_PREC_FP64)
-            fail(OLCAO_ERR_UNSUPPORTED, "only the FP64 precision form is available");
+        if (c->precision != OLCAO_PREC_FP64 && c->precision != OLCAO_PREC_MIXED)
+            fail(OLCAO_ERR_INPUT, "precision must be OLCAO_PREC_FP64 or OLCAO_PREC_MIXED");
         HIPCHK(hipStreamCreate(&c->stream));
         for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
         if (rocblas_create_handle(&c->blas) != rocblas_status_success)
@@ -1212,7 +1213,7 @@ olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates
             launch_fill(reinterpret_cast<double *>(c->dCounters.p), 32, 0.0, s);
             HIPCHK(hipEventRecord(c->ev[2 + 3 * q], s));
             launch_three_center(c->dev, c->cls, K3_HAM, 0, 0, c->dCoef.p + (size_t)q * c->potDim,
-                                1, c->dBlocks.p, s);
+                                1, c->dBlocks.p, c->precision == OLCAO_PREC_MIXED, s);
             HIPCHK(hipEventRecord(c->ev[3 + 3 * q], s));
             contract_scatter(*c, c->H[q].p, c->CV_H[q].p, c->CC_H[q].p);
             ortho(*c, c->H[q].p, c->CV_H[q].p, c->CC_H[q].p, false);
