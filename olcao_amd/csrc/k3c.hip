@@ -33,9 +33,12 @@ namespace olcao {
 // items are computed once per lane into an LDS table (they depend on the
 // pair and the potential exponent only), which takes the division and the
 // square root out of the item loop.  MIX: mixed-precision exponential.
+// One lane per chunk of a record's image list (chRec, chBeg, chEnd; n
+// chunks); the chunk's Hermite sums go to W[e][chunk].
 template <int LA, int LB, bool NUC, bool TAB, bool MIX>
 __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs, int n,
-                                                     const long long *ioff,
+                                                     const int *chRec, const long long *chBeg,
+                                                     const long long *chEnd,
                                                      const unsigned long long *img, int kind,
                                                      int termType, int termIdx,
                                                      const double *coef, int addW, double *W)
@@ -73,7 +76,7 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
     const int i = blockIdx.x * 64 + lane;
     const bool active = i < n;
     const int ri = active ? i : 0;
-    const PPRec rec = recs[ri];
+    const PPRec rec = recs[chRec[ri]];
     const PairConst q = make_pair(rec.a1, rec.a2, rec.A, rec.B);
     // term tables [type * maxPA + alpha][lane] of mu and pref0, after the sites
     const int nTab = c.nPotTypes * c.maxPA * 64;
@@ -100,8 +103,8 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
 #pragma unroll
     for (int e = 0; e < NH; ++e) acc[e] = 0.0;
 
-    long long p = active ? ioff[ri] : 0;
-    const long long pe = active ? ioff[ri + 1] : 0;
+    long long p = active ? chBeg[ri] : 0;
+    const long long pe = active ? chEnd[ri] : 0;
     unsigned long long nxt = p < pe ? img[p] : 0ull;
     int nt = 0, nn = 0, t = 0;
     double C[3] = {0.0, 0.0, 0.0};
@@ -176,9 +179,12 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
 // per lane per element.
 constexpr int kFinishStage = 81;
 
+// W of record i: the sum of its chunks' W in entry order (rcOff, rcList;
+// nChunk columns).
 template <int LA, int LB>
 __global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const double *W, int nrec,
-                                                    int accumulate, double *blocks)
+                                                    const int *rcOff, const int *rcList,
+                                                    int nChunk, int accumulate, double *blocks)
 {
     constexpr int L = LA + LB, NH = nherm(L);
     constexpr int NA = ncart(LA), SA = nsh(LA), SB = nsh(LB);
@@ -192,7 +198,13 @@ __global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const dou
     const PairConst q = make_pair(rec.a1, rec.a2, rec.A, rec.B);
     double Wl[NH];
 #pragma unroll
-    for (int e = 0; e < NH; ++e) Wl[e] = active ? W[(long long)e * nrec + i] : 0.0;
+    for (int e = 0; e < NH; ++e) Wl[e] = 0.0;
+    if (active)
+        for (int q = rcOff[i]; q < rcOff[i + 1]; ++q) {
+            const int col = rcList[q];
+#pragma unroll
+            for (int e = 0; e < NH; ++e) Wl[e] += W[(long long)e * nChunk + col];
+        }
     double E[3][LA + 1][LB + 1][L + 1];
 #pragma unroll
     for (int x = 0; x < 3; ++x) herm_e<LA, LB>(q.P[x] - q.A[x], q.P[x] - q.B[x], 0.5 / q.zeta, E[x]);
@@ -263,7 +275,8 @@ static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, i
     if (n == 0) return;
     constexpr int NH = nherm(LA + LB);
     double *W = cl.W[k];
-    const dim3 grid((n + 63) / 64);
+    const int nch = cl.nChunk[k];
+    const dim3 grid((nch + 63) / 64), gridRec((n + 63) / 64);
     const size_t dyn = site_bytes(c) + 16;
     const bool tab = table_bytes(c) <= kTableLdsMax;
     const size_t dynTab = site_bytes(c) + table_bytes(c) + 16;
@@ -272,20 +285,24 @@ static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, i
     if (e3c) {
         if (tab && mixed)
             hipLaunchKernelGGL((k_three_center<LA, LB, false, true, true>), grid, dim3(64), dynTab,
-                               s, c, cl.rec[k], n, cl.ioff[k], cl.img[k], kind, tt, ti, coef, 0, W);
+                               s, c, cl.rec[k], nch, cl.chRec[k], cl.chBeg[k], cl.chEnd[k], cl.img[k], kind,
+                               tt, ti, coef, 0, W);
         else if (tab)
             hipLaunchKernelGGL((k_three_center<LA, LB, false, true, false>), grid, dim3(64), dynTab,
-                               s, c, cl.rec[k], n, cl.ioff[k], cl.img[k], kind, tt, ti, coef, 0, W);
+                               s, c, cl.rec[k], nch, cl.chRec[k], cl.chBeg[k], cl.chEnd[k], cl.img[k], kind,
+                               tt, ti, coef, 0, W);
         else
             hipLaunchKernelGGL((k_three_center<LA, LB, false, false, false>), grid, dim3(64), dyn,
-                               s, c, cl.rec[k], n, cl.ioff[k], cl.img[k], kind, tt, ti, coef, 0, W);
+                               s, c, cl.rec[k], nch, cl.chRec[k], cl.chBeg[k], cl.chEnd[k], cl.img[k], kind,
+                               tt, ti, coef, 0, W);
     }
     if (nuc)
         hipLaunchKernelGGL((k_three_center<LA, LB, true, false, false>), grid, dim3(64), dyn, s, c,
-                           cl.rec[k], n, cl.ioff[k], cl.img[k], kind, tt, ti, coef, e3c ? 1 : 0, W);
-    if (!e3c && !nuc) launch_fill(W, (long long)NH * n, 0.0, s);
-    hipLaunchKernelGGL((k_herm_finish<LA, LB>), grid, dim3(64), 0, s, cl.rec[k], W, n,
-                       accumulate, blocks);
+                           cl.rec[k], nch, cl.chRec[k], cl.chBeg[k], cl.chEnd[k], cl.img[k], kind,
+                           tt, ti, coef, e3c ? 1 : 0, W);
+    if (!e3c && !nuc) launch_fill(W, (long long)NH * nch, 0.0, s);
+    hipLaunchKernelGGL((k_herm_finish<LA, LB>), gridRec, dim3(64), 0, s, cl.rec[k], W, n,
+                       cl.rcOff[k], cl.rcList[k], nch, accumulate, blocks);
 }
 
 #define OLCAO_CAT2(a, b) a##b

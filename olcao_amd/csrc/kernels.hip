@@ -17,6 +17,8 @@
 //                   intgSaving.F90:132-655)
 //   k_assemble      H = NP + KE + sum_j c_j V_j over full matrices
 //                   (secularEqn.F90:147-170)
+#include <algorithm>
+
 #include "device_common.hpp"
 
 namespace olcao {
@@ -169,11 +171,22 @@ __global__ __launch_bounds__(64) void k_contract_sum(DevCtx c, const int4 *multi
 }
 
 // ---------------------------------------------------------------------------
-// Bloch sum + origin shift + scatter, one workgroup per atom pair
-__global__ __launch_bounds__(256) void k_bloch(DevCtx c, const double *X, double *vv,
-                                               double *cv, double *cc, int gamma)
+// Bloch sum + origin shift + scatter.  One wavefront per (atom pair, tile of
+// KT k-points): the pair's contracted blocks X(cell) and the phases
+// exp(i k.L_cell) of the tile are staged through LDS kBlochCells cells at a
+// time, each lane accumulating up to kBlochOut outputs (k, s1, s2) with s1
+// fastest, so that the scatter into the column-major matrices is
+// contiguous.  Cells are summed in the pair's cell order.
+constexpr int kBlochCells = 16;
+constexpr int kBlochOut = 8;
+
+__global__ __launch_bounds__(64) void k_bloch(DevCtx c, const double *X, double *vv,
+                                              double *cv, double *cc, int gamma, int KT,
+                                              int nTiles)
 {
-    const int pr = blockIdx.x;
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const int pr = blockIdx.x / nTiles, tile = blockIdx.x - pr * nTiles;
     if (pr >= c.nPairs) return;
     const int i = c.pairI[pr], j = c.pairJ[pr];
     const int ti = c.atomType[i], tj = c.atomType[j];
@@ -185,60 +198,100 @@ __global__ __launch_bounds__(256) void k_bloch(DevCtx c, const double *X, double
     const int p0 = c.pairPc[pr], p1 = c.pairPc[pr + 1];
     const int ms = c.maxS;
     const int nk = gamma ? 1 : c.nk;
-    const double lv[3] = {c.pairLv[3 * pr], c.pairLv[3 * pr + 1], c.pairLv[3 * pr + 2]};
+    const int k0 = tile * KT, nkt = min(nk, k0 + KT) - k0;
+    if (nkt <= 0) return;
     const int per = nt1 * nt2;
-    for (int o = threadIdx.x; o < nk * per; o += blockDim.x) {
-        const int k = o / per;
-        const int rem = o - k * per;
-        const int s1 = rem / nt2, s2 = rem - s1 * nt2;
-        double re = 0.0, im = 0.0;
-        if (gamma) {
-            for (int pc = p0; pc < p1; ++pc) re += X[(long long)pc * ms * ms + s1 * ms + s2];
-        } else {
-            for (int pc = p0; pc < p1; ++pc) {
-                const double x = X[(long long)pc * ms * ms + s1 * ms + s2];
-                const double *ph = c.phase + 2 * ((long long)c.pcCell[pc] * c.nk + k);
-                re += ph[0] * x;
-                im += ph[1] * x;
+    const int nout = nkt * per;
+    double *sX = lds;                          // [kBlochCells][per]
+    double *sP = sX + kBlochCells * per;       // [kBlochCells][nkt][2]
+    const double lv[3] = {c.pairLv[3 * pr], c.pairLv[3 * pr + 1], c.pairLv[3 * pr + 2]};
+    for (int g = 0; g < nout; g += 64 * kBlochOut) {
+        double re[kBlochOut], im[kBlochOut];
+#pragma unroll
+        for (int r = 0; r < kBlochOut; ++r) re[r] = im[r] = 0.0;
+        for (int q0 = p0; q0 < p1; q0 += kBlochCells) {
+            const int ncl = min(kBlochCells, p1 - q0);
+            for (int q = lane; q < ncl * per; q += 64) {
+                const int cl = q / per, e = q - cl * per;
+                const int s1 = e / nt2, s2 = e - s1 * nt2;
+                sX[q] = X[(long long)(q0 + cl) * ms * ms + s1 * ms + s2];
             }
-            const double *kp = c.kcart + 3 * k;
-            const double dot = kp[0] * lv[0] + kp[1] * lv[1] + kp[2] * lv[2];
-            if (fabs(dot) >= kSmallThresh) {
-                const double cr = cos(dot), ci = sin(dot);
-                const double r2 = cr * re - ci * im;
-                im = cr * im + ci * re;
-                re = r2;
+            if (!gamma)
+                for (int q = lane; q < ncl * nkt; q += 64) {
+                    const int cl = q / nkt, kk = q - cl * nkt;
+                    const double *ph = c.phase + 2 * ((long long)c.pcCell[q0 + cl] * c.nk + k0 + kk);
+                    sP[2 * q] = ph[0];
+                    sP[2 * q + 1] = ph[1];
+                }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < kBlochOut; ++r) {
+                const int o = g + r * 64 + lane;
+                if (o >= nout) continue;
+                const int kk = o / per, e = o - kk * per;
+                const int s2 = e / nt1, s1 = e - s2 * nt1;
+                const double *xs = sX + s1 * nt2 + s2;
+                if (gamma) {
+                    for (int cl = 0; cl < ncl; ++cl) re[r] += xs[cl * per];
+                } else {
+                    const double *ps = sP + 2 * kk;
+                    for (int cl = 0; cl < ncl; ++cl) {
+                        const double x = xs[cl * per];
+                        re[r] += ps[2 * cl * nkt] * x;
+                        im[r] += ps[2 * cl * nkt + 1] * x;
+                    }
+                }
             }
+            __syncthreads();
         }
-        const int cp = gamma ? 1 : 2;
-        const bool a_core = s1 >= nv1, b_core = s2 >= nv2;
-        if (!a_core && !b_core) {
-            const long long idx = (v1 + s1) + (long long)(v2 + s2) * n + (long long)k * n * n;
-            const bool zero = (i == j) && (s1 > s2);
-            vv[idx * cp] = zero ? 0.0 : re;
-            if (!gamma) vv[idx * cp + 1] = zero ? 0.0 : im;
-        } else if (a_core && !b_core) {
-            if (i != j) {   // for i == j the dagger assignment below wins
-                const long long idx = (c1 + s1 - nv1) + (long long)(v2 + s2) * nc +
-                                      (long long)k * nc * n;
-                cv[idx * cp] = re;
-                if (!gamma) cv[idx * cp + 1] = im;
+#pragma unroll
+        for (int r = 0; r < kBlochOut; ++r) {
+            const int o = g + r * 64 + lane;
+            if (o >= nout) continue;
+            const int kk = o / per, e = o - kk * per;
+            const int s2 = e / nt1, s1 = e - s2 * nt1;
+            const int k = k0 + kk;
+            double vre = re[r], vim = im[r];
+            if (!gamma) {
+                const double *kp = c.kcart + 3 * k;
+                const double dot = kp[0] * lv[0] + kp[1] * lv[1] + kp[2] * lv[2];
+                if (fabs(dot) >= kSmallThresh) {
+                    const double cr = cos(dot), ci = sin(dot);
+                    const double r2 = cr * vre - ci * vim;
+                    vim = cr * vim + ci * vre;
+                    vre = r2;
+                }
             }
-        } else if (!a_core && b_core) {
-            const long long idx = (c2 + s2 - nv2) + (long long)(v1 + s1) * nc +
-                                  (long long)k * nc * n;
-            cv[idx * cp] = re;
-            if (!gamma) cv[idx * cp + 1] = -im;
-        } else {
-            const long long idx = (c1 + s1 - nv1) + (long long)(c2 + s2 - nv2) * nc +
-                                  (long long)k * nc * nc;
-            cc[idx * cp] = re;
-            if (!gamma) cc[idx * cp + 1] = im;
-            if (i != j) {
-                const long long id2 = (c2 + s2 - nv2) + (long long)(c1 + s1 - nv1) * nc +
+            const int cp = gamma ? 1 : 2;
+            const bool a_core = s1 >= nv1, b_core = s2 >= nv2;
+            if (!a_core && !b_core) {
+                const long long idx = (v1 + s1) + (long long)(v2 + s2) * n + (long long)k * n * n;
+                const bool zero = (i == j) && (s1 > s2);
+                vv[idx * cp] = zero ? 0.0 : vre;
+                if (!gamma) vv[idx * cp + 1] = zero ? 0.0 : vim;
+            } else if (a_core && !b_core) {
+                if (i != j) {   // for i == j the dagger assignment below wins
+                    const long long idx = (c1 + s1 - nv1) + (long long)(v2 + s2) * nc +
+                                          (long long)k * nc * n;
+                    cv[idx * cp] = vre;
+                    if (!gamma) cv[idx * cp + 1] = vim;
+                }
+            } else if (!a_core && b_core) {
+                const long long idx = (c2 + s2 - nv2) + (long long)(v1 + s1) * nc +
+                                      (long long)k * nc * n;
+                cv[idx * cp] = vre;
+                if (!gamma) cv[idx * cp + 1] = -vim;
+            } else {
+                const long long idx = (c1 + s1 - nv1) + (long long)(c2 + s2 - nv2) * nc +
                                       (long long)k * nc * nc;
-                cc[id2 * cp] = re;
-                if (!gamma) cc[id2 * cp + 1] = -im;
+                cc[idx * cp] = vre;
+                if (!gamma) cc[idx * cp + 1] = vim;
+                if (i != j) {
+                    const long long id2 = (c2 + s2 - nv2) + (long long)(c1 + s1 - nv1) * nc +
+                                          (long long)k * nc * nc;
+                    cc[id2 * cp] = vre;
+                    if (!gamma) cc[id2 * cp + 1] = -vim;
+                }
             }
         }
     }
@@ -507,7 +560,15 @@ void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv, double *
                           double *cc, int gamma, hipStream_t s)
 {
     if (c.nPairs == 0) return;
-    hipLaunchKernelGGL(k_bloch, dim3(c.nPairs), dim3(256), 0, s, c, X, vv, cv, cc, gamma);
+    // k tile: about 512 outputs per wavefront, and at least ~1024 wavefronts
+    const int nk = gamma ? 1 : c.nk;
+    const int per = c.maxS * c.maxS;
+    int KT = std::max(1, std::min(nk, 64 * kBlochOut / std::max(per, 1)));
+    while (KT > 1 && (long long)c.nPairs * ((nk + KT - 1) / KT) < 1024) KT = (KT + 1) / 2;
+    const int nTiles = (nk + KT - 1) / KT;
+    const size_t lds = ((size_t)kBlochCells * per + 2 * (size_t)kBlochCells * KT) * sizeof(double);
+    hipLaunchKernelGGL(k_bloch, dim3(c.nPairs * nTiles), dim3(64), lds, s, c, X, vv, cv, cc,
+                       gamma, KT, nTiles);
 }
 
 void launch_fill(double *p, long long n, double v, hipStream_t s)

@@ -117,13 +117,24 @@ inline __host__ __device__ int img_nucham(unsigned long long e) { return (int)((
 inline __host__ __device__ int img_nucnp(unsigned long long e) { return (int)((e >> 55) & 1); }
 
 // Kernel launchers (kernels.hip).  All asynchronous on `s`.
+// The image list of a record is cut into chunks of about kChunkItems
+// fused-sweep items; each chunk is one lane of the three-centre kernels, so
+// that a diffuse pair with thousands of images does not hold a wave alone.
+constexpr int kChunkItems = 192;
 struct ClassLists {
     const int *idx[16];         // device list of primitive pairs per class
     const PPRec *rec[16];       // the class's pairs as packed records, heaviest first
     const long long *ioff[16];  // [count + 1] image-list offsets per record
     const unsigned long long *img[16];   // image entries
-    double *W[16];              // Hermite sums [nherm(la+lb)][count] per class
+    double *W[16];              // Hermite sums [nherm(la+lb)][nChunk] per class
     int count[16];
+    // chunks, heaviest first: record, entry range
+    const int *chRec[16];
+    const long long *chBeg[16], *chEnd[16];
+    int nChunk[16];
+    // per record: its chunks' positions, in entry order
+    const int *rcOff[16];       // [count + 1]
+    const int *rcList[16];      // [nChunk]
 };
 
 void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int accumulate,

@@ -143,6 +143,8 @@ struct olcao_ctx {
     DBuf<int> dClass[16];
     DBuf<PPRec> dRec[16];
     DBuf<long long> dIoff[16];
+    DBuf<int> dChRec[16], dRcOff[16], dRcList[16];
+    DBuf<long long> dChBeg[16], dChEnd[16];
     DBuf<unsigned long long> dImg[16];
     DBuf<double> dW;
     long long imageEntries = 0;
@@ -578,16 +580,6 @@ void upload(olcao_ctx &c)
         c.dRec[k].upload(recs, s);
         c.cls.rec[k] = c.dRec[k].p;
     }
-    // Hermite-sum scratch of the three-centre sweeps, one slice per class
-    size_t wTotal = 0;
-    size_t wOff[16];
-    for (int k = 0; k < 16; ++k) {
-        const int L = k / 4 + k % 4;
-        wOff[k] = wTotal;
-        wTotal += (size_t)(L + 1) * (L + 2) * (L + 3) / 6 * c.classIdx[k].size();
-    }
-    c.dW.alloc(std::max<size_t>(wTotal, 1));
-    for (int k = 0; k < 16; ++k) c.cls.W[k] = c.dW.p + wOff[k];
     c.dBlocks.alloc(std::max<long long>(c.blockDoubles, 1));
     c.dX.alloc(std::max<size_t>(c.hPcPair.size() * (size_t)c.maxS * c.maxS, 1));
     {   // contraction segments (at most kSegPP primitive pairs each)
@@ -692,7 +684,59 @@ void build_image_lists(olcao_ctx &c)
         c.cls.img[k] = c.dImg[k].p;
         c.imageEntries += off[n];
         launch_image_fill(c.dev, c.cls.rec[k], n, c.cls.ioff[k], c.dImg[k].p, dOverflow.p, s);
+        // chunks of about kChunkItems items, entries split evenly within a record
+        struct Chunk { int rec; long long beg, end; double est; };
+        std::vector<Chunk> ch;
+        ch.reserve(n);
+        for (int q = 0; q < n; ++q) {
+            const int o = order[q];
+            const long long ne = cnt[o];
+            const int nch = (int)std::max<long long>(1, std::min<long long>(ne, (items[o] + kChunkItems - 1) / kChunkItems));
+            for (int u = 0; u < nch; ++u) {
+                const long long b = off[q] + ne * u / nch, e = off[q] + ne * (u + 1) / nch;
+                ch.push_back({q, b, e, ne ? (double)items[o] * (double)(e - b) / (double)ne : 0.0});
+            }
+        }
+        std::vector<int> corder(ch.size());
+        for (size_t u = 0; u < ch.size(); ++u) corder[u] = (int)u;
+        std::stable_sort(corder.begin(), corder.end(), [&](int x, int y) { return ch[x].est > ch[y].est; });
+        const int nc = (int)ch.size();
+        std::vector<int> chRec(nc), rcOff(n + 1, 0), rcList(nc), pos(nc);
+        std::vector<long long> chBeg(nc), chEnd(nc);
+        for (int u = 0; u < nc; ++u) {
+            const Chunk &x = ch[corder[u]];
+            chRec[u] = x.rec;
+            chBeg[u] = x.beg;
+            chEnd[u] = x.end;
+            pos[corder[u]] = u;
+        }
+        // chunks were generated record by record in entry order
+        for (int u = 0; u < nc; ++u) rcOff[ch[u].rec + 1]++;
+        for (int q = 0; q < n; ++q) rcOff[q + 1] += rcOff[q];
+        for (int u = 0; u < nc; ++u) rcList[u] = pos[u];
+        c.dChRec[k].upload(chRec, s);
+        c.dChBeg[k].upload(chBeg, s);
+        c.dChEnd[k].upload(chEnd, s);
+        c.dRcOff[k].upload(rcOff, s);
+        c.dRcList[k].upload(rcList, s);
+        HIPCHK(hipStreamSynchronize(s));
+        c.cls.chRec[k] = c.dChRec[k].p;
+        c.cls.chBeg[k] = c.dChBeg[k].p;
+        c.cls.chEnd[k] = c.dChEnd[k].p;
+        c.cls.rcOff[k] = c.dRcOff[k].p;
+        c.cls.rcList[k] = c.dRcList[k].p;
+        c.cls.nChunk[k] = nc;
     }
+    // Hermite-sum scratch of the three-centre sweeps, one slice per class
+    size_t wTotal = 0;
+    size_t wOff[16];
+    for (int k = 0; k < 16; ++k) {
+        const int L = k / 4 + k % 4;
+        wOff[k] = wTotal;
+        wTotal += (size_t)(L + 1) * (L + 2) * (L + 3) / 6 * (size_t)c.cls.nChunk[k];
+    }
+    c.dW.alloc(std::max<size_t>(wTotal, 1));
+    for (int k = 0; k < 16; ++k) c.cls.W[k] = c.dW.p + wOff[k];
     int overflow = 0;
     HIPCHK(hipMemcpyAsync(&overflow, dOverflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
