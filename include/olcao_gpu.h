@@ -86,9 +86,16 @@ typedef struct {
     int spin;                     /* 1 or 2 potential-coefficient sets */
 } olcao_system;
 
+/* Generalised eigensolve: OLCAO_EIG_AUTO uses the batched in-LDS solver
+ * (one workgroup per k-point: Cholesky, reduction, Jacobi) where the matrices
+ * fit (complex n <= 69, real n <= 98) and rocSOLVER otherwise;
+ * OLCAO_EIG_LIBRARY always uses rocSOLVER. */
+enum { OLCAO_EIG_AUTO = 0, OLCAO_EIG_LIBRARY = 1 };
+
 typedef struct {
     int precision;                /* OLCAO_PREC_* */
-    int reserved[7];
+    int eigensolver;              /* OLCAO_EIG_* */
+    int reserved[6];
 } olcao_options;
 
 /* Context lifetime.  Replaces the derived setup of OLCAO.F90:setupSCF

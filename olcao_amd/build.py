@@ -32,6 +32,7 @@ UNITS = [("kernels.hip", "kernels.o", []),
          ("k3c.hip", "k3c_la1.o", ["-DOLCAO_LA=1"]),
          ("k3c.hip", "k3c_la2.o", ["-DOLCAO_LA=2"]),
          ("k3c.hip", "k3c_la3.o", ["-DOLCAO_LA=3"]),
+         ("ksolve.hip", "ksolve.o", []),
          ("runtime.cpp", "runtime.o", ["-x", "hip"])]
 
 

@@ -176,6 +176,11 @@ void launch_prim_batch(int op, int n, const double *a, const double *A,
                        const int *classCount, double *sh, hipStream_t s);
 void launch_boys_batch(int n, const double *T, double *F, hipStream_t s);
 void launch_herm_fill(double *A, int n, int nb, int cplx, hipStream_t s);
+// Batched small-matrix generalised eigensolve in LDS (ksolve.hip): A (H upper
+// in, eigenvectors out), B (S upper in, U out), w eigenvalues, info per k.
+bool small_hegv_fits(int n, bool cplx);
+void launch_small_hegv(bool cplx, int n, int nb, double *A, double *B, double *w, int *info,
+                       hipStream_t s);
 void launch_extract_evecs(const double *A, int n, int nk, int numStates,
                           int cplx, double *out, hipStream_t s);
 

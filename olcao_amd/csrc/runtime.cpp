@@ -163,6 +163,7 @@ struct olcao_ctx {
 
     DevCtx dev{};
     ClassLists cls{};
+    bool eigLibrary = false;     // always rocSOLVER, also where the LDS solver fits
     ContractPlan cplan{};
     DBuf<int4> dSegs, dMulti;
     DBuf<double> dPartial;
@@ -913,10 +914,20 @@ void ensure_mats(olcao_ctx &c)
 // by the eigenvectors (all n columns) and B by U.  info[k] > n flags a B(k)
 // that is not positive definite, as zhegv does.
 void gen_eig(rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *A, double *B,
-             double *D, double *E, int *dinfo, std::vector<int> &info)
+             double *D, double *E, int *dinfo, std::vector<int> &info, bool library)
 {
     const rocblas_stride st = (rocblas_stride)n * n;
     info.assign(nb, 0);
+    if (!library && small_hegv_fits(n, cplx)) {
+        // one workgroup per matrix, everything in LDS (ksolve.hip)
+        launch_small_hegv(cplx, n, nb, A, B, D, dinfo, s);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int k = 0; k < nb; ++k)
+            if (info[k] < 0) info[k] = 1;   // Jacobi did not converge: reported as zhegv info in 1..n
+        return;
+    }
     rocblas_status r;
     if (cplx)
         r = rocsolver_zpotrf_strided_batched(h, rocblas_fill_upper, n, (rocblas_double_complex *)B, n,
@@ -988,7 +999,8 @@ void solve_device(olcao_ctx &c, int numStates, const double *Hsrc, const double 
     HIPCHK(hipMemcpyAsync(c.Aw.p, Hsrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
     std::vector<int> info;
-    gen_eig(c.blas, s, !c.gamma, n, nk, c.Aw.p, c.Bw.p, c.D.p, c.E.p, c.dInfo.p, info);
+    gen_eig(c.blas, s, !c.gamma, n, nk, c.Aw.p, c.Bw.p, c.D.p, c.E.p, c.dInfo.p, info,
+            c.eigLibrary);
     std::vector<double> w((size_t)n * nk);
     HIPCHK(hipMemcpyAsync(w.data(), c.D.p, w.size() * 8, hipMemcpyDeviceToHost, s));
     if (evecs) {
@@ -1045,6 +1057,7 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
     olcao_ctx *c = new olcao_ctx();
     olcao_status st = guard(c, [&] {
         c->precision = opt ? opt->precision : 0;
+        c->eigLibrary = opt && opt->eigensolver == OLCAO_EIG_LIBRARY;
         if (c->precision != OLCAO_PREC_FP64 && c->precision != OLCAO_PREC_MIXED)
             fail(OLCAO_ERR_INPUT, "precision must be OLCAO_PREC_FP64 or OLCAO_PREC_MIXED");
         HIPCHK(hipStreamCreate(&c->stream));
@@ -1323,7 +1336,7 @@ static olcao_status solve_host(int n, int nb, double *A, double *B, double *w, b
         HIPCHK(hipMemcpyAsync(dA.p, A, per * nb * 8, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(dB.p, B, per * nb * 8, hipMemcpyHostToDevice, s));
         std::vector<int> info;
-        gen_eig(h, s, cplx, n, nb, dA.p, dB.p, dD.p, dE.p, dI.p, info);
+        gen_eig(h, s, cplx, n, nb, dA.p, dB.p, dD.p, dE.p, dI.p, info, false);
         HIPCHK(hipMemcpyAsync(A, dA.p, per * nb * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(w, dD.p, (size_t)n * nb * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));

@@ -47,7 +47,8 @@ module O_GPUFacade
 
    type, bind(C) :: olcao_options_t
       integer(c_int) :: precision
-      integer(c_int) :: reserved(7)
+      integer(c_int) :: eigensolver
+      integer(c_int) :: reserved(6)
    end type olcao_options_t
 
    type(c_ptr), save :: ctx = c_null_ptr
@@ -207,6 +208,7 @@ contains
       if (present(rel)) s%rel = rel
       if (present(spin)) s%spin = spin
       o%precision = 0
+      o%eigensolver = 0
       o%reserved = 0
       st = c_ctx_create(s, o, ctx)
       if (st /= 0) call gpuStop(st, 'gpuInit')

@@ -57,7 +57,10 @@ class _System(C.Structure):
 
 
 class _Options(C.Structure):
-    _fields_ = [("precision", C.c_int), ("reserved", C.c_int * 7)]
+    _fields_ = [("precision", C.c_int), ("eigensolver", C.c_int), ("reserved", C.c_int * 6)]
+
+
+EIG_AUTO, EIG_LIBRARY = 0, 1
 
 
 _lib = None
@@ -167,7 +170,7 @@ def boys_batch(T):
 class OlcaoGPU:
     """One device context for one system (setupSCF's derived state)."""
 
-    def __init__(self, sysin, precision=PREC_FP64):
+    def __init__(self, sysin, precision=PREC_FP64, eigensolver=EIG_AUTO):
         self.sysin = sysin
         f = sysin.flat()
         self._keep = f
@@ -190,6 +193,7 @@ class OlcaoGPU:
         self._sys = s
         opt = _Options()
         opt.precision = precision
+        opt.eigensolver = eigensolver
         ctx = C.c_void_p()
         code = lib().olcao_ctx_create(C.byref(s), C.byref(opt), C.byref(ctx))
         _check(code, None)
