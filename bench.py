@@ -123,7 +123,22 @@ def main():
     ap.add_argument("--workload", default="si64", choices=sorted(WORKLOADS),
                     help="si64 is the benchmark workload; the others are measured the same "
                          "way for the design notes")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="time the FP64 form only (for counter runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES once on the device (the reference kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described, not run).", file=sys.stderr)
@@ -136,6 +151,10 @@ def main():
     s = getattr(configs, args.workload)()
     print(f"[bench] setting up {s.name}: valeDim={s.vale_dim} nk={s.nk}", flush=True)
     fp64 = run_form(s, gpu.PREC_FP64, args.steps, args.warmup, "fp64")
+    if args.fp64_only:
+        print(json.dumps({"workload": args.workload, "fp64_s_per_iter": fp64["sec"],
+                          "stage_ms_fp64": fp64["stage"]}), flush=True)
+        return
     mixed = run_form(s, gpu.PREC_MIXED, args.steps, args.warmup, "mixed")
     HARTREE_EV = 27.211386245988
     n_el = sum(s.pot_types[t].Z for t in s.pot_type)

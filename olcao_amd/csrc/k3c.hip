@@ -183,8 +183,9 @@ constexpr int kFinishStage = 81;
 // nChunk columns).
 template <int LA, int LB>
 __global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const double *W, int nrec,
-                                                    const int *rcOff, const int *rcList,
-                                                    int nChunk, int accumulate, double *blocks)
+                                                    const int *finOrder, const int *rcOff,
+                                                    const int *rcList, int nChunk,
+                                                    int accumulate, double *blocks)
 {
     constexpr int L = LA + LB, NH = nherm(L);
     constexpr int NA = ncart(LA), SA = nsh(LA), SB = nsh(LB);
@@ -192,9 +193,10 @@ __global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const dou
     __shared__ double sOut[STAGE ? 64 * SA * SB + 1 : 1];
     __shared__ long long sOff[64];
     const int lane = threadIdx.x;
-    const int i = blockIdx.x * 64 + lane;
-    const bool active = i < nrec;
-    const PPRec rec = recs[active ? i : nrec - 1];
+    const int li = blockIdx.x * 64 + lane;
+    const bool active = li < nrec;
+    const int i = finOrder[active ? li : nrec - 1];
+    const PPRec rec = recs[i];
     const PairConst q = make_pair(rec.a1, rec.a2, rec.A, rec.B);
     double Wl[NH];
 #pragma unroll
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const dou
     if (STAGE) {
         sOff[lane] = active ? rec.off : -1;
         __syncthreads();
-        const int nr = min(64, nrec - blockIdx.x * 64);
+        const int nr = min(64, nrec - (int)blockIdx.x * 64);
         for (int r = 0; r < nr; ++r) {
             double *p = blocks + sOff[r];
             for (int e = lane; e < SA * SB; e += 64) {
@@ -302,7 +304,7 @@ static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, i
                            tt, ti, coef, e3c ? 1 : 0, W);
     if (!e3c && !nuc) launch_fill(W, (long long)NH * nch, 0.0, s);
     hipLaunchKernelGGL((k_herm_finish<LA, LB>), gridRec, dim3(64), 0, s, cl.rec[k], W, n,
-                       cl.rcOff[k], cl.rcList[k], nch, accumulate, blocks);
+                       cl.finOrder[k], cl.rcOff[k], cl.rcList[k], nch, accumulate, blocks);
 }
 
 #define OLCAO_CAT2(a, b) a##b

@@ -135,6 +135,9 @@ struct ClassLists {
     // per record: its chunks' positions, in entry order
     const int *rcOff[16];       // [count + 1]
     const int *rcList[16];      // [nChunk]
+    // records in the order of their first chunk, so that the finish kernel's
+    // lanes read neighbouring W columns
+    const int *finOrder[16];    // [count]
 };
 
 void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int accumulate,

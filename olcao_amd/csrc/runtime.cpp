@@ -143,7 +143,7 @@ struct olcao_ctx {
     DBuf<int> dClass[16];
     DBuf<PPRec> dRec[16];
     DBuf<long long> dIoff[16];
-    DBuf<int> dChRec[16], dRcOff[16], dRcList[16];
+    DBuf<int> dChRec[16], dRcOff[16], dRcList[16], dFinOrder[16];
     DBuf<long long> dChBeg[16], dChEnd[16];
     DBuf<unsigned long long> dImg[16];
     DBuf<double> dW;
@@ -715,6 +715,17 @@ void build_image_lists(olcao_ctx &c)
         for (int u = 0; u < nc; ++u) rcOff[ch[u].rec + 1]++;
         for (int q = 0; q < n; ++q) rcOff[q + 1] += rcOff[q];
         for (int u = 0; u < nc; ++u) rcList[u] = pos[u];
+        std::vector<int> finOrder;
+        finOrder.reserve(n);
+        {
+            std::vector<char> seen(n, 0);
+            for (int u = 0; u < nc; ++u)
+                if (!seen[chRec[u]]) {
+                    seen[chRec[u]] = 1;
+                    finOrder.push_back(chRec[u]);
+                }
+        }
+        c.dFinOrder[k].upload(finOrder, s);
         c.dChRec[k].upload(chRec, s);
         c.dChBeg[k].upload(chBeg, s);
         c.dChEnd[k].upload(chEnd, s);
@@ -726,6 +737,7 @@ void build_image_lists(olcao_ctx &c)
         c.cls.chEnd[k] = c.dChEnd[k].p;
         c.cls.rcOff[k] = c.dRcOff[k].p;
         c.cls.rcList[k] = c.dRcList[k].p;
+        c.cls.finOrder[k] = c.dFinOrder[k].p;
         c.cls.nChunk[k] = nc;
     }
     // Hermite-sum scratch of the three-centre sweeps, one slice per class
