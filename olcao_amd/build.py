@@ -33,6 +33,8 @@ UNITS = [("kernels.hip", "kernels.o", []),
          ("k3c.hip", "k3c_la2.o", ["-DOLCAO_LA=2"]),
          ("k3c.hip", "k3c_la3.o", ["-DOLCAO_LA=3"]),
          ("ksolve.hip", "ksolve.o", []),
+         ("ktrd.hip", "ktrd.o", []),
+         ("kdc.hip", "kdc.o", []),
          ("runtime.cpp", "runtime.o", ["-x", "hip"])]
 
 

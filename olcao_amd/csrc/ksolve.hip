@@ -20,6 +20,7 @@
 // column-major in LDS with leading dimension n + 1.
 #include <hip/hip_runtime.h>
 
+#include "numtraits.hpp"
 #include "olcao_internal.h"
 
 namespace olcao {
@@ -28,47 +29,6 @@ namespace {
 
 constexpr int kSolveThreads = 1024;
 constexpr int kMaxSweeps = 40;
-
-template <bool CPLX>
-struct Num;
-template <>
-struct Num<true> {
-    struct T {
-        double x, y;
-    };
-    static __device__ T zero() { return {0.0, 0.0}; }
-    static __device__ T mk(double r, double i) { return {r, i}; }
-    static __device__ T add(T a, T b) { return {a.x + b.x, a.y + b.y}; }
-    static __device__ T sub(T a, T b) { return {a.x - b.x, a.y - b.y}; }
-    static __device__ T mul(T a, T b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
-    static __device__ T cmul(T a, T b) { return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }   // conj(a) b
-    static __device__ T scale(T a, double s) { return {a.x * s, a.y * s}; }
-    static __device__ T conj(T a) { return {a.x, -a.y}; }
-    static __device__ double re(T a) { return a.x; }
-    static __device__ double abs2(T a) { return a.x * a.x + a.y * a.y; }
-    static __device__ T load(const double *p, long long i) { return {p[2 * i], p[2 * i + 1]}; }
-    static __device__ void store(double *p, long long i, T v)
-    {
-        p[2 * i] = v.x;
-        p[2 * i + 1] = v.y;
-    }
-};
-template <>
-struct Num<false> {
-    typedef double T;
-    static __device__ T zero() { return 0.0; }
-    static __device__ T mk(double r, double) { return r; }
-    static __device__ T add(T a, T b) { return a + b; }
-    static __device__ T sub(T a, T b) { return a - b; }
-    static __device__ T mul(T a, T b) { return a * b; }
-    static __device__ T cmul(T a, T b) { return a * b; }
-    static __device__ T scale(T a, double s) { return a * s; }
-    static __device__ T conj(T a) { return a; }
-    static __device__ double re(T a) { return a; }
-    static __device__ double abs2(T a) { return a * a; }
-    static __device__ T load(const double *p, long long i) { return p[i]; }
-    static __device__ void store(double *p, long long i, T v) { p[i] = v; }
-};
 
 // round-robin (circle method) pair i of round r among m players (m even)
 __device__ __forceinline__ void rr_pair(int r, int i, int m, int &p, int &q)

@@ -1,0 +1,604 @@
+// Batched reduction of Hermitian (real symmetric) matrices to real
+// tridiagonal form, and the blocked back-transform of the tridiagonal
+// eigenvectors: the zhetrd/dsytrd (uplo 'L') and zunmtr/dormtr steps inside
+// zhegv/dsygv (interfaces.F90:73-200), for matrices too large for one
+// workgroup's LDS.
+//
+// Reduction.  One group of G workgroups per matrix, the whole grid launched
+// cooperatively; groups walk through the batch.  Workgroup b owns the rows
+// r = b, b+G, ... of the full Hermitian matrix: columns 0..63 in registers,
+// columns 64..n-1 in LDS, so the matrix is read from global memory once and
+// never written back.  Step i builds H_i = I - tau_i v_i v_i^H with
+// v_i(i+1) = 1, zeroing A(i+2:n, i) (zhetd2's order), in one pass and one
+// group barrier:
+//   pass     each owned row r > i takes the pending rank-2 update of step
+//            i-1, A(r,:) -= v(r) w^H + w(r) v^H, in place, and accumulates
+//            p(r) = tau_i A(r,:) v_i; p(r), A(r, i+1) and the wave's share of
+//            p^H v_i are published
+//   barrier  the group's arrival counter
+//   vectors  every wave reads p and column i+1 and forms, redundantly,
+//            w = p - tau_i/2 (p^H v_i) v_i, the updated column i+1, d(i+1)
+//            and the next reflector (zlarfg)
+// v, w and the column are register vectors, lane l holding entries l, l+64, ...
+//
+// Back-transform.  Q = H_0 H_1 ... H_{n-2}; for blocks of kTrdBlock
+// reflectors Q_b = I - V_b T_b V_b^H (zlarft, forward, columnwise).  The
+// tfactor kernel forms U_b = V_b T_b; the host then applies
+// X <- X - U_b (V_b^H X) block by block with batched GEMMs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "numtraits.hpp"
+#include "olcao_internal.h"
+
+namespace olcao {
+
+namespace {
+
+constexpr int kTrdThreads = 256;
+constexpr int kTrdWaves = kTrdThreads / 64;
+constexpr int kTrdK = 10;        // 64-column chunks of a row: n <= 640
+constexpr int kTrdRpw = 5;       // owned rows per wave: ceil(n / G) <= 20
+constexpr size_t kTrdLdsMax = 160 * 1024 - 256;
+constexpr long long kSpinLimit = 1ll << 22;
+constexpr int kRs = 6;           // per-row scalars: v, pending v, pending w, p, column, next column
+enum { RV, RVP, RWP, PR, COLR, RCN };
+
+template <bool CPLX>
+__device__ __forceinline__ typename Num<CPLX>::T wshfl(typename Num<CPLX>::T a, int lane)
+{
+    if constexpr (CPLX)
+        return {__shfl(a.x, lane, 64), __shfl(a.y, lane, 64)};
+    else
+        return __shfl(a, lane, 64);
+}
+
+template <bool CPLX>
+__device__ __forceinline__ typename Num<CPLX>::T wsum(typename Num<CPLX>::T a)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        if constexpr (CPLX) {
+            a.x += __shfl_xor(a.x, off, 64);
+            a.y += __shfl_xor(a.y, off, 64);
+        } else {
+            a += __shfl_xor(a, off, 64);
+        }
+    }
+    return a;
+}
+
+__device__ __forceinline__ double wsum_real(double a)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    return a;
+}
+
+// entry idx of a lane-distributed register vector, on every lane
+template <bool CPLX, int KC>
+__device__ __forceinline__ typename Num<CPLX>::T bcast(const typename Num<CPLX>::T (&v)[KC],
+                                                       int idx)
+{
+    const int kk = idx >> 6;
+    typename Num<CPLX>::T s = Num<CPLX>::zero();
+#pragma unroll
+    for (int k = 0; k < KC; ++k)
+        if (k == kk) s = v[k];
+    return wshfl<CPLX>(s, idx & 63);
+}
+
+// Publication stores and loads.  FL 0: plain accesses ordered by the
+// barrier's agent-scope release/acquire fences.  FL 1: agent-scope relaxed
+// atomics, coherent across the device's L2 caches without the fences' cache
+// write-back and invalidation; the barrier then orders them by waiting for
+// every outstanding store before arriving.
+template <int FL>
+__device__ __forceinline__ void pst(double *p, double v)
+{
+    if constexpr (FL == 1)
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        *p = v;
+}
+template <int FL>
+__device__ __forceinline__ double pld(const double *p)
+{
+    if constexpr (FL == 1)
+        return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return *p;
+}
+template <bool CPLX, int FL>
+__device__ __forceinline__ void pstT(double *p, long long i, typename Num<CPLX>::T v)
+{
+    if constexpr (CPLX) {
+        pst<FL>(p + 2 * i, v.x);
+        pst<FL>(p + 2 * i + 1, v.y);
+    } else {
+        pst<FL>(p + i, v);
+    }
+}
+template <bool CPLX, int FL>
+__device__ __forceinline__ typename Num<CPLX>::T pldT(const double *p, long long i)
+{
+    if constexpr (CPLX)
+        return {pld<FL>(p + 2 * i), pld<FL>(p + 2 * i + 1)};
+    else
+        return pld<FL>(p + i);
+}
+
+// Group barrier: the workgroup arrives once its waves' publications are
+// complete and leaves when all G workgroups of the group have arrived.  The
+// spin is bounded; false means the group did not assemble.
+template <int FL>
+__device__ __forceinline__ bool group_barrier(unsigned *cnt, unsigned target, int *bad)
+{
+    if constexpr (FL == 1) {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = 0;
+        if constexpr (FL == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        long long sp = 0;
+        while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++sp > kSpinLimit) {
+                b = 1;
+                break;
+            }
+        }
+        if constexpr (FL == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *bad = b;
+    }
+    __syncthreads();
+    if constexpr (FL == 1) __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    return *bad == 0;
+}
+
+// zlarfg/dlarfg on column i of the current matrix, held in cn (entries > i):
+// alpha = cn(i+1), x = cn(i+2:n).  Returns v (v(i+1) = 1, zero above), tau
+// and beta (the new subdiagonal entry e(i)).
+template <bool CPLX, int KC>
+__device__ __forceinline__ void reflector(int i, int n, int lane,
+                                          const typename Num<CPLX>::T (&cn)[KC],
+                                          typename Num<CPLX>::T (&v)[KC],
+                                          typename Num<CPLX>::T &tau, double &beta,
+                                          typename Num<CPLX>::T &scale)
+{
+    using N = Num<CPLX>;
+    using T = typename N::T;
+    const T alpha = bcast<CPLX>(cn, i + 1);
+    double part = 0.0;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+        const int c = 64 * k + lane;
+        if (c > i + 1 && c < n) part += N::abs2(cn[k]);
+    }
+    const double xnorm2 = wsum_real(part);
+    double ar, ai;
+    if constexpr (CPLX) {
+        ar = alpha.x;
+        ai = alpha.y;
+    } else {
+        ar = alpha;
+        ai = 0.0;
+    }
+    scale = N::mk(1.0, 0.0);
+    if (xnorm2 == 0.0 && ai == 0.0) {
+        tau = N::zero();
+        beta = ar;
+    } else {
+        const double nrm = sqrt(ar * ar + ai * ai + xnorm2);
+        beta = ar >= 0.0 ? -nrm : nrm;
+        tau = N::mk((beta - ar) / beta, -ai / beta);
+        // 1 / (alpha - beta)
+        const double zr = ar - beta, zi = ai;
+        const double den = zr * zr + zi * zi;
+        scale = N::mk(zr / den, -zi / den);
+    }
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+        const int c = 64 * k + lane;
+        v[k] = c > i + 1 && c < n ? N::mul(cn[k], scale)
+               : c == i + 1       ? N::mk(1.0, 0.0)
+                                  : N::zero();
+    }
+}
+
+template <bool CPLX, int FL, int KC>
+__global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ Ain, double *Vout,
+                                                     double *dOut, double *eOut, double *tauOut,
+                                                     double *pub, unsigned *cnt, int *info, int n,
+                                                     int nb, int nGroups, int G)
+{
+    using N = Num<CPLX>;
+    using T = typename N::T;
+    constexpr int E = CPLX ? 2 : 1;
+    extern __shared__ double lds_raw[];
+    T *rows = reinterpret_cast<T *>(lds_raw);
+    __shared__ int bad;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = blockIdx.x % nGroups, b = blockIdx.x / nGroups;
+    const int Lr = n > 64 ? n - 64 : 0;
+    const int rowsPer = (n + G - 1) / G;
+    // wave-uniform per-row scalars of this wave's rows, after the rows:
+    // [row j][RV, RVP, RWP, PR, COLR, RCN]
+    T *rsc = rows + rowsPer * Lr + wv * kTrdRpw * kRs;
+    unsigned *myCnt = cnt + 64 * g;
+    // two publication buffers per group: [p (n) | column (n) | wave dots (4G)]
+    const long long pubLen = (long long)(2 * n + kTrdWaves * G) * E;
+    double *pubG = pub + (long long)g * 2 * pubLen;
+    unsigned epoch = 0;
+    const bool writer = b == 0 && wv == 0;
+
+    for (int mat = g; mat < nb; mat += nGroups) {
+        const long long per = (long long)n * n;
+        const double *A = Ain + E * per * mat;
+        double *V = Vout + E * per * mat;
+        double *d = dOut + (long long)n * mat;
+        double *e = eOut + (long long)n * mat;
+        double *tauG = tauOut + (long long)E * n * mat;
+
+        // owned rows of the full Hermitian matrix from the lower triangle;
+        // row j of this wave is r = b + (wv + 4 j) G; columns 0..63 to
+        // registers, the rest to LDS (one row at a time)
+        T rreg[kTrdRpw];
+#pragma unroll
+        for (int j = 0; j < kTrdRpw; ++j) {
+            const int r = b + (wv + kTrdWaves * j) * G, c = lane;
+            T a = N::zero();
+            if (r < n && c < n) {
+                a = c <= r ? N::load(A, r + (long long)c * n) : N::conj(N::load(A, c + (long long)r * n));
+                if (c == r) a = N::mk(N::re(a), 0.0);
+            }
+            rreg[j] = a;
+        }
+#pragma unroll 1
+        for (int j = 0; j < kTrdRpw; ++j) {
+            const int s = wv + kTrdWaves * j, r = b + s * G;
+            if (r >= n) break;
+#pragma unroll
+            for (int k = 1; k < KC; ++k) {
+                const int c = 64 * k + lane;
+                if (c < n) {
+                    T a = c <= r ? N::load(A, r + (long long)c * n)
+                                 : N::conj(N::load(A, c + (long long)r * n));
+                    if (c == r) a = N::mk(N::re(a), 0.0);
+                    rows[s * Lr + c - 64] = a;
+                }
+            }
+        }
+        // column 0 and reflector 0; per-row scalars of the current (rv) and
+        // pending (rvp, rwp) reflector and update vectors
+        T cn[KC], vc[KC], vp[KC], wp[KC];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int c = 64 * k + lane;
+            cn[k] = c < n ? N::load(A, c) : N::zero();
+            vp[k] = N::zero();
+            wp[k] = N::zero();
+        }
+        if (writer && lane == 0) d[0] = N::re(cn[0]);
+        T tau, scale;
+        double beta;
+        reflector<CPLX, KC>(0, n, lane, cn, vc, tau, beta, scale);
+        if (lane < kTrdRpw) {
+            const int j = lane, r = b + (wv + kTrdWaves * j) * G;
+            const T cr = r < n ? N::load(A, r) : N::zero();
+            rsc[j * kRs + RV] = r > 1 ? N::mul(cr, scale) : r == 1 ? N::mk(1.0, 0.0) : N::zero();
+            rsc[j * kRs + RVP] = N::zero();
+            rsc[j * kRs + RWP] = N::zero();
+        }
+
+        for (int i = 0; i < n - 1; ++i) {
+            double *pb = pubG + (epoch & 1) * pubLen;
+            double *pP = pb, *pC = pb + (long long)E * n, *pPart = pb + 2LL * E * n;
+            if (writer) {
+#pragma unroll
+                for (int k = 0; k < KC; ++k) {
+                    const int c = 64 * k + lane;
+                    if (c < n) N::store(V, c + (long long)i * n, vc[k]);
+                }
+                if (lane == 0) {
+                    e[i] = beta;
+                    N::store(tauG, i, tau);
+                }
+            }
+            // ---- pass over the owned rows: pending update, p(r), column i+1 ----
+            // Entries at columns <= i carry zero v and w, so whole live chunks
+            // are processed without per-lane column tests.
+            const int kLive = (i + 1) >> 6, ci = (i + 1) & 63;
+            T dotw = N::zero();
+#pragma unroll
+            for (int j = 0; j < kTrdRpw; ++j) {
+                const int s = wv + kTrdWaves * j, r = b + s * G;
+                if (r < n && r > i) {
+                    const T vr = rsc[j * kRs + RVP], wr = rsc[j * kRs + RWP];
+                    T acc = N::zero(), col = N::zero();
+#pragma unroll
+                    for (int k = 0; k < KC; ++k) {
+                        if (k >= kLive) {
+                            const int c = 64 * k + lane;
+                            if (k < KC - 1 || c < n) {
+                                T a = k == 0 ? rreg[j] : rows[s * Lr + c - 64];
+                                a = N::sub(N::sub(a, N::cmul(wp[k], vr)), N::cmul(vp[k], wr));
+                                if (k == 0)
+                                    rreg[j] = a;
+                                else
+                                    rows[s * Lr + c - 64] = a;
+                                acc = N::add(acc, N::mul(a, vc[k]));
+                                if (k == kLive) col = a;
+                            }
+                        }
+                    }
+                    const T prj = N::mul(tau, wsum<CPLX>(acc));
+                    const T colj = wshfl<CPLX>(col, ci);
+                    if (lane == 0) {
+                        pstT<CPLX, FL>(pP, r, prj);
+                        pstT<CPLX, FL>(pC, r, colj);
+                        rsc[j * kRs + PR] = prj;
+                        rsc[j * kRs + COLR] = colj;
+                    }
+                    dotw = N::add(dotw, N::cmul(prj, rsc[j * kRs + RV]));
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (lane == 0) pstT<CPLX, FL>(pPart, b * kTrdWaves + wv, dotw);
+            ++epoch;
+            if (!group_barrier<FL>(myCnt, epoch * (unsigned)G, &bad)) {
+                if (threadIdx.x == 0) info[mat] = -2;
+                return;
+            }
+            // ---- vectors: w, column i+1, next reflector ----
+            T part = N::zero();
+            for (int q = lane; q < kTrdWaves * G; q += 64) part = N::add(part, pldT<CPLX, FL>(pPart, q));
+            const T dot = wsum<CPLX>(part);
+            const T alpha = N::scale(N::mul(tau, dot), -0.5);
+            // w(i+1) = p(i+1) + alpha since v_i(i+1) = 1
+            const T w1 = N::add(pldT<CPLX, FL>(pP, i + 1), alpha);
+#pragma unroll
+            for (int k = 0; k < KC; ++k) {
+                const int c = 64 * k + lane;
+                T wk = N::zero(), ck = N::zero();
+                if (c < n && c > i) {
+                    wk = N::add(pldT<CPLX, FL>(pP, c), N::mul(alpha, vc[k]));
+                    ck = N::sub(N::sub(pldT<CPLX, FL>(pC, c), N::cmul(w1, vc[k])), wk);
+                }
+                vp[k] = vc[k];
+                wp[k] = wk;
+                cn[k] = ck;
+            }
+            // the same for this wave's rows (lane j handles row j)
+            if (lane < kTrdRpw) {
+                const int j = lane, r = b + (wv + kTrdWaves * j) * G;
+                const bool live = r < n && r > i;
+                const T rvj = rsc[j * kRs + RV];
+                const T wr = live ? N::add(rsc[j * kRs + PR], N::mul(alpha, rvj)) : N::zero();
+                rsc[j * kRs + RCN] =
+                    live ? N::sub(N::sub(rsc[j * kRs + COLR], N::cmul(w1, rvj)), wr) : N::zero();
+                rsc[j * kRs + RVP] = rvj;
+                rsc[j * kRs + RWP] = wr;
+            }
+            if (writer) {
+                const double dNext = N::re(bcast<CPLX, KC>(cn, i + 1));
+                if (lane == 0) d[i + 1] = dNext;
+            }
+            if (i + 1 < n - 1) {
+                reflector<CPLX, KC>(i + 1, n, lane, cn, vc, tau, beta, scale);
+                if (lane < kTrdRpw) {
+                    const int j = lane, r = b + (wv + kTrdWaves * j) * G;
+                    rsc[j * kRs + RV] = r > i + 2    ? N::mul(rsc[j * kRs + RCN], scale)
+                                        : r == i + 2 ? N::mk(1.0, 0.0)
+                                                     : N::zero();
+                }
+            }
+        }
+        if (threadIdx.x == 0) info[mat] = 0;
+    }
+}
+
+// U_b = V_b T_b for one block of kTrdBlock reflectors of one matrix.
+// V: explicit reflectors (column j = v_j, zero above row j+1, 1 at j+1).
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_trd_tfactor(const double *__restrict__ Vall,
+                                                     const double *__restrict__ tauAll,
+                                                     double *Uall, int n)
+{
+    using N = Num<CPLX>;
+    using T = typename N::T;
+    constexpr int E = CPLX ? 2 : 1;
+    constexpr int NB = kTrdBlock;
+    constexpr int NP = NB * (NB - 1) / 2;
+    __shared__ T Vs[64][NB + 1];
+    __shared__ T Gm[NB][NB + 1];
+    __shared__ T Tm[NB][NB + 1];
+    const int tid = threadIdx.x;
+    const int j0 = blockIdx.x * NB, mat = blockIdx.y;
+    const int nref = n - 1;
+    const int mb = min(NB, nref - j0);
+    const long long per = (long long)n * n;
+    const double *V = Vall + E * per * mat;
+    const double *tau = tauAll + (long long)E * n * mat;
+    double *U = Uall + E * per * mat;
+    const int r0 = j0 + 1;   // first row of the block's reflectors
+
+    // Gram matrix of the block: Gm(a, j) = v_a^H v_j, a < j
+    constexpr int PPT = (NP + 255) / 256;
+    T acc[PPT];
+    int pa[PPT], pj[PPT];
+#pragma unroll
+    for (int t = 0; t < PPT; ++t) {
+        acc[t] = N::zero();
+        int q = tid + 256 * t, j = 1;
+        while (q >= j && j < NB) {
+            q -= j;
+            ++j;
+        }
+        pa[t] = q;
+        pj[t] = j;
+    }
+    for (int rb = r0; rb < n; rb += 64) {
+        for (int e = tid; e < 64 * NB; e += 256) {
+            const int rr = e % 64, a = e / 64, r = rb + rr;
+            Vs[rr][a] = (r < n && a < mb) ? N::load(V, r + (long long)(j0 + a) * n) : N::zero();
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < PPT; ++t) {
+            if (tid + 256 * t < NP)
+                for (int rr = 0; rr < 64; ++rr)
+                    acc[t] = N::add(acc[t], N::cmul(Vs[rr][pa[t]], Vs[rr][pj[t]]));
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < PPT; ++t)
+        if (tid + 256 * t < NP) Gm[pa[t]][pj[t]] = acc[t];
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int a = e % NB, j = e / NB;
+        Tm[a][j] = N::zero();
+    }
+    __syncthreads();
+    // T(j,j) = tau_j; T(0:j, j) = -tau_j T(0:j, 0:j) Gm(0:j, j)
+    for (int j = 0; j < mb; ++j) {
+        const T tj = N::load(tau, j0 + j);
+        if (tid < j) {
+            T s = N::zero();
+            for (int q = tid; q < j; ++q) s = N::add(s, N::mul(Tm[tid][q], Gm[q][j]));
+            Tm[tid][j] = N::scale(N::mul(tj, s), -1.0);
+        }
+        if (tid == j) Tm[j][j] = tj;
+        __syncthreads();
+    }
+    // U = V_b T for rows r0..n-1
+    for (int rb = r0; rb < n; rb += 64) {
+        for (int e = tid; e < 64 * NB; e += 256) {
+            const int rr = e % 64, a = e / 64, r = rb + rr;
+            Vs[rr][a] = (r < n && a < mb) ? N::load(V, r + (long long)(j0 + a) * n) : N::zero();
+        }
+        __syncthreads();
+        const int rr = tid % 64, jg = tid / 64;
+        const int r = rb + rr;
+        if (r < n) {
+            for (int jj = 0; jj < NB / 4; ++jj) {
+                const int j = jg * (NB / 4) + jj;
+                if (j >= mb) break;
+                T s = N::zero();
+                for (int a = 0; a <= j; ++a) s = N::add(s, N::mul(Vs[rr][a], Tm[a][j]));
+                N::store(U, r + (long long)(j0 + j) * n, s);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// X (field of the eigenvector matrices, complex or real) = Z (real)
+__global__ void k_real_to_field(const double *Z, double *X, long long count, int cplx)
+{
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    if (cplx) {
+        X[2 * t] = Z[t];
+        X[2 * t + 1] = 0.0;
+    } else {
+        X[t] = Z[t];
+    }
+}
+
+template <bool CPLX, int FL, int KC>
+const void *trd_fn()
+{
+    return reinterpret_cast<const void *>(&k_trd<CPLX, FL, KC>);
+}
+
+template <bool CPLX, int FL>
+const void *trd_fn_kc(int kc)
+{
+    switch (kc) {
+    case 2: return trd_fn<CPLX, FL, 2>();
+    case 3: return trd_fn<CPLX, FL, 3>();
+    case 4: return trd_fn<CPLX, FL, 4>();
+    case 5: return trd_fn<CPLX, FL, 5>();
+    case 6: return trd_fn<CPLX, FL, 6>();
+    case 7: return trd_fn<CPLX, FL, 7>();
+    case 8: return trd_fn<CPLX, FL, 8>();
+    case 9: return trd_fn<CPLX, FL, 9>();
+    default: return trd_fn<CPLX, FL, 10>();
+    }
+}
+
+// column chunks of 64: 2..kTrdK
+const void *trd_kernel(bool cplx, int flavor, int n)
+{
+    const int kc = std::max(2, (n + 63) / 64);
+    if (cplx) return flavor ? trd_fn_kc<true, 1>(kc) : trd_fn_kc<true, 0>(kc);
+    return flavor ? trd_fn_kc<false, 1>(kc) : trd_fn_kc<false, 0>(kc);
+}
+
+}  // namespace
+
+bool trd_plan(int n, int nb, bool cplx, int flavor, TrdPlan &p)
+{
+    p = TrdPlan{};
+    if (n < 2 || nb < 1 || n > 64 * kTrdK) return false;
+    const size_t el = cplx ? 16 : 8;
+    const size_t Lr = n > 64 ? (size_t)n - 64 : 0;
+    int G = (n + kTrdWaves * kTrdRpw - 1) / (kTrdWaves * kTrdRpw);
+    for (; G <= 64; ++G) {
+        const size_t rowsPer = ((size_t)n + G - 1) / G;
+        if (rowsPer * Lr * el + (size_t)kTrdWaves * kTrdRpw * kRs * el <= kTrdLdsMax) break;
+    }
+    if (G > 64) return false;
+    const size_t lds = (((size_t)n + G - 1) / G) * Lr * el + (size_t)kTrdWaves * kTrdRpw * kRs * el;
+    int dev = 0, cus = 0, perCU = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return false;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, trd_kernel(cplx, flavor, n), kTrdThreads,
+                                                     lds) != hipSuccess)
+        return false;
+    const int groups = std::min(nb, perCU * cus / G);
+    if (groups < 1) return false;
+    p.G = G;
+    p.nGroups = groups;
+    p.lds = lds;
+    p.flavor = flavor;
+    p.pubDoubles = (size_t)groups * 2 * (2 * (size_t)n + kTrdWaves * G) * (cplx ? 2 : 1);
+    p.counters = (size_t)groups * 64;
+    return true;
+}
+
+hipError_t launch_trd(bool cplx, const TrdPlan &p, int n, int nb, const double *A, double *V,
+                      double *d, double *e, double *tau, double *pub, unsigned *cnt, int *info,
+                      hipStream_t s)
+{
+    int G = p.G, nGroups = p.nGroups;
+    void *args[] = {(void *)&A, (void *)&V, (void *)&d, (void *)&e, (void *)&tau, (void *)&pub,
+                    (void *)&cnt, (void *)&info, (void *)&n, (void *)&nb, (void *)&nGroups,
+                    (void *)&G};
+    return hipLaunchCooperativeKernel(trd_kernel(cplx, p.flavor, n), dim3(nGroups * G),
+                                      dim3(kTrdThreads), args, (unsigned)p.lds, s);
+}
+
+void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double *tau, double *U,
+                        hipStream_t s)
+{
+    const int blocks = (n - 1 + kTrdBlock - 1) / kTrdBlock;
+    if (cplx)
+        hipLaunchKernelGGL(k_trd_tfactor<true>, dim3(blocks, nb), dim3(256), 0, s, V, tau, U, n);
+    else
+        hipLaunchKernelGGL(k_trd_tfactor<false>, dim3(blocks, nb), dim3(256), 0, s, V, tau, U, n);
+}
+
+void launch_real_to_field(const double *Z, double *X, long long count, bool cplx, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_real_to_field, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, Z,
+                       X, count, cplx ? 1 : 0);
+}
+
+}  // namespace olcao

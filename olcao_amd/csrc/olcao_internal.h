@@ -2,6 +2,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <vector>
 #include <stdint.h>
 
 namespace olcao {
@@ -186,5 +188,47 @@ void launch_small_hegv(bool cplx, int n, int nb, double *A, double *B, double *w
                        hipStream_t s);
 void launch_extract_evecs(const double *A, int n, int nk, int numStates,
                           int cplx, double *out, hipStream_t s);
+
+// Cooperative tridiagonal reduction with the matrix resident in LDS across a
+// group of workgroups (ktrd.hip).  A: full Hermitian (lower triangle read);
+// V: explicit reflectors, column i = v_i; d, e: [nb][n]; tau: [nb][n]
+// (complex for cplx); info[k] = -2 when a group barrier timed out.
+constexpr int kTrdBlock = 32;   // reflectors per back-transform block
+struct TrdPlan {
+    int G = 0;            // workgroups per matrix
+    int nGroups = 0;      // matrices in flight
+    int flavor = 0;       // 0: fenced barrier, 1: atomic publications
+    size_t lds = 0;       // dynamic LDS bytes per workgroup
+    size_t pubDoubles = 0;
+    size_t counters = 0;  // unsigned barrier counters (zeroed before launch)
+};
+bool trd_plan(int n, int nb, bool cplx, int flavor, TrdPlan &p);
+hipError_t launch_trd(bool cplx, const TrdPlan &p, int n, int nb, const double *A, double *V,
+                      double *d, double *e, double *tau, double *pub, unsigned *cnt, int *info,
+                      hipStream_t s);
+// U_b = V_b T_b per block of kTrdBlock reflectors (rows j0+1..n-1 of columns
+// j0..j0+kTrdBlock-1 of U)
+void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double *tau, double *U,
+                        hipStream_t s);
+void launch_real_to_field(const double *Z, double *X, long long count, bool cplx, hipStream_t s);
+
+// Batched divide and conquer eigensolver for real symmetric tridiagonal
+// matrices (kdc.hip): D (diagonal in, eigenvalues out, ascending), E
+// (couplings), Q (eigenvectors out, column-major n x n per matrix); Qt, W, S:
+// n x n work per matrix, ord: n ints per matrix.
+constexpr int kDcMaxN = 64 * 10;
+struct DcLevel {
+    int nodeBase = 0, nNodes = 0, tileBase = 0, nTiles = 0;
+};
+struct DcPlan {
+    int n = 0;
+    std::vector<int4> nodes;   // (offset, first child size, size, -) by height
+    std::vector<int4> tiles;   // (node, row tile, column tile, -)
+    std::vector<DcLevel> levels;
+};
+void dc_plan(int n, DcPlan &p);
+void launch_dc(const DcPlan &p, const int4 *dNodes, const int4 *dTiles, int nb, double *D,
+               const double *E, double *Q, double *Qt, double *W, double *S, int *ord,
+               hipStream_t s);
 
 }  // namespace olcao

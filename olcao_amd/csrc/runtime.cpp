@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -83,6 +84,16 @@ struct DBuf {
         n = 0;
     }
     ~DBuf() { release(); }
+};
+
+// Buffers of the tridiagonal eigensolver path, kept between calls.
+struct EigWork {
+    DBuf<double> V, U, W, tau, pub;        // reflectors, block factors, GEMM work
+    DBuf<double> Z, Zt, Wd, Sd;             // divide and conquer: vectors, work
+    DBuf<unsigned> cnt;
+    DBuf<int> ord;
+    DcPlan dc;
+    DBuf<int4> dcNodes, dcTiles;
 };
 
 double angnorm(int lm)   // lm = 1..16, basis.f90:86-101
@@ -164,6 +175,7 @@ struct olcao_ctx {
     DevCtx dev{};
     ClassLists cls{};
     bool eigLibrary = false;     // always rocSOLVER, also where the LDS solver fits
+    EigWork eig;
     ContractPlan cplan{};
     DBuf<int4> dSegs, dMulti;
     DBuf<double> dPartial;
@@ -919,14 +931,93 @@ void ensure_mats(olcao_ctx &c)
     }
 }
 
+// C = Y diag(w) Y^H for the Hermitian C in A (full): the cooperative
+// tridiagonal reduction C = Q T Q^H (ktrd.hip), the batched divide and
+// conquer T = Z diag(w) Z^T (kdc.hip), and Y = Q Z by blocks of reflectors,
+// X <- X - U_b (V_b^H X), with batched GEMMs.  D receives w, A receives Y,
+// dinfo the reduction's status.
+void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, bool cplx, int n,
+             int nb, double *A, double *D, double *E, int *dinfo)
+{
+    const int el = cplx ? 2 : 1;
+    const size_t per = (size_t)n * n;
+    ew.V.alloc(per * el * nb);
+    ew.U.alloc(per * el * nb);
+    ew.W.alloc((size_t)kTrdBlock * n * el * nb);
+    ew.tau.alloc((size_t)n * el * nb);
+    ew.pub.alloc(plan.pubDoubles);
+    ew.cnt.alloc(plan.counters);
+    ew.Z.alloc(per * nb);
+    ew.Zt.alloc(per * nb);
+    ew.Wd.alloc(per * nb);
+    ew.Sd.alloc(per * nb);
+    ew.ord.alloc((size_t)n * nb);
+    if (ew.dc.n != n) {
+        dc_plan(n, ew.dc);
+        ew.dcNodes.upload(ew.dc.nodes, s);
+        ew.dcTiles.upload(ew.dc.tiles, s);
+    }
+    HIPCHK(hipMemsetAsync(ew.cnt.p, 0, plan.counters * sizeof(unsigned), s));
+    HIPCHK(launch_trd(cplx, plan, n, nb, A, ew.V.p, D, E, ew.tau.p, ew.pub.p, ew.cnt.p, dinfo, s));
+    launch_dc(ew.dc, ew.dcNodes.p, ew.dcTiles.p, nb, D, E, ew.Z.p, ew.Zt.p, ew.Wd.p, ew.Sd.p,
+              ew.ord.p, s);
+    HIPCHK(hipGetLastError());
+    launch_trd_tfactor(cplx, n, nb, ew.V.p, ew.tau.p, ew.U.p, s);
+    HIPCHK(hipGetLastError());
+    launch_real_to_field(ew.Z.p, A, (long long)per * nb, cplx, s);
+    HIPCHK(hipGetLastError());
+    const rocblas_stride st = (rocblas_stride)per, stW = (rocblas_stride)kTrdBlock * n;
+    const int nref = n - 1;
+    for (int j0 = ((nref - 1) / kTrdBlock) * kTrdBlock; j0 >= 0; j0 -= kTrdBlock) {
+        const int mb = std::min(kTrdBlock, nref - j0), m = n - 1 - j0;
+        const size_t offV = (size_t)(j0 + 1) + (size_t)j0 * n, offX = (size_t)(j0 + 1);
+        rocblas_status r;
+        if (cplx) {
+            const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0}, mone = {-1.0, 0.0};
+            auto *Vz = (const rocblas_double_complex *)ew.V.p, *Uz = (const rocblas_double_complex *)ew.U.p;
+            auto *Xz = (rocblas_double_complex *)A, *Wz = (rocblas_double_complex *)ew.W.p;
+            r = rocblas_zgemm_strided_batched(h, rocblas_operation_conjugate_transpose,
+                                              rocblas_operation_none, mb, n, m, &one, Vz + offV, n,
+                                              st, Xz + offX, n, st, &zero, Wz, kTrdBlock, stW, nb);
+            if (r == rocblas_status_success)
+                r = rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m,
+                                                  n, mb, &mone, Uz + offV, n, st, Wz, kTrdBlock, stW,
+                                                  &one, Xz + offX, n, st, nb);
+        } else {
+            const double one = 1.0, zero = 0.0, mone = -1.0;
+            r = rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none,
+                                              mb, n, m, &one, ew.V.p + offV, n, st, A + offX, n, st,
+                                              &zero, ew.W.p, kTrdBlock, stW, nb);
+            if (r == rocblas_status_success)
+                r = rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m,
+                                                  n, mb, &mone, ew.U.p + offV, n, st, ew.W.p, kTrdBlock,
+                                                  stW, &one, A + offX, n, st, nb);
+        }
+        if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "back-transform gemm status %d", (int)r);
+    }
+}
+
+// Which solver the tridiagonal path would use for (n, nb), or false.
+bool trd_fits(int n, int nb, bool cplx, TrdPlan &plan)
+{
+    static const int flavor = [] {
+        const char *f = std::getenv("OLCAO_TRD_FLAVOR");
+        return f && f[0] == '1' ? 1 : 0;
+    }();
+    return trd_plan(n, nb, cplx, flavor, plan);
+}
+
 // Batched generalised eigensolve A x = w B x from the upper triangles
 // (what zhegv/dsygv with itype 1, 'V', 'U' compute, interfaces.F90:73-200):
 // B = U^H U (potrf), C = U^-H A U^-1 (two BLAS-3 triangular solves on the
-// Hermitian-filled A), C = Y diag(w) Y^H (heevd), X = U^-1 Y.  A is replaced
-// by the eigenvectors (all n columns) and B by U.  info[k] > n flags a B(k)
-// that is not positive definite, as zhegv does.
-void gen_eig(rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *A, double *B,
-             double *D, double *E, int *dinfo, std::vector<int> &info, bool library)
+// Hermitian-filled A), C = Y diag(w) Y^H, X = U^-1 Y.  A is replaced by the
+// eigenvectors (all n columns) and B by U.  info[k] > n flags a B(k) that is
+// not positive definite, as zhegv does.  C = Y diag(w) Y^H is the one-
+// workgroup LDS solver for small n, else the tridiagonal path (trd_eig) where
+// the matrix fits its registers and LDS, else rocSOLVER's heevd; library
+// selects heevd always.
+void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *A,
+             double *B, double *D, double *E, int *dinfo, std::vector<int> &info, bool library)
 {
     const rocblas_stride st = (rocblas_stride)n * n;
     info.assign(nb, 0);
@@ -955,6 +1046,8 @@ void gen_eig(rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *
             return;
         }
     launch_herm_fill(A, n, nb, cplx ? 2 : 1, s);
+    TrdPlan plan;
+    const bool useTrd = !library && trd_fits(n, nb, cplx, plan);
     if (cplx) {
         const rocblas_double_complex one = {1.0, 0.0};
         auto *Az = (rocblas_double_complex *)A;
@@ -968,8 +1061,11 @@ void gen_eig(rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n,
                                               n, &one, Bz, n, st, Az, n, st, nb);
         if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "trsm status %d", (int)r);
-        r = rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n, Az,
-                                             n, st, D, n, E, n, dinfo, nb);
+        if (useTrd)
+            trd_eig(ew, plan, h, s, true, n, nb, A, D, E, dinfo);
+        else
+            r = rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n, Az,
+                                                 n, st, D, n, E, n, dinfo, nb);
         if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "heevd status %d", (int)r);
         r = rocblas_ztrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper,
                                           rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
@@ -984,8 +1080,11 @@ void gen_eig(rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n,
                                               n, &one, B, n, st, A, n, st, nb);
         if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "trsm status %d", (int)r);
-        r = rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n, A, n,
-                                             st, D, n, E, n, dinfo, nb);
+        if (useTrd)
+            trd_eig(ew, plan, h, s, false, n, nb, A, D, E, dinfo);
+        else
+            r = rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n, A,
+                                                 n, st, D, n, E, n, dinfo, nb);
         if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "syevd status %d", (int)r);
         r = rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper,
                                           rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
@@ -994,6 +1093,9 @@ void gen_eig(rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *
     if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "trsm status %d", (int)r);
     HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    if (useTrd)
+        for (int k = 0; k < nb; ++k)
+            if (info[k] == -2) info[k] = 1;   // a group barrier timed out
 }
 
 // Batched generalised eigensolve of H, S (upper) on the device.
@@ -1011,7 +1113,7 @@ void solve_device(olcao_ctx &c, int numStates, const double *Hsrc, const double 
     HIPCHK(hipMemcpyAsync(c.Aw.p, Hsrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
     std::vector<int> info;
-    gen_eig(c.blas, s, !c.gamma, n, nk, c.Aw.p, c.Bw.p, c.D.p, c.E.p, c.dInfo.p, info,
+    gen_eig(c.eig, c.blas, s, !c.gamma, n, nk, c.Aw.p, c.Bw.p, c.D.p, c.E.p, c.dInfo.p, info,
             c.eigLibrary);
     std::vector<double> w((size_t)n * nk);
     HIPCHK(hipMemcpyAsync(w.data(), c.D.p, w.size() * 8, hipMemcpyDeviceToHost, s));
@@ -1348,7 +1450,8 @@ static olcao_status solve_host(int n, int nb, double *A, double *B, double *w, b
         HIPCHK(hipMemcpyAsync(dA.p, A, per * nb * 8, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(dB.p, B, per * nb * 8, hipMemcpyHostToDevice, s));
         std::vector<int> info;
-        gen_eig(h, s, cplx, n, nb, dA.p, dB.p, dD.p, dE.p, dI.p, info, false);
+        EigWork ew;
+        gen_eig(ew, h, s, cplx, n, nb, dA.p, dB.p, dD.p, dE.p, dI.p, info, false);
         HIPCHK(hipMemcpyAsync(A, dA.p, per * nb * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(w, dD.p, (size_t)n * nb * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));

@@ -250,6 +250,82 @@ def test_drop_in_solvers(gpu):
         assert np.allclose(wr[k], eigh(Ar[k], Br[k], eigvals_only=True), atol=1e-9)
 
 
+def _pencil(rng, n, nb, cplx, degenerate):
+    """Random Hermitian pencils (A, B), B positive definite; with
+    degenerate, A has every eigenvalue (of B^-1 A) three times over."""
+    def rnd(*shape):
+        x = rng.normal(size=shape)
+        return x + 1j * rng.normal(size=shape) if cplx else x
+    Z = rnd(nb, n, n)
+    B = Z @ np.conj(Z.transpose(0, 2, 1)) / n + np.eye(n)
+    if degenerate:
+        L = np.linalg.cholesky(B)
+        Q, _ = np.linalg.qr(rnd(nb, n, n))
+        lam = np.repeat(rng.normal(size=(nb, (n + 2) // 3)), 3, axis=1)[:, :n]
+        C = Q @ (lam[:, :, None] * np.conj(Q.transpose(0, 2, 1)))
+        A = L @ C @ np.conj(L.transpose(0, 2, 1))
+        A = 0.5 * (A + np.conj(A.transpose(0, 2, 1)))
+    else:
+        A = rnd(nb, n, n)
+        A = A + np.conj(A.transpose(0, 2, 1))
+    return A, B
+
+
+@pytest.mark.parametrize("n,nb,cplx,degenerate", [(100, 3, True, False), (331, 2, True, True),
+                                                  (576, 9, True, False), (200, 2, False, False),
+                                                  (640, 2, False, True)])
+def test_tridiagonal_path_solvers(gpu, n, nb, cplx, degenerate):
+    """Sizes past the one-workgroup LDS solver take the cooperative
+    tridiagonal reduction: eigenvalues against LAPACK (scipy eigh) within
+    1e-10 of the spectrum's scale, residuals |A x - w B x| and
+    |X^H B X - I| within 1e-9 (relative), for random and for threefold
+    degenerate spectra, complex and real, and batches larger than the
+    matrices in flight at once (576 x 9)."""
+    from scipy.linalg import eigh
+    _log(f"tridiagonal path n={n} nb={nb} cplx={cplx} degenerate={degenerate}")
+    A, B = _pencil(np.random.default_rng(n + nb), n, nb, cplx, degenerate)
+    solve = gpu.solveZHEGV if cplx else gpu.solveDSYGV
+    w, X = solve(np.triu(A), np.triu(B))
+    for k in range(nb):
+        wr = eigh(A[k], B[k], eigvals_only=True)
+        scale = np.abs(wr).max()
+        assert np.abs(w[k] - wr).max() < 1e-10 * scale, (k, np.abs(w[k] - wr).max())
+        R = A[k] @ X[k] - B[k] @ X[k] * w[k]
+        assert np.abs(R).max() < 1e-9 * scale * np.abs(B[k]).max()
+        O = np.conj(X[k].T) @ B[k] @ X[k]
+        assert np.abs(O - np.eye(n)).max() < 1e-9
+
+
+def _tridiagonal_cases(rng):
+    w21 = np.abs(np.arange(21) - 10.0)
+    yield "glued Wilkinson", np.tile(w21, 7), np.concatenate([np.ones(20), [1e-8]] * 7)[:-1]
+    yield "graded", 10.0 ** rng.uniform(-3, 3, 150), 10.0 ** rng.uniform(-3, 1, 149)
+    yield "tiny couplings", np.repeat(rng.normal(size=50), 3), np.full(149, 1e-15)
+    yield "wide spread", np.concatenate([rng.normal(size=300), 1e3 * rng.random(276)]), rng.normal(size=575)
+
+
+def test_tridiagonal_divide_and_conquer_cases(gpu):
+    """Tridiagonal A with B = I through the real drop-in solver: the
+    reduction returns A's own tridiagonal form, so the divide and conquer
+    eigensolver meets its hard cases directly: glued Wilkinson matrices
+    (close pairs, rotation deflation), graded entries, negligible couplings
+    with repeated diagonal entries (small-z deflation) and a wide spectrum.
+    Eigenvalues against scipy within 1e-12 of the scale; residual and
+    orthogonality within 1e-11."""
+    from scipy.linalg import eigh_tridiagonal
+    rng = np.random.default_rng(7)
+    for name, d, e in _tridiagonal_cases(rng):
+        n = len(d)
+        _log(f"tridiagonal case {name} n={n}")
+        A = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+        w, X = gpu.solveDSYGV(np.triu(A)[None], np.eye(n)[None])
+        wr = eigh_tridiagonal(d, e, eigvals_only=True)
+        scale = np.abs(wr).max()
+        assert np.abs(w[0] - wr).max() < 1e-12 * scale, (name, np.abs(w[0] - wr).max())
+        assert np.abs(A @ X[0] - X[0] * w[0]).max() < 1e-11 * scale, name
+        assert np.abs(X[0].T @ X[0] - np.eye(n)).max() < 1e-11, name
+
+
 def test_error_paths(gpu):
     # indefinite overlap -> NOT_SPD, the condition on which the reference stops
     with pytest.raises(gpu.OlcaoError) as e:
@@ -326,11 +402,13 @@ def test_mixed_precision_form_tolerances(gpu, make):
     gmx.close()
 
 
-@pytest.mark.parametrize("make", ["diamond_c8", "quartz"])
+@pytest.mark.parametrize("make", ["diamond_c8", "quartz", "si64"])
 def test_lds_eigensolver_matches_library(gpu, make):
-    """The batched in-LDS generalised eigensolver (one workgroup per k:
-    Cholesky, reduction, Jacobi) against rocSOLVER's zpotrf/ztrsm/zheevd on
-    the same device-resident H(k), S(k): all eigenvalues within 1e-6 eV."""
+    """The device eigensolvers against rocSOLVER's zpotrf/ztrsm/zheevd on
+    the same device-resident H(k), S(k): the batched in-LDS solver (one
+    workgroup per k: Cholesky, reduction, Jacobi) for diamond and quartz, the
+    cooperative tridiagonal path for Si64 (n = 576): all eigenvalues within
+    1e-6 eV."""
     s = getattr(configs, make)()
     g1 = gpu.OlcaoGPU(s)
     g2 = gpu.OlcaoGPU(s, eigensolver=gpu.EIG_LIBRARY)
