@@ -35,6 +35,7 @@ UNITS = [("kernels.hip", "kernels.o", []),
          ("ksolve.hip", "ksolve.o", []),
          ("ktrd.hip", "ktrd.o", []),
          ("kdc.hip", "kdc.o", []),
+         ("kchol.hip", "kchol.o", []),
          ("runtime.cpp", "runtime.o", ["-x", "hip"])]
 
 

@@ -31,12 +31,13 @@ namespace olcao {
 
 namespace {
 
-constexpr int kDcThreads = 1024;
-constexpr int kDcWaves = kDcThreads / 64;
-constexpr int kDcMax = kDcMaxN;
 constexpr double kEps = 1.1102230246251565e-16;   // 2^-53
+constexpr int kDcMax = kDcMaxN;
 constexpr int kSecularIters = 64;
 constexpr int kTile = 64;
+constexpr int kDcBig = 1024;     // threads of a merge of more than kDcSmallM
+constexpr int kDcSmall = 256;
+constexpr int kDcSmallM = 128;
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -50,9 +51,27 @@ __device__ __forceinline__ double wave_max(double v)
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
 }
+// Four wave sums in seven exchanges: halves, then quarters, hold one sum each.
+__device__ __forceinline__ void wave_sum4(double &a, double &b, double &c, double &d)
+{
+    const int lane = threadIdx.x & 63;
+    const bool lo32 = lane < 32, lo16 = (lane & 16) == 0;
+    double x1 = lo32 ? a : c, x2 = lo32 ? b : d;
+    const double y1 = lo32 ? c : a, y2 = lo32 ? d : b;
+    x1 += __shfl_xor(y1, 32, 64);
+    x2 += __shfl_xor(y2, 32, 64);
+    double v = lo16 ? x1 : x2;
+    v += __shfl_xor(lo16 ? x2 : x1, 16, 64);
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    a = __shfl(v, 0, 64);
+    b = __shfl(v, 16, 64);
+    c = __shfl(v, 32, 64);
+    d = __shfl(v, 48, 64);
+}
 
 // Block-wide reduction through LDS (all threads call it).
-template <bool MAX>
+template <bool MAX, int NT>
 __device__ double block_reduce(double v, double *red)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -60,9 +79,38 @@ __device__ double block_reduce(double v, double *red)
     __syncthreads();
     if (lane == 0) red[wv] = v;
     __syncthreads();
-    double r = MAX ? 0.0 : 0.0;
-    for (int q = 0; q < kDcWaves; ++q) r = MAX ? fmax(r, red[q]) : r + red[q];
+    double r = 0.0;
+    for (int q = 0; q < NT / 64; ++q) r = MAX ? fmax(r, red[q]) : r + red[q];
     return r;
+}
+
+// Exclusive prefix count of flags over [0, m) in LDS (m <= kDcMax): per-thread
+// segments, a wave scan of the segment sums, then the waves' totals.
+template <int NT>
+__device__ void block_scan(const int *flag, int *pre, int m, int *red)
+{
+    constexpr int kSeg = (kDcMax + NT - 1) / NT;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int b0 = t * kSeg, b1 = min(m, b0 + kSeg);
+    int sum = 0;
+    for (int p = b0; p < b1; ++p) sum += flag[p];
+    int inc = sum;   // inclusive scan within the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+    }
+    __syncthreads();
+    if (lane == 63) red[wv] = inc;
+    __syncthreads();
+    int base = 0;
+    for (int q = 0; q < wv; ++q) base += red[q];
+    int run = base + inc - sum;
+    for (int p = b0; p < b1; ++p) {
+        pre[p] = run;
+        run += flag[p];
+    }
+    __syncthreads();
 }
 
 __global__ void k_dc_init(double *Dall, const double *Eall, double *Qall, int n)
@@ -85,10 +133,12 @@ __global__ void k_dc_init(double *Dall, const double *Eall, double *Qall, int n)
     }
 }
 
-__global__ __launch_bounds__(kDcThreads) void k_dc_merge(const int4 *nodes, int n, double *Dall,
-                                                         const double *Eall, const double *Qall,
-                                                         double *Wall, double *Sall, int *ordAll)
+template <int NT>
+__global__ __launch_bounds__(NT) void k_dc_merge(const int4 *nodes, int n, double *Dall,
+                                                 const double *Eall, const double *Qall,
+                                                 double *Wall, double *Sall, int *ordAll)
 {
+    constexpr int NW = NT / 64;
     const int4 nd = nodes[blockIdx.x];
     const int off = nd.x, m1 = nd.y, m = nd.z;
     const int mat = blockIdx.y;
@@ -106,26 +156,27 @@ __global__ __launch_bounds__(kDcThreads) void k_dc_merge(const int4 *nodes, int 
     __shared__ int order[kDcMax], keep[kDcMax], rowKeep[kDcMax], defl[kDcMax], colInfo[kDcMax];
     __shared__ int rotP[kDcMax], rotQ[kDcMax];
     __shared__ double rotC[kDcMax], rotS[kDcMax];
-    __shared__ double red[kDcWaves];
-    __shared__ int sK, sNd, sNr;
+    __shared__ double red[NW];
+    __shared__ int ired[NW];
+    __shared__ int sK, sNd, sNr, sClose, sUnsorted;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const double beta = E[off + m1 - 1];
     const double sgn = beta < 0.0 ? -1.0 : 1.0;
 
     // children's eigenvalues and z = (last row of Q1, sign(beta) first row of Q2)
     double nzp = 0.0;
-    for (int i = tid; i < m; i += kDcThreads) {
+    for (int i = tid; i < m; i += NT) {
         sd[i] = D[off + i];
         const double v = i < m1 ? Q[(size_t)(off + m1 - 1) + (size_t)n * (off + i)]
                                 : sgn * Q[(size_t)(off + m1) + (size_t)n * (off + i)];
         sz[i] = v;
         nzp += v * v;
     }
-    const double nz2 = block_reduce<false>(nzp, red);
+    const double nz2 = block_reduce<false, NT>(nzp, red);
     const double nz = sqrt(nz2);
     const double rho = fabs(beta) * nz2;
     // merge order of the two sorted children (ties: first child first)
-    for (int i = tid; i < m; i += kDcThreads) {
+    for (int i = tid; i < m; i += NT) {
         const double v = sd[i];
         int lo, hi;
         if (i < m1) {   // count second-child entries < v
@@ -148,73 +199,100 @@ __global__ __launch_bounds__(kDcThreads) void k_dc_merge(const int4 *nodes, int 
     }
     __syncthreads();
     double dmx = 0.0, zmx = 0.0;
-    for (int p = tid; p < m; p += kDcThreads) {
+    for (int p = tid; p < m; p += NT) {
         ds[p] = sd[order[p]];
         zs[p] = nz > 0.0 ? sz[order[p]] / nz : 0.0;
         dmx = fmax(dmx, fabs(ds[p]));
         zmx = fmax(zmx, fabs(zs[p]));
     }
-    dmx = block_reduce<true>(dmx, red);
-    zmx = block_reduce<true>(zmx, red);
+    dmx = block_reduce<true, NT>(dmx, red);
+    zmx = block_reduce<true, NT>(zmx, red);
     const double tol = 8.0 * kEps * fmax(dmx, zmx);
+    const bool allDefl = !(rho * zmx > tol);
 
-    // deflation scan (dlaed2's rules), positions in the sorted order
-    if (tid == 0) {
-        int K = 0, nd = 0, nr = 0;
-        if (!(rho * zmx > tol)) {
-            for (int p = 0; p < m; ++p) defl[nd++] = p;
-        } else {
-            int pj = -1;
-            for (int nj = 0; nj < m; ++nj) {
-                if (rho * fabs(zs[nj]) <= tol) {
-                    defl[nd++] = nj;
-                    continue;
-                }
-                if (pj < 0) {
-                    pj = nj;
-                    continue;
-                }
-                double s = zs[pj], c = zs[nj];
-                const double tau = sqrt(c * c + s * s);
-                const double t = ds[nj] - ds[pj];
-                c /= tau;
-                s = -s / tau;
-                if (fabs(t * c * s) <= tol) {
-                    zs[nj] = tau;
-                    zs[pj] = 0.0;
-                    rotP[nr] = pj;
-                    rotQ[nr] = nj;
-                    rotC[nr] = c;
-                    rotS[nr] = s;
-                    ++nr;
-                    const double t2 = ds[pj] * c * c + ds[nj] * s * s;
-                    ds[nj] = ds[pj] * s * s + ds[nj] * c * c;
-                    ds[pj] = t2;
-                    defl[nd++] = pj;
-                } else {
-                    keep[K++] = pj;
-                }
-                pj = nj;
-            }
-            if (pj >= 0) keep[K++] = pj;
+    // Deflation (dlaed2's rules) in the sorted order.  Parallel pass: small
+    // rho z(p) entries, and whether any two consecutive remaining poles are
+    // close enough to need a rotation; only then the sequential scan.
+    int *big = colInfo, *pre = rowKeep;   // scratch until their own use below
+    for (int p = tid; p < m; p += NT) big[p] = !allDefl && rho * fabs(zs[p]) > tol;
+    if (tid == 0) sClose = 0;
+    block_scan<NT>(big, pre, m, ired);
+    const int nBig = m > 0 ? pre[m - 1] + big[m - 1] : 0;
+    for (int p = tid; p < m; p += NT)
+        if (big[p]) keep[pre[p]] = p;
+    __syncthreads();
+    for (int q = tid + 1; q < nBig; q += NT) {
+        const int pj = keep[q - 1], nj = keep[q];
+        double s = zs[pj], c = zs[nj];
+        const double tau = sqrt(c * c + s * s);
+        c /= tau;
+        s = -s / tau;
+        if (fabs((ds[nj] - ds[pj]) * c * s) <= tol) sClose = 1;
+    }
+    __syncthreads();
+    if (!sClose) {
+        for (int p = tid; p < m; p += NT)
+            if (!big[p]) defl[p - pre[p]] = p;
+        if (tid == 0) {
+            sK = nBig;
+            sNd = m - nBig;
+            sNr = 0;
         }
+    } else if (tid == 0) {
+        int K = 0, nd = 0, nr = 0;
+        int pj = -1;
+        for (int nj = 0; nj < m; ++nj) {
+            if (!big[nj]) {
+                defl[nd++] = nj;
+                continue;
+            }
+            if (pj < 0) {
+                pj = nj;
+                continue;
+            }
+            double s = zs[pj], c = zs[nj];
+            const double tau = sqrt(c * c + s * s);
+            const double t = ds[nj] - ds[pj];
+            c /= tau;
+            s = -s / tau;
+            if (fabs(t * c * s) <= tol) {
+                zs[nj] = tau;
+                zs[pj] = 0.0;
+                rotP[nr] = pj;
+                rotQ[nr] = nj;
+                rotC[nr] = c;
+                rotS[nr] = s;
+                ++nr;
+                const double t2 = ds[pj] * c * c + ds[nj] * s * s;
+                ds[nj] = ds[pj] * s * s + ds[nj] * c * c;
+                ds[pj] = t2;
+                defl[nd++] = pj;
+            } else {
+                keep[K++] = pj;
+            }
+            pj = nj;
+        }
+        if (pj >= 0) keep[K++] = pj;
         sK = K;
         sNd = nd;
         sNr = nr;
     }
     __syncthreads();
     const int K = sK, Nd = sNd, Nr = sNr;
-    for (int p = tid; p < m; p += kDcThreads) rowKeep[p] = -1;
+    for (int p = tid; p < m; p += NT) rowKeep[p] = -1;
+    if (tid == 0) sUnsorted = 0;
     __syncthreads();
-    for (int ii = tid; ii < K; ii += kDcThreads) {
+    for (int ii = tid; ii < K; ii += NT) {
         dk[ii] = ds[keep[ii]];
         zk[ii] = zs[keep[ii]];
         rowKeep[keep[ii]] = ii;
     }
+    for (int q = tid; q + 1 < Nd; q += NT)
+        if (ds[defl[q]] > ds[defl[q + 1]]) sUnsorted = 1;
     __syncthreads();
 
     // secular equation, one wave per root: Delta(i, j) = dk(i) - lambda_j into S
-    for (int j = wv; j < K; j += kDcWaves) {
+    for (int j = wv; j < K; j += NW) {
         int org;
         double a, b;
         if (j < K - 1) {
@@ -243,7 +321,8 @@ __global__ __launch_bounds__(kDcThreads) void k_dc_merge(const int4 *nodes, int 
         const double dj1 = j < K - 1 ? dk[j + 1] - dorg : 0.0;
         double tau = 0.5 * (a + b);
         for (int it = 0; it < kSecularIters; ++it) {
-            double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, sabs = 0.0;
+            // left poles (ii <= j) give negative terms, right poles positive
+            double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
             for (int ii = lane; ii < K; ii += 64) {
                 const double r = 1.0 / ((dk[ii] - dorg) - tau);
                 const double t = rho * zk[ii] * zk[ii] * r;
@@ -254,16 +333,12 @@ __global__ __launch_bounds__(kDcThreads) void k_dc_merge(const int4 *nodes, int 
                     phi += t;
                     dphi += t * r;
                 }
-                sabs += fabs(t);
             }
-            psi = wave_sum(psi);
-            dpsi = wave_sum(dpsi);
-            phi = wave_sum(phi);
-            dphi = wave_sum(dphi);
-            sabs = wave_sum(sabs);
+            wave_sum4(psi, dpsi, phi, dphi);
             const double g = 1.0 + psi + phi;
             if (g < 0.0) a = tau; else b = tau;
-            if (b - a <= 4.0 * kEps * fmax(fabs(a), fabs(b)) || fabs(g) <= 8.0 * kEps * (1.0 + sabs))
+            if (b - a <= 4.0 * kEps * fmax(fabs(a), fabs(b)) ||
+                fabs(g) <= 8.0 * kEps * (1.0 + phi - psi))
                 break;
             // two-pole rational model through (psi, psi') and (phi, phi')
             const double dl = dj - tau;
@@ -315,27 +390,30 @@ __global__ __launch_bounds__(kDcThreads) void k_dc_merge(const int4 *nodes, int 
     __syncthreads();
 
     // Gu-Eisenstat: zhat(i)^2 = (lambda_i - d_i)/rho prod_{j!=i} (lambda_j - d_i)/(d_j - d_i)
-    for (int ii = tid; ii < K; ii += kDcThreads) {
+    for (int ii = tid; ii < K; ii += NT) {
         double prod = -Sat(ii, ii) / rho;
         for (int j = 0; j < K; ++j)
             if (j != ii) prod *= -Sat(ii, j) / (dk[j] - dk[ii]);
         zh[ii] = copysign(sqrt(fmax(prod, 0.0)), zk[ii]);
     }
-    // deflated values in ascending order (rank sort), staged in sd (values)
-    // and order-free slots of sz (positions)
-    for (int q = tid; q < Nd; q += kDcThreads) {
+    // deflated values ascending into sd (values) and sz (positions): as
+    // scanned unless a rotation moved one out of order
+    for (int q = tid; q < Nd; q += NT) {
         const double v = ds[defl[q]];
-        int rank = 0;
-        for (int q2 = 0; q2 < Nd; ++q2) {
-            const double u = ds[defl[q2]];
-            rank += (u < v) || (u == v && q2 < q);
+        int rank = q;
+        if (sUnsorted) {
+            rank = 0;
+            for (int q2 = 0; q2 < Nd; ++q2) {
+                const double u = ds[defl[q2]];
+                rank += (u < v) || (u == v && q2 < q);
+            }
         }
         sd[rank] = v;
         sz[rank] = (double)defl[q];
     }
     __syncthreads();
     // final column of every eigenpair: roots before equal deflated values
-    for (int j = tid; j < K; j += kDcThreads) {
+    for (int j = tid; j < K; j += NT) {
         const double v = lam[j];
         int lo = 0, hi = Nd;
         while (lo < hi) {
@@ -345,7 +423,7 @@ __global__ __launch_bounds__(kDcThreads) void k_dc_merge(const int4 *nodes, int 
         colInfo[j + lo] = j;
         D[off + j + lo] = v;
     }
-    for (int q = tid; q < Nd; q += kDcThreads) {
+    for (int q = tid; q < Nd; q += NT) {
         const double v = sd[q];
         int lo = 0, hi = K;
         while (lo < hi) {
@@ -356,21 +434,27 @@ __global__ __launch_bounds__(kDcThreads) void k_dc_merge(const int4 *nodes, int 
         D[off + q + lo] = v;
     }
     __syncthreads();
-    // W, one wave per column
-    for (int s = wv; s < m; s += kDcWaves) {
+    // W, one wave per column: kept rows u(i) = zhat(i) / Delta(i, j),
+    // normalised; deflated rows zero, or the unit entry of a deflated column
+    for (int s = wv; s < m; s += NW) {
         const int info = colInfo[s];
         if (info >= 0) {
             const int j = info;
+            double u[kDcMax / 64];
             double part = 0.0;
-            for (int ii = lane; ii < K; ii += 64) {
-                const double u = zh[ii] / Sat(ii, j);
-                part += u * u;
+#pragma unroll
+            for (int q = 0; q < kDcMax / 64; ++q) {
+                const int ii = lane + 64 * q;
+                u[q] = ii < K ? zh[ii] / Sat(ii, j) : 0.0;
+                part += u[q] * u[q];
             }
             const double inv = 1.0 / sqrt(wave_sum(part));
-            for (int p = lane; p < m; p += 64) {
-                const int ii = rowKeep[p];
-                Wat(p, s) = ii >= 0 ? zh[ii] / Sat(ii, j) * inv : 0.0;
+#pragma unroll
+            for (int q = 0; q < kDcMax / 64; ++q) {
+                const int ii = lane + 64 * q;
+                if (ii < K) Wat(keep[ii], s) = u[q] * inv;
             }
+            for (int q = lane; q < Nd; q += 64) Wat(defl[q], s) = 0.0;
         } else {
             const int pd = -1 - info;
             for (int p = lane; p < m; p += 64) Wat(p, s) = p == pd ? 1.0 : 0.0;
@@ -380,7 +464,7 @@ __global__ __launch_bounds__(kDcThreads) void k_dc_merge(const int4 *nodes, int 
     // deflation rotations on W's rows, last first; a chain (p1,p2),(p2,p3),...
     // keeps its running row in a register
     if (Nr > 0) {
-        for (int s = tid; s < m; s += kDcThreads) {
+        for (int s = tid; s < m; s += NT) {
             int carryRow = -1;
             double carry = 0.0;
             for (int q = Nr - 1; q >= 0; --q) {
@@ -401,7 +485,7 @@ __global__ __launch_bounds__(kDcThreads) void k_dc_merge(const int4 *nodes, int 
             Wat(carryRow, s) = carry;
         }
     }
-    for (int p = tid; p < m; p += kDcThreads) ord[off + p] = off + order[p];
+    for (int p = tid; p < m; p += NT) ord[off + p] = off + order[p];
 }
 
 // Qt(node) = Q(:, ord) W over node blocks; tiles: x node, y row tile, z col tile
@@ -506,6 +590,7 @@ void dc_plan(int n, DcPlan &p)
             for (int a = 0; a < nt; ++a)
                 for (int b2 = 0; b2 < nt; ++b2) p.tiles.push_back(make_int4(L.nodeBase + q, a, b2, 0));
             p.nodes.push_back(lv[q]);
+            L.maxM = std::max(L.maxM, m);
         }
         L.nTiles = (int)p.tiles.size() - L.tileBase;
         p.levels.push_back(L);
@@ -519,8 +604,12 @@ void launch_dc(const DcPlan &p, const int4 *dNodes, const int4 *dTiles, int nb, 
     const int n = p.n;
     hipLaunchKernelGGL(k_dc_init, dim3(64, nb), dim3(256), 0, s, D, E, Q, n);
     for (const DcLevel &L : p.levels) {
-        hipLaunchKernelGGL(k_dc_merge, dim3(L.nNodes, nb), dim3(kDcThreads), 0, s,
-                           dNodes + L.nodeBase, n, D, E, Q, W, S, ord);
+        if (L.maxM > kDcSmallM)
+            hipLaunchKernelGGL(k_dc_merge<kDcBig>, dim3(L.nNodes, nb), dim3(kDcBig), 0, s,
+                               dNodes + L.nodeBase, n, D, E, Q, W, S, ord);
+        else
+            hipLaunchKernelGGL(k_dc_merge<kDcSmall>, dim3(L.nNodes, nb), dim3(kDcSmall), 0, s,
+                               dNodes + L.nodeBase, n, D, E, Q, W, S, ord);
         hipLaunchKernelGGL(k_dc_gemm, dim3(L.nTiles, nb), dim3(256), 0, s, dTiles + L.tileBase,
                            dNodes, n, Q, W, ord, Qt);
         hipLaunchKernelGGL(k_dc_copy, dim3(L.nTiles, nb), dim3(256), 0, s, dTiles + L.tileBase,

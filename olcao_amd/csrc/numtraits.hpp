@@ -20,6 +20,7 @@ struct Num<true> {
     static __device__ T cmul(T a, T b) { return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x}; }   // conj(a) b
     static __device__ T scale(T a, double s) { return {a.x * s, a.y * s}; }
     static __device__ T conj(T a) { return {a.x, -a.y}; }
+    static __device__ T sel(bool c, T a, T b) { return {c ? a.x : b.x, c ? a.y : b.y}; }
     static __device__ double re(T a) { return a.x; }
     static __device__ double abs2(T a) { return a.x * a.x + a.y * a.y; }
     static __device__ T load(const double *p, long long i) { return {p[2 * i], p[2 * i + 1]}; }
@@ -40,6 +41,7 @@ struct Num<false> {
     static __device__ T cmul(T a, T b) { return a * b; }
     static __device__ T scale(T a, double s) { return a * s; }
     static __device__ T conj(T a) { return a; }
+    static __device__ T sel(bool c, T a, T b) { return c ? a : b; }
     static __device__ double re(T a) { return a; }
     static __device__ double abs2(T a) { return a * a; }
     static __device__ T load(const double *p, long long i) { return p[i]; }

@@ -197,12 +197,11 @@ constexpr int kTrdBlock = 32;   // reflectors per back-transform block
 struct TrdPlan {
     int G = 0;            // workgroups per matrix
     int nGroups = 0;      // matrices in flight
-    int flavor = 0;       // 0: fenced barrier, 1: atomic publications
     size_t lds = 0;       // dynamic LDS bytes per workgroup
     size_t pubDoubles = 0;
     size_t counters = 0;  // unsigned barrier counters (zeroed before launch)
 };
-bool trd_plan(int n, int nb, bool cplx, int flavor, TrdPlan &p);
+bool trd_plan(int n, int nb, bool cplx, TrdPlan &p);
 hipError_t launch_trd(bool cplx, const TrdPlan &p, int n, int nb, const double *A, double *V,
                       double *d, double *e, double *tau, double *pub, unsigned *cnt, int *info,
                       hipStream_t s);
@@ -212,13 +211,21 @@ void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double 
                         hipStream_t s);
 void launch_real_to_field(const double *Z, double *X, long long count, bool cplx, hipStream_t s);
 
+// Blocked batched Cholesky, diagonal step (kchol.hip): factors the updated
+// diagonal block [j0, j0 + jb) of B (upper, column-major) into U and writes
+// its inverse into Dinv (rows j0.., columns 0..jb-1; n x n per matrix);
+// info takes the first non-positive pivot column (1-based).
+constexpr int kCholBlock = 64;
+void launch_chol_diag(bool cplx, const double *B, double *U, double *Dinv, int n, int nb, int j0,
+                      int jb, int *info, hipStream_t s);
+
 // Batched divide and conquer eigensolver for real symmetric tridiagonal
 // matrices (kdc.hip): D (diagonal in, eigenvalues out, ascending), E
 // (couplings), Q (eigenvectors out, column-major n x n per matrix); Qt, W, S:
 // n x n work per matrix, ord: n ints per matrix.
 constexpr int kDcMaxN = 64 * 10;
 struct DcLevel {
-    int nodeBase = 0, nNodes = 0, tileBase = 0, nTiles = 0;
+    int nodeBase = 0, nNodes = 0, tileBase = 0, nTiles = 0, maxM = 0;
 };
 struct DcPlan {
     int n = 0;

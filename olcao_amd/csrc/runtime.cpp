@@ -90,6 +90,7 @@ struct DBuf {
 struct EigWork {
     DBuf<double> V, U, W, tau, pub;        // reflectors, block factors, GEMM work
     DBuf<double> Z, Zt, Wd, Sd;             // divide and conquer: vectors, work
+    DBuf<double> Uc, Dinv, Y;               // Cholesky factor, inverted blocks, solve work
     DBuf<unsigned> cnt;
     DBuf<int> ord;
     DcPlan dc;
@@ -997,25 +998,113 @@ void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, 
     }
 }
 
-// Which solver the tridiagonal path would use for (n, nb), or false.
-bool trd_fits(int n, int nb, bool cplx, TrdPlan &plan)
+// Batched GEMM on column-major n x n matrices (stride n^2), offsets in
+// elements: C = alpha op(A) op(B) + beta C, op = N or C (T for real).
+void gemm_nn(rocblas_handle h, bool cplx, bool ta, bool tb, int m, int nc, int k, double alpha,
+             const double *A, const double *B, double beta, double *C, int n, int nb)
 {
-    static const int flavor = [] {
-        const char *f = std::getenv("OLCAO_TRD_FLAVOR");
-        return f && f[0] == '1' ? 1 : 0;
-    }();
-    return trd_plan(n, nb, cplx, flavor, plan);
+    const rocblas_stride st = (rocblas_stride)n * n;
+    rocblas_status r;
+    if (cplx) {
+        const rocblas_double_complex al = {alpha, 0.0}, be = {beta, 0.0};
+        r = rocblas_zgemm_strided_batched(
+            h, ta ? rocblas_operation_conjugate_transpose : rocblas_operation_none,
+            tb ? rocblas_operation_conjugate_transpose : rocblas_operation_none, m, nc, k, &al,
+            (const rocblas_double_complex *)A, n, st, (const rocblas_double_complex *)B, n, st, &be,
+            (rocblas_double_complex *)C, n, st, nb);
+    } else {
+        r = rocblas_dgemm_strided_batched(h, ta ? rocblas_operation_transpose : rocblas_operation_none,
+                                          tb ? rocblas_operation_transpose : rocblas_operation_none, m,
+                                          nc, k, &alpha, A, n, st, B, n, st, &beta, C, n, st, nb);
+    }
+    if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "gemm status %d", (int)r);
 }
+
+// S = U^H U for the batch in B (upper triangle read; B is overwritten), the
+// factor into U (upper triangle; the rest is not touched), left-looking by
+// 64-wide block rows: a GEMM applies the finished rows to block row J, the
+// diagonal kernel factors and inverts the diagonal block (Dinv), a GEMM
+// forms the row panel U(J, >J) = Dinv_J^H S(J, >J).  dinfo: first failing
+// column (1-based).
+void chol_blocked(rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *B, double *U,
+                  double *Dinv, int *dinfo)
+{
+    const int el = cplx ? 2 : 1;
+    HIPCHK(hipMemsetAsync(dinfo, 0, nb * sizeof(int), s));
+    for (int j0 = 0; j0 < n; j0 += kCholBlock) {
+        const int jb = std::min(kCholBlock, n - j0), rest = n - j0 - jb;
+        if (j0 > 0)
+            gemm_nn(h, cplx, true, false, jb, n - j0, j0, -1.0, U + el * (size_t)n * j0,
+                    U + el * (size_t)n * j0, 1.0, B + el * ((size_t)j0 + (size_t)n * j0), n, nb);
+        launch_chol_diag(cplx, B, U, Dinv, n, nb, j0, jb, dinfo, s);
+        HIPCHK(hipGetLastError());
+        if (rest > 0)
+            gemm_nn(h, cplx, true, false, jb, rest, jb, 1.0, Dinv + el * (size_t)j0,
+                    B + el * ((size_t)j0 + (size_t)n * (j0 + jb)), 0.0,
+                    U + el * ((size_t)j0 + (size_t)n * (j0 + jb)), n, nb);
+    }
+}
+
+// The three triangular solves by 64-wide blocks with the inverted diagonal
+// blocks (GEMMs only).  Dinv holds block J's inverse in rows [j0, j0 + jb),
+// columns [0, jb) of an n x n matrix per k.  Block J = rows/columns
+// [j0, j0 + jb).
+//   left:  Y = U^-H A      Y_J = Dinv_J^H (A_J - U_{<J,J}^H Y_{<J})  (A destroyed)
+//   right: C = Y U^-1      C_J = (Y_J - C_{<J} U_{<J,J}) Dinv_J      (Y destroyed)
+//   back:  X = U^-1 Z      X_J = Dinv_J (Z_J - U_{J,>J} X_{>J})       (Z destroyed)
+void solve_left_h(rocblas_handle h, bool cplx, int n, int nb, const double *U, const double *Dinv,
+                  double *A, double *Y)
+{
+    const int el = cplx ? 2 : 1;
+    for (int j0 = 0; j0 < n; j0 += kCholBlock) {
+        const int jb = std::min(kCholBlock, n - j0);
+        if (j0 > 0)
+            gemm_nn(h, cplx, true, false, jb, n, j0, -1.0, U + el * (size_t)n * j0, Y, 1.0,
+                    A + el * (size_t)j0, n, nb);
+        gemm_nn(h, cplx, true, false, jb, n, jb, 1.0, Dinv + el * (size_t)j0, A + el * (size_t)j0,
+                0.0, Y + el * (size_t)j0, n, nb);
+    }
+}
+void solve_right(rocblas_handle h, bool cplx, int n, int nb, const double *U, const double *Dinv,
+                 double *Y, double *C)
+{
+    const int el = cplx ? 2 : 1;
+    for (int j0 = 0; j0 < n; j0 += kCholBlock) {
+        const int jb = std::min(kCholBlock, n - j0);
+        if (j0 > 0)
+            gemm_nn(h, cplx, false, false, n, jb, j0, -1.0, C, U + el * (size_t)n * j0, 1.0,
+                    Y + el * (size_t)n * j0, n, nb);
+        gemm_nn(h, cplx, false, false, n, jb, jb, 1.0, Y + el * (size_t)n * j0, Dinv + el * (size_t)j0,
+                0.0, C + el * (size_t)n * j0, n, nb);
+    }
+}
+void solve_back(rocblas_handle h, bool cplx, int n, int nb, const double *U, const double *Dinv,
+                double *Z, double *X)
+{
+    const int el = cplx ? 2 : 1;
+    const int last = ((n - 1) / kCholBlock) * kCholBlock;
+    for (int j0 = last; j0 >= 0; j0 -= kCholBlock) {
+        const int jb = std::min(kCholBlock, n - j0), rest = n - j0 - jb;
+        if (rest > 0)
+            gemm_nn(h, cplx, false, false, jb, n, rest, -1.0, U + el * ((size_t)j0 + (size_t)n * (j0 + jb)),
+                    X + el * (size_t)(j0 + jb), 1.0, Z + el * (size_t)j0, n, nb);
+        gemm_nn(h, cplx, false, false, jb, n, jb, 1.0, Dinv + el * (size_t)j0, Z + el * (size_t)j0, 0.0,
+                X + el * (size_t)j0, n, nb);
+    }
+}
+
+// Whether the tridiagonal path takes (n, nb), and its launch plan.
+bool trd_fits(int n, int nb, bool cplx, TrdPlan &plan) { return trd_plan(n, nb, cplx, plan); }
 
 // Batched generalised eigensolve A x = w B x from the upper triangles
 // (what zhegv/dsygv with itype 1, 'V', 'U' compute, interfaces.F90:73-200):
-// B = U^H U (potrf), C = U^-H A U^-1 (two BLAS-3 triangular solves on the
-// Hermitian-filled A), C = Y diag(w) Y^H, X = U^-1 Y.  A is replaced by the
-// eigenvectors (all n columns) and B by U.  info[k] > n flags a B(k) that is
-// not positive definite, as zhegv does.  C = Y diag(w) Y^H is the one-
-// workgroup LDS solver for small n, else the tridiagonal path (trd_eig) where
-// the matrix fits its registers and LDS, else rocSOLVER's heevd; library
-// selects heevd always.
+// B = U^H U, C = U^-H A U^-1 (triangular solves on the Hermitian-filled A),
+// C = Y diag(w) Y^H, X = U^-1 Y.  A is replaced by the eigenvectors (all n
+// columns); B is overwritten.  info[k] > n flags a B(k) that is not positive
+// definite, as zhegv does.  Three paths: the one-workgroup LDS solver for
+// small n (everything in ksolve.hip); the device path where the tridiagonal
+// reduction fits (blocked Cholesky, GEMM triangular solves, trd_eig); else,
+// and always when library is set, rocSOLVER (potrf, trsm, heevd).
 void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *A,
              double *B, double *D, double *E, int *dinfo, std::vector<int> &info, bool library)
 {
@@ -1029,6 +1118,35 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
         HIPCHK(hipStreamSynchronize(s));
         for (int k = 0; k < nb; ++k)
             if (info[k] < 0) info[k] = 1;   // Jacobi did not converge: reported as zhegv info in 1..n
+        return;
+    }
+    TrdPlan plan;
+    if (!library && trd_fits(n, nb, cplx, plan)) {
+        // blocked Cholesky, GEMM triangular solves, cooperative reduction,
+        // divide and conquer, blocked back-transforms (all batched)
+        const int el = cplx ? 2 : 1;
+        ew.Dinv.alloc((size_t)n * n * el * nb);
+        ew.Y.alloc((size_t)n * n * el * nb);
+        ew.Uc.alloc((size_t)n * n * el * nb);
+        chol_blocked(h, s, cplx, n, nb, B, ew.Uc.p, ew.Dinv.p, dinfo);
+        HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int k = 0; k < nb; ++k)
+            if (info[k] != 0) {
+                info[k] += n;   // zhegv's convention: B not positive definite
+                return;
+            }
+        launch_herm_fill(A, n, nb, el, s);
+        solve_left_h(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
+        solve_right(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, ew.Y.p, A);
+        trd_eig(ew, plan, h, s, cplx, n, nb, A, D, E, dinfo);
+        solve_back(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
+        HIPCHK(hipMemcpyAsync(A, ew.Y.p, (size_t)n * n * el * nb * sizeof(double),
+                              hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int k = 0; k < nb; ++k)
+            if (info[k] == -2) info[k] = 1;   // a group barrier timed out
         return;
     }
     rocblas_status r;
@@ -1046,8 +1164,6 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
             return;
         }
     launch_herm_fill(A, n, nb, cplx ? 2 : 1, s);
-    TrdPlan plan;
-    const bool useTrd = !library && trd_fits(n, nb, cplx, plan);
     if (cplx) {
         const rocblas_double_complex one = {1.0, 0.0};
         auto *Az = (rocblas_double_complex *)A;
@@ -1061,11 +1177,8 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n,
                                               n, &one, Bz, n, st, Az, n, st, nb);
         if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "trsm status %d", (int)r);
-        if (useTrd)
-            trd_eig(ew, plan, h, s, true, n, nb, A, D, E, dinfo);
-        else
-            r = rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n, Az,
-                                                 n, st, D, n, E, n, dinfo, nb);
+        r = rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n, Az, n,
+                                             st, D, n, E, n, dinfo, nb);
         if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "heevd status %d", (int)r);
         r = rocblas_ztrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper,
                                           rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
@@ -1080,11 +1193,8 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
                                               rocblas_operation_none, rocblas_diagonal_non_unit, n,
                                               n, &one, B, n, st, A, n, st, nb);
         if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "trsm status %d", (int)r);
-        if (useTrd)
-            trd_eig(ew, plan, h, s, false, n, nb, A, D, E, dinfo);
-        else
-            r = rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n, A,
-                                                 n, st, D, n, E, n, dinfo, nb);
+        r = rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, n, A, n,
+                                             st, D, n, E, n, dinfo, nb);
         if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "syevd status %d", (int)r);
         r = rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper,
                                           rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
@@ -1093,9 +1203,6 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
     if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "trsm status %d", (int)r);
     HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (useTrd)
-        for (int k = 0; k < nb; ++k)
-            if (info[k] == -2) info[k] = 1;   // a group barrier timed out
 }
 
 // Batched generalised eigensolve of H, S (upper) on the device.

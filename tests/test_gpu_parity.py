@@ -326,6 +326,19 @@ def test_tridiagonal_divide_and_conquer_cases(gpu):
         assert np.abs(X[0].T @ X[0] - np.eye(n)).max() < 1e-11, name
 
 
+def test_blocked_cholesky_reports_indefinite_overlap(gpu):
+    """Sizes on the tridiagonal path factor S with the blocked Cholesky: an
+    overlap whose leading minor fails in the second block row (column 71) is
+    NOT_SPD, as zhegv reports info = n + 71."""
+    n = 150
+    B = np.eye(n, dtype=complex)
+    B[70, 70] = -1.0
+    A = np.diag(np.arange(n, dtype=float)).astype(complex)
+    with pytest.raises(gpu.OlcaoError) as e:
+        gpu.solveZHEGV(A[None], B[None])
+    assert e.value.code == 5
+
+
 def test_error_paths(gpu):
     # indefinite overlap -> NOT_SPD, the condition on which the reference stops
     with pytest.raises(gpu.OlcaoError) as e:
