@@ -70,6 +70,18 @@ __device__ __forceinline__ void wave_sum4(double &a, double &b, double &c, doubl
     d = __shfl(v, 48, 64);
 }
 
+// 1/x from the hardware estimate and two Newton steps (about 1 ulp; the
+// arguments here are differences of eigenvalue estimates, far from the
+// denormal range)
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 // Block-wide reduction through LDS (all threads call it).
 template <bool MAX, int NT>
 __device__ double block_reduce(double v, double *red)
@@ -133,328 +145,363 @@ __global__ void k_dc_init(double *Dall, const double *Eall, double *Qall, int n)
     }
 }
 
-template <int NT>
-__global__ __launch_bounds__(NT) void k_dc_merge(const int4 *nodes, int n, double *Dall,
-                                                 const double *Eall, const double *Qall,
-                                                 double *Wall, double *Sall, int *ordAll)
-{
-    constexpr int NW = NT / 64;
-    const int4 nd = nodes[blockIdx.x];
-    const int off = nd.x, m1 = nd.y, m = nd.z;
-    const int mat = blockIdx.y;
-    double *D = Dall + (size_t)n * mat;
-    const double *E = Eall + (size_t)n * mat;
-    const double *Q = Qall + (size_t)n * n * mat;
-    double *W = Wall + (size_t)n * n * mat;
-    double *S = Sall + (size_t)n * n * mat;
-    int *ord = ordAll + (size_t)n * mat;
-    auto Wat = [&](int row, int col) -> double & { return W[(size_t)(off + row) + (size_t)n * (off + col)]; };
-    auto Sat = [&](int row, int col) -> double & { return S[(size_t)(off + row) + (size_t)n * (off + col)]; };
+// LDS state of one merge.
+struct MergeLds {
+    double sd[kDcMax], sz[kDcMax], ds[kDcMax], zs[kDcMax];
+    double dk[kDcMax], zk[kDcMax], zz[kDcMax];
+    int order[kDcMax], keep[kDcMax], rowKeep[kDcMax], defl[kDcMax], scratch[kDcMax];
+    double dv[kDcMax];                 // deflated values, ascending
+    int dp[kDcMax];                    // their sorted-pole positions
+    int rotP[kDcMax], rotQ[kDcMax];
+    double rotC[kDcMax], rotS[kDcMax];
+    double red[16];
+    int ired[16];
+    int K, Nd, Nr, close, unsorted;
+    double rho;
+};
 
-    __shared__ double sd[kDcMax], sz[kDcMax], ds[kDcMax], zs[kDcMax];
-    __shared__ double dk[kDcMax], zk[kDcMax], lam[kDcMax], zh[kDcMax];
-    __shared__ int order[kDcMax], keep[kDcMax], rowKeep[kDcMax], defl[kDcMax], colInfo[kDcMax];
-    __shared__ int rotP[kDcMax], rotQ[kDcMax];
-    __shared__ double rotC[kDcMax], rotS[kDcMax];
-    __shared__ double red[NW];
-    __shared__ int ired[NW];
-    __shared__ int sK, sNd, sNr, sClose, sUnsorted;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+// Sort the children's poles, form z, deflate (dlaed2's rules: small rho z,
+// then Givens rotations of poles closer than the tolerance, in one ascending
+// scan that runs only when a parallel test finds such a pair), and leave the
+// kept poles dk, z, rho z^2 and the ascending deflated values in L.
+template <int NT>
+__device__ void dc_prepare(int n, int off, int m1, int m, const double *D, const double *E,
+                           const double *Q, int *ord, MergeLds &L)
+{
+    const int tid = threadIdx.x;
     const double beta = E[off + m1 - 1];
     const double sgn = beta < 0.0 ? -1.0 : 1.0;
-
     // children's eigenvalues and z = (last row of Q1, sign(beta) first row of Q2)
     double nzp = 0.0;
     for (int i = tid; i < m; i += NT) {
-        sd[i] = D[off + i];
+        L.sd[i] = D[off + i];
         const double v = i < m1 ? Q[(size_t)(off + m1 - 1) + (size_t)n * (off + i)]
                                 : sgn * Q[(size_t)(off + m1) + (size_t)n * (off + i)];
-        sz[i] = v;
+        L.sz[i] = v;
         nzp += v * v;
     }
-    const double nz2 = block_reduce<false, NT>(nzp, red);
+    const double nz2 = block_reduce<false, NT>(nzp, L.red);
     const double nz = sqrt(nz2);
     const double rho = fabs(beta) * nz2;
     // merge order of the two sorted children (ties: first child first)
     for (int i = tid; i < m; i += NT) {
-        const double v = sd[i];
+        const double v = L.sd[i];
         int lo, hi;
         if (i < m1) {   // count second-child entries < v
             lo = m1;
             hi = m;
             while (lo < hi) {
                 const int md = (lo + hi) >> 1;
-                if (sd[md] < v) lo = md + 1; else hi = md;
+                if (L.sd[md] < v) lo = md + 1; else hi = md;
             }
-            order[i + (lo - m1)] = i;
+            L.order[i + (lo - m1)] = i;
         } else {        // count first-child entries <= v
             lo = 0;
             hi = m1;
             while (lo < hi) {
                 const int md = (lo + hi) >> 1;
-                if (sd[md] <= v) lo = md + 1; else hi = md;
+                if (L.sd[md] <= v) lo = md + 1; else hi = md;
             }
-            order[(i - m1) + lo] = i;
+            L.order[(i - m1) + lo] = i;
         }
     }
     __syncthreads();
     double dmx = 0.0, zmx = 0.0;
     for (int p = tid; p < m; p += NT) {
-        ds[p] = sd[order[p]];
-        zs[p] = nz > 0.0 ? sz[order[p]] / nz : 0.0;
-        dmx = fmax(dmx, fabs(ds[p]));
-        zmx = fmax(zmx, fabs(zs[p]));
+        L.ds[p] = L.sd[L.order[p]];
+        L.zs[p] = nz > 0.0 ? L.sz[L.order[p]] / nz : 0.0;
+        dmx = fmax(dmx, fabs(L.ds[p]));
+        zmx = fmax(zmx, fabs(L.zs[p]));
+        ord[off + p] = off + L.order[p];
     }
-    dmx = block_reduce<true, NT>(dmx, red);
-    zmx = block_reduce<true, NT>(zmx, red);
+    dmx = block_reduce<true, NT>(dmx, L.red);
+    zmx = block_reduce<true, NT>(zmx, L.red);
     const double tol = 8.0 * kEps * fmax(dmx, zmx);
     const bool allDefl = !(rho * zmx > tol);
-
-    // Deflation (dlaed2's rules) in the sorted order.  Parallel pass: small
-    // rho z(p) entries, and whether any two consecutive remaining poles are
-    // close enough to need a rotation; only then the sequential scan.
-    int *big = colInfo, *pre = rowKeep;   // scratch until their own use below
-    for (int p = tid; p < m; p += NT) big[p] = !allDefl && rho * fabs(zs[p]) > tol;
-    if (tid == 0) sClose = 0;
-    block_scan<NT>(big, pre, m, ired);
+    int *big = L.scratch, *pre = L.rowKeep;   // rowKeep is set below
+    for (int p = tid; p < m; p += NT) big[p] = !allDefl && rho * fabs(L.zs[p]) > tol;
+    if (tid == 0) L.close = 0;
+    block_scan<NT>(big, pre, m, L.ired);
     const int nBig = m > 0 ? pre[m - 1] + big[m - 1] : 0;
     for (int p = tid; p < m; p += NT)
-        if (big[p]) keep[pre[p]] = p;
+        if (big[p]) L.keep[pre[p]] = p;
     __syncthreads();
     for (int q = tid + 1; q < nBig; q += NT) {
-        const int pj = keep[q - 1], nj = keep[q];
-        double s = zs[pj], c = zs[nj];
+        const int pj = L.keep[q - 1], nj = L.keep[q];
+        double s = L.zs[pj], c = L.zs[nj];
         const double tau = sqrt(c * c + s * s);
         c /= tau;
         s = -s / tau;
-        if (fabs((ds[nj] - ds[pj]) * c * s) <= tol) sClose = 1;
+        if (fabs((L.ds[nj] - L.ds[pj]) * c * s) <= tol) L.close = 1;
     }
     __syncthreads();
-    if (!sClose) {
+    if (!L.close) {
         for (int p = tid; p < m; p += NT)
-            if (!big[p]) defl[p - pre[p]] = p;
+            if (!big[p]) L.defl[p - pre[p]] = p;
         if (tid == 0) {
-            sK = nBig;
-            sNd = m - nBig;
-            sNr = 0;
+            L.K = nBig;
+            L.Nd = m - nBig;
+            L.Nr = 0;
         }
     } else if (tid == 0) {
         int K = 0, nd = 0, nr = 0;
         int pj = -1;
         for (int nj = 0; nj < m; ++nj) {
             if (!big[nj]) {
-                defl[nd++] = nj;
+                L.defl[nd++] = nj;
                 continue;
             }
             if (pj < 0) {
                 pj = nj;
                 continue;
             }
-            double s = zs[pj], c = zs[nj];
+            double s = L.zs[pj], c = L.zs[nj];
             const double tau = sqrt(c * c + s * s);
-            const double t = ds[nj] - ds[pj];
+            const double t = L.ds[nj] - L.ds[pj];
             c /= tau;
             s = -s / tau;
             if (fabs(t * c * s) <= tol) {
-                zs[nj] = tau;
-                zs[pj] = 0.0;
-                rotP[nr] = pj;
-                rotQ[nr] = nj;
-                rotC[nr] = c;
-                rotS[nr] = s;
+                L.zs[nj] = tau;
+                L.zs[pj] = 0.0;
+                L.rotP[nr] = pj;
+                L.rotQ[nr] = nj;
+                L.rotC[nr] = c;
+                L.rotS[nr] = s;
                 ++nr;
-                const double t2 = ds[pj] * c * c + ds[nj] * s * s;
-                ds[nj] = ds[pj] * s * s + ds[nj] * c * c;
-                ds[pj] = t2;
-                defl[nd++] = pj;
+                const double t2 = L.ds[pj] * c * c + L.ds[nj] * s * s;
+                L.ds[nj] = L.ds[pj] * s * s + L.ds[nj] * c * c;
+                L.ds[pj] = t2;
+                L.defl[nd++] = pj;
             } else {
-                keep[K++] = pj;
+                L.keep[K++] = pj;
             }
             pj = nj;
         }
-        if (pj >= 0) keep[K++] = pj;
-        sK = K;
-        sNd = nd;
-        sNr = nr;
+        if (pj >= 0) L.keep[K++] = pj;
+        L.K = K;
+        L.Nd = nd;
+        L.Nr = nr;
+    }
+    if (tid == 0) {
+        L.unsorted = 0;
+        L.rho = rho;
     }
     __syncthreads();
-    const int K = sK, Nd = sNd, Nr = sNr;
-    for (int p = tid; p < m; p += NT) rowKeep[p] = -1;
-    if (tid == 0) sUnsorted = 0;
+    const int K = L.K, Nd = L.Nd;
+    for (int p = tid; p < m; p += NT) L.rowKeep[p] = -1;
     __syncthreads();
     for (int ii = tid; ii < K; ii += NT) {
-        dk[ii] = ds[keep[ii]];
-        zk[ii] = zs[keep[ii]];
-        rowKeep[keep[ii]] = ii;
+        L.dk[ii] = L.ds[L.keep[ii]];
+        L.zk[ii] = L.zs[L.keep[ii]];
+        L.zz[ii] = rho * L.zk[ii] * L.zk[ii];
+        L.rowKeep[L.keep[ii]] = ii;
     }
     for (int q = tid; q + 1 < Nd; q += NT)
-        if (ds[defl[q]] > ds[defl[q + 1]]) sUnsorted = 1;
+        if (L.ds[L.defl[q]] > L.ds[L.defl[q + 1]]) L.unsorted = 1;
     __syncthreads();
-
-    // secular equation, one wave per root: Delta(i, j) = dk(i) - lambda_j into S
-    for (int j = wv; j < K; j += NW) {
-        int org;
-        double a, b;
-        if (j < K - 1) {
-            const double mid = 0.5 * (dk[j] + dk[j + 1]);
-            double part = 0.0;
-            for (int ii = lane; ii < K; ii += 64) part += rho * zk[ii] * zk[ii] / (dk[ii] - mid);
-            const double g = 1.0 + wave_sum(part);
-            if (g >= 0.0) {
-                org = j;
-                a = 0.0;
-                b = mid - dk[j];
-            } else {
-                org = j + 1;
-                a = mid - dk[j + 1];
-                b = 0.0;
-            }
-        } else {
-            double part = 0.0;
-            for (int ii = lane; ii < K; ii += 64) part += rho * zk[ii] * zk[ii];
-            org = j;
-            a = 0.0;
-            b = wave_sum(part);
-        }
-        const double dorg = dk[org];
-        const double dj = dk[j] - dorg;
-        const double dj1 = j < K - 1 ? dk[j + 1] - dorg : 0.0;
-        double tau = 0.5 * (a + b);
-        for (int it = 0; it < kSecularIters; ++it) {
-            // left poles (ii <= j) give negative terms, right poles positive
-            double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
-            for (int ii = lane; ii < K; ii += 64) {
-                const double r = 1.0 / ((dk[ii] - dorg) - tau);
-                const double t = rho * zk[ii] * zk[ii] * r;
-                if (ii <= j) {
-                    psi += t;
-                    dpsi += t * r;
-                } else {
-                    phi += t;
-                    dphi += t * r;
-                }
-            }
-            wave_sum4(psi, dpsi, phi, dphi);
-            const double g = 1.0 + psi + phi;
-            if (g < 0.0) a = tau; else b = tau;
-            if (b - a <= 4.0 * kEps * fmax(fabs(a), fabs(b)) ||
-                fabs(g) <= 8.0 * kEps * (1.0 + phi - psi))
-                break;
-            // two-pole rational model through (psi, psi') and (phi, phi')
-            const double dl = dj - tau;
-            const double s1 = dpsi * dl * dl, c1 = psi - s1 / dl;
-            double x = 0.0;
-            bool ok = false;
-            if (j < K - 1) {
-                const double dr = dj1 - tau;
-                const double s2 = dphi * dr * dr, c2 = phi - s2 / dr;
-                const double C = 1.0 + c1 + c2;
-                const double B2 = -(C * (dj + dj1) + s1 + s2);
-                const double C2 = C * dj * dj1 + s1 * dj1 + s2 * dj;
-                if (C != 0.0) {
-                    const double sq = sqrt(fmax(B2 * B2 - 4.0 * C * C2, 0.0));
-                    const double q = -0.5 * (B2 + copysign(sq, B2));
-                    const double x1 = q / C;
-                    const double x2 = q != 0.0 ? C2 / q : INFINITY;
-                    const bool ok1 = a < x1 && x1 < b, ok2 = a < x2 && x2 < b;
-                    if (ok1 && ok2) {
-                        x = fabs(x1 - tau) <= fabs(x2 - tau) ? x1 : x2;
-                        ok = true;
-                    } else if (ok1) {
-                        x = x1;
-                        ok = true;
-                    } else if (ok2) {
-                        x = x2;
-                        ok = true;
-                    }
-                } else if (B2 != 0.0) {
-                    x = -C2 / B2;
-                    ok = a < x && x < b;
-                }
-            } else {
-                const double C = 1.0 + c1 + phi;
-                if (C != 0.0) {
-                    x = dj + s1 / C;
-                    ok = a < x && x < b;
-                }
-            }
-            if (ok && fabs(x - tau) <= 2.0 * kEps * fabs(x)) {
-                tau = x;
-                break;
-            }
-            tau = ok ? x : 0.5 * (a + b);
-        }
-        for (int ii = lane; ii < K; ii += 64) Sat(ii, j) = (dk[ii] - dorg) - tau;
-        if (lane == 0) lam[j] = dorg + tau;
-    }
-    __syncthreads();
-
-    // Gu-Eisenstat: zhat(i)^2 = (lambda_i - d_i)/rho prod_{j!=i} (lambda_j - d_i)/(d_j - d_i)
-    for (int ii = tid; ii < K; ii += NT) {
-        double prod = -Sat(ii, ii) / rho;
-        for (int j = 0; j < K; ++j)
-            if (j != ii) prod *= -Sat(ii, j) / (dk[j] - dk[ii]);
-        zh[ii] = copysign(sqrt(fmax(prod, 0.0)), zk[ii]);
-    }
-    // deflated values ascending into sd (values) and sz (positions): as
-    // scanned unless a rotation moved one out of order
+    // deflated values ascending: as scanned unless a rotation moved one
     for (int q = tid; q < Nd; q += NT) {
-        const double v = ds[defl[q]];
+        const double v = L.ds[L.defl[q]];
         int rank = q;
-        if (sUnsorted) {
+        if (L.unsorted) {
             rank = 0;
             for (int q2 = 0; q2 < Nd; ++q2) {
-                const double u = ds[defl[q2]];
+                const double u = L.ds[L.defl[q2]];
                 rank += (u < v) || (u == v && q2 < q);
             }
         }
-        sd[rank] = v;
-        sz[rank] = (double)defl[q];
+        L.dv[rank] = v;
+        L.dp[rank] = L.defl[q];
     }
     __syncthreads();
-    // final column of every eigenpair: roots before equal deflated values
+}
+
+// Root j of 1 + sum zz / (dk - lambda) = 0 by one wave: bracketed, in
+// coordinates shifted to the nearer pole (org), two-pole rational model
+// steps; lambda = dk(org) + tau.
+__device__ void dc_root(int j, int K, const double *dk, const double *zz, int &orgOut,
+                        double &tauOut)
+{
+    const int lane = threadIdx.x & 63;
+    int org;
+    double a, b;
+    if (j < K - 1) {
+        const double mid = 0.5 * (dk[j] + dk[j + 1]);
+        double part = 0.0;
+        for (int ii = lane; ii < K; ii += 64) part += zz[ii] / (dk[ii] - mid);
+        const double g = 1.0 + wave_sum(part);
+        if (g >= 0.0) {
+            org = j;
+            a = 0.0;
+            b = mid - dk[j];
+        } else {
+            org = j + 1;
+            a = mid - dk[j + 1];
+            b = 0.0;
+        }
+    } else {
+        double part = 0.0;
+        for (int ii = lane; ii < K; ii += 64) part += zz[ii];
+        org = j;
+        a = 0.0;
+        b = wave_sum(part);
+    }
+    const double dorg = dk[org];
+    const double dj = dk[j] - dorg;
+    const double dj1 = j < K - 1 ? dk[j + 1] - dorg : 0.0;
+    double tau = 0.5 * (a + b);
+    for (int it = 0; it < kSecularIters; ++it) {
+        // left poles (ii <= j) give negative terms, right poles positive;
+        // the lane's terms unrolled so their reciprocals overlap
+        double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
+#pragma unroll
+        for (int q = 0; q < kDcMax / 64; ++q) {
+            const int ii = lane + 64 * q;
+            if (64 * q < K) {
+                const bool on = ii < K;
+                const int iu = on ? ii : 0;
+                const double r = rcp_nr((dk[iu] - dorg) - tau);
+                const double t = on ? zz[iu] * r : 0.0;
+                const bool left = ii <= j;
+                psi += left ? t : 0.0;
+                dpsi += left ? t * r : 0.0;
+                phi += left ? 0.0 : t;
+                dphi += left ? 0.0 : t * r;
+            }
+        }
+        wave_sum4(psi, dpsi, phi, dphi);
+        const double g = 1.0 + psi + phi;
+        if (g < 0.0) a = tau; else b = tau;
+        if (b - a <= 4.0 * kEps * fmax(fabs(a), fabs(b)) || fabs(g) <= 8.0 * kEps * (1.0 + phi - psi))
+            break;
+        // two-pole rational model through (psi, psi') and (phi, phi')
+        const double dl = dj - tau;
+        const double s1 = dpsi * dl * dl, c1 = psi - s1 / dl;
+        double x = 0.0;
+        bool ok = false;
+        if (j < K - 1) {
+            const double dr = dj1 - tau;
+            const double s2 = dphi * dr * dr, c2 = phi - s2 / dr;
+            const double C = 1.0 + c1 + c2;
+            const double B2 = -(C * (dj + dj1) + s1 + s2);
+            const double C2 = C * dj * dj1 + s1 * dj1 + s2 * dj;
+            if (C != 0.0) {
+                const double sq = sqrt(fmax(B2 * B2 - 4.0 * C * C2, 0.0));
+                const double q = -0.5 * (B2 + copysign(sq, B2));
+                const double x1 = q / C;
+                const double x2 = q != 0.0 ? C2 / q : INFINITY;
+                const bool ok1 = a < x1 && x1 < b, ok2 = a < x2 && x2 < b;
+                if (ok1 && ok2) {
+                    x = fabs(x1 - tau) <= fabs(x2 - tau) ? x1 : x2;
+                    ok = true;
+                } else if (ok1) {
+                    x = x1;
+                    ok = true;
+                } else if (ok2) {
+                    x = x2;
+                    ok = true;
+                }
+            } else if (B2 != 0.0) {
+                x = -C2 / B2;
+                ok = a < x && x < b;
+            }
+        } else {
+            const double C = 1.0 + c1 + phi;
+            if (C != 0.0) {
+                x = dj + s1 / C;
+                ok = a < x && x < b;
+            }
+        }
+        if (ok && fabs(x - tau) <= 2.0 * kEps * fabs(x)) {
+            tau = x;
+            break;
+        }
+        tau = ok ? x : 0.5 * (a + b);
+    }
+    orgOut = org;
+    tauOut = tau;
+}
+
+// Gu-Eisenstat: zhat(i)^2 = (lambda_i - d_i)/rho prod_{j!=i} (lambda_j - d_i)/(d_j - d_i),
+// with d_i - lambda_j = (dk(i) - dk(org_j)) - tau_j as the secular iteration formed it.
+__device__ double dc_zhat(int ii, int K, const double *dk, const double *zk, double rho,
+                          const int *rorg, const double *rtau)
+{
+    auto delta = [&](int i, int j) { return (dk[i] - dk[rorg[j]]) - rtau[j]; };
+    double p0 = -delta(ii, ii) / rho, p1 = 1.0;
+    int j = 0;
+    for (; j + 1 < K; j += 2) {
+        const double a0 = j == ii ? 1.0 : -delta(ii, j) * rcp_nr(dk[j] - dk[ii]);
+        const double a1 = j + 1 == ii ? 1.0 : -delta(ii, j + 1) * rcp_nr(dk[j + 1] - dk[ii]);
+        p0 *= a0;
+        p1 *= a1;
+    }
+    if (j < K && j != ii) p0 *= -delta(ii, j) * rcp_nr(dk[j] - dk[ii]);
+    return copysign(sqrt(fmax(p0 * p1, 0.0)), zk[ii]);
+}
+
+// Final columns [c0, c1) of the merge: the eigenvalue of each (roots before
+// equal deflated values), W's column in the sorted-pole basis (kept rows
+// zhat(i) / (d_i - lambda_j) normalised, or the deflated unit vector), and the
+// deflation rotations on W's rows.  colInfo: LDS scratch of kDcMax ints.
+template <int NT>
+__device__ void dc_columns(int n, int off, int m, int c0, int c1, const MergeLds &L,
+                           const double *lam, const int *rorg, const double *rtau,
+                           const double *zh, int *colInfo, double *D, double *W)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int NW = NT / 64;
+    const int K = L.K, Nd = L.Nd, Nr = L.Nr;
+    auto Wat = [&](int row, int col) -> double & { return W[(size_t)(off + row) + (size_t)n * (off + col)]; };
     for (int j = tid; j < K; j += NT) {
         const double v = lam[j];
         int lo = 0, hi = Nd;
         while (lo < hi) {
             const int md = (lo + hi) >> 1;
-            if (sd[md] < v) lo = md + 1; else hi = md;
+            if (L.dv[md] < v) lo = md + 1; else hi = md;
         }
-        colInfo[j + lo] = j;
-        D[off + j + lo] = v;
+        const int s = j + lo;
+        if (s >= c0 && s < c1) {
+            colInfo[s] = j;
+            D[off + s] = v;
+        }
     }
     for (int q = tid; q < Nd; q += NT) {
-        const double v = sd[q];
+        const double v = L.dv[q];
         int lo = 0, hi = K;
         while (lo < hi) {
             const int md = (lo + hi) >> 1;
             if (lam[md] <= v) lo = md + 1; else hi = md;
         }
-        colInfo[q + lo] = -1 - (int)sz[q];
-        D[off + q + lo] = v;
+        const int s = q + lo;
+        if (s >= c0 && s < c1) {
+            colInfo[s] = -1 - L.dp[q];
+            D[off + s] = v;
+        }
     }
     __syncthreads();
-    // W, one wave per column: kept rows u(i) = zhat(i) / Delta(i, j),
-    // normalised; deflated rows zero, or the unit entry of a deflated column
-    for (int s = wv; s < m; s += NW) {
+    for (int s = c0 + wv; s < c1; s += NW) {
         const int info = colInfo[s];
         if (info >= 0) {
             const int j = info;
+            const double dj = L.dk[rorg[j]], tj = rtau[j];
             double u[kDcMax / 64];
             double part = 0.0;
 #pragma unroll
             for (int q = 0; q < kDcMax / 64; ++q) {
                 const int ii = lane + 64 * q;
-                u[q] = ii < K ? zh[ii] / Sat(ii, j) : 0.0;
+                const bool on = ii < K;
+                const int iu = on ? ii : 0;
+                u[q] = on ? zh[iu] * rcp_nr((L.dk[iu] - dj) - tj) : 0.0;
                 part += u[q] * u[q];
             }
             const double inv = 1.0 / sqrt(wave_sum(part));
 #pragma unroll
             for (int q = 0; q < kDcMax / 64; ++q) {
                 const int ii = lane + 64 * q;
-                if (ii < K) Wat(keep[ii], s) = u[q] * inv;
+                if (ii < K) Wat(L.keep[ii], s) = u[q] * inv;
             }
-            for (int q = lane; q < Nd; q += 64) Wat(defl[q], s) = 0.0;
+            for (int q = lane; q < Nd; q += 64) Wat(L.defl[q], s) = 0.0;
         } else {
             const int pd = -1 - info;
             for (int p = lane; p < m; p += 64) Wat(p, s) = p == pd ? 1.0 : 0.0;
@@ -464,12 +511,12 @@ __global__ __launch_bounds__(NT) void k_dc_merge(const int4 *nodes, int n, doubl
     // deflation rotations on W's rows, last first; a chain (p1,p2),(p2,p3),...
     // keeps its running row in a register
     if (Nr > 0) {
-        for (int s = tid; s < m; s += NT) {
+        for (int s = c0 + tid; s < c1; s += NT) {
             int carryRow = -1;
             double carry = 0.0;
             for (int q = Nr - 1; q >= 0; --q) {
-                const int p = rotP[q], r = rotQ[q];
-                const double c = rotC[q], sn = rotS[q];
+                const int p = L.rotP[q], r = L.rotQ[q];
+                const double c = L.rotC[q], sn = L.rotS[q];
                 double y;
                 if (r == carryRow) {
                     y = carry;
@@ -485,7 +532,174 @@ __global__ __launch_bounds__(NT) void k_dc_merge(const int4 *nodes, int n, doubl
             Wat(carryRow, s) = carry;
         }
     }
-    for (int p = tid; p < m; p += NT) ord[off + p] = off + order[p];
+}
+
+// One whole merge in one workgroup (small merges).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_dc_merge(const int4 *nodes, int n, double *Dall,
+                                                 const double *Eall, const double *Qall,
+                                                 double *Wall, int *ordAll)
+{
+    constexpr int NW = NT / 64;
+    const int4 nd = nodes[blockIdx.x];
+    const int off = nd.x, m1 = nd.y, m = nd.z;
+    const int mat = blockIdx.y;
+    double *D = Dall + (size_t)n * mat;
+    __shared__ MergeLds L;
+    __shared__ double lam[kDcMax], zh[kDcMax], rtau[kDcMax];
+    __shared__ int rorg[kDcMax], colInfo[kDcMax];
+    dc_prepare<NT>(n, off, m1, m, D, Eall + (size_t)n * mat, Qall + (size_t)n * n * mat,
+                   ordAll + (size_t)n * mat, L);
+    const int K = L.K, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = wv; j < K; j += NW) {
+        int org;
+        double tau;
+        dc_root(j, K, L.dk, L.zz, org, tau);
+        if (lane == 0) {
+            lam[j] = L.dk[org] + tau;
+            rorg[j] = org;
+            rtau[j] = tau;
+        }
+    }
+    __syncthreads();
+    for (int ii = threadIdx.x; ii < K; ii += NT) zh[ii] = dc_zhat(ii, K, L.dk, L.zk, L.rho, rorg, rtau);
+    __syncthreads();
+    dc_columns<NT>(n, off, m, 0, m, L, lam, rorg, rtau, zh, colInfo, D, Wall + (size_t)n * n * mat);
+}
+
+// Large merges in three launches, so roots and columns spread over many
+// workgroups.  The node's prepared state moves through global memory:
+//   st[off + i] (per matrix, n entries each): dk, zk, dv, lam, tau (double)
+//   ist[...]: keep, dp, defl, org, rowKeep (int); node scalars in ns[node]
+struct DcState {
+    double *dk, *zk, *dv, *lam, *tau, *rotC, *rotS;
+    int *keep, *dp, *defl, *org, *rowKeep, *rotP, *rotQ;
+    int4 *cnt;      // per node x matrix: K, Nd, Nr
+    double *rho;    // per node x matrix
+};
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_dc_prep(const int4 *nodes, int nodeBase, int nNodes, int n,
+                                                const double *Dall, const double *Eall,
+                                                const double *Qall, int *ordAll, DcState st)
+{
+    const int node = nodeBase + blockIdx.x, mat = blockIdx.y;
+    const int4 nd = nodes[node];
+    const int off = nd.x, m1 = nd.y, m = nd.z;
+    __shared__ MergeLds L;
+    dc_prepare<NT>(n, off, m1, m, Dall + (size_t)n * mat, Eall + (size_t)n * mat,
+                   Qall + (size_t)n * n * mat, ordAll + (size_t)n * mat, L);
+    const size_t b = (size_t)n * mat + off;
+    for (int i = threadIdx.x; i < m; i += NT) {
+        if (i < L.K) {
+            st.dk[b + i] = L.dk[i];
+            st.zk[b + i] = L.zk[i];
+            st.keep[b + i] = L.keep[i];
+        }
+        if (i < L.Nd) {
+            st.dv[b + i] = L.dv[i];
+            st.dp[b + i] = L.dp[i];
+            st.defl[b + i] = L.defl[i];
+        }
+        if (i < L.Nr) {
+            st.rotP[b + i] = L.rotP[i];
+            st.rotQ[b + i] = L.rotQ[i];
+            st.rotC[b + i] = L.rotC[i];
+            st.rotS[b + i] = L.rotS[i];
+        }
+        st.rowKeep[b + i] = L.rowKeep[i];
+    }
+    if (threadIdx.x == 0) {
+        st.cnt[(size_t)mat * nNodes + blockIdx.x] = make_int4(L.K, L.Nd, L.Nr, 0);
+        st.rho[(size_t)mat * nNodes + blockIdx.x] = L.rho;
+    }
+}
+
+constexpr int kRootsPerWg = 16;
+constexpr int kColsPerWg = 64;
+
+// tiles: (node index within the level, chunk)
+__global__ __launch_bounds__(256) void k_dc_roots(const int2 *tiles, const int4 *nodes, int nodeBase,
+                                                  int nNodes, int n, DcState st)
+{
+    const int2 tl = tiles[blockIdx.x];
+    const int mat = blockIdx.y;
+    const int4 nd = nodes[nodeBase + tl.x];
+    const int off = nd.x;
+    const int4 c = st.cnt[(size_t)mat * nNodes + tl.x];
+    const int K = c.x;
+    const int j0 = tl.y * kRootsPerWg;
+    if (j0 >= K) return;
+    const double rho = st.rho[(size_t)mat * nNodes + tl.x];
+    __shared__ double dk[kDcMax], zz[kDcMax];
+    const size_t b = (size_t)n * mat + off;
+    for (int i = threadIdx.x; i < K; i += 256) {
+        dk[i] = st.dk[b + i];
+        const double z = st.zk[b + i];
+        zz[i] = rho * z * z;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = j0 + wv; j < min(K, j0 + kRootsPerWg); j += 4) {
+        int org;
+        double tau;
+        dc_root(j, K, dk, zz, org, tau);
+        if (lane == 0) {
+            st.lam[b + j] = dk[org] + tau;
+            st.org[b + j] = org;
+            st.tau[b + j] = tau;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dc_vecs(const int2 *tiles, const int4 *nodes, int nodeBase,
+                                                 int nNodes, int n, double *Dall, double *Wall,
+                                                 DcState st)
+{
+    const int2 tl = tiles[blockIdx.x];
+    const int mat = blockIdx.y;
+    const int4 nd = nodes[nodeBase + tl.x];
+    const int off = nd.x, m = nd.z;
+    const int c0 = tl.y * kColsPerWg, c1 = min(m, c0 + kColsPerWg);
+    if (c0 >= m) return;
+    const int4 cn = st.cnt[(size_t)mat * nNodes + tl.x];
+    __shared__ MergeLds L;
+    __shared__ double lam[kDcMax], zh[kDcMax], rtau[kDcMax];
+    __shared__ int rorg[kDcMax], colInfo[kDcMax];
+    const size_t b = (size_t)n * mat + off;
+    const int K = cn.x, Nd = cn.y, Nr = cn.z;
+    if (threadIdx.x == 0) {
+        L.K = K;
+        L.Nd = Nd;
+        L.Nr = Nr;
+        L.rho = st.rho[(size_t)mat * nNodes + tl.x];
+    }
+    for (int i = threadIdx.x; i < m; i += 256) {
+        if (i < K) {
+            L.dk[i] = st.dk[b + i];
+            L.zk[i] = st.zk[b + i];
+            L.keep[i] = st.keep[b + i];
+            lam[i] = st.lam[b + i];
+            rorg[i] = st.org[b + i];
+            rtau[i] = st.tau[b + i];
+        }
+        if (i < Nd) {
+            L.dv[i] = st.dv[b + i];
+            L.dp[i] = st.dp[b + i];
+            L.defl[i] = st.defl[b + i];
+        }
+        if (i < Nr) {
+            L.rotP[i] = st.rotP[b + i];
+            L.rotQ[i] = st.rotQ[b + i];
+            L.rotC[i] = st.rotC[b + i];
+            L.rotS[i] = st.rotS[b + i];
+        }
+    }
+    __syncthreads();
+    for (int ii = threadIdx.x; ii < K; ii += 256) zh[ii] = dc_zhat(ii, K, L.dk, L.zk, L.rho, rorg, rtau);
+    __syncthreads();
+    dc_columns<256>(n, off, m, c0, c1, L, lam, rorg, rtau, zh, colInfo, Dall + (size_t)n * mat,
+                    Wall + (size_t)n * n * mat);
 }
 
 // Qt(node) = Q(:, ord) W over node blocks; tiles: x node, y row tile, z col tile
@@ -585,35 +799,66 @@ void dc_plan(int n, DcPlan &p)
         L.nodeBase = (int)p.nodes.size();
         L.nNodes = (int)lv.size();
         L.tileBase = (int)p.tiles.size();
+        L.rootBase = (int)p.rootTiles.size();
+        L.colBase = (int)p.colTiles.size();
         for (int q = 0; q < L.nNodes; ++q) {
             const int m = lv[q].z, nt = (m + kTile - 1) / kTile;
             for (int a = 0; a < nt; ++a)
                 for (int b2 = 0; b2 < nt; ++b2) p.tiles.push_back(make_int4(L.nodeBase + q, a, b2, 0));
+            for (int a = 0; a < (m + kRootsPerWg - 1) / kRootsPerWg; ++a) p.rootTiles.push_back(make_int2(q, a));
+            for (int a = 0; a < (m + kColsPerWg - 1) / kColsPerWg; ++a) p.colTiles.push_back(make_int2(q, a));
             p.nodes.push_back(lv[q]);
             L.maxM = std::max(L.maxM, m);
         }
         L.nTiles = (int)p.tiles.size() - L.tileBase;
+        L.nRoot = (int)p.rootTiles.size() - L.rootBase;
+        L.nCol = (int)p.colTiles.size() - L.colBase;
         p.levels.push_back(L);
     }
 }
 
-void launch_dc(const DcPlan &p, const int4 *dNodes, const int4 *dTiles, int nb, double *D,
-               const double *E, double *Q, double *Qt, double *W, double *S, int *ord,
-               hipStream_t s)
+size_t dc_state_doubles(int n, int nb) { return (size_t)7 * n * nb; }
+size_t dc_state_ints(int n, int nb) { return (size_t)7 * n * nb; }
+
+void launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double *E, double *Q,
+               double *Qt, double *W, int *ord, hipStream_t s)
 {
     const int n = p.n;
+    const size_t nn = (size_t)n * nb;
+    DcState st;
+    st.dk = dv.stD;
+    st.zk = st.dk + nn;
+    st.dv = st.zk + nn;
+    st.lam = st.dv + nn;
+    st.tau = st.lam + nn;
+    st.rotC = st.tau + nn;
+    st.rotS = st.rotC + nn;
+    st.keep = dv.stI;
+    st.dp = st.keep + nn;
+    st.defl = st.dp + nn;
+    st.org = st.defl + nn;
+    st.rowKeep = st.org + nn;
+    st.rotP = st.rowKeep + nn;
+    st.rotQ = st.rotP + nn;
+    st.cnt = dv.cnt;
+    st.rho = dv.rho;
     hipLaunchKernelGGL(k_dc_init, dim3(64, nb), dim3(256), 0, s, D, E, Q, n);
     for (const DcLevel &L : p.levels) {
-        if (L.maxM > kDcSmallM)
-            hipLaunchKernelGGL(k_dc_merge<kDcBig>, dim3(L.nNodes, nb), dim3(kDcBig), 0, s,
-                               dNodes + L.nodeBase, n, D, E, Q, W, S, ord);
-        else
+        if (L.maxM <= kDcSmallM) {
             hipLaunchKernelGGL(k_dc_merge<kDcSmall>, dim3(L.nNodes, nb), dim3(kDcSmall), 0, s,
-                               dNodes + L.nodeBase, n, D, E, Q, W, S, ord);
-        hipLaunchKernelGGL(k_dc_gemm, dim3(L.nTiles, nb), dim3(256), 0, s, dTiles + L.tileBase,
-                           dNodes, n, Q, W, ord, Qt);
-        hipLaunchKernelGGL(k_dc_copy, dim3(L.nTiles, nb), dim3(256), 0, s, dTiles + L.tileBase,
-                           dNodes, n, Qt, Q);
+                               dv.nodes + L.nodeBase, n, D, E, Q, W, ord);
+        } else {
+            hipLaunchKernelGGL(k_dc_prep<kDcBig>, dim3(L.nNodes, nb), dim3(kDcBig), 0, s, dv.nodes,
+                               L.nodeBase, L.nNodes, n, D, E, Q, ord, st);
+            hipLaunchKernelGGL(k_dc_roots, dim3(L.nRoot, nb), dim3(256), 0, s, dv.rootTiles + L.rootBase,
+                               dv.nodes, L.nodeBase, L.nNodes, n, st);
+            hipLaunchKernelGGL(k_dc_vecs, dim3(L.nCol, nb), dim3(256), 0, s, dv.colTiles + L.colBase,
+                               dv.nodes, L.nodeBase, L.nNodes, n, D, W, st);
+        }
+        hipLaunchKernelGGL(k_dc_gemm, dim3(L.nTiles, nb), dim3(256), 0, s, dv.tiles + L.tileBase,
+                           dv.nodes, n, Q, W, ord, Qt);
+        hipLaunchKernelGGL(k_dc_copy, dim3(L.nTiles, nb), dim3(256), 0, s, dv.tiles + L.tileBase,
+                           dv.nodes, n, Qt, Q);
     }
 }
 

@@ -221,21 +221,34 @@ void launch_chol_diag(bool cplx, const double *B, double *U, double *Dinv, int n
 
 // Batched divide and conquer eigensolver for real symmetric tridiagonal
 // matrices (kdc.hip): D (diagonal in, eigenvalues out, ascending), E
-// (couplings), Q (eigenvectors out, column-major n x n per matrix); Qt, W, S:
+// (couplings), Q (eigenvectors out, column-major n x n per matrix); Qt, W:
 // n x n work per matrix, ord: n ints per matrix.
 constexpr int kDcMaxN = 64 * 10;
 struct DcLevel {
     int nodeBase = 0, nNodes = 0, tileBase = 0, nTiles = 0, maxM = 0;
+    int rootBase = 0, nRoot = 0, colBase = 0, nCol = 0;
 };
 struct DcPlan {
     int n = 0;
-    std::vector<int4> nodes;   // (offset, first child size, size, -) by height
-    std::vector<int4> tiles;   // (node, row tile, column tile, -)
+    std::vector<int4> nodes;       // (offset, first child size, size, -) by height
+    std::vector<int4> tiles;       // GEMM tiles: (node, row tile, column tile, -)
+    std::vector<int2> rootTiles;   // (node within level, root chunk)
+    std::vector<int2> colTiles;    // (node within level, column chunk)
     std::vector<DcLevel> levels;
 };
+// device copies of the plan's lists and the state of split merges
+struct DcDev {
+    const int4 *nodes, *tiles;
+    const int2 *rootTiles, *colTiles;
+    double *stD;      // dc_state_doubles
+    int *stI;         // dc_state_ints
+    int4 *cnt;        // nodes x nb
+    double *rho;      // nodes x nb
+};
 void dc_plan(int n, DcPlan &p);
-void launch_dc(const DcPlan &p, const int4 *dNodes, const int4 *dTiles, int nb, double *D,
-               const double *E, double *Q, double *Qt, double *W, double *S, int *ord,
-               hipStream_t s);
+size_t dc_state_doubles(int n, int nb);
+size_t dc_state_ints(int n, int nb);
+void launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double *E, double *Q,
+               double *Qt, double *W, int *ord, hipStream_t s);
 
 }  // namespace olcao

@@ -89,12 +89,15 @@ struct DBuf {
 // Buffers of the tridiagonal eigensolver path, kept between calls.
 struct EigWork {
     DBuf<double> V, U, W, tau, pub;        // reflectors, block factors, GEMM work
-    DBuf<double> Z, Zt, Wd, Sd;             // divide and conquer: vectors, work
+    DBuf<double> Z, Zt, Wd;                 // divide and conquer: vectors, work
     DBuf<double> Uc, Dinv, Y;               // Cholesky factor, inverted blocks, solve work
     DBuf<unsigned> cnt;
     DBuf<int> ord;
     DcPlan dc;
-    DBuf<int4> dcNodes, dcTiles;
+    DBuf<int4> dcNodes, dcTiles, dcCnt;
+    DBuf<int2> dcRoot, dcCol;
+    DBuf<double> dcStD, dcRho;
+    DBuf<int> dcStI;
 };
 
 double angnorm(int lm)   // lm = 1..16, basis.f90:86-101
@@ -951,17 +954,23 @@ void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, 
     ew.Z.alloc(per * nb);
     ew.Zt.alloc(per * nb);
     ew.Wd.alloc(per * nb);
-    ew.Sd.alloc(per * nb);
     ew.ord.alloc((size_t)n * nb);
     if (ew.dc.n != n) {
         dc_plan(n, ew.dc);
         ew.dcNodes.upload(ew.dc.nodes, s);
         ew.dcTiles.upload(ew.dc.tiles, s);
+        ew.dcRoot.upload(ew.dc.rootTiles, s);
+        ew.dcCol.upload(ew.dc.colTiles, s);
     }
     HIPCHK(hipMemsetAsync(ew.cnt.p, 0, plan.counters * sizeof(unsigned), s));
     HIPCHK(launch_trd(cplx, plan, n, nb, A, ew.V.p, D, E, ew.tau.p, ew.pub.p, ew.cnt.p, dinfo, s));
-    launch_dc(ew.dc, ew.dcNodes.p, ew.dcTiles.p, nb, D, E, ew.Z.p, ew.Zt.p, ew.Wd.p, ew.Sd.p,
-              ew.ord.p, s);
+    ew.dcStD.alloc(dc_state_doubles(n, nb));
+    ew.dcStI.alloc(dc_state_ints(n, nb));
+    ew.dcCnt.alloc(ew.dc.nodes.size() * nb);
+    ew.dcRho.alloc(ew.dc.nodes.size() * nb);
+    const DcDev dv{ew.dcNodes.p, ew.dcTiles.p, ew.dcRoot.p, ew.dcCol.p, ew.dcStD.p, ew.dcStI.p,
+                   ew.dcCnt.p, ew.dcRho.p};
+    launch_dc(ew.dc, dv, nb, D, E, ew.Z.p, ew.Zt.p, ew.Wd.p, ew.ord.p, s);
     HIPCHK(hipGetLastError());
     launch_trd_tfactor(cplx, n, nb, ew.V.p, ew.tau.p, ew.U.p, s);
     HIPCHK(hipGetLastError());
