@@ -344,23 +344,32 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
                 return;
             }
             // ---- vectors: w, column i+1, next reflector ----
+            // every publication load is issued before any of them is used, so
+            // the step pays one memory round trip
+            T pk[KC], ck[KC];
+#pragma unroll
+            for (int k = 0; k < KC; ++k) {
+                const int c = 64 * k + lane;
+                const int cc = c < n && c > i ? c : i + 1;   // in range, value unused
+                pk[k] = pldT<CPLX>(pP, cc);
+                ck[k] = pldT<CPLX>(pC, cc);
+            }
+            const T p1 = pldT<CPLX>(pP, i + 1);
             T part = N::zero();
             for (int q = lane; q < kTrdWaves * G; q += 64) part = N::add(part, pldT<CPLX>(pPart, q));
             const T dot = wsum<CPLX>(part);
             const T alpha = N::scale(N::mul(tau, dot), -0.5);
             // w(i+1) = p(i+1) + alpha since v_i(i+1) = 1
-            const T w1 = N::add(pldT<CPLX>(pP, i + 1), alpha);
+            const T w1 = N::add(p1, alpha);
 #pragma unroll
             for (int k = 0; k < KC; ++k) {
                 const int c = 64 * k + lane;
-                T wk = N::zero(), ck = N::zero();
-                if (c < n && c > i) {
-                    wk = N::add(pldT<CPLX>(pP, c), N::mul(alpha, vc[k]));
-                    ck = N::sub(N::sub(pldT<CPLX>(pC, c), N::cmul(w1, vc[k])), wk);
-                }
+                const bool on = c < n && c > i;
+                const T wk = N::add(pk[k], N::mul(alpha, vc[k]));
+                const T cv = N::sub(N::sub(ck[k], N::cmul(w1, vc[k])), wk);
                 vp[k] = vc[k];
-                wp[k] = wk;
-                cn[k] = ck;
+                wp[k] = N::sel(on, wk, N::zero());
+                cn[k] = N::sel(on, cv, N::zero());
             }
             // the same for this wave's rows (lane j handles row j)
             if (lane < kTrdRpw) {
