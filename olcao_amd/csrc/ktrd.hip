@@ -402,19 +402,22 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
 
 // U_b = V_b T_b for one block of kTrdBlock reflectors of one matrix.
 // V: explicit reflectors (column j = v_j, zero above row j+1, 1 at j+1).
+// The Gram matrix G(a, j) = v_a^H v_j (a < j) is overwritten column by column
+// with T (zlarft: T(j, j) = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) G(0:j, j)).
 template <bool CPLX>
 __global__ __launch_bounds__(256) void k_trd_tfactor(const double *__restrict__ Vall,
                                                      const double *__restrict__ tauAll,
-                                                     double *Uall, int n)
+                                                     double *Tall, int n)
 {
     using N = Num<CPLX>;
     using T = typename N::T;
     constexpr int E = CPLX ? 2 : 1;
     constexpr int NB = kTrdBlock;
     constexpr int NP = NB * (NB - 1) / 2;
+    constexpr int PPT = (NP + 255) / 256;
     __shared__ T Vs[64][NB + 1];
     __shared__ T Gm[NB][NB + 1];
-    __shared__ T Tm[NB][NB + 1];
+    __shared__ T tcol[NB];
     const int tid = threadIdx.x;
     const int j0 = blockIdx.x * NB, mat = blockIdx.y;
     const int nref = n - 1;
@@ -422,11 +425,9 @@ __global__ __launch_bounds__(256) void k_trd_tfactor(const double *__restrict__ 
     const long long per = (long long)n * n;
     const double *V = Vall + E * per * mat;
     const double *tau = tauAll + (long long)E * n * mat;
-    double *U = Uall + E * per * mat;
     const int r0 = j0 + 1;   // first row of the block's reflectors
 
     // Gram matrix of the block: Gm(a, j) = v_a^H v_j, a < j
-    constexpr int PPT = (NP + 255) / 256;
     T acc[PPT];
     int pa[PPT], pj[PPT];
 #pragma unroll
@@ -446,52 +447,73 @@ __global__ __launch_bounds__(256) void k_trd_tfactor(const double *__restrict__ 
             Vs[rr][a] = (r < n && a < mb) ? N::load(V, r + (long long)(j0 + a) * n) : N::zero();
         }
         __syncthreads();
+#pragma unroll 1
+        for (int rr = 0; rr < 64; ++rr) {
 #pragma unroll
-        for (int t = 0; t < PPT; ++t) {
-            if (tid + 256 * t < NP)
-                for (int rr = 0; rr < 64; ++rr)
-                    acc[t] = N::add(acc[t], N::cmul(Vs[rr][pa[t]], Vs[rr][pj[t]]));
+            for (int t = 0; t < PPT; ++t)
+                acc[t] = N::add(acc[t], N::cmul(Vs[rr][pa[t] < NB ? pa[t] : 0], Vs[rr][pj[t] < NB ? pj[t] : 0]));
         }
         __syncthreads();
     }
 #pragma unroll
     for (int t = 0; t < PPT; ++t)
         if (tid + 256 * t < NP) Gm[pa[t]][pj[t]] = acc[t];
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int a = e % NB, j = e / NB;
-        Tm[a][j] = N::zero();
-    }
     __syncthreads();
-    // T(j,j) = tau_j; T(0:j, j) = -tau_j T(0:j, 0:j) Gm(0:j, j)
+    // T over G, column by column (columns < j already hold T)
     for (int j = 0; j < mb; ++j) {
         const T tj = N::load(tau, j0 + j);
         if (tid < j) {
-            T s = N::zero();
-            for (int q = tid; q < j; ++q) s = N::add(s, N::mul(Tm[tid][q], Gm[q][j]));
-            Tm[tid][j] = N::scale(N::mul(tj, s), -1.0);
+            T sum = N::zero();
+            for (int q = tid; q < j; ++q) sum = N::add(sum, N::mul(Gm[tid][q], Gm[q][j]));
+            tcol[tid] = N::scale(N::mul(tj, sum), -1.0);
         }
-        if (tid == j) Tm[j][j] = tj;
+        __syncthreads();
+        if (tid < j) Gm[tid][j] = tcol[tid];
+        if (tid == j) Gm[j][j] = tj;
         __syncthreads();
     }
-    // U = V_b T for rows r0..n-1
-    for (int rb = r0; rb < n; rb += 64) {
-        for (int e = tid; e < 64 * NB; e += 256) {
-            const int rr = e % 64, a = e / 64, r = rb + rr;
-            Vs[rr][a] = (r < n && a < mb) ? N::load(V, r + (long long)(j0 + a) * n) : N::zero();
-        }
-        __syncthreads();
-        const int rr = tid % 64, jg = tid / 64;
-        const int r = rb + rr;
-        if (r < n) {
-            for (int jj = 0; jj < NB / 4; ++jj) {
-                const int j = jg * (NB / 4) + jj;
-                if (j >= mb) break;
-                T s = N::zero();
-                for (int a = 0; a <= j; ++a) s = N::add(s, N::mul(Vs[rr][a], Tm[a][j]));
-                N::store(U, r + (long long)(j0 + j) * n, s);
-            }
-        }
-        __syncthreads();
+    // T (upper triangle) out for the row-parallel product U = V_b T
+    double *Tg = Tall + E * (long long)NB * NB * (blockIdx.x + gridDim.x * mat);
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int a = e % NB, j = e / NB;
+        N::store(Tg, e, a <= j && j < mb ? Gm[a][j] : N::zero());
+    }
+}
+
+// U_b = V_b T_b: workgroup (block, matrix, 64-row chunk), thread (row, 16 columns)
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_trd_vt(const double *__restrict__ Vall,
+                                                const double *__restrict__ Tall, double *Uall, int n)
+{
+    using N = Num<CPLX>;
+    using T = typename N::T;
+    constexpr int E = CPLX ? 2 : 1;
+    constexpr int NB = kTrdBlock;
+    __shared__ T Vs[64][NB + 1];
+    __shared__ T Ts[NB][NB + 1];
+    const int tid = threadIdx.x, blk = blockIdx.x, mat = blockIdx.y;
+    const int j0 = blk * NB, mb = min(NB, n - 1 - j0);
+    const int rb = j0 + 1 + 64 * blockIdx.z;
+    if (rb >= n) return;
+    const long long per = (long long)n * n;
+    const double *V = Vall + E * per * mat;
+    const double *Tg = Tall + E * (long long)NB * NB * (blk + gridDim.x * mat);
+    double *U = Uall + E * per * mat;
+    for (int e = tid; e < NB * NB; e += 256) Ts[e % NB][e / NB] = N::load(Tg, e);
+    for (int e = tid; e < 64 * NB; e += 256) {
+        const int rr = e % 64, a = e / 64, r = rb + rr;
+        Vs[rr][a] = (r < n && a < mb) ? N::load(V, r + (long long)(j0 + a) * n) : N::zero();
+    }
+    __syncthreads();
+    const int rr = tid % 64, jg = tid / 64;
+    const int r = rb + rr;
+    if (r >= n) return;
+    for (int jj = 0; jj < NB / 4; ++jj) {
+        const int j = jg * (NB / 4) + jj;
+        if (j >= mb) break;
+        T sum = N::zero();
+        for (int a = 0; a <= j; ++a) sum = N::add(sum, N::mul(Vs[rr][a], Ts[a][j]));
+        N::store(U, r + (long long)(j0 + j) * n, sum);
     }
 }
 
@@ -581,14 +603,18 @@ hipError_t launch_trd(bool cplx, const TrdPlan &p, int n, int nb, const double *
                                       dim3(kTrdThreads), args, (unsigned)p.lds, s);
 }
 
-void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double *tau, double *U,
-                        hipStream_t s)
+void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double *tau, double *Tw,
+                        double *U, hipStream_t s)
 {
     const int blocks = (n - 1 + kTrdBlock - 1) / kTrdBlock;
-    if (cplx)
-        hipLaunchKernelGGL(k_trd_tfactor<true>, dim3(blocks, nb), dim3(256), 0, s, V, tau, U, n);
-    else
-        hipLaunchKernelGGL(k_trd_tfactor<false>, dim3(blocks, nb), dim3(256), 0, s, V, tau, U, n);
+    const dim3 g2(blocks, nb, (n + 63) / 64);
+    if (cplx) {
+        hipLaunchKernelGGL(k_trd_tfactor<true>, dim3(blocks, nb), dim3(256), 0, s, V, tau, Tw, n);
+        hipLaunchKernelGGL(k_trd_vt<true>, g2, dim3(256), 0, s, V, Tw, U, n);
+    } else {
+        hipLaunchKernelGGL(k_trd_tfactor<false>, dim3(blocks, nb), dim3(256), 0, s, V, tau, Tw, n);
+        hipLaunchKernelGGL(k_trd_vt<false>, g2, dim3(256), 0, s, V, Tw, U, n);
+    }
 }
 
 void launch_real_to_field(const double *Z, double *X, long long count, bool cplx, hipStream_t s)

@@ -193,7 +193,7 @@ void launch_extract_evecs(const double *A, int n, int nk, int numStates,
 // group of workgroups (ktrd.hip).  A: full Hermitian (lower triangle read);
 // V: explicit reflectors, column i = v_i; d, e: [nb][n]; tau: [nb][n]
 // (complex for cplx); info[k] = -2 when a group barrier timed out.
-constexpr int kTrdBlock = 32;   // reflectors per back-transform block
+constexpr int kTrdBlock = 64;   // reflectors per back-transform block
 struct TrdPlan {
     int G = 0;            // workgroups per matrix
     int nGroups = 0;      // matrices in flight
@@ -206,9 +206,9 @@ hipError_t launch_trd(bool cplx, const TrdPlan &p, int n, int nb, const double *
                       double *d, double *e, double *tau, double *pub, unsigned *cnt, int *info,
                       hipStream_t s);
 // U_b = V_b T_b per block of kTrdBlock reflectors (rows j0+1..n-1 of columns
-// j0..j0+kTrdBlock-1 of U)
-void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double *tau, double *U,
-                        hipStream_t s);
+// j0..j0+kTrdBlock-1 of U); Tw: kTrdBlock^2 per block and matrix (work)
+void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double *tau, double *Tw,
+                        double *U, hipStream_t s);
 void launch_real_to_field(const double *Z, double *X, long long count, bool cplx, hipStream_t s);
 
 // Blocked batched Cholesky, diagonal step (kchol.hip): factors the updated

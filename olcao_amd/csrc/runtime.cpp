@@ -88,7 +88,7 @@ struct DBuf {
 
 // Buffers of the tridiagonal eigensolver path, kept between calls.
 struct EigWork {
-    DBuf<double> V, U, W, tau, pub;        // reflectors, block factors, GEMM work
+    DBuf<double> V, U, W, Tw, tau, pub;    // reflectors, block factors, GEMM work
     DBuf<double> Z, Zt, Wd;                 // divide and conquer: vectors, work
     DBuf<double> Uc, Dinv, Y;               // Cholesky factor, inverted blocks, solve work
     DBuf<unsigned> cnt;
@@ -972,7 +972,8 @@ void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, 
                    ew.dcCnt.p, ew.dcRho.p};
     launch_dc(ew.dc, dv, nb, D, E, ew.Z.p, ew.Zt.p, ew.Wd.p, ew.ord.p, s);
     HIPCHK(hipGetLastError());
-    launch_trd_tfactor(cplx, n, nb, ew.V.p, ew.tau.p, ew.U.p, s);
+    ew.Tw.alloc((size_t)((n - 1 + kTrdBlock - 1) / kTrdBlock) * kTrdBlock * kTrdBlock * el * nb);
+    launch_trd_tfactor(cplx, n, nb, ew.V.p, ew.tau.p, ew.Tw.p, ew.U.p, s);
     HIPCHK(hipGetLastError());
     launch_real_to_field(ew.Z.p, A, (long long)per * nb, cplx, s);
     HIPCHK(hipGetLastError());
