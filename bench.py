@@ -43,6 +43,13 @@ WORKLOADS = {
     "quartz": "quartz: alpha-quartz SiO2 (9 atoms), Si s+p+d, O 1s 2s 2p, 8x8x8 k (512 k), n=57",
 }
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes of one mixed-form three-centre sweep (all k_three_center and
+# k_herm_finish dispatches), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# runs scaled by a 256 MiB device copy's counter readings (fetch x2.00,
+# write x1.00); profiles/k_three_center_si64_hbm_traffic.md has the commands
+# and the per-kernel figures.  Counters cannot be read inside a timed run, so
+# the figure is carried here; other workloads report null.
+SWEEP_HBM_BYTES = {"si64": 8.2945e9}
 
 
 def cpu_baseline(sysin, total_triples, stride):
@@ -125,6 +132,8 @@ def main():
                          "way for the design notes")
     ap.add_argument("--fp64-only", action="store_true",
                     help="time the FP64 form only (for counter runs)")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="time the mixed form only (for counter runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES once on the device (the reference kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -150,6 +159,11 @@ def main():
 
     s = getattr(configs, args.workload)()
     print(f"[bench] setting up {s.name}: valeDim={s.vale_dim} nk={s.nk}", flush=True)
+    if args.mixed_only:
+        mixed = run_form(s, gpu.PREC_MIXED, args.steps, args.warmup, "mixed")
+        print(json.dumps({"workload": args.workload, "mixed_s_per_iter": mixed["sec"],
+                          "stage_ms_mixed": mixed["stage"]}), flush=True)
+        return
     fp64 = run_form(s, gpu.PREC_FP64, args.steps, args.warmup, "fp64")
     if args.fp64_only:
         print(json.dumps({"workload": args.workload, "fp64_s_per_iter": fp64["sec"],
@@ -195,7 +209,8 @@ def main():
                    "mixed_vs_fp64_band_energy_err_Ha_per_atom": de,
                    "stage_ms_mixed": mixed["stage"], "stage_ms_fp64": fp64["stage"]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": SWEEP_HBM_BYTES.get(args.workload),
                      "kernel": "k_three_center (all classes, one sweep, mixed form)",
                      "flops_per_launch": flops, "launch_ms": t3c * 1e3},
     }

@@ -88,7 +88,9 @@ typedef struct {
 
 /* Generalised eigensolve: OLCAO_EIG_AUTO uses the batched in-LDS solver
  * (one workgroup per k-point: Cholesky, reduction, Jacobi) where the matrices
- * fit (complex n <= 69, real n <= 98) and rocSOLVER otherwise;
+ * fit (complex n <= 69, real n <= 98), else the device tridiagonal path
+ * (blocked Cholesky, cooperative Householder reduction, divide and conquer,
+ * blocked back-transform) up to n = 640, and rocSOLVER beyond;
  * OLCAO_EIG_LIBRARY always uses rocSOLVER. */
 enum { OLCAO_EIG_AUTO = 0, OLCAO_EIG_LIBRARY = 1 };
 
