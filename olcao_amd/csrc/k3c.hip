@@ -309,14 +309,16 @@ static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, i
 
 #define OLCAO_CAT2(a, b) a##b
 #define OLCAO_CAT(a, b) OLCAO_CAT2(a, b)
+// The classes (OLCAO_LA, lb) whose bit 4 OLCAO_LA + lb is set in mask.
 void OLCAO_CAT(launch_3c_la, OLCAO_LA)(const DevCtx &c, const ClassLists &cl, int kind,
                                         int tt, int ti, const double *coef, int accumulate,
-                                        double *blocks, int mixed, hipStream_t s)
+                                        double *blocks, int mixed, unsigned mask, hipStream_t s)
 {
-    three_one<OLCAO_LA, 0>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
-    three_one<OLCAO_LA, 1>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
-    three_one<OLCAO_LA, 2>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
-    three_one<OLCAO_LA, 3>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+    constexpr int k0 = 4 * OLCAO_LA;
+    if (mask >> (k0 + 0) & 1u) three_one<OLCAO_LA, 0>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+    if (mask >> (k0 + 1) & 1u) three_one<OLCAO_LA, 1>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+    if (mask >> (k0 + 2) & 1u) three_one<OLCAO_LA, 2>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+    if (mask >> (k0 + 3) & 1u) three_one<OLCAO_LA, 3>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
 }
 
 }  // namespace olcao

@@ -389,22 +389,22 @@ void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int acc, d
 }
 
 void launch_3c_la0(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
-                   double *, int, hipStream_t);
+                   double *, int, unsigned, hipStream_t);
 void launch_3c_la1(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
-                   double *, int, hipStream_t);
+                   double *, int, unsigned, hipStream_t);
 void launch_3c_la2(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
-                   double *, int, hipStream_t);
+                   double *, int, unsigned, hipStream_t);
 void launch_3c_la3(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
-                   double *, int, hipStream_t);
+                   double *, int, unsigned, hipStream_t);
 
 void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind, int tt, int ti,
                          const double *coef, int accumulate, double *blocks, int mixed,
-                         hipStream_t s)
+                         unsigned mask, hipStream_t s)
 {
-    launch_3c_la0(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
-    launch_3c_la1(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
-    launch_3c_la2(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
-    launch_3c_la3(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+    if (mask & 0x000fu) launch_3c_la0(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, mask, s);
+    if (mask & 0x00f0u) launch_3c_la1(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, mask, s);
+    if (mask & 0x0f00u) launch_3c_la2(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, mask, s);
+    if (mask & 0xf000u) launch_3c_la3(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, mask, s);
 }
 
 // ---------------------------------------------------------------------------

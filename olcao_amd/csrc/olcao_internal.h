@@ -145,9 +145,12 @@ struct ClassLists {
 void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int accumulate,
                        double *blocks, hipStream_t s);
 // mixed: three-Gaussian exponentials by the mixed-precision evaluation.
+// Launches the classes k = 4 la + lb whose bit k is set in mask; different
+// classes touch disjoint memory (their own W slice, their own primitive-pair
+// blocks, their own work counters), so they may run on different streams.
 void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind, int termType,
                          int termIdx, const double *coef, int accumulate, double *blocks,
-                         int mixed, hipStream_t s);
+                         int mixed, unsigned mask, hipStream_t s);
 // Image lists: pass 1 counts entries and fused-sweep items per record, pass 2
 // writes the entries at off[i]; *overflow is set if a lattice index does not
 // fit the entry.

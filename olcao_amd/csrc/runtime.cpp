@@ -131,6 +131,10 @@ struct olcao_ctx {
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
     hipEvent_t ev[12] = {};
+    // side streams: the three-centre classes fork onto them and join back
+    static constexpr int kSide = 3;
+    hipStream_t side[kSide] = {};
+    hipEvent_t evFork = nullptr, evJoin[kSide] = {};
 
     // host-side derived setup
     int nAtoms = 0, nT = 0, maxA = 0, maxS = 0, nPT = 0, maxPA = 0, nPS = 0;
@@ -776,13 +780,57 @@ void build_image_lists(olcao_ctx &c)
     c.listMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// The three-centre sweep, its 16 angular classes spread over the context
+// stream and the side streams and joined back onto the context stream.  The
+// classes are independent (each has its own W slice, primitive-pair blocks
+// and counters).  All but the d-d class are small launches whose time is the
+// serial chain of one lane, not the chip's throughput; beside the d-d class
+// they cost little.  Longest first onto the least-loaded stream, weight =
+// chunks x Hermite sums per item.
+void three_center_forked(olcao_ctx &c, int kind, int tt, int ti, const double *coef,
+                         int accumulate)
+{
+    constexpr int nS = 1 + olcao_ctx::kSide;
+    hipStream_t st[nS] = {c.stream};
+    for (int i = 1; i < nS; ++i) st[i] = c.side[i - 1];
+    int order[16];
+    double w[16];
+    for (int k = 0; k < 16; ++k) {
+        const int L = k / 4 + k % 4;
+        order[k] = k;
+        w[k] = c.cls.count[k] ? (double)c.cls.nChunk[k] * ((L + 1) * (L + 2) * (L + 3) / 6) : 0.0;
+    }
+    std::stable_sort(order, order + 16, [&](int a, int b) { return w[a] > w[b]; });
+    double load[nS] = {};
+    unsigned mask[nS] = {};
+    for (int k : order) {
+        if (w[k] == 0.0) continue;
+        const int i = (int)(std::min_element(load, load + nS) - load);
+        mask[i] |= 1u << k;
+        load[i] += w[k];
+    }
+    const int mixed = c.precision == OLCAO_PREC_MIXED;
+    HIPCHK(hipEventRecord(c.evFork, c.stream));
+    for (int i = 1; i < nS; ++i)
+        if (mask[i]) HIPCHK(hipStreamWaitEvent(st[i], c.evFork, 0));
+    for (int i = 0; i < nS; ++i)
+        if (mask[i])
+            launch_three_center(c.dev, c.cls, kind, tt, ti, coef, accumulate, c.dBlocks.p, mixed,
+                                mask[i], st[i]);
+    HIPCHK(hipGetLastError());
+    for (int i = 1; i < nS; ++i)
+        if (mask[i]) {
+            HIPCHK(hipEventRecord(c.evJoin[i - 1], st[i]));
+            HIPCHK(hipStreamWaitEvent(c.stream, c.evJoin[i - 1], 0));
+        }
+}
+
 // Three-centre sweep with its work counters cleared first (a fill kernel,
 // so the sequence stays capturable in a graph).
 void three_center(olcao_ctx &c, int kind, int tt, int ti, const double *coef, int accumulate)
 {
     launch_fill(reinterpret_cast<double *>(c.dCounters.p), 32, 0.0, c.stream);
-    launch_three_center(c.dev, c.cls, kind, tt, ti, coef, accumulate, c.dBlocks.p,
-                        c.precision == OLCAO_PREC_MIXED, c.stream);
+    three_center_forked(c, kind, tt, ti, coef, accumulate);
 }
 
 // ---------------------------------------------------------------------------
@@ -1293,6 +1341,9 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
             fail(OLCAO_ERR_INPUT, "precision must be OLCAO_PREC_FP64 or OLCAO_PREC_MIXED");
         HIPCHK(hipStreamCreate(&c->stream));
         for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+        for (auto &q : c->side) HIPCHK(hipStreamCreate(&q));
+        HIPCHK(hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming));
+        for (auto &e : c->evJoin) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         if (rocblas_create_handle(&c->blas) != rocblas_status_success)
             fail(OLCAO_ERR_HIP, "rocblas_create_handle failed");
         rocblas_set_stream(c->blas, c->stream);
@@ -1314,9 +1365,16 @@ void olcao_ctx_destroy(olcao_ctx *c)
 {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto q : c->side)
+        if (q) (void)hipStreamSynchronize(q);
     if (c->blas) rocblas_destroy_handle(c->blas);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->evFork) (void)hipEventDestroy(c->evFork);
+    for (auto e : c->evJoin)
+        if (e) (void)hipEventDestroy(e);
+    for (auto q : c->side)
+        if (q) (void)hipStreamDestroy(q);
     hipStream_t s = c->stream;
     delete c;
     if (s) (void)hipStreamDestroy(s);
@@ -1500,8 +1558,7 @@ olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates
             if (c->rel) launch_two_center(c->dev, c->cls, OP_MV, 1, c->dBlocks.p, s);
             launch_fill(reinterpret_cast<double *>(c->dCounters.p), 32, 0.0, s);
             HIPCHK(hipEventRecord(c->ev[2 + 3 * q], s));
-            launch_three_center(c->dev, c->cls, K3_HAM, 0, 0, c->dCoef.p + (size_t)q * c->potDim,
-                                1, c->dBlocks.p, c->precision == OLCAO_PREC_MIXED, s);
+            three_center_forked(*c, K3_HAM, 0, 0, c->dCoef.p + (size_t)q * c->potDim, 1);
             HIPCHK(hipEventRecord(c->ev[3 + 3 * q], s));
             contract_scatter(*c, c->H[q].p, c->CV_H[q].p, c->CC_H[q].p);
             ortho(*c, c->H[q].p, c->CV_H[q].p, c->CC_H[q].p, false);
