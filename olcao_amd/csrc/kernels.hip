@@ -84,37 +84,37 @@ __global__ __launch_bounds__(64) void k_two_center(DevCtx c, const int *list, in
 // contraction: X(s1,s2) = sum over the pair-cell's primitive pairs of
 //   C1(a1,s1) * block(lm(s1), lm(s2)) * C2(a2,s2)
 // The primitive pairs of a pair-cell are cut into segments of at most
-// kSegPP (an on-site pair-cell has several hundred, a distant one a few), one
-// wavefront per segment, so that no pair-cell serialises the launch.  Lane q
-// holds the record of the segment's q-th primitive pair and the wave walks
-// them in order, reading each record from that lane; each lane sums up to
-// kContractOut outputs at a time.  A single-segment pair-cell writes X
-// directly; otherwise each segment writes a partial block and k_contract_sum
-// adds the partials in segment order (deterministic).
-constexpr int kContractOut = 4;
+// kSegPP pairs (an on-site pair-cell has several hundred, a distant one a
+// few), one wavefront per segment, so that no pair-cell serialises the
+// launch.  Lane q holds the q-th pair's record; each lane sums up to
+// kContractOut outputs at a time, walking the pairs in order in batches of
+// kQBatch whose block loads are all issued before the first is used (the
+// lanes of one pair read neighbouring words of its block).  The two atom
+// types' coefficient tables C(a, s) are staged in LDS.  A single-segment
+// pair-cell writes X directly; otherwise each segment writes a partial block
+// and k_contract_sum adds the partials in segment order (deterministic).
+constexpr int kContractOut = 2;
+constexpr int kQBatch = 16;
 
-static __device__ __forceinline__ long long readlane64(long long v, int l)
+__global__ __launch_bounds__(64) void k_contract_seg(DevCtx c, const ContractSeg *segs, int nSeg,
+                                                     const double *blocks, double *X,
+                                                     double *partial)
 {
-    const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), l);
-    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
-    return ((long long)hi << 32) | (unsigned int)lo;
-}
-
-__global__ __launch_bounds__(256) void k_contract_seg(DevCtx c, const int4 *segs, int nSeg,
-                                                      const double *blocks, double *X,
-                                                      double *partial)
-{
-    const int lane = threadIdx.x & 63;
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const int w = blockIdx.x;
     if (w >= nSeg) return;
-    const int4 sg = segs[w];
-    const int pc = sg.x, p0 = sg.y, np = sg.z, part = sg.w;
-    const int pr = c.pcPair[pc];
-    const int ti = c.atomType[c.pairI[pr]], tj = c.atomType[c.pairJ[pr]];
-    const int nt1 = c.nVale[ti] + c.nCore[ti], nt2 = c.nVale[tj] + c.nCore[tj];
+    const ContractSeg sg = segs[w];
+    const int pc = sg.pc, p0 = sg.p0, np = sg.np, part = sg.part;
+    const int ti = sg.ti, tj = sg.tj, nt1 = sg.nt1, nt2 = sg.nt2;
     const int ms = c.maxS, ma = c.maxA;
+    double *sC1 = lds, *sC2 = lds + ms * ma;
+    const double *bseg = blocks + sg.base;
     const int myAB = lane < np ? c.ppAB[p0 + lane] : 0;
-    const long long myOff = lane < np ? c.ppOff[p0 + lane] : 0;
+    const int myRel = lane < np ? (int)(c.ppOff[p0 + lane] - sg.base) : 0;
+    for (int e = lane; e < nt1 * ma; e += 64) sC1[e] = c.basis[(long long)ti * ms * ma + e];
+    for (int e = lane; e < nt2 * ma; e += 64) sC2[e] = c.basis[(long long)tj * ms * ma + e];
+    __syncthreads();
     double *dst = (part < 0 ? X + (long long)pc * ms * ms : partial + (long long)part * ms * ms);
     const int nout = nt1 * nt2;
     for (int g = 0; g < nout; g += 64 * kContractOut) {
@@ -130,18 +130,32 @@ __global__ __launch_bounds__(256) void k_contract_seg(DevCtx c, const int4 *segs
             lm1v[r] = ok ? c.lmIndex[ti * ms + s1] - 1 : 99;
             lm2v[r] = ok ? c.lmIndex[tj * ms + s2] - 1 : 99;
             dv[r] = ok ? s1 * ms + s2 : -1;
-            c1v[r] = c.basis + (long long)(ti * ms + s1) * ma;
-            c2v[r] = c.basis + (long long)(tj * ms + s2) * ma;
+            c1v[r] = sC1 + s1 * ma;
+            c2v[r] = sC2 + s2 * ma;
         }
-        for (int q = 0; q < np; ++q) {
-            const int ab = __builtin_amdgcn_readlane(myAB, q);
-            const long long off = readlane64(myOff, q);
-            const int n1 = nsh(pp_la(ab)), n2 = nsh(pp_lb(ab));
-            const int a1 = pp_a1(ab), a2 = pp_a2(ab);
+        for (int q0 = 0; q0 < np; q0 += kQBatch) {
+            double v[kQBatch][kContractOut];
 #pragma unroll
-            for (int r = 0; r < kContractOut; ++r)
-                if (lm1v[r] < n1 && lm2v[r] < n2)
-                    acc[r] += c1v[r][a1] * blocks[off + lm1v[r] * n2 + lm2v[r]] * c2v[r][a2];
+            for (int u = 0; u < kQBatch; ++u) {
+                const int ab = __builtin_amdgcn_readlane(myAB, q0 + u);
+                const int rel = __builtin_amdgcn_readlane(myRel, q0 + u);
+                const int n1 = nsh(pp_la(ab)), n2 = nsh(pp_lb(ab));
+#pragma unroll
+                for (int r = 0; r < kContractOut; ++r) {
+                    const bool ok = q0 + u < np && lm1v[r] < n1 && lm2v[r] < n2;
+                    v[u][r] = ok ? bseg[rel + lm1v[r] * n2 + lm2v[r]] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kQBatch; ++u) {
+                if (q0 + u >= np) break;
+                const int ab = __builtin_amdgcn_readlane(myAB, q0 + u);
+                const int n1 = nsh(pp_la(ab)), n2 = nsh(pp_lb(ab));
+                const int a1 = pp_a1(ab), a2 = pp_a2(ab);
+#pragma unroll
+                for (int r = 0; r < kContractOut; ++r)
+                    if (lm1v[r] < n1 && lm2v[r] < n2) acc[r] += c1v[r][a1] * v[u][r] * c2v[r][a2];
+            }
         }
 #pragma unroll
         for (int r = 0; r < kContractOut; ++r)
@@ -376,16 +390,15 @@ static void two_one(const DevCtx &c, const ClassLists &cl, int op, int acc, doub
 }
 
 void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int acc, double *blocks,
-                       hipStream_t s)
+                       unsigned mask, hipStream_t s)
 {
-    two_one<0, 0>(c, cl, op, acc, blocks, s); two_one<0, 1>(c, cl, op, acc, blocks, s);
-    two_one<0, 2>(c, cl, op, acc, blocks, s); two_one<0, 3>(c, cl, op, acc, blocks, s);
-    two_one<1, 0>(c, cl, op, acc, blocks, s); two_one<1, 1>(c, cl, op, acc, blocks, s);
-    two_one<1, 2>(c, cl, op, acc, blocks, s); two_one<1, 3>(c, cl, op, acc, blocks, s);
-    two_one<2, 0>(c, cl, op, acc, blocks, s); two_one<2, 1>(c, cl, op, acc, blocks, s);
-    two_one<2, 2>(c, cl, op, acc, blocks, s); two_one<2, 3>(c, cl, op, acc, blocks, s);
-    two_one<3, 0>(c, cl, op, acc, blocks, s); two_one<3, 1>(c, cl, op, acc, blocks, s);
-    two_one<3, 2>(c, cl, op, acc, blocks, s); two_one<3, 3>(c, cl, op, acc, blocks, s);
+#define OLCAO_TWO(la, lb) \
+    if (mask >> (4 * la + lb) & 1u) two_one<la, lb>(c, cl, op, acc, blocks, s);
+    OLCAO_TWO(0, 0) OLCAO_TWO(0, 1) OLCAO_TWO(0, 2) OLCAO_TWO(0, 3)
+    OLCAO_TWO(1, 0) OLCAO_TWO(1, 1) OLCAO_TWO(1, 2) OLCAO_TWO(1, 3)
+    OLCAO_TWO(2, 0) OLCAO_TWO(2, 1) OLCAO_TWO(2, 2) OLCAO_TWO(2, 3)
+    OLCAO_TWO(3, 0) OLCAO_TWO(3, 1) OLCAO_TWO(3, 2) OLCAO_TWO(3, 3)
+#undef OLCAO_TWO
 }
 
 void launch_3c_la0(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
@@ -549,8 +562,9 @@ void launch_contract(const DevCtx &c, const ContractPlan &cp, const double *bloc
                      hipStream_t s)
 {
     if (cp.nSeg == 0) return;
-    hipLaunchKernelGGL(k_contract_seg, dim3((cp.nSeg + 3) / 4), dim3(256), 0, s, c, cp.segs,
-                       cp.nSeg, blocks, X, cp.partial);
+    const size_t lds = (size_t)(2 * c.maxS * c.maxA) * sizeof(double);
+    hipLaunchKernelGGL(k_contract_seg, dim3(cp.nSeg), dim3(64), lds, s, c, cp.segs, cp.nSeg,
+                       blocks, X, cp.partial);
     if (cp.nMulti > 0)
         hipLaunchKernelGGL(k_contract_sum, dim3(cp.nMulti), dim3(64), 0, s, c, cp.multi,
                            cp.nMulti, cp.partial, X);

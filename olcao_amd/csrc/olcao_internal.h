@@ -142,8 +142,10 @@ struct ClassLists {
     const int *finOrder[16];    // [count]
 };
 
+// The classes k = 4 la + lb whose bit k is set in mask (independent, as for
+// launch_three_center below).
 void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int accumulate,
-                       double *blocks, hipStream_t s);
+                       double *blocks, unsigned mask, hipStream_t s);
 // mixed: three-Gaussian exponentials by the mixed-precision evaluation.
 // Launches the classes k = 4 la + lb whose bit k is set in mask; different
 // classes touch disjoint memory (their own W slice, their own primitive-pair
@@ -158,12 +160,20 @@ void launch_image_count(const DevCtx &c, const PPRec *recs, int n, int *nEnt, in
                         hipStream_t s);
 void launch_image_fill(const DevCtx &c, const PPRec *recs, int n, const long long *off,
                        unsigned long long *ent, int *overflow, hipStream_t s);
-// Contraction work: segments {pair-cell, first primitive pair, count,
-// partial index or -1} of at most kSegPP primitive pairs, and the pair-cells
-// that have several segments {pair-cell, first partial, segment count}.
+// Contraction work: segments of at most kSegPP primitive pairs, and the
+// pair-cells that have several segments {pair-cell, first partial, segment
+// count}.
 constexpr int kSegPP = 32;
+// One contraction segment with everything its wave needs before the first
+// block load: the atom types and state counts of its pair and the store
+// offset of its first block.
+struct ContractSeg {
+    int pc, p0, np, part;       // pair-cell, first primitive pair, count, partial or -1
+    int ti, tj, nt1, nt2;       // atom types, states (valence + core) per side
+    long long base;             // ppOff[p0]
+};
 struct ContractPlan {
-    const int4 *segs;
+    const ContractSeg *segs;
     int nSeg;
     const int4 *multi;
     int nMulti;

@@ -185,7 +185,8 @@ struct olcao_ctx {
     bool eigLibrary = false;     // always rocSOLVER, also where the LDS solver fits
     EigWork eig;
     ContractPlan cplan{};
-    DBuf<int4> dSegs, dMulti;
+    DBuf<ContractSeg> dSegs;
+    DBuf<int4> dMulti;
     DBuf<double> dPartial;
 
     size_t matDoubles() const { return (size_t)(gamma ? 1 : 2) * valeDim * valeDim * (gamma ? 1 : nk); }
@@ -604,20 +605,33 @@ void upload(olcao_ctx &c)
     c.dBlocks.alloc(std::max<long long>(c.blockDoubles, 1));
     c.dX.alloc(std::max<size_t>(c.hPcPair.size() * (size_t)c.maxS * c.maxS, 1));
     {   // contraction segments (at most kSegPP primitive pairs each)
-        std::vector<int4> segs, multi;
+        if (2 * (size_t)c.maxS * c.maxA * sizeof(double) > 64 * 1024)
+            fail(OLCAO_ERR_INPUT, "basis too large for the contraction kernel (states x exponents)");
+        std::vector<ContractSeg> segs;
+        std::vector<int4> multi;
+        std::vector<std::pair<int, int>> cut;
         int parts = 0;
         for (size_t pc = 0; pc < c.hPcPair.size(); ++pc) {
-            const int p0 = c.hPcPp[pc], np = c.hPcPp[pc + 1] - p0;
-            const int nseg = std::max(1, (np + kSegPP - 1) / kSegPP);
-            if (nseg == 1) {
-                segs.push_back(make_int4((int)pc, p0, np, -1));
+            const int p0 = c.hPcPp[pc], p1 = c.hPcPp[pc + 1];
+            cut.clear();
+            for (int q = p0; q < p1; q += kSegPP) cut.emplace_back(q, std::min(kSegPP, p1 - q));
+            const int pr = c.hPcPair[pc];
+            const int ti = c.hAtomType[c.hPairI[pr]], tj = c.hAtomType[c.hPairJ[pr]];
+            auto seg = [&](int q0, int nq, int part) {
+                ContractSeg g;
+                g.pc = (int)pc; g.p0 = q0; g.np = nq; g.part = part;
+                g.ti = ti; g.tj = tj;
+                g.nt1 = c.hNVale[ti] + c.hNCore[ti]; g.nt2 = c.hNVale[tj] + c.hNCore[tj];
+                g.base = q0 < (int)c.hPpOff.size() ? c.hPpOff[q0] : c.blockDoubles;
+                segs.push_back(g);
+            };
+            if (cut.size() <= 1) {
+                seg(p0, p1 - p0, -1);
                 continue;
             }
-            multi.push_back(make_int4((int)pc, parts, nseg, 0));
-            for (int q = 0; q < nseg; ++q)
-                segs.push_back(make_int4((int)pc, p0 + q * kSegPP, std::min(kSegPP, np - q * kSegPP),
-                                         parts + q));
-            parts += nseg;
+            multi.push_back(make_int4((int)pc, parts, (int)cut.size(), 0));
+            for (size_t q = 0; q < cut.size(); ++q) seg(cut[q].first, cut[q].second, parts + (int)q);
+            parts += (int)cut.size();
         }
         c.dSegs.upload(segs, s);
         c.dMulti.upload(multi, s);
@@ -780,26 +794,20 @@ void build_image_lists(olcao_ctx &c)
     c.listMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// The three-centre sweep, its 16 angular classes spread over the context
-// stream and the side streams and joined back onto the context stream.  The
-// classes are independent (each has its own W slice, primitive-pair blocks
-// and counters).  All but the d-d class are small launches whose time is the
-// serial chain of one lane, not the chip's throughput; beside the d-d class
-// they cost little.  Longest first onto the least-loaded stream, weight =
-// chunks x Hermite sums per item.
-void three_center_forked(olcao_ctx &c, int kind, int tt, int ti, const double *coef,
-                         int accumulate)
+// Per-class work spread over the context stream and the side streams and
+// joined back onto the context stream.  The 16 angular classes are
+// independent (each has its own primitive-pair blocks, W slice and
+// counters).  Most of them are small launches whose time is the serial
+// chain of one lane, not the chip's throughput; beside the largest class
+// they cost little.  Longest first onto the least-loaded stream.
+template <class F>
+void fork_classes(olcao_ctx &c, const double (&w)[16], F launch)
 {
     constexpr int nS = 1 + olcao_ctx::kSide;
     hipStream_t st[nS] = {c.stream};
     for (int i = 1; i < nS; ++i) st[i] = c.side[i - 1];
     int order[16];
-    double w[16];
-    for (int k = 0; k < 16; ++k) {
-        const int L = k / 4 + k % 4;
-        order[k] = k;
-        w[k] = c.cls.count[k] ? (double)c.cls.nChunk[k] * ((L + 1) * (L + 2) * (L + 3) / 6) : 0.0;
-    }
+    for (int k = 0; k < 16; ++k) order[k] = k;
     std::stable_sort(order, order + 16, [&](int a, int b) { return w[a] > w[b]; });
     double load[nS] = {};
     unsigned mask[nS] = {};
@@ -809,20 +817,44 @@ void three_center_forked(olcao_ctx &c, int kind, int tt, int ti, const double *c
         mask[i] |= 1u << k;
         load[i] += w[k];
     }
-    const int mixed = c.precision == OLCAO_PREC_MIXED;
     HIPCHK(hipEventRecord(c.evFork, c.stream));
     for (int i = 1; i < nS; ++i)
         if (mask[i]) HIPCHK(hipStreamWaitEvent(st[i], c.evFork, 0));
     for (int i = 0; i < nS; ++i)
-        if (mask[i])
-            launch_three_center(c.dev, c.cls, kind, tt, ti, coef, accumulate, c.dBlocks.p, mixed,
-                                mask[i], st[i]);
+        if (mask[i]) launch(mask[i], st[i]);
     HIPCHK(hipGetLastError());
     for (int i = 1; i < nS; ++i)
         if (mask[i]) {
             HIPCHK(hipEventRecord(c.evJoin[i - 1], st[i]));
             HIPCHK(hipStreamWaitEvent(c.stream, c.evJoin[i - 1], 0));
         }
+}
+
+// Two-centre blocks of op (accumulated onto the store when acc), classes
+// weighted by pairs x block size.
+void two_center_forked(olcao_ctx &c, int op, int acc)
+{
+    double w[16];
+    for (int k = 0; k < 16; ++k) w[k] = (double)c.cls.count[k] * nshh(k / 4) * nshh(k % 4);
+    fork_classes(c, w, [&](unsigned mask, hipStream_t s) {
+        launch_two_center(c.dev, c.cls, op, acc, c.dBlocks.p, mask, s);
+    });
+}
+
+// The three-centre sweep, classes weighted by chunks x Hermite sums per item.
+void three_center_forked(olcao_ctx &c, int kind, int tt, int ti, const double *coef,
+                         int accumulate)
+{
+    double w[16];
+    for (int k = 0; k < 16; ++k) {
+        const int L = k / 4 + k % 4;
+        w[k] = c.cls.count[k] ? (double)c.cls.nChunk[k] * ((L + 1) * (L + 2) * (L + 3) / 6) : 0.0;
+    }
+    const int mixed = c.precision == OLCAO_PREC_MIXED;
+    fork_classes(c, w, [&](unsigned mask, hipStream_t s) {
+        launch_three_center(c.dev, c.cls, kind, tt, ti, coef, accumulate, c.dBlocks.p, mixed, mask,
+                            s);
+    });
 }
 
 // Three-centre sweep with its work counters cleared first (a fill kernel,
@@ -848,9 +880,9 @@ void sweep_blocks(olcao_ctx &c, Sweep kind, int term, int spinIdx)
     hipStream_t s = c.stream;
     const DevCtx &d = c.dev;
     switch (kind) {
-    case SW_OL: launch_two_center(d, c.cls, OP_OL, 0, c.dBlocks.p, s); break;
-    case SW_KE: launch_two_center(d, c.cls, OP_KE, 0, c.dBlocks.p, s); break;
-    case SW_MV: launch_two_center(d, c.cls, OP_MV, 0, c.dBlocks.p, s); break;
+    case SW_OL: two_center_forked(c, OP_OL, 0); break;
+    case SW_KE: two_center_forked(c, OP_KE, 0); break;
+    case SW_MV: two_center_forked(c, OP_MV, 0); break;
     case SW_NP: three_center(c, K3_NP, 0, 0, nullptr, 0); break;
     case SW_EP: {
         int t = 0;
@@ -859,8 +891,8 @@ void sweep_blocks(olcao_ctx &c, Sweep kind, int term, int spinIdx)
         break;
     }
     case SW_HAM:
-        launch_two_center(d, c.cls, OP_KE, 0, c.dBlocks.p, s);
-        if (c.rel) launch_two_center(d, c.cls, OP_MV, 1, c.dBlocks.p, s);
+        two_center_forked(c, OP_KE, 0);
+        if (c.rel) two_center_forked(c, OP_MV, 1);
         three_center(c, K3_HAM, 0, 0, c.dCoef.p + (size_t)spinIdx * c.potDim, 1);
         break;
     }
@@ -1554,8 +1586,8 @@ olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates
         // three-centre sweep, 3+3q after it, 4+3q after contraction, Bloch
         // sum and orthogonalisation; 8 after the solve
         for (int q = 0; q < c->spin; ++q) {
-            launch_two_center(c->dev, c->cls, OP_KE, 0, c->dBlocks.p, s);
-            if (c->rel) launch_two_center(c->dev, c->cls, OP_MV, 1, c->dBlocks.p, s);
+            two_center_forked(*c, OP_KE, 0);
+            if (c->rel) two_center_forked(*c, OP_MV, 1);
             launch_fill(reinterpret_cast<double *>(c->dCounters.p), 32, 0.0, s);
             HIPCHK(hipEventRecord(c->ev[2 + 3 * q], s));
             three_center_forked(*c, K3_HAM, 0, 0, c->dCoef.p + (size_t)q * c->potDim, 1);
