@@ -70,4 +70,42 @@ static __device__ __forceinline__ PPInfo load_pp(const DevCtx &c, int pp)
     return r;
 }
 
+// Writes (accumulate = 0) or adds onto (accumulate = 1) the primitive-pair
+// blocks of up to 64 records staged in LDS by a 64-lane workgroup: word k of
+// sOut is element k % NE of record k / NE, whose block starts at sOff[k / NE].
+// The records' blocks are disjoint, so DEPTH loads are issued before any
+// store; a load-add-store loop over the records one by one would wait a
+// full memory round trip per record.  Lanes write consecutive words.
+template <int NE, int DEPTH = 16>
+static __device__ __forceinline__ void store_staged_blocks(const double *sOut,
+                                                           const long long *sOff, int nr,
+                                                           int accumulate, double *blocks)
+{
+    const int lane = threadIdx.x;
+    const int tot = nr * NE;
+    for (int k0 = 0; k0 < tot; k0 += 64 * DEPTH) {
+        double v[DEPTH], o[DEPTH];
+        long long a[DEPTH];
+#pragma unroll
+        for (int u = 0; u < DEPTH; ++u) {
+            const int k = k0 + u * 64 + lane;
+            const int r = k / NE;
+            const bool ok = k < tot;
+            a[u] = ok ? sOff[r] + (k - r * NE) : 0;
+            v[u] = ok ? sOut[k] : 0.0;
+        }
+        if (accumulate) {
+#pragma unroll
+            for (int u = 0; u < DEPTH; ++u) o[u] = k0 + u * 64 + lane < tot ? blocks[a[u]] : 0.0;
+#pragma unroll
+            for (int u = 0; u < DEPTH; ++u)
+                if (k0 + u * 64 + lane < tot) blocks[a[u]] = o[u] + v[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < DEPTH; ++u)
+                if (k0 + u * 64 + lane < tot) blocks[a[u]] = v[u];
+        }
+    }
+}
+
 }  // namespace olcao

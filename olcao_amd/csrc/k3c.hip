@@ -174,9 +174,9 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
 // (column) at a time so that only one Cartesian column is live.  All loops
 // are compile-time (static_for), so W and the E tables stay in registers.
 // Blocks of up to kFinishStage doubles are staged in LDS and written (or
-// accumulated) block by block with the whole wave, so that the stores to
-// the primitive-pair store are contiguous runs instead of one 8-byte word
-// per lane per element.
+// accumulated) by the whole wave with store_staged_blocks, so that the
+// stores to the primitive-pair store are contiguous runs instead of one
+// 8-byte word per lane per element.
 constexpr int kFinishStage = 81;
 
 // W of record i: the sum of its chunks' W in entry order (rcOff, rcList;
@@ -244,14 +244,8 @@ __global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const dou
     if (STAGE) {
         sOff[lane] = active ? rec.off : -1;
         __syncthreads();
-        const int nr = min(64, nrec - (int)blockIdx.x * 64);
-        for (int r = 0; r < nr; ++r) {
-            double *p = blocks + sOff[r];
-            for (int e = lane; e < SA * SB; e += 64) {
-                const double v = sOut[r * SA * SB + e];
-                p[e] = accumulate ? p[e] + v : v;
-            }
-        }
+        store_staged_blocks<SA * SB>(sOut, sOff, min(64, nrec - (int)blockIdx.x * 64), accumulate,
+                                     blocks);
     }
 }
 

@@ -62,14 +62,8 @@ __global__ __launch_bounds__(64) void k_two_center(DevCtx c, const int *list, in
             for (int j = 0; j < SB; ++j) sOut[lane * SA * SB + i * SB + j] = sg * sh[i][j];
         sOff[lane] = off;
         __syncthreads();
-        const int nr = min(64, n - (int)blockIdx.x * 64);
-        for (int r = 0; r < nr; ++r) {
-            double *o = blocks + sOff[r];
-            for (int e = lane; e < SA * SB; e += 64) {
-                const double v = sOut[r * SA * SB + e];
-                o[e] = accumulate ? o[e] + v : v;
-            }
-        }
+        store_staged_blocks<SA * SB>(sOut, sOff, min(64, n - (int)blockIdx.x * 64), accumulate,
+                                     blocks);
     } else if (active) {
         double *o = blocks + off;
 #pragma unroll
