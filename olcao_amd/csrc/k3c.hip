@@ -155,10 +155,18 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
         }
     }
     if (active) {
+        // all NH loads before the first store: the compiler cannot tell that
+        // the NH words differ, and would otherwise wait for each in turn
+        double *pw = W + i;
+        if (NUC && addW) {   // only the nuclear pass adds onto the three-Gaussian pass's W
+            double old[NH];
 #pragma unroll
-        for (int e = 0; e < NH; ++e) {
-            double *pw = W + (long long)e * n + i;
-            *pw = addW ? *pw + acc[e] : acc[e];
+            for (int e = 0; e < NH; ++e) old[e] = pw[(long long)e * n];
+#pragma unroll
+            for (int e = 0; e < NH; ++e) pw[(long long)e * n] = old[e] + acc[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < NH; ++e) pw[(long long)e * n] = acc[e];
         }
     }
     if (c.counters) {
