@@ -45,38 +45,56 @@ constexpr long long kSpinLimit = 1ll << 22;
 constexpr int kRs = 6;           // per-row scalars: v, pending v, pending w, p, column, next column
 enum { RV, RVP, RWP, PR, COLR, RCN };
 
+// Cross-lane operations without LDS round trips: a broadcast from a
+// wave-uniform lane is a readlane, and a wave sum is four DPP steps inside
+// each row of 16 lanes (swap neighbours, swap pairs, mirror halves, mirror
+// the row) followed by the sum of the four row totals, so that every lane
+// holds the same value.
+__device__ __forceinline__ double rlane(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double wsum_real(double a)
+{
+    a += dpp<0xb1>(a);    // quad_perm [1,0,3,2]
+    a += dpp<0x4e>(a);    // quad_perm [2,3,0,1]
+    a += dpp<0x141>(a);   // row_half_mirror
+    a += dpp<0x140>(a);   // row_mirror
+    return (rlane(a, 0) + rlane(a, 16)) + (rlane(a, 32) + rlane(a, 48));
+}
+
 template <bool CPLX>
 __device__ __forceinline__ typename Num<CPLX>::T wshfl(typename Num<CPLX>::T a, int lane)
 {
     if constexpr (CPLX)
-        return {__shfl(a.x, lane, 64), __shfl(a.y, lane, 64)};
+        return {rlane(a.x, lane), rlane(a.y, lane)};
     else
-        return __shfl(a, lane, 64);
+        return rlane(a, lane);
 }
 
 template <bool CPLX>
 __device__ __forceinline__ typename Num<CPLX>::T wsum(typename Num<CPLX>::T a)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        if constexpr (CPLX) {
-            a.x += __shfl_xor(a.x, off, 64);
-            a.y += __shfl_xor(a.y, off, 64);
-        } else {
-            a += __shfl_xor(a, off, 64);
-        }
-    }
-    return a;
+    if constexpr (CPLX)
+        return {wsum_real(a.x), wsum_real(a.y)};
+    else
+        return wsum_real(a);
 }
 
-__device__ __forceinline__ double wsum_real(double a)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-    return a;
-}
-
-// entry idx of a lane-distributed register vector, on every lane
+// entry idx (wave-uniform) of a lane-distributed register vector, on every lane
 template <bool CPLX, int KC>
 __device__ __forceinline__ typename Num<CPLX>::T bcast(const typename Num<CPLX>::T (&v)[KC],
                                                        int idx)
@@ -211,7 +229,9 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
     extern __shared__ double lds_raw[];
     T *rows = reinterpret_cast<T *>(lds_raw);
     __shared__ int bad;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the wave index is uniform across a wave; readfirstlane lets the
+    // compiler treat every row-dependent branch as a scalar branch
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g = blockIdx.x % nGroups, b = blockIdx.x / nGroups;
     const int Lr = n > 64 ? n - 64 : 0;
     const int rowsPer = (n + G - 1) / G;
@@ -302,41 +322,54 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
             // Entries at columns <= i carry zero v and w, so whole live chunks
             // are processed without per-lane column tests.
             const int kLive = (i + 1) >> 6, ci = (i + 1) & 63;
-            T dotw = N::zero();
+            // the wave's rows side by side: per chunk, all row loads before
+            // any store, so that the rows' latencies overlap
+            bool live[kTrdRpw];
+            T vr[kTrdRpw], wr[kTrdRpw], acc[kTrdRpw], colj[kTrdRpw];
 #pragma unroll
             for (int j = 0; j < kTrdRpw; ++j) {
-                const int s = wv + kTrdWaves * j, r = b + s * G;
-                if (r < n && r > i) {
-                    const T vr = rsc[j * kRs + RVP], wr = rsc[j * kRs + RWP];
-                    T acc = N::zero(), col = N::zero();
+                const int r = b + (wv + kTrdWaves * j) * G;
+                live[j] = r < n && r > i;
+                vr[j] = live[j] ? rsc[j * kRs + RVP] : N::zero();
+                wr[j] = live[j] ? rsc[j * kRs + RWP] : N::zero();
+                acc[j] = N::zero();
+                colj[j] = N::zero();
+            }
 #pragma unroll
-                    for (int k = 0; k < KC; ++k) {
-                        if (k >= kLive) {
-                            const int c = 64 * k + lane;
-                            if (k < KC - 1 || c < n) {
-                                T a = k == 0 ? rreg[j] : rows[s * Lr + c - 64];
-                                a = N::sub(N::sub(a, N::cmul(wp[k], vr)), N::cmul(vp[k], wr));
-                                if (k == 0)
-                                    rreg[j] = a;
-                                else
-                                    rows[s * Lr + c - 64] = a;
-                                acc = N::add(acc, N::mul(a, vc[k]));
-                                if (k == kLive) col = a;
-                            }
+            for (int k = 0; k < KC; ++k) {
+                const int c = 64 * k + lane;
+                if (k >= kLive && (k < KC - 1 || c < n)) {
+                    T a[kTrdRpw];
+#pragma unroll
+                    for (int j = 0; j < kTrdRpw; ++j)
+                        if (live[j]) a[j] = k == 0 ? rreg[j] : rows[(wv + kTrdWaves * j) * Lr + c - 64];
+#pragma unroll
+                    for (int j = 0; j < kTrdRpw; ++j)
+                        if (live[j]) {
+                            a[j] = N::sub(N::sub(a[j], N::cmul(wp[k], vr[j])), N::cmul(vp[k], wr[j]));
+                            if (k == 0)
+                                rreg[j] = a[j];
+                            else
+                                rows[(wv + kTrdWaves * j) * Lr + c - 64] = a[j];
+                            acc[j] = N::add(acc[j], N::mul(a[j], vc[k]));
+                            if (k == kLive) colj[j] = wshfl<CPLX>(a[j], ci);
                         }
-                    }
-                    const T prj = N::mul(tau, wsum<CPLX>(acc));
-                    const T colj = wshfl<CPLX>(col, ci);
+                }
+            }
+            T dotw = N::zero();
+#pragma unroll
+            for (int j = 0; j < kTrdRpw; ++j)
+                if (live[j]) {
+                    const int r = b + (wv + kTrdWaves * j) * G;
+                    const T prj = N::mul(tau, wsum<CPLX>(acc[j]));
                     if (lane == 0) {
                         pstT<CPLX>(pP, r, prj);
-                        pstT<CPLX>(pC, r, colj);
+                        pstT<CPLX>(pC, r, colj[j]);
                         rsc[j * kRs + PR] = prj;
-                        rsc[j * kRs + COLR] = colj;
+                        rsc[j * kRs + COLR] = colj[j];
                     }
                     dotw = N::add(dotw, N::cmul(prj, rsc[j * kRs + RV]));
                 }
-                __builtin_amdgcn_sched_barrier(0);
-            }
             if (lane == 0) pstT<CPLX>(pPart, b * kTrdWaves + wv, dotw);
             ++epoch;
             if (!group_barrier(myCnt, epoch * (unsigned)G, &bad)) {
