@@ -26,6 +26,7 @@
 #include <cmath>
 
 #include "olcao_internal.h"
+#include "wave.hpp"
 
 namespace olcao {
 
@@ -39,35 +40,13 @@ constexpr int kDcBig = 1024;     // threads of a merge of more than kDcSmallM
 constexpr int kDcSmall = 256;
 constexpr int kDcSmallM = 128;
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ double wave_max(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-// Four wave sums in seven exchanges: halves, then quarters, hold one sum each.
+// Four wave sums side by side.
 __device__ __forceinline__ void wave_sum4(double &a, double &b, double &c, double &d)
 {
-    const int lane = threadIdx.x & 63;
-    const bool lo32 = lane < 32, lo16 = (lane & 16) == 0;
-    double x1 = lo32 ? a : c, x2 = lo32 ? b : d;
-    const double y1 = lo32 ? c : a, y2 = lo32 ? d : b;
-    x1 += __shfl_xor(y1, 32, 64);
-    x2 += __shfl_xor(y2, 32, 64);
-    double v = lo16 ? x1 : x2;
-    v += __shfl_xor(lo16 ? x2 : x1, 16, 64);
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    a = __shfl(v, 0, 64);
-    b = __shfl(v, 16, 64);
-    c = __shfl(v, 32, 64);
-    d = __shfl(v, 48, 64);
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    d = wave_sum(d);
 }
 
 // 1/x from the hardware estimate and two Newton steps (about 1 ulp; the
@@ -86,7 +65,7 @@ __device__ __forceinline__ double rcp_nr(double x)
 template <bool MAX, int NT>
 __device__ double block_reduce(double v, double *red)
 {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     v = MAX ? wave_max(v) : wave_sum(v);
     __syncthreads();
     if (lane == 0) red[wv] = v;
@@ -550,7 +529,7 @@ __global__ __launch_bounds__(NT) void k_dc_merge(const int4 *nodes, int n, doubl
     __shared__ int rorg[kDcMax], colInfo[kDcMax];
     dc_prepare<NT>(n, off, m1, m, D, Eall + (size_t)n * mat, Qall + (size_t)n * n * mat,
                    ordAll + (size_t)n * mat, L);
-    const int K = L.K, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int K = L.K, lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int j = wv; j < K; j += NW) {
         int org;
         double tau;
@@ -639,7 +618,7 @@ __global__ __launch_bounds__(256) void k_dc_roots(const int2 *tiles, const int4 
         zz[i] = rho * z * z;
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int j = j0 + wv; j < min(K, j0 + kRootsPerWg); j += 4) {
         int org;
         double tau;

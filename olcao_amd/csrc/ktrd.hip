@@ -31,6 +31,7 @@
 
 #include "numtraits.hpp"
 #include "olcao_internal.h"
+#include "wave.hpp"
 
 namespace olcao {
 
@@ -45,37 +46,6 @@ constexpr long long kSpinLimit = 1ll << 22;
 constexpr int kRs = 6;           // per-row scalars: v, pending v, pending w, p, column, next column
 enum { RV, RVP, RWP, PR, COLR, RCN };
 
-// Cross-lane operations without LDS round trips: a broadcast from a
-// wave-uniform lane is a readlane, and a wave sum is four DPP steps inside
-// each row of 16 lanes (swap neighbours, swap pairs, mirror halves, mirror
-// the row) followed by the sum of the four row totals, so that every lane
-// holds the same value.
-__device__ __forceinline__ double rlane(double v, int lane)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-template <int CTRL>
-__device__ __forceinline__ double dpp(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-__device__ __forceinline__ double wsum_real(double a)
-{
-    a += dpp<0xb1>(a);    // quad_perm [1,0,3,2]
-    a += dpp<0x4e>(a);    // quad_perm [2,3,0,1]
-    a += dpp<0x141>(a);   // row_half_mirror
-    a += dpp<0x140>(a);   // row_mirror
-    return (rlane(a, 0) + rlane(a, 16)) + (rlane(a, 32) + rlane(a, 48));
-}
-
 template <bool CPLX>
 __device__ __forceinline__ typename Num<CPLX>::T wshfl(typename Num<CPLX>::T a, int lane)
 {
@@ -89,9 +59,9 @@ template <bool CPLX>
 __device__ __forceinline__ typename Num<CPLX>::T wsum(typename Num<CPLX>::T a)
 {
     if constexpr (CPLX)
-        return {wsum_real(a.x), wsum_real(a.y)};
+        return {wave_sum(a.x), wave_sum(a.y)};
     else
-        return wsum_real(a);
+        return wave_sum(a);
 }
 
 // entry idx (wave-uniform) of a lane-distributed register vector, on every lane
@@ -186,7 +156,7 @@ __device__ __forceinline__ void reflector(int i, int n, int lane,
         const int c = 64 * k + lane;
         if (c > i + 1 && c < n) part += N::abs2(cn[k]);
     }
-    const double xnorm2 = wsum_real(part);
+    const double xnorm2 = wave_sum(part);
     double ar, ai;
     if constexpr (CPLX) {
         ar = alpha.x;
