@@ -300,9 +300,9 @@ static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, i
                                s, c, cl.rec[k], nch, cl.chRec[k], cl.chBeg[k], cl.chEnd[k], cl.img[k], kind,
                                tt, ti, coef, 0, W);
     }
-    if (nuc)
+    if (nuc)   // the compacted nuclear entries: chunk u spans nOff[u] .. nOff[u + 1]
         hipLaunchKernelGGL((k_three_center<LA, LB, true, false, false>), grid, dim3(64), dyn, s, c,
-                           cl.rec[k], nch, cl.chRec[k], cl.chBeg[k], cl.chEnd[k], cl.img[k], kind,
+                           cl.rec[k], nch, cl.chRec[k], cl.nOff[k], cl.nOff[k] + 1, cl.imgN[k], kind,
                            tt, ti, coef, e3c ? 1 : 0, W);
     if (!e3c && !nuc) launch_fill(W, (long long)NH * nch, 0.0, s);
     hipLaunchKernelGGL((k_herm_finish<LA, LB>), gridRec, dim3(64), 0, s, cl.rec[k], W, n,

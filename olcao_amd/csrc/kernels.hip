@@ -552,6 +552,37 @@ void launch_image_fill(const DevCtx &c, const PPRec *recs, int n, const long lon
                        1, nullptr, nullptr, off, ent, overflow);
 }
 
+// One thread per chunk: the chunk's entries with nuclear work, in entry
+// order (the nuclear passes read only these; most image entries lie outside
+// the nuclear range and carry three-Gaussian terms only).
+__global__ __launch_bounds__(256) void k_nuc_compact(const unsigned long long *img,
+                                                     const long long *beg, const long long *end,
+                                                     int nc, int *cnt, const long long *nOff,
+                                                     unsigned long long *outN)
+{
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    if (u >= nc) return;
+    long long w = outN ? nOff[u] : 0;
+    int k = 0;
+    for (long long p = beg[u]; p < end[u]; ++p) {
+        const unsigned long long e = img[p];
+        if (img_nucham(e) | img_nucnp(e)) {
+            if (outN) outN[w + k] = e;
+            ++k;
+        }
+    }
+    if (!outN) cnt[u] = k;
+}
+
+void launch_nuc_compact(const unsigned long long *img, const long long *beg, const long long *end,
+                        int nc, int *cnt, const long long *nOff, unsigned long long *outN,
+                        hipStream_t s)
+{
+    if (nc <= 0) return;
+    hipLaunchKernelGGL(k_nuc_compact, dim3((nc + 255) / 256), dim3(256), 0, s, img, beg, end, nc,
+                       cnt, nOff, outN);
+}
+
 void launch_contract(const DevCtx &c, const ContractPlan &cp, const double *blocks, double *X,
                      hipStream_t s)
 {

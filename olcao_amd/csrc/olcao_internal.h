@@ -140,6 +140,10 @@ struct ClassLists {
     // records in the order of their first chunk, so that the finish kernel's
     // lanes read neighbouring W columns
     const int *finOrder[16];    // [count]
+    // the entries of each chunk that carry nuclear work (for either sweep
+    // kind), compacted: chunk u's are imgN[nOff[u] .. nOff[u + 1])
+    const unsigned long long *imgN[16];
+    const long long *nOff[16];  // [nChunk + 1]
 };
 
 // The classes k = 4 la + lb whose bit k is set in mask (independent, as for
@@ -160,6 +164,11 @@ void launch_image_count(const DevCtx &c, const PPRec *recs, int n, int *nEnt, in
                         hipStream_t s);
 void launch_image_fill(const DevCtx &c, const PPRec *recs, int n, const long long *off,
                        unsigned long long *ent, int *overflow, hipStream_t s);
+// Nuclear entries of each chunk [beg[u], end[u]) of img: counted into
+// cnt[u] (outN null), or copied to outN from nOff[u] on (outN set).
+void launch_nuc_compact(const unsigned long long *img, const long long *beg, const long long *end,
+                        int nc, int *cnt, const long long *nOff, unsigned long long *outN,
+                        hipStream_t s);
 // Contraction work: segments of at most kSegPP primitive pairs, and the
 // pair-cells that have several segments {pair-cell, first partial, segment
 // count}.

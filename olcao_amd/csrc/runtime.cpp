@@ -164,7 +164,8 @@ struct olcao_ctx {
     DBuf<long long> dIoff[16];
     DBuf<int> dChRec[16], dRcOff[16], dRcList[16], dFinOrder[16];
     DBuf<long long> dChBeg[16], dChEnd[16];
-    DBuf<unsigned long long> dImg[16];
+    DBuf<unsigned long long> dImg[16], dImgN[16];
+    DBuf<long long> dNOff[16];
     DBuf<double> dW;
     long long imageEntries = 0;
     double listMs = 0.0;
@@ -773,6 +774,23 @@ void build_image_lists(olcao_ctx &c)
         c.cls.rcList[k] = c.dRcList[k].p;
         c.cls.finOrder[k] = c.dFinOrder[k].p;
         c.cls.nChunk[k] = nc;
+        {   // compacted nuclear entries per chunk
+            DBuf<int> dNc;
+            dNc.alloc(std::max(nc, 1));
+            launch_nuc_compact(c.dImg[k].p, c.dChBeg[k].p, c.dChEnd[k].p, nc, dNc.p, nullptr, nullptr, s);
+            std::vector<int> ncnt(nc);
+            HIPCHK(hipMemcpyAsync(ncnt.data(), dNc.p, nc * sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            std::vector<long long> noff(nc + 1, 0);
+            for (int u = 0; u < nc; ++u) noff[u + 1] = noff[u] + ncnt[u];
+            c.dNOff[k].upload(noff, s);
+            c.dImgN[k].alloc(std::max<long long>(noff[nc], 1));
+            launch_nuc_compact(c.dImg[k].p, c.dChBeg[k].p, c.dChEnd[k].p, nc, nullptr, c.dNOff[k].p,
+                               c.dImgN[k].p, s);
+            HIPCHK(hipStreamSynchronize(s));
+            c.cls.imgN[k] = c.dImgN[k].p;
+            c.cls.nOff[k] = c.dNOff[k].p;
+        }
     }
     // Hermite-sum scratch of the three-centre sweeps, one slice per class
     size_t wTotal = 0;
