@@ -193,7 +193,7 @@ template <int LA, int LB>
 __global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const double *W, int nrec,
                                                     const int *finOrder, const int *rcOff,
                                                     const int *rcList, int nChunk,
-                                                    int accumulate, double *blocks)
+                                                    int accumulate, int addKE, double *blocks)
 {
     constexpr int L = LA + LB, NH = nherm(L);
     constexpr int NA = ncart(LA), SA = nsh(LA), SB = nsh(LB);
@@ -250,6 +250,16 @@ __global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const dou
         }
     });
     if (STAGE) {
+        if (addKE) {   // + the kinetic block of the same primitive pair
+            double pk[NA][ncart(LB)];
+            two_center_cart<LA, LB, true>(q, pk);
+            double sk[SA][SB];
+            cart_to_sh<LA, LB>(pk, sk);
+#pragma unroll
+            for (int ii = 0; ii < SA; ++ii)
+#pragma unroll
+                for (int j = 0; j < SB; ++j) sOut[lane * SA * SB + ii * SB + j] += sk[ii][j];
+        }
         sOff[lane] = active ? rec.off : -1;
         __syncthreads();
         store_staged_blocks<SA * SB>(sOut, sOff, min(64, nrec - (int)blockIdx.x * 64), accumulate,
@@ -305,8 +315,13 @@ static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, i
                            cl.rec[k], nch, cl.chRec[k], cl.nOff[k], cl.nOff[k] + 1, cl.imgN[k], kind,
                            tt, ti, coef, e3c ? 1 : 0, W);
     if (!e3c && !nuc) launch_fill(W, (long long)NH * nch, 0.0, s);
+    // classes up to d-d stage their blocks and can take the kinetic term here
+    constexpr bool kFuse = nsh(LA) * nsh(LB) <= kFinishStage;
+    static_assert(kFuse == ((kFusedKEClasses >> (4 * LA + LB)) & 1u), "kinetic-fused class set");
+    const bool addKE = kFuse && (accumulate & kAccAddKE);
     hipLaunchKernelGGL((k_herm_finish<LA, LB>), gridRec, dim3(64), 0, s, cl.rec[k], W, n,
-                       cl.finOrder[k], cl.rcOff[k], cl.rcList[k], nch, accumulate, blocks);
+                       cl.finOrder[k], cl.rcOff[k], cl.rcList[k], nch,
+                       addKE ? 0 : (accumulate & 1), addKE ? 1 : 0, blocks);
 }
 
 #define OLCAO_CAT2(a, b) a##b

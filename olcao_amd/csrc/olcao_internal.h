@@ -150,6 +150,14 @@ struct ClassLists {
 // launch_three_center below).
 void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int accumulate,
                        double *blocks, unsigned mask, hipStream_t s);
+// accumulate: bit 0 adds the blocks onto the store; with kAccAddKE set the
+// classes whose blocks are staged in LDS (nsh(la) nsh(lb) <= 81: all but
+// d-f, f-d and f-f) instead add the kinetic block of each primitive pair
+// themselves and write the sum (the fused PSCF sweep's kinetic term,
+// integrals.F90:3444-3490), the other classes adding onto the store as with
+// bit 0.  kFusedKEClasses marks the classes k = 4 la + lb that do so.
+constexpr int kAccAddKE = 2;
+constexpr unsigned kFusedKEClasses = 0x37ffu;
 // mixed: three-Gaussian exponentials by the mixed-precision evaluation.
 // Launches the classes k = 4 la + lb whose bit k is set in mask; different
 // classes touch disjoint memory (their own W slice, their own primitive-pair

@@ -848,12 +848,13 @@ void fork_classes(olcao_ctx &c, const double (&w)[16], F launch)
         }
 }
 
-// Two-centre blocks of op (accumulated onto the store when acc), classes
-// weighted by pairs x block size.
-void two_center_forked(olcao_ctx &c, int op, int acc)
+// Two-centre blocks of op (accumulated onto the store when acc) for the
+// classes in classMask, weighted by pairs x block size.
+void two_center_forked(olcao_ctx &c, int op, int acc, unsigned classMask = 0xffffu)
 {
     double w[16];
-    for (int k = 0; k < 16; ++k) w[k] = (double)c.cls.count[k] * nshh(k / 4) * nshh(k % 4);
+    for (int k = 0; k < 16; ++k)
+        w[k] = (classMask >> k & 1u) ? (double)c.cls.count[k] * nshh(k / 4) * nshh(k % 4) : 0.0;
     fork_classes(c, w, [&](unsigned mask, hipStream_t s) {
         launch_two_center(c.dev, c.cls, op, acc, c.dBlocks.p, mask, s);
     });
@@ -909,9 +910,14 @@ void sweep_blocks(olcao_ctx &c, Sweep kind, int term, int spinIdx)
         break;
     }
     case SW_HAM:
-        two_center_forked(c, OP_KE, 0);
-        if (c.rel) two_center_forked(c, OP_MV, 1);
-        three_center(c, K3_HAM, 0, 0, c.dCoef.p + (size_t)spinIdx * c.potDim, 1);
+        if (c.rel) {
+            two_center_forked(c, OP_KE, 0);
+            two_center_forked(c, OP_MV, 1);
+            three_center(c, K3_HAM, 0, 0, c.dCoef.p + (size_t)spinIdx * c.potDim, 1);
+        } else {   // the finish kernels add the kinetic blocks of their classes
+            two_center_forked(c, OP_KE, 0, ~kFusedKEClasses & 0xffffu);
+            three_center(c, K3_HAM, 0, 0, c.dCoef.p + (size_t)spinIdx * c.potDim, 1 | kAccAddKE);
+        }
         break;
     }
 }
@@ -1604,11 +1610,15 @@ olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates
         // three-centre sweep, 3+3q after it, 4+3q after contraction, Bloch
         // sum and orthogonalisation; 8 after the solve
         for (int q = 0; q < c->spin; ++q) {
-            two_center_forked(*c, OP_KE, 0);
+            // kinetic blocks: here for every class when rel = 1 (the
+            // mass-velocity term follows them); otherwise only for the
+            // classes whose finish kernels do not add them
+            two_center_forked(*c, OP_KE, 0, c->rel ? 0xffffu : ~kFusedKEClasses & 0xffffu);
             if (c->rel) two_center_forked(*c, OP_MV, 1);
             launch_fill(reinterpret_cast<double *>(c->dCounters.p), 32, 0.0, s);
             HIPCHK(hipEventRecord(c->ev[2 + 3 * q], s));
-            three_center_forked(*c, K3_HAM, 0, 0, c->dCoef.p + (size_t)q * c->potDim, 1);
+            three_center_forked(*c, K3_HAM, 0, 0, c->dCoef.p + (size_t)q * c->potDim,
+                                c->rel ? 1 : 1 | kAccAddKE);
             HIPCHK(hipEventRecord(c->ev[3 + 3 * q], s));
             contract_scatter(*c, c->H[q].p, c->CV_H[q].p, c->CC_H[q].p);
             ortho(*c, c->H[q].p, c->CV_H[q].p, c->CC_H[q].p, false);
