@@ -192,6 +192,7 @@ __device__ void dc_prepare(int n, int off, int m1, int m, const double *D, const
         dmx = fmax(dmx, fabs(L.ds[p]));
         zmx = fmax(zmx, fabs(L.zs[p]));
         ord[off + p] = off + L.order[p];
+        ord[n + off + L.order[p]] = p;   // inverse: sorted position of pole L.order[p]
     }
     dmx = block_reduce<true, NT>(dmx, L.red);
     zmx = block_reduce<true, NT>(zmx, L.red);
@@ -528,7 +529,7 @@ __global__ __launch_bounds__(NT) void k_dc_merge(const int4 *nodes, int n, doubl
     __shared__ double lam[kDcMax], zh[kDcMax], rtau[kDcMax];
     __shared__ int rorg[kDcMax], colInfo[kDcMax];
     dc_prepare<NT>(n, off, m1, m, D, Eall + (size_t)n * mat, Qall + (size_t)n * n * mat,
-                   ordAll + (size_t)n * mat, L);
+                   ordAll + (size_t)2 * n * mat, L);
     const int K = L.K, lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int j = wv; j < K; j += NW) {
         int org;
@@ -567,7 +568,7 @@ __global__ __launch_bounds__(NT) void k_dc_prep(const int4 *nodes, int nodeBase,
     const int off = nd.x, m1 = nd.y, m = nd.z;
     __shared__ MergeLds L;
     dc_prepare<NT>(n, off, m1, m, Dall + (size_t)n * mat, Eall + (size_t)n * mat,
-                   Qall + (size_t)n * n * mat, ordAll + (size_t)n * mat, L);
+                   Qall + (size_t)n * n * mat, ordAll + (size_t)2 * n * mat, L);
     const size_t b = (size_t)n * mat + off;
     for (int i = threadIdx.x; i < m; i += NT) {
         if (i < L.K) {
@@ -681,48 +682,76 @@ __global__ __launch_bounds__(256) void k_dc_vecs(const int2 *tiles, const int4 *
                     Wall + (size_t)n * n * mat);
 }
 
-// Qt(node) = Q(:, ord) W over node blocks; tiles: x node, y row tile, z col tile
+// Qt(node) = Q(node) P W over node blocks; tiles: x node, y row tile, z col
+// tile.  Before the merge Q(node) = diag(Q1, Q2) in natural order, so a row
+// of child c takes k over that child's columns only: a tile's k range is
+// the children its rows span (half the products of the full m x m x m
+// product are exact zeros).  Row j of P W is W(inv[j], :), inv[j] being the
+// sorted position of pole j.  The next k tile is loaded into registers
+// while the current one is multiplied from LDS.
 __global__ __launch_bounds__(256) void k_dc_gemm(const int4 *tiles, const int4 *nodes, int n,
                                                  const double *Qall, const double *Wall,
                                                  const int *ordAll, double *Qtall)
 {
     const int4 tl = tiles[blockIdx.x];
     const int4 nd = nodes[tl.x];
-    const int off = nd.x, m = nd.z;
+    const int off = nd.x, m1 = nd.y, m = nd.z;
     const int mat = blockIdx.y;
     const double *Q = Qall + (size_t)n * n * mat;
     const double *W = Wall + (size_t)n * n * mat;
-    const int *ord = ordAll + (size_t)n * mat;
+    const int *inv = ordAll + (size_t)2 * n * mat + n;
     double *Qt = Qtall + (size_t)n * n * mat;
     const int r0 = tl.y * kTile, c0 = tl.z * kTile;
-    __shared__ double As[16][kTile + 1];
-    __shared__ double Bs[16][kTile + 1];
+    const int rLast = min(r0 + kTile, m) - 1;
+    const int kb = r0 < m1 ? 0 : m1, ke = rLast < m1 ? m1 : m;
+    __shared__ double As[2][16][kTile + 1];
+    __shared__ double Bs[2][16][kTile + 1];
     const int tid = threadIdx.x;
     const int tr = tid % 16, tc = tid / 16;   // 4 x 4 outputs each: rows tr + 16 a, cols tc + 16 b
     double acc[4][4] = {};
-    for (int k0 = 0; k0 < m; k0 += 16) {
-        for (int e = tid; e < 16 * kTile; e += 256) {
+    double ra[4], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
             const int rr = e % kTile, kk = e / kTile;
             const int r = r0 + rr, k = k0 + kk;
-            As[kk][rr] = (r < m && k < m) ? Q[(size_t)(off + r) + (size_t)n * ord[off + k]] : 0.0;
-            const int kb = e % 16, cc = e / 16;
-            const int k2 = k0 + kb, c = c0 + cc;
-            Bs[kb][cc] = (k2 < m && c < m) ? W[(size_t)(off + k2) + (size_t)n * (off + c)] : 0.0;
+            ra[u] = (r < m && k < ke) ? Q[(size_t)(off + r) + (size_t)n * (off + k)] : 0.0;
+            const int kq = e % 16, cc = e / 16;
+            const int k2 = k0 + kq, c = c0 + cc;
+            rb[u] = (k2 < ke && c < m) ? W[(size_t)(off + inv[off + k2]) + (size_t)n * (off + c)] : 0.0;
         }
-        __syncthreads();
+    };
+    auto store = [&](int buf) {
 #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            As[buf][e / kTile][e % kTile] = ra[u];
+            Bs[buf][e % 16][e / 16] = rb[u];
+        }
+    };
+    int buf = 0;
+    load(kb);
+    store(0);
+    __syncthreads();
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        const bool more = k0 + 16 < ke;
+        if (more) load(k0 + 16);
+#pragma unroll 4
         for (int kk = 0; kk < 16; ++kk) {
             double av[4], bv[4];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) av[a] = As[kk][tr + 16 * a];
+            for (int a = 0; a < 4; ++a) av[a] = As[buf][kk][tr + 16 * a];
 #pragma unroll
-            for (int bb = 0; bb < 4; ++bb) bv[bb] = Bs[kk][tc + 16 * bb];
+            for (int bb = 0; bb < 4; ++bb) bv[bb] = Bs[buf][kk][tc + 16 * bb];
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int bb = 0; bb < 4; ++bb) acc[a][bb] = fma(av[a], bv[bb], acc[a][bb]);
         }
+        if (more) store(buf ^ 1);
         __syncthreads();
+        buf ^= 1;
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a)

@@ -1058,7 +1058,7 @@ void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, 
     ew.Z.alloc(per * nb);
     ew.Zt.alloc(per * nb);
     ew.Wd.alloc(per * nb);
-    ew.ord.alloc((size_t)n * nb);
+    ew.ord.alloc((size_t)2 * n * nb);   // per matrix: sort order, then its inverse
     if (ew.dc.n != n) {
         dc_plan(n, ew.dc);
         ew.dcNodes.upload(ew.dc.nodes, s);
