@@ -55,6 +55,15 @@ __device__ __forceinline__ typename Num<CPLX>::T wshfl(typename Num<CPLX>::T a, 
         return rlane(a, lane);
 }
 
+template <bool CPLX, int CTRL>
+__device__ __forceinline__ typename Num<CPLX>::T quad_swap(typename Num<CPLX>::T a)
+{
+    if constexpr (CPLX)
+        return {dpp<CTRL>(a.x), dpp<CTRL>(a.y)};
+    else
+        return dpp<CTRL>(a);
+}
+
 template <bool CPLX>
 __device__ __forceinline__ typename Num<CPLX>::T wsum(typename Num<CPLX>::T a)
 {
@@ -462,17 +471,22 @@ __global__ __launch_bounds__(256) void k_trd_tfactor(const double *__restrict__ 
     for (int t = 0; t < PPT; ++t)
         if (tid + 256 * t < NP) Gm[pa[t]][pj[t]] = acc[t];
     __syncthreads();
-    // T over G, column by column (columns < j already hold T)
+    // T over G, column by column (columns < j already hold T):
+    // T(a, j) = -tau_j sum_{a <= q < j} T(a, q) G(q, j); row a's sum is split
+    // over the four lanes 4a .. 4a + 3 and combined within the quad
+    if (tid < mb) tcol[tid] = N::load(tau, j0 + tid);
+    __syncthreads();
+    const int ra = tid >> 2, part = tid & 3;
     for (int j = 0; j < mb; ++j) {
-        const T tj = N::load(tau, j0 + j);
-        if (tid < j) {
-            T sum = N::zero();
-            for (int q = tid; q < j; ++q) sum = N::add(sum, N::mul(Gm[tid][q], Gm[q][j]));
-            tcol[tid] = N::scale(N::mul(tj, sum), -1.0);
-        }
+        T sum = N::zero();
+        if (ra < j)
+            for (int q = ra + part; q < j; q += 4) sum = N::add(sum, N::mul(Gm[ra][q], Gm[q][j]));
+        sum = N::add(sum, quad_swap<CPLX, kDppQuadSwap1>(sum));
+        sum = N::add(sum, quad_swap<CPLX, kDppQuadSwap2>(sum));
         __syncthreads();
-        if (tid < j) Gm[tid][j] = tcol[tid];
-        if (tid == j) Gm[j][j] = tj;
+        const T tj = tcol[j];
+        if (part == 0 && ra < j) Gm[ra][j] = N::scale(N::mul(tj, sum), -1.0);
+        if (tid == 0) Gm[j][j] = tj;
         __syncthreads();
     }
     // T (upper triangle) out for the row-parallel product U = V_b T
