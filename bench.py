@@ -41,6 +41,8 @@ WORKLOADS = {
     "diamond_c8": "diamond_c8: diamond C8, 1s 2s 2p from 20 exponents 0.12..1e6, "
                   "16 potential Gaussians, 4x4x4 k (64 k), n=40",
     "quartz": "quartz: alpha-quartz SiO2 (9 atoms), Si s+p+d, O 1s 2s 2p, 8x8x8 k (512 k), n=57",
+    "asio2_1000": "asio2_1000: amorphous SiO2 cube, 333 Si + 667 O at 2.2 g/cm^3, minimal basis "
+                  "(Si 1s-3p, O 2s 2p + 1s core), Gamma only (real), n=5665, 667 core states",
 }
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes of one mixed-form three-centre sweep (all k_three_center and
@@ -171,7 +173,7 @@ def main():
         return
     mixed = run_form(s, gpu.PREC_MIXED, args.steps, args.warmup, "mixed")
     HARTREE_EV = 27.211386245988
-    n_el = sum(s.pot_types[t].Z for t in s.pot_type)
+    n_el = sum(s.pot_types[t].Z for t in s.pot_type) - 2 * s.core_dim
     n_occ = min(s.vale_dim, int(round(n_el / 2)))
     eig_err_mev = float(np.abs(mixed["evals"][:, :n_occ] - fp64["evals"][:, :n_occ]).max()) \
         * HARTREE_EV * 1e3

@@ -1,6 +1,6 @@
 """Seeded synthetic systems for the integral -> eigenvalue path.
 
-Five workloads, in order:
+Four of the five workloads, in order:
   diamond_c8   diamond-cubic C, 8-atom cell, 1s 2s 2p from 20 even-tempered
                exponents 0.12..1e6, 16 potential Gaussians 0.1..1e5,
                4x4x4 Monkhorst-Pack k-mesh (64 k)                    n = 40
@@ -8,8 +8,9 @@ Five workloads, in order:
                Si (1s 2p 3d) from 24 exponents 0.05..1e7, 2x2x2 k   n = 576
   quartz       alpha-quartz SiO2 (P3_1 21, 9 atoms), Si s+p+d, O 1s 2s 2p,
                8x8x8 k-mesh (512 k)                                  n = 57
-  uneven_batch 64 seeded cells of 8..216 atoms (C/Si/O), 1..64 k each
-  asio2_1000   1000-atom amorphous SiO2 cube, Gamma only (real)
+  asio2_1000   1000-atom amorphous SiO2 cube at 2.2 g/cm^3, Si 1s 2s 2p 3s 3p,
+               O 2s 2p + 1s core, Gamma only (real)            n = 5665
+(the fourth, 64 uneven cells in one launch, is not generated here)
 
 plus small variants used by the tests.  Exponents are even-tempered,
 alpha_i = amin (amax/amin)^(i/(N-1)), the rule of makePotAlphas
@@ -218,4 +219,41 @@ def small_test(seed=7, kmesh=(2, 2, 1), n_alpha=8, lmax=1, hexa=False,
                      gamma=gamma, rel=rel, spin=spin)
 
 
-CONFIGS = {"diamond_c8": diamond_c8, "si64": si64, "quartz": quartz}
+def asio2(seed=5, n_si=333, n_o=667, density=2.2, dmin=1.45, n_alpha_si=12, n_alpha_o=10,
+          n_pot=8):
+    """C5: amorphous SiO2 cube at `density` g/cm^3, Gamma only (real H and
+    S).  Positions uniform from the seed with a `dmin` A minimum distance
+    (minimum image) enforced by rejection.  Minimal basis: Si 1s 2s 2p 3s
+    3p (9 orbitals) from `n_alpha_si` exponents 0.2..1e6; O 2s 2p valence
+    (4 orbitals) with a 1s core state from `n_alpha_o` exponents 0.25..1e6.
+    333 Si + 667 O: 5665 valence and 667 core orbitals."""
+    rng = np.random.default_rng(seed)
+    mass = n_si * 28.0855 + n_o * 15.999
+    side = (mass * 1.66053906660e-24 / density * 1e24) ** (1.0 / 3.0)   # A
+    pos = np.empty((0, 3))
+    while len(pos) < n_si + n_o:
+        p = rng.uniform(0.0, side, size=3)
+        d = pos - p
+        d -= side * np.round(d / side)
+        if len(pos) == 0 or (d * d).sum(axis=1).min() >= dmin * dmin:
+            pos = np.vstack([pos, p])
+    cell = cubic(side * BOHR_PER_ANGSTROM)
+    pos = pos * BOHR_PER_ANGSTROM
+    tsi = make_type(rng, ELEM_SI, even_tempered(0.2, 1.0e6, n_alpha_si),
+                    (n_alpha_si, n_alpha_si - 2, 0, 0), [0, 0, 1, 0, 1])
+    to = make_type(rng, ELEM_O, even_tempered(0.25, 1.0e6, n_alpha_o),
+                   (n_alpha_o, n_alpha_o - 2, 0, 0), [0, 1], core_l=[0])
+    psi = PotType(ELEM_SI, 14.0, ALPHA_NUC, even_tempered(0.2, 1.0e4, n_pot))
+    po = PotType(ELEM_O, 8.0, ALPHA_NUC, even_tempered(0.25, 1.0e4, n_pot))
+    types = [0] * n_si + [1] * n_o
+    k, w = np.zeros((1, 3)), np.array([2.0])
+    return _assemble("asio2", cell, pos, types, [tsi, to], [psi, po], {0: 0, 1: 1}, k, w, rng,
+                     gamma=True)
+
+
+def asio2_1000(seed=5):
+    """C5 at its full size: 333 Si + 667 O."""
+    return asio2(seed=seed)
+
+
+CONFIGS = {"diamond_c8": diamond_c8, "si64": si64, "quartz": quartz, "asio2_1000": asio2_1000}
