@@ -43,15 +43,18 @@ WORKLOADS = {
     "quartz": "quartz: alpha-quartz SiO2 (9 atoms), Si s+p+d, O 1s 2s 2p, 8x8x8 k (512 k), n=57",
     "asio2_1000": "asio2_1000: amorphous SiO2 cube, 333 Si + 667 O at 2.2 g/cm^3, minimal basis "
                   "(Si 1s-3p, O 2s 2p + 1s core), Gamma only (real), n=5665, 667 core states",
+    "uneven_batch": "uneven_batch: 64 seeded cells of 8-216 C/Si/O atoms on jittered diamond/"
+                    "zincblende lattices at 2.2-3.5 g/cm^3, minimal or extended basis, 1-64 k "
+                    "each (n = 40..1296), one batched call per iteration",
 }
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes of one mixed-form three-centre sweep (all k_three_center and
 # k_herm_finish dispatches), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # runs scaled by a 256 MiB device copy's counter readings (fetch x2.00,
-# write x1.00); profiles/k_three_center_si64_hbm_traffic.md has the commands
-# and the per-kernel figures.  Counters cannot be read inside a timed run, so
-# the figure is carried here; other workloads report null.
-SWEEP_HBM_BYTES = {"si64": 8.2945e9}
+# write x1.00): 4.39 GB fetched + 2.12 GB written.  Counters cannot be read
+# inside a timed run, so the figure is carried here; other workloads report
+# null.
+SWEEP_HBM_BYTES = {"si64": 6.5101e9}
 
 
 def cpu_baseline(sysin, total_triples, stride):
@@ -117,6 +120,101 @@ def run_form(s, precision, steps, warmup, label):
     return info
 
 
+def run_batch_form(cells, precision, steps, warmup, label, workers):
+    """Per-cell isolated iterations (for the per-cell time distribution),
+    then `steps` timed batched iterations of all cells after `warmup`."""
+    import numpy as np
+
+    from olcao_amd import gpu, workmodel
+    ctxs = []
+    t0 = time.perf_counter()
+    for s in cells:
+        ctxs.append(gpu.OlcaoGPU(s, precision=precision))
+    print(f"[bench] {label}: {len(ctxs)} contexts set up in {time.perf_counter() - t0:.1f} s",
+          flush=True)
+    ns = [g.vale_dim for g in ctxs]
+    single, flops, t3c = [], 0.0, 0.0
+    for g, n in zip(ctxs, ns):
+        g.scf_iteration(g.sysin.pot_coeffs, n)          # first call allocates
+        g.scf_iteration(g.sysin.pot_coeffs, n)
+        t = g.timings()
+        single.append(t["total_ms"])
+        flops += workmodel.sweep_flops(g.work_counts())
+        t3c += t["three_center_ms"]
+    print(f"[bench] {label}: isolated per-cell ms p50 {np.percentile(single, 50):.2f} "
+          f"p99 {np.percentile(single, 99):.2f} sum {sum(single):.1f}", flush=True)
+    for w in range(warmup):
+        gpu.scf_iteration_batch(ctxs, ns, workers)
+        print(f"[bench] {label} batch warmup {w + 1}/{warmup}", flush=True)
+    secs, lat = [], []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        evals, ms = gpu.scf_iteration_batch(ctxs, ns, workers)
+        secs.append(time.perf_counter() - t0)
+        lat.append(ms)
+        print(f"[bench] {label} batch step {k + 1}/{steps} {secs[-1] * 1e3:.1f} ms", flush=True)
+    for g in ctxs:
+        g.close()
+    lat = np.concatenate(lat)
+    return dict(sec=float(np.mean(secs)), evals=[np.asarray(e) for e in evals],
+                single_ms=np.asarray(single), latency_ms=lat, flops=flops, t3c_ms=t3c)
+
+
+def main_batch(args):
+    import numpy as np
+
+    from olcao_amd import configs
+    cells = configs.uneven_batch()
+    fp64 = run_batch_form(cells, 0, args.steps, args.warmup, "fp64", args.workers)
+    mixed = None if args.fp64_only else run_batch_form(cells, 1, args.steps, args.warmup,
+                                                       "mixed", args.workers)
+    if mixed is None:
+        print(json.dumps({"workload": "uneven_batch", "fp64_s_per_iter": fp64["sec"]}), flush=True)
+        return
+    HARTREE_EV = 27.211386245988
+    eig_err, de = 0.0, 0.0
+    for s, a, b in zip(cells, mixed["evals"], fp64["evals"]):
+        n_el = sum(s.pot_types[t].Z for t in s.pot_type) - 2 * s.core_dim
+        n_occ = min(s.vale_dim, int(round(n_el / 2)))
+        a, b = a.reshape(s.nk, -1), b.reshape(s.nk, -1)
+        eig_err = max(eig_err, float(np.abs(a[:, :n_occ] - b[:, :n_occ]).max()))
+        de = max(de, abs(band_energy(a, s.kweights, n_occ) - band_energy(b, s.kweights, n_occ))
+                 / s.n_atoms)
+    pct = lambda a, q: float(np.percentile(a, q))   # noqa: E731
+    achieved = mixed["flops"] / (mixed["t3c_ms"] * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": mixed["sec"], "unit": "s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": mixed["sec"] * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (f32 exponential in the three-centre items)", "data": "synthetic (seeded)",
+        "config": {"workload": WORKLOADS["uneven_batch"], "cells": len(cells),
+                   "form": "mixed (value) beside plain FP64", "mode": "B, all cells per step",
+                   "workers": args.workers,
+                   "atoms": int(sum(s.n_atoms for s in cells)),
+                   "orbital_k_points": int(sum(s.vale_dim * s.nk for s in cells)),
+                   "fp64_s_per_iter": fp64["sec"],
+                   "mixed_speedup_vs_fp64": fp64["sec"] / mixed["sec"],
+                   "cells_per_s_mixed": len(cells) / mixed["sec"],
+                   "cells_per_s_fp64": len(cells) / fp64["sec"],
+                   "isolated_sum_s_fp64": float(fp64["single_ms"].sum()) * 1e-3,
+                   "isolated_cell_ms_fp64": {"p50": pct(fp64["single_ms"], 50),
+                                             "p99": pct(fp64["single_ms"], 99)},
+                   "isolated_cell_ms_mixed": {"p50": pct(mixed["single_ms"], 50),
+                                              "p99": pct(mixed["single_ms"], 99)},
+                   "batched_cell_latency_ms_mixed": {"p50": pct(mixed["latency_ms"], 50),
+                                                     "p99": pct(mixed["latency_ms"], 99)},
+                   "mixed_vs_fp64_max_eig_err_meV_occupied": eig_err * HARTREE_EV * 1e3,
+                   "mixed_vs_fp64_band_energy_err_Ha_per_atom": de},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_three_center (all classes, every cell's sweep run alone, "
+                               "mixed form)",
+                     "flops_per_launch": mixed["flops"], "launch_ms": mixed["t3c_ms"]},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def band_energy(w, kweights, n_occ):
     import numpy as np
     return float(np.sum(np.asarray(kweights)[:, None] * np.asarray(w)[:, :n_occ]))
@@ -136,6 +234,8 @@ def main():
                     help="time the FP64 form only (for counter runs)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="time the mixed form only (for counter runs)")
+    ap.add_argument("--workers", type=int, default=8,
+                    help="host threads of the batched call (uneven_batch workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES once on the device (the reference kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -154,6 +254,10 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described, not run).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "uneven_batch":
+        main_batch(args)
+        return
 
     import numpy as np
 

@@ -172,12 +172,45 @@ olcao_status olcao_setup_scf(olcao_ctx *ctx);
 olcao_status olcao_secular_eqn_scf(olcao_ctx *ctx, const double *potCoeffs,
                                    int numStates, double *evals, double *evecs);
 
+/* Valence density matrix and its traces (makeValenceRho with inSCF = 1,
+ * valeCharge.F90:34-526): rho(k) = sum_j f_jk x_jk x_jk^H per spin from the
+ * eigenvectors of the last solve (:214-232; dsyr at Gamma :266-277), then
+ * Tr[M rho] summed over k as matrixElementMult does on packed matrices
+ * (matrixSubs.F90:66-180) for M = S, NP, KE, MV and every V_j (:330-418).
+ * At spin = 2 the density is first rewritten as up + down and up - down
+ * (:306-314).  Needs olcao_setup_scf (the term matrices) and a solve.
+ * occupancy: [nk][spin][numStates], electronPopulation's order (:160-169);
+ * a state whose populations sum below 1e-8 in magnitude is skipped.
+ * electronEnergy: [spin] sum f * eigenvalue per spin; traces: [spin][4] =
+ * chargeDensityTrace, nucPotTrace, kineticEnergyTrace, massVelocityTrace
+ * (0 when rel = 0); potRho: [spin][potDim]; index 0 is the total density and
+ * index 1 (spin = 2) the difference.  Any output pointer may be NULL. */
+olcao_status olcao_vale_charge(olcao_ctx *ctx, int numStates, const double *occupancy,
+                               double *electronEnergy, double *traces, double *potRho);
+
 /* Mode B iteration, the unit the benchmark times: overlap sweep, fused
  * Hamiltonian sweep per spin and the solve for all k (intgPSCF + bandPSCF,
  * OLCAO.F90:533-770).  potCoeffs: [spin][potDim]; evals:
  * [spin][nk][numStates]. */
 olcao_status olcao_scf_iteration(olcao_ctx *ctx, const double *potCoeffs,
                                  int numStates, double *evals);
+
+/* Batched Mode-B iteration over independent cells (contexts), the batched
+ * form for many uneven cells.  It has no counterpart in the reference, which
+ * runs one cell per process through OLCAO.F90:533-770; each cell's result is
+ * what olcao_scf_iteration gives for it.  nWorkers host threads (1..16) take
+ * the cells heaviest first, each cell on its own streams, so the device runs
+ * several cells at once: first every cell's integral build, then the solves
+ * (those of the tridiagonal path, a cooperative launch, one at a time).
+ * potCoeffs[i], numStates[i], evals[i] as olcao_scf_iteration for cell i.
+ * cellMs (may be NULL): [nCells] each cell's time from its first to its last
+ * event within the batch (its latency while sharing the device).  On failure
+ * *failedCell names the cell and olcao_last_error(NULL) its message.  The
+ * contexts must be distinct and not used by another thread meanwhile. */
+olcao_status olcao_scf_iteration_batch(int nCells, olcao_ctx *const *ctxs,
+                                       const double *const *potCoeffs, const int *numStates,
+                                       double *const *evals, int nWorkers, double *cellMs,
+                                       int *failedCell);
 
 /* Stage times (ms, HIP events) of the last olcao_scf_iteration:
  * t[0] overlap sweep (incl. its contraction and Bloch sum), t[1] Hamiltonian

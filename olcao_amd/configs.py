@@ -256,4 +256,73 @@ def asio2_1000(seed=5):
     return asio2(seed=seed)
 
 
+# conventional diamond cells per supercell, 8..216 atoms
+BATCH_SUPERCELLS = [(1, 1, 1), (2, 1, 1), (3, 1, 1), (2, 2, 1), (3, 2, 1), (2, 2, 2),
+                    (3, 3, 1), (3, 2, 2), (3, 3, 2), (3, 3, 3)]
+BATCH_MESHES = [(1, 1, 1), (2, 1, 1), (2, 2, 1), (2, 2, 2), (3, 2, 2), (3, 3, 2), (3, 3, 3),
+                (4, 3, 3), (4, 4, 3), (4, 4, 4)]
+ATOMIC_MASS = {ELEM_C: 12.011, ELEM_O: 15.999, ELEM_SI: 28.0855}
+ELEM_BASIS = {   # exponent range (bohr^-2), extended-basis exponent count, potential range
+    ELEM_C: (0.12, 1.0e6, 20, 0.1),
+    ELEM_O: (0.15, 1.0e6, 20, 0.12),
+    ELEM_SI: (0.05, 1.0e7, 24, 0.1),
+}
+
+
+def batch_cell(rng, index, k_budget=512):
+    """One cell of the uneven batch.  A diamond (one element) or zincblende
+    (two elements on the two fcc sublattices) supercell of 8..216 atoms of
+    C, Si or O, lattice constant set by a density drawn from 2.2..3.5
+    g/cm^3, 0.1 A Gaussian jitter.  Basis chosen at random: minimal (1s 2s
+    2p per atom, 4 fewer exponents) or extended (s + p + d, as the Si64
+    workload).  k-points: a Monkhorst-Pack mesh of 1..64 points drawn from
+    the meshes with nk * atoms <= k_budget (a k density that falls with the
+    cell size); one point is the real Gamma build."""
+    reps = BATCH_SUPERCELLS[rng.integers(len(BATCH_SUPERCELLS))]
+    elems = [ELEM_C, ELEM_SI, ELEM_O]
+    e1, e2 = elems[rng.integers(3)], elems[rng.integers(3)]
+    density = rng.uniform(2.2, 3.5)
+    m_avg = 0.5 * (ATOMIC_MASS[e1] + ATOMIC_MASS[e2])
+    a = (8 * m_avg * 1.66053906660e-24 / density * 1e24) ** (1.0 / 3.0)   # A
+    frac, sub = [], []
+    for i in range(reps[0]):
+        for j in range(reps[1]):
+            for k in range(reps[2]):
+                frac.append(DIAMOND_FRAC + np.array([i, j, k], float))
+                sub.extend([0] * 4 + [1] * 4)
+    frac = np.vstack(frac) / np.asarray(reps, float)
+    cell = np.diag(np.asarray(reps, float) * a * BOHR_PER_ANGSTROM)
+    pos = frac @ cell + rng.normal(0.0, 0.1 * BOHR_PER_ANGSTROM, size=frac.shape)
+    extended = bool(rng.integers(2))
+    uniq = sorted({e1, e2}, key=[e1, e2].index)
+    types, pots = [], []
+    for e in uniq:
+        amin, amax, na, pmin = ELEM_BASIS[e]
+        if extended:
+            t = make_type(rng, e, even_tempered(amin, amax, na), (na, na - 2, na - 4, 0), [0, 1, 2])
+        else:
+            na -= 4
+            t = make_type(rng, e, even_tempered(amin, amax, na), (na, na - 2, 0, 0), [0, 0, 1])
+        types.append(t)
+        pots.append(PotType(e, float(e), ALPHA_NUC, even_tempered(pmin, 1.0e5, 16)))
+    atom_types = [uniq.index(e1 if s == 0 else e2) for s in sub]
+    n_atoms = len(pos)
+    meshes = [m for m in BATCH_MESHES if m[0] * m[1] * m[2] * n_atoms <= k_budget]
+    mesh = meshes[rng.integers(len(meshes))]
+    gamma = mesh == (1, 1, 1)
+    if gamma:
+        k, w = np.zeros((1, 3)), np.array([2.0])
+    else:
+        k, w = mp_mesh(mesh)
+    name = f"batch{index:02d}"
+    return _assemble(name, cell, pos, atom_types, types, pots,
+                     {i: i for i in range(len(uniq))}, k, w, rng, gamma=gamma)
+
+
+def uneven_batch(seed=4, n_cells=64):
+    """C4: `n_cells` independent cells drawn from one seed (batch_cell)."""
+    rng = np.random.default_rng(seed)
+    return [batch_cell(rng, i) for i in range(n_cells)]
+
+
 CONFIGS = {"diamond_c8": diamond_c8, "si64": si64, "quartz": quartz, "asio2_1000": asio2_1000}

@@ -668,4 +668,91 @@ void launch_extract_evecs(const double *A, int n, int nk, int ns, int cplx, doub
                        out);
 }
 
+// Y(:, j, k) = f(j, k) X(:, j, k) for the first ns eigenvectors of each
+// k-point; X and Y are n x n column-major per k (cplx = 2 interleaved).
+__global__ void k_scale_states(const double *X, const double *f, int n, int ns, int nk, int cplx,
+                               double *Y)
+{
+    const long long total = (long long)nk * ns * n * cplx;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long e = t / cplx;
+        const int r = (int)(e % n);
+        const long long jk = e / n;
+        const int j = (int)(jk % ns), k = (int)(jk / ns);
+        const size_t off = ((size_t)k * n * n + (size_t)j * n + r) * cplx + t % cplx;
+        Y[off] = f[(size_t)k * ns + j] * X[off];
+    }
+}
+
+void launch_scale_states(const double *X, const double *f, int n, int ns, int nk, int cplx,
+                         double *Y, hipStream_t s)
+{
+    long long total = (long long)nk * ns * n * cplx;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_scale_states, dim3((unsigned)blocks), dim3(256), 0, s, X, f, n, ns, nk,
+                       cplx, Y);
+}
+
+// Partial traces Tr[M_m rho] over the upper triangles of nk matrices, rho =
+// R0 + sgn R1 (R1 may be null): off-diagonal elements count twice with
+// re*re + im*im, diagonal elements once with their real parts, as
+// matrixElementMult sums a packed pair (matrixSubs.F90:66-112).  Block b of
+// matrix m writes partial[m * gridDim.x + b]; k_trace_sum adds them in order.
+__global__ __launch_bounds__(256) void k_trace_part(const double *const *M, const double *R0,
+                                                    const double *R1, double sgn, int n, int nk,
+                                                    int cplx, double *partial)
+{
+    const int m = blockIdx.y;
+    const double *A = M[m];
+    const long long total = (long long)nk * n * n;
+    double acc = 0.0;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long rc = e % ((long long)n * n);
+        const int r = (int)(rc % n), c = (int)(rc / n);
+        if (r > c) continue;
+        const size_t o = (size_t)e * cplx;
+        double rr = R0[o], ri = cplx == 2 ? R0[o + 1] : 0.0;
+        if (R1) {
+            rr += sgn * R1[o];
+            if (cplx == 2) ri += sgn * R1[o + 1];
+        }
+        if (r == c)
+            acc += A[o] * rr;
+        else
+            acc += 2.0 * (A[o] * rr + (cplx == 2 ? A[o + 1] * ri : 0.0));
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)m * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ void k_trace_sum(const double *partial, int parts, int nMat, double *out)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nMat) return;
+    double acc = 0.0;
+    for (int b = 0; b < parts; ++b) acc += partial[(size_t)m * parts + b];
+    out[m] = acc;
+}
+
+constexpr int kTraceParts = 512;
+
+void launch_traces(const double *const *M, int nMat, const double *R0, const double *R1,
+                   double sgn, int n, int nk, int cplx, double *partial, double *out,
+                   hipStream_t s)
+{
+    hipLaunchKernelGGL(k_trace_part, dim3(kTraceParts, nMat), dim3(256), 0, s, M, R0, R1, sgn, n,
+                       nk, cplx, partial);
+    hipLaunchKernelGGL(k_trace_sum, dim3((nMat + 63) / 64), dim3(64), 0, s, partial, kTraceParts,
+                       nMat, out);
+}
+
 }  // namespace olcao

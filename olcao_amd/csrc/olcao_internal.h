@@ -218,6 +218,15 @@ void launch_small_hegv(bool cplx, int n, int nb, double *A, double *B, double *w
                        hipStream_t s);
 void launch_extract_evecs(const double *A, int n, int nk, int numStates,
                           int cplx, double *out, hipStream_t s);
+// Valence density: Y(:, j, k) = f(j, k) X(:, j, k) for j < ns.
+void launch_scale_states(const double *X, const double *f, int n, int ns, int nk, int cplx,
+                         double *Y, hipStream_t s);
+// out[m] = Tr[M_m (R0 + sgn R1)] summed over the nk upper triangles
+// (M: device array of nMat matrix pointers; R1 may be null); partial holds
+// 512 doubles per matrix.
+void launch_traces(const double *const *M, int nMat, const double *R0, const double *R1,
+                   double sgn, int n, int nk, int cplx, double *partial, double *out,
+                   hipStream_t s);
 
 // Cooperative tridiagonal reduction with the matrix resident in LDS across a
 // group of workgroups (ktrd.hip).  A: full Hermitian (lower triangle read);

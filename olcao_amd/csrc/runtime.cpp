@@ -17,6 +17,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -24,6 +25,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/olcao_gpu.h"
@@ -171,6 +173,15 @@ struct olcao_ctx {
     double listMs = 0.0;
     DBuf<double> dBlocks, dX, dCoef;
     DBuf<double> S, Aw, Bw, D, E, CV_S, CC_S, Tmp;
+    // eigenvectors of spin 2 (spin 1's stay in Aw) and every spin's
+    // eigenvalues [nk][n] from the last solve, for the valence density
+    DBuf<double> Aw2;
+    std::vector<double> hEv[2];
+    bool vecsValid[2] = {false, false};
+    // valence density work: rho per spin, scaled eigenvectors, occupancies,
+    // trace partial sums, results and the term-matrix pointer list
+    DBuf<double> Rho[2], Ysc, dOcc, dTrPart, dTrOut;
+    DBuf<const double *> dTrPtr;
     DBuf<double> H[2], CV_H[2], CC_H[2];   // per spin
     DBuf<int> dInfo;
     DBuf<unsigned long long> dCounters;
@@ -1320,21 +1331,24 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
 }
 
 // Batched generalised eigensolve of H, S (upper) on the device.
-void solve_device(olcao_ctx &c, int numStates, const double *Hsrc, const double *Ssrc,
+void solve_device(olcao_ctx &c, int q, int numStates, const double *Hsrc, const double *Ssrc,
                   double *evals, double *evecs)
 {
     hipStream_t s = c.stream;
     const int n = c.valeDim, nk = c.nk;
     if (numStates <= 0 || numStates > n) fail(OLCAO_ERR_INPUT, "numStates must lie in 1..%d", n);
-    c.Aw.alloc(c.matDoubles());
+    // spin q's eigenvectors stay in Aw (q = 0) or Aw2 (q = 1) for the density
+    DBuf<double> &Aq = q ? c.Aw2 : c.Aw;
+    c.vecsValid[q] = false;
+    Aq.alloc(c.matDoubles());
     c.Bw.alloc(c.matDoubles());
     c.D.alloc((size_t)n * nk);
     c.E.alloc((size_t)n * nk);
     c.dInfo.alloc(nk);
-    HIPCHK(hipMemcpyAsync(c.Aw.p, Hsrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(Aq.p, Hsrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
     std::vector<int> info;
-    gen_eig(c.eig, c.blas, s, !c.gamma, n, nk, c.Aw.p, c.Bw.p, c.D.p, c.E.p, c.dInfo.p, info,
+    gen_eig(c.eig, c.blas, s, !c.gamma, n, nk, Aq.p, c.Bw.p, c.D.p, c.E.p, c.dInfo.p, info,
             c.eigLibrary);
     std::vector<double> w((size_t)n * nk);
     HIPCHK(hipMemcpyAsync(w.data(), c.D.p, w.size() * 8, hipMemcpyDeviceToHost, s));
@@ -1342,7 +1356,7 @@ void solve_device(olcao_ctx &c, int numStates, const double *Hsrc, const double 
         DBuf<double> tmp;
         const int cp = c.gamma ? 1 : 2;
         tmp.alloc((size_t)nk * numStates * n * cp);
-        launch_extract_evecs(c.Aw.p, n, nk, numStates, cp, tmp.p, s);
+        launch_extract_evecs(Aq.p, n, nk, numStates, cp, tmp.p, s);
         HIPCHK(hipMemcpyAsync(evecs, tmp.p, tmp.n * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
@@ -1354,9 +1368,12 @@ void solve_device(olcao_ctx &c, int numStates, const double *Hsrc, const double 
     if (evals)
         for (int k = 0; k < nk; ++k)
             std::memcpy(evals + (size_t)k * numStates, w.data() + (size_t)k * n, numStates * sizeof(double));
+    c.hEv[q].swap(w);
+    c.vecsValid[q] = true;
 }
 
 thread_local std::string g_noctx_err;   // message of context-free calls
+std::atomic<int> g_streamRotation{0};   // contexts created so far (stream placement)
 
 template <class F>
 olcao_status guard(olcao_ctx *c, F &&f)
@@ -1395,9 +1412,19 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
         c->eigLibrary = opt && opt->eigensolver == OLCAO_EIG_LIBRARY;
         if (c->precision != OLCAO_PREC_FP64 && c->precision != OLCAO_PREC_MIXED)
             fail(OLCAO_ERR_INPUT, "precision must be OLCAO_PREC_FP64 or OLCAO_PREC_MIXED");
-        HIPCHK(hipStreamCreate(&c->stream));
+        // Non-blocking streams: a synchronous copy on the null stream
+        // elsewhere in the process (libraries issue them) neither waits for
+        // nor stalls this context's work.  The runtime spreads a process's
+        // streams over a few hardware queues in creation order, so the main
+        // stream is created after a rotating number of side streams: the
+        // main streams of successive contexts (where their solves run) land
+        // on different hardware queues and can run at the same time.
+        const int rot = g_streamRotation.fetch_add(1) % (olcao_ctx::kSide + 1);
+        for (int q = 0; q < rot; ++q) HIPCHK(hipStreamCreateWithFlags(&c->side[q], hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (int q = rot; q < olcao_ctx::kSide; ++q)
+            HIPCHK(hipStreamCreateWithFlags(&c->side[q], hipStreamNonBlocking));
         for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
-        for (auto &q : c->side) HIPCHK(hipStreamCreate(&q));
         HIPCHK(hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming));
         for (auto &e : c->evJoin) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         if (rocblas_create_handle(&c->blas) != rocblas_status_success)
@@ -1530,7 +1557,7 @@ static void solve_spins(olcao_ctx *c, int numStates, double *evals, double *evec
     const size_t ev = (size_t)c->nk * numStates;
     const size_t vec = ev * c->valeDim * (c->gamma ? 1 : 2);
     for (int q = 0; q < c->spin; ++q)
-        solve_device(*c, numStates, c->H[q].p, c->S.p, evals ? evals + q * ev : nullptr,
+        solve_device(*c, q, numStates, c->H[q].p, c->S.p, evals ? evals + q * ev : nullptr,
                      evecs ? evecs + q * vec : nullptr);
 }
 
@@ -1595,57 +1622,223 @@ olcao_status olcao_secular_eqn_scf(olcao_ctx *c, const double *coef, int numStat
     });
 }
 
+olcao_status olcao_vale_charge(olcao_ctx *c, int numStates, const double *occupancy,
+                               double *electronEnergy, double *traces, double *potRho)
+{
+    if (!c || !occupancy) return OLCAO_ERR_INPUT;
+    return guard(c, [&] {
+        const int n = c->valeDim, nk = c->nk, sp = c->spin, cp = c->gamma ? 1 : 2;
+        if (!c->haveScf) fail(OLCAO_ERR_STATE, "call olcao_setup_scf first (the term matrices)");
+        for (int q = 0; q < sp; ++q)
+            if (!c->vecsValid[q]) fail(OLCAO_ERR_STATE, "solve for the eigenvectors first");
+        if (numStates <= 0 || numStates > n) fail(OLCAO_ERR_INPUT, "numStates must lie in 1..%d", n);
+        hipStream_t s = c->stream;
+        // populations arrive in electronPopulation order [nk][spin][state]
+        // (valeCharge.F90:160-169); a state whose populations sum below
+        // smallThresh in magnitude is skipped (:214-216)
+        const double smallThresh = 1e-8;
+        std::vector<double> f((size_t)sp * nk * numStates, 0.0);   // [spin][nk][state]
+        double ee[2] = {0.0, 0.0};
+        for (int k = 0; k < nk; ++k)
+            for (int j = 0; j < numStates; ++j) {
+                double sum = 0.0;
+                for (int q = 0; q < sp; ++q) sum += std::fabs(occupancy[((size_t)k * sp + q) * numStates + j]);
+                if (sum < smallThresh) continue;
+                for (int q = 0; q < sp; ++q) {
+                    const double fq = occupancy[((size_t)k * sp + q) * numStates + j];
+                    f[((size_t)q * nk + k) * numStates + j] = fq;
+                    ee[q] += fq * c->hEv[q][(size_t)k * n + j];
+                }
+            }
+        // rho(k) = X diag(f) X^H per spin (the zher / dsyr updates, :228-232, :273-277)
+        c->dOcc.upload(f, s);
+        c->Ysc.alloc(c->matDoubles());
+        const int nb = c->gamma ? 1 : nk;
+        for (int q = 0; q < sp; ++q) {
+            const double *X = q ? c->Aw2.p : c->Aw.p;
+            c->Rho[q].alloc(c->matDoubles());
+            launch_scale_states(X, c->dOcc.p + (size_t)q * nk * numStates, n, numStates, nb, cp,
+                                c->Ysc.p, s);
+            HIPCHK(hipGetLastError());
+            gemm_nn(c->blas, !c->gamma, false, true, n, n, numStates, 1.0, c->Ysc.p, X, 0.0,
+                    c->Rho[q].p, n, nb);
+        }
+        // traces against S, NP, KE, MV and every V_j (:330-418), for the
+        // total and, at spin 2, the difference density (:306-314)
+        std::vector<const double *> mats = {c->S.p, c->NPm.p, c->KEm.p, c->rel ? c->MVm.p : c->KEm.p};
+        for (int j = 0; j < c->potDim; ++j) mats.push_back(c->Vm[j].p);
+        const int nMat = (int)mats.size();
+        c->dTrPtr.upload(mats, s);
+        c->dTrPart.alloc((size_t)512 * nMat);
+        c->dTrOut.alloc((size_t)sp * nMat);
+        for (int t = 0; t < sp; ++t) {
+            launch_traces(c->dTrPtr.p, nMat, c->Rho[0].p, sp == 2 ? c->Rho[1].p : nullptr,
+                          t == 0 ? 1.0 : -1.0, n, nb, cp, c->dTrPart.p, c->dTrOut.p + (size_t)t * nMat, s);
+            HIPCHK(hipGetLastError());
+        }
+        std::vector<double> out((size_t)sp * nMat);
+        HIPCHK(hipMemcpyAsync(out.data(), c->dTrOut.p, out.size() * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int t = 0; t < sp; ++t) {
+            const double *o = out.data() + (size_t)t * nMat;
+            if (electronEnergy) electronEnergy[t] = ee[t];
+            if (traces) {
+                traces[4 * t + 0] = o[0];
+                traces[4 * t + 1] = o[1];
+                traces[4 * t + 2] = o[2];
+                traces[4 * t + 3] = c->rel ? o[3] : 0.0;
+            }
+            if (potRho)
+                for (int j = 0; j < c->potDim; ++j) potRho[(size_t)t * c->potDim + j] = o[4 + j];
+        }
+    });
+}
+
+// First half of a Mode-B iteration: potential coefficients in, overlap
+// sweep, fused Hamiltonian sweep per spin, contraction, Bloch sums and
+// orthogonalisation, all queued on the context's streams.
+static void iteration_build(olcao_ctx &c, const double *coef)
+{
+    ensure_mats(c);
+    hipStream_t s = c.stream;
+    HIPCHK(hipMemcpyAsync(c.dCoef.p, coef, (size_t)c.potDim * c.spin * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(c.ev[0], s));
+    overlap_sweep(c);
+    HIPCHK(hipEventRecord(c.ev[1], s));
+    // events: 0 start, 1 overlap done; per spin q: 2+3q before the
+    // three-centre sweep, 3+3q after it, 4+3q after contraction, Bloch
+    // sum and orthogonalisation; 8 after the solve
+    for (int q = 0; q < c.spin; ++q) {
+        // kinetic blocks: here for every class when rel = 1 (the
+        // mass-velocity term follows them); otherwise only for the
+        // classes whose finish kernels do not add them
+        two_center_forked(c, OP_KE, 0, c.rel ? 0xffffu : ~kFusedKEClasses & 0xffffu);
+        if (c.rel) two_center_forked(c, OP_MV, 1);
+        launch_fill(reinterpret_cast<double *>(c.dCounters.p), 32, 0.0, s);
+        HIPCHK(hipEventRecord(c.ev[2 + 3 * q], s));
+        three_center_forked(c, K3_HAM, 0, 0, c.dCoef.p + (size_t)q * c.potDim, c.rel ? 1 : 1 | kAccAddKE);
+        HIPCHK(hipEventRecord(c.ev[3 + 3 * q], s));
+        contract_scatter(c, c.H[q].p, c.CV_H[q].p, c.CC_H[q].p);
+        ortho(c, c.H[q].p, c.CV_H[q].p, c.CC_H[q].p, false);
+        HIPCHK(hipEventRecord(c.ev[4 + 3 * q], s));
+    }
+    c.haveH = true;
+}
+
+// Second half: the solve for all k and spins, then the stage times.
+static void iteration_solve(olcao_ctx &c, int numStates, double *evals)
+{
+    hipStream_t s = c.stream;
+    solve_spins(&c, numStates, evals, nullptr);
+    HIPCHK(hipEventRecord(c.ev[8], s));
+    HIPCHK(hipEventSynchronize(c.ev[8]));
+    auto el = [&](int a, int b) {
+        float v = 0.f;
+        HIPCHK(hipEventElapsedTime(&v, c.ev[a], c.ev[b]));
+        return (double)v;
+    };
+    double ham = 0.0, t3c = 0.0, cb = 0.0;
+    for (int q = 0; q < c.spin; ++q) {
+        ham += el(q == 0 ? 1 : 1 + 3 * q, 3 + 3 * q);
+        t3c += el(2 + 3 * q, 3 + 3 * q);
+        cb += el(3 + 3 * q, 4 + 3 * q);
+    }
+    c.timings[0] = el(0, 1);
+    c.timings[1] = ham;
+    c.timings[2] = cb;
+    c.timings[3] = el(1 + 3 * c.spin, 8);
+    c.timings[4] = el(0, 8);
+    c.timings[5] = t3c;
+}
+
 olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates, double *evals)
 {
     if (!c || !coef) return OLCAO_ERR_INPUT;
     return guard(c, [&] {
-        ensure_mats(*c);
-        hipStream_t s = c->stream;
-        HIPCHK(hipMemcpyAsync(c->dCoef.p, coef, (size_t)c->potDim * c->spin * 8,
-                              hipMemcpyHostToDevice, s));
-        HIPCHK(hipEventRecord(c->ev[0], s));
-        overlap_sweep(*c);
-        HIPCHK(hipEventRecord(c->ev[1], s));
-        // events: 0 start, 1 overlap done; per spin q: 2+3q before the
-        // three-centre sweep, 3+3q after it, 4+3q after contraction, Bloch
-        // sum and orthogonalisation; 8 after the solve
-        for (int q = 0; q < c->spin; ++q) {
-            // kinetic blocks: here for every class when rel = 1 (the
-            // mass-velocity term follows them); otherwise only for the
-            // classes whose finish kernels do not add them
-            two_center_forked(*c, OP_KE, 0, c->rel ? 0xffffu : ~kFusedKEClasses & 0xffffu);
-            if (c->rel) two_center_forked(*c, OP_MV, 1);
-            launch_fill(reinterpret_cast<double *>(c->dCounters.p), 32, 0.0, s);
-            HIPCHK(hipEventRecord(c->ev[2 + 3 * q], s));
-            three_center_forked(*c, K3_HAM, 0, 0, c->dCoef.p + (size_t)q * c->potDim,
-                                c->rel ? 1 : 1 | kAccAddKE);
-            HIPCHK(hipEventRecord(c->ev[3 + 3 * q], s));
-            contract_scatter(*c, c->H[q].p, c->CV_H[q].p, c->CC_H[q].p);
-            ortho(*c, c->H[q].p, c->CV_H[q].p, c->CC_H[q].p, false);
-            HIPCHK(hipEventRecord(c->ev[4 + 3 * q], s));
-        }
-        c->haveH = true;
-        solve_spins(c, numStates, evals, nullptr);
-        HIPCHK(hipEventRecord(c->ev[8], s));
-        HIPCHK(hipEventSynchronize(c->ev[8]));
-        auto el = [&](int a, int b) {
-            float v = 0.f;
-            HIPCHK(hipEventElapsedTime(&v, c->ev[a], c->ev[b]));
-            return (double)v;
-        };
-        double ham = 0.0, t3c = 0.0, cb = 0.0;
-        for (int q = 0; q < c->spin; ++q) {
-            ham += el(q == 0 ? 1 : 1 + 3 * q, 3 + 3 * q);
-            t3c += el(2 + 3 * q, 3 + 3 * q);
-            cb += el(3 + 3 * q, 4 + 3 * q);
-        }
-        c->timings[0] = el(0, 1);
-        c->timings[1] = ham;
-        c->timings[2] = cb;
-        c->timings[3] = el(1 + 3 * c->spin, 8);
-        c->timings[4] = el(0, 8);
-        c->timings[5] = t3c;
+        iteration_build(*c, coef);
+        iteration_solve(*c, numStates, evals);
     });
+}
+
+// Whether the context's solve takes the tridiagonal path, whose reduction
+// kernel is a cooperative launch that needs all of its workgroups resident
+// at once (ktrd.hip).
+static bool cooperative_solve(const olcao_ctx &c)
+{
+    TrdPlan plan;
+    return !c.eigLibrary && !small_hegv_fits(c.valeDim, !c.gamma) &&
+           trd_fits(c.valeDim, c.nk, !c.gamma, plan);
+}
+
+olcao_status olcao_scf_iteration_batch(int nCells, olcao_ctx *const *ctxs,
+                                       const double *const *potCoeffs, const int *numStates,
+                                       double *const *evals, int nWorkers, double *cellMs,
+                                       int *failedCell)
+{
+    if (failedCell) *failedCell = -1;
+    if (nCells <= 0 || !ctxs || !potCoeffs || !numStates || !evals) return OLCAO_ERR_INPUT;
+    for (int i = 0; i < nCells; ++i)
+        if (!ctxs[i] || !potCoeffs[i] || !evals[i]) {
+            if (failedCell) *failedCell = i;
+            return OLCAO_ERR_INPUT;
+        }
+    nWorkers = std::max(1, std::min(nWorkers, 16));
+    // heaviest cells first: three-centre image entries plus the solve's n^3
+    std::vector<int> order(nCells);
+    std::vector<double> weight(nCells);
+    for (int i = 0; i < nCells; ++i) {
+        const olcao_ctx &c = *ctxs[i];
+        weight[i] = (double)c.imageEntries +
+                    (double)c.valeDim * c.valeDim * c.valeDim * c.nk * c.spin / 16.0;
+        order[i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return weight[a] > weight[b]; });
+    std::vector<olcao_status> st(nCells, OLCAO_OK);
+    std::atomic<bool> failed{false};
+    // one pass over `list`: nWorkers host threads take cells in order, each
+    // cell on its own streams, so the device runs several cells at once
+    auto run = [&](const std::vector<int> &list, int workers, auto &&step) {
+        std::atomic<int> next{0};
+        auto work = [&] {
+            for (int t; (t = next.fetch_add(1)) < (int)list.size();) {
+                if (failed.load()) return;
+                const int i = list[t];
+                st[i] = guard(ctxs[i], [&] { step(i); });
+                if (st[i] != OLCAO_OK) failed.store(true);
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int w = 1; w < std::min(workers, (int)list.size()); ++w) pool.emplace_back(work);
+        work();
+        for (auto &th : pool) th.join();
+    };
+    // 1. every cell's integral build, concurrently
+    run(order, nWorkers, [&](int i) { iteration_build(*ctxs[i], potCoeffs[i]); });
+    // 2. the solves that are ordinary launches, concurrently; then the
+    // cooperative ones one at a time with nothing else on the device
+    std::vector<int> plain, coop;
+    for (int i : order) (cooperative_solve(*ctxs[i]) ? coop : plain).push_back(i);
+    if (!failed.load())
+        run(plain, nWorkers, [&](int i) { iteration_solve(*ctxs[i], numStates[i], evals[i]); });
+    // every other cell's work has ended (each plain solve waits for its own
+    // stream); the cooperative cells' builds finish before the first of them
+    for (int i : coop)
+        if (st[i] == OLCAO_OK &&
+            (st[i] = guard(ctxs[i], [&] { HIPCHK(hipStreamSynchronize(ctxs[i]->stream)); })) != OLCAO_OK)
+            failed.store(true);
+    if (!failed.load())
+        run(coop, 1, [&](int i) { iteration_solve(*ctxs[i], numStates[i], evals[i]); });
+    for (int t = 0; t < nCells; ++t) {
+        const int i = order[t];
+        if (st[i] != OLCAO_OK) {
+            if (failedCell) *failedCell = i;
+            g_noctx_err = ctxs[i]->err;
+            return st[i];
+        }
+    }
+    if (cellMs)
+        for (int i = 0; i < nCells; ++i) cellMs[i] = ctxs[i]->timings[4];
+    return OLCAO_OK;
 }
 
 olcao_status olcao_get_work_counts(olcao_ctx *c, unsigned long long counts[32])

@@ -78,6 +78,9 @@ FUNCTIONS = {
     "olcao_setup_scf": [C.c_void_p],
     "olcao_secular_eqn_scf": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
     "olcao_scf_iteration": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p],
+    "olcao_vale_charge": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "olcao_scf_iteration_batch": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_int, C.c_void_p, C.c_void_p],
     "olcao_get_timings": [C.c_void_p, C.c_void_p],
     "olcao_get_work_counts": [C.c_void_p, C.c_void_p],
     "olcao_solve_zhegv": [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
@@ -342,6 +345,21 @@ class OlcaoGPU:
         _check(lib().olcao_scf_iteration(self.ctx, _p(c), num_states, _p(w)), self.ctx)
         return self._evals(w)
 
+    def makeValenceRho(self, occupancy):
+        """Valence density traces from the last solve (makeValenceRho,
+        valeCharge.F90:34).  occupancy: [nk, spin, numStates] populations.
+        Returns dict(electron_energy [spin], charge, nuclear, kinetic,
+        mass_velocity [spin] (0 = total, 1 = difference), pot_rho [spin,
+        potDim])."""
+        occ = np.ascontiguousarray(occupancy, np.float64).reshape(self.nk, self.spin, -1)
+        ns = occ.shape[2]
+        ee = np.zeros(self.spin)
+        tr = np.zeros((self.spin, 4))
+        pr = np.zeros((self.spin, self.pot_dim))
+        _check(lib().olcao_vale_charge(self.ctx, ns, _p(occ), _p(ee), _p(tr), _p(pr)), self.ctx)
+        return dict(electron_energy=ee, charge=tr[:, 0], nuclear=tr[:, 1], kinetic=tr[:, 2],
+                    mass_velocity=tr[:, 3], pot_rho=pr)
+
     def work_counts(self):
         """Three-centre evaluations of the last sweep per class (la, lb):
         array [4, 4, 2] of (three-Gaussian overlap, nuclear)."""
@@ -354,3 +372,26 @@ class OlcaoGPU:
         _check(lib().olcao_get_timings(self.ctx, _p(t)), self.ctx)
         return dict(overlap_ms=t[0], hamiltonian_ms=t[1], contract_bloch_ms=t[2],
                     solve_ms=t[3], total_ms=t[4], three_center_ms=t[5])
+
+
+def scf_iteration_batch(cells, num_states, workers=8):
+    """Batched Mode-B iteration over independent cells (OlcaoGPU contexts,
+    one per cell) in one call: olcao_scf_iteration_batch.  num_states: one
+    count per cell.  Returns (eigenvalues per cell as scf_iteration gives
+    them, each cell's latency in ms within the batch)."""
+    n = len(cells)
+    coefs = [c._coeffs(c.sysin.pot_coeffs) for c in cells]
+    ws = [np.zeros((c.spin, c.nk, ns)) for c, ns in zip(cells, num_states)]
+    ctxs = (C.c_void_p * n)(*[c.ctx for c in cells])
+    cp = (C.c_void_p * n)(*[_p(a) for a in coefs])
+    wp = (C.c_void_p * n)(*[_p(a) for a in ws])
+    ns = np.asarray(num_states, np.int32)
+    ms = np.zeros(n)
+    bad = np.full(1, -1, np.int32)
+    code = lib().olcao_scf_iteration_batch(n, C.cast(ctxs, C.c_void_p), C.cast(cp, C.c_void_p),
+                                           _p(ns), C.cast(wp, C.c_void_p), workers, _p(ms),
+                                           _p(bad))
+    if code != 0:
+        msg = lib().olcao_last_error(None).decode()
+        raise OlcaoError(code, f"cell {int(bad[0])}: {msg}")
+    return [c._evals(w) for c, w in zip(cells, ws)], ms

@@ -383,3 +383,69 @@ def solve(H, S, num_states=None):
     if single:
         return evals[0], evecs[0]
     return np.array(evals), np.array(evecs)
+
+
+def matrix_element_mult(m1, m2):
+    """Tr[M rho] of two packed Hermitian matrices as matrixElementMult sums
+    them (matrixSubs.F90:66-112): off-diagonal pairs re*re + im*im, diagonal
+    real parts halved, the whole doubled.  Real packed matrices follow
+    matrixElementMultGamma (:143-179): twice the sum of all products minus
+    the diagonal products."""
+    n = int(round((math.sqrt(8 * m1.shape[-1] + 1) - 1) / 2))
+    diag = np.cumsum(np.arange(1, n + 1)) - 1
+    if np.iscomplexobj(m1) or np.iscomplexobj(m2):
+        prod = m1.real * m2.real + m1.imag * m2.imag
+        off = prod.sum() - prod[diag].sum()
+        return 2.0 * (off + (m1.real[diag] * m2.real[diag]).sum() / 2.0)
+    return 2.0 * float(np.sum(m1 * m2)) - float(np.sum(m1[diag] * m2[diag]))
+
+
+def vale_charge(evals, evecs, occupancy, S, NP, KE, V, MV=None):
+    """makeValenceRho with inSCF = 1 (valeCharge.F90:34-526) on the CPU.
+
+    evals: [spin][nk][numStates]; evecs: [spin][nk][n][numStates] (columns
+    are states; real at Gamma); occupancy: [nk][spin][numStates]
+    (electronPopulation order, :160-169).  S, NP, KE, MV, V[j]: the term
+    matrices as [nk][n][n] upper triangles (or [n][n] at Gamma).
+
+    Per k: states whose populations sum below smallThresh in magnitude are
+    skipped (:214-216); electronEnergy += f * eigenvalue (:218-219); the
+    upper triangle of rho gets f x x^H (zher / dsyr, :228-232, :273-277); rho
+    is packed (:263-265) and, at spin 2, rewritten as up + down and up - down
+    (:306-314); then each trace accumulates matrixElementMult of the packed
+    term matrix against the packed density (:330-418).  Returns
+    (electronEnergy [spin], traces [spin][4] = charge, nuclear, kinetic,
+    mass velocity, potRho [spin][potDim])."""
+    occ = np.asarray(occupancy, np.float64)
+    nk, spin, ns = occ.shape
+    gamma = not np.iscomplexobj(np.asarray(evecs))
+    mats = lambda M: (np.asarray(M)[None] if gamma else np.asarray(M))   # noqa: E731
+    S, NP, KE = mats(S), mats(NP), mats(KE)
+    MV = mats(MV) if MV is not None else None
+    V = [mats(v) for v in V]
+    n = S.shape[-1]
+    ee = np.zeros(spin)
+    traces = np.zeros((spin, 4))
+    pot_rho = np.zeros((spin, len(V)))
+    for k in range(nk):
+        rho = np.zeros((spin, n, n), dtype=np.float64 if gamma else np.complex128)
+        for j in range(ns):
+            f = occ[k, :, j]
+            if np.sum(np.abs(f)) < SMALL_THRESH:
+                continue
+            ee += f * np.array([evals[q][k][j] for q in range(spin)])
+            for q in range(spin):
+                x = np.asarray(evecs[q][k])[:, j]
+                rho[q] += np.triu(f[q] * np.outer(x, np.conj(x)))
+        packed = np.array([pack_upper(rho[q]) for q in range(spin)])
+        if spin == 2:
+            packed = np.array([packed[0] + packed[1], packed[0] - packed[1]])
+        for t in range(spin):
+            traces[t, 0] += matrix_element_mult(pack_upper(S[k]), packed[t])
+            traces[t, 1] += matrix_element_mult(pack_upper(NP[k]), packed[t])
+            traces[t, 2] += matrix_element_mult(pack_upper(KE[k]), packed[t])
+            if MV is not None:
+                traces[t, 3] += matrix_element_mult(pack_upper(MV[k]), packed[t])
+            for jj, v in enumerate(V):
+                pot_rho[t, jj] += matrix_element_mult(pack_upper(v[k]), packed[t])
+    return ee, traces, pot_rho

@@ -41,6 +41,9 @@ def test_null_arguments_are_rejected():
     assert lib.olcao_ctx_create(None, None, None) == 3
     assert lib.olcao_get_dims(None, None) == 3
     assert lib.olcao_scf_iteration(None, None, 1, None) == 3
+    bad = C.c_int(7)
+    assert lib.olcao_scf_iteration_batch(0, None, None, None, None, 4, None, C.byref(bad)) == 3
+    assert bad.value == -1
 
 
 def test_fortran_facade_source_names_reference_procedures():

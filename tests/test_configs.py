@@ -24,3 +24,33 @@ def test_amorphous_sio2_cube():
     # the generator is deterministic in its seed
     t = configs.asio2_1000()
     assert np.array_equal(np.asarray(t.atom_pos), np.asarray(s.atom_pos))
+
+
+def test_uneven_batch():
+    cells = configs.uneven_batch()
+    assert len(cells) == 64
+    masses = configs.ATOMIC_MASS
+    sizes, nks, kinds = set(), set(), set()
+    for s in cells:
+        s.validate()
+        n = s.n_atoms
+        assert 8 <= n <= 216 and n % 8 == 0
+        assert 1 <= s.nk <= 64 and s.nk * n <= 512
+        assert s.gamma == (s.nk == 1)
+        if not s.gamma:
+            assert abs(np.sum(s.kweights) - 2.0) < 1e-12
+        elems = [s.types[t].element for t in s.atom_type]
+        assert set(elems) <= {configs.ELEM_C, configs.ELEM_SI, configs.ELEM_O}
+        vol = abs(np.linalg.det(s.cell)) / BOHR_PER_ANGSTROM ** 3 * 1e-24   # cm^3
+        rho = sum(masses[e] for e in elems) * 1.66053906660e-24 / vol
+        assert 2.2 - 1e-9 <= rho <= 3.5 + 1e-9
+        per_atom = {t.num_vale_states for t in s.types}
+        assert per_atom <= {5, 9}
+        sizes.add(n)
+        nks.add(s.nk)
+        kinds.add(per_atom.pop())
+    # the batch spans the stated ranges
+    assert min(sizes) == 8 and max(sizes) == 216 and 1 in nks and 64 in nks
+    assert kinds == {5, 9}
+    t = configs.uneven_batch()
+    assert all(np.array_equal(a.atom_pos, b.atom_pos) for a, b in zip(cells, t))

@@ -250,6 +250,127 @@ def test_amorphous_sio2_1000_full_size(gpu):
     assert err < 1.0
 
 
+BATCH_ORACLE_CELLS = (2, 6, 20, 59)   # the batch's 8-atom cells: minimal and extended
+                                     # bases, Gamma (real) and 4 / 8 k-points
+
+
+def test_uneven_batch_cells_match_oracle(gpu):
+    """Cells of the uneven batch that the oracle finishes in seconds: S(k),
+    H(k) and the Mode-B eigenvalues against the oracle, then the same cells
+    through the batched call against their single-context iterations."""
+    cells = configs.uneven_batch()
+    ctxs = []
+    for i in BATCH_ORACLE_CELLS:
+        s = cells[i]
+        _log(f"batch cell {i}: {s.n_atoms} atoms, n={s.vale_dim}, nk={s.nk}, gamma={s.gamma}")
+        g, o = _pair(gpu, s)
+        Sg = g.gaussOverlapOL()[0]
+        Hg = g.gaussOverlapHamPSCF(s.pot_coeffs)[0]
+        So = o.integral(oracle.OL)[0]
+        Ho = o.integral(oracle.HAM, coef=s.pot_coeffs)[0]
+        assert block_rel_err(Sg, So) < 1e-10, i
+        assert block_rel_err(Hg, Ho) < 1e-10, i
+        wo = np.asarray(oracle.solve(Ho, So)[0]).reshape(-1)
+        w = np.asarray(g.scf_iteration(s.pot_coeffs, g.vale_dim)).reshape(-1)
+        err = np.abs(w - wo).max() * HARTREE_EV
+        _log(f"  eigenvalue error {err:.2e} eV")
+        assert err < 1e-6, i
+        ctxs.append((g, w))
+    wb, ms = gpu.scf_iteration_batch([g for g, _ in ctxs], [g.vale_dim for g, _ in ctxs], 4)
+    for (g, w), b in zip(ctxs, wb):
+        assert np.abs(np.asarray(b).reshape(-1) - w).max() * HARTREE_EV < 1e-9
+        g.close()
+    assert np.all(ms > 0)
+
+
+def test_uneven_batch_equals_single_cells(gpu):
+    """All 64 cells of the uneven batch (8..216 atoms, n = 40..1296, 1..64
+    k-points; every eigensolver path) in one batched call, twice in a row:
+    each cell's eigenvalues equal its single-context Mode-B iteration's, and
+    they are finite and ascending."""
+    cells = configs.uneven_batch()
+    _log("uneven batch: creating 64 contexts")
+    ctxs = [gpu.OlcaoGPU(s) for s in cells]
+    ns = [g.vale_dim for g in ctxs]
+    single = []
+    for i, (g, n) in enumerate(zip(ctxs, ns)):
+        single.append(np.asarray(g.scf_iteration(g.sysin.pot_coeffs, n)))
+        _log(f"  cell {i} n={n} nk={g.nk}: {g.timings()['total_ms']:.2f} ms")
+    for rep in range(2):
+        wb, ms = gpu.scf_iteration_batch(ctxs, ns, 8)
+        worst = 0.0
+        for w1, w2 in zip(single, wb):
+            w2 = np.asarray(w2)
+            assert w2.shape == w1.shape
+            assert np.all(np.isfinite(w2)) and np.all(np.diff(w2, axis=-1) >= 0.0)
+            worst = max(worst, float(np.abs(w2 - w1).max()))
+        _log(f"  batch pass {rep}: max |batch - single| {worst * HARTREE_EV:.2e} eV, "
+             f"latency p50 {np.percentile(ms, 50):.1f} ms p99 {np.percentile(ms, 99):.1f} ms")
+        assert worst * HARTREE_EV < 1e-9
+    for g in ctxs:
+        g.close()
+
+
+VALE_CASES = {"cubic_sp": dict(lmax=1, kmesh=(2, 2, 1)),
+              "core_gamma": dict(core=True, gamma=True, two_types=True),
+              "rel_spin2": dict(lmax=1, rel=1, spin=2, kmesh=(2, 1, 1))}
+
+
+@pytest.mark.parametrize("case", list(VALE_CASES))
+def test_vale_charge_matches_oracle(gpu, case):
+    """Valence density traces (makeValenceRho) after a Mode A solve on the
+    device against the oracle's from its own terms and solve, with the
+    same Fermi-smeared populations: electron energy, charge, nuclear,
+    kinetic and mass-velocity traces and potRho to 1e-10 of their scale."""
+    from tests.conftest import fermi_occupancy, mode_a_hamiltonian, mode_a_terms
+    _log(f"vale charge {case}")
+    s = configs.small_test(**VALE_CASES[case])
+    g, o = _pair(gpu, s)
+    t = mode_a_terms(o, s)
+    coefs = np.asarray(s.pot_coeffs).reshape(s.spin, -1)
+    ws, vs = [], []
+    for q in range(s.spin):
+        w, v = oracle.solve(mode_a_hamiltonian(t, coefs[q]), t["S"])
+        ws.append(np.asarray(w).reshape(s.nk, -1))
+        vs.append(np.asarray(v).reshape(s.nk, s.vale_dim, -1))
+    n_el = sum(s.pot_types[p].Z for p in s.pot_type) - 2 * s.core_dim
+    occ = fermi_occupancy(ws, s.kweights, n_el, s.spin)
+    ee, tr, pr = oracle.vale_charge(ws, vs, occ, t["S"], t["NP"], t["KE"], t["V"], t["MV"])
+    g.setupSCF()
+    g.secularEqnSCF(s.pot_coeffs, g.vale_dim)
+    d = g.makeValenceRho(occ)
+    scale = max(1.0, np.abs(ee).max())
+    assert np.abs(d["electron_energy"] - ee).max() < 1e-10 * scale
+    got = np.stack([d["charge"], d["nuclear"], d["kinetic"], d["mass_velocity"]], axis=1)
+    assert np.abs(got - tr).max() < 1e-10 * max(1.0, np.abs(tr).max())
+    assert np.abs(d["pot_rho"] - pr).max() < 1e-10 * max(1.0, np.abs(pr).max())
+    g.close()
+
+
+def test_vale_charge_si64_invariants(gpu):
+    """Si64 at full size (576 orbitals, 8 k): after a Mode A solve the
+    charge trace equals the summed populations and the Hamiltonian traces
+    sum_j c_j potRho_j + KE + NP equal sum f * eigenvalue."""
+    import time
+    from tests.conftest import fermi_occupancy
+    _log("vale charge si64")
+    s = configs.si64()
+    g = gpu.OlcaoGPU(s)
+    g.setupSCF()
+    w = g.secularEqnSCF(s.pot_coeffs, g.vale_dim)
+    n_el = sum(s.pot_types[p].Z for p in s.pot_type)
+    occ = fermi_occupancy([w], s.kweights, n_el, 1)
+    g.makeValenceRho(occ)
+    t0 = time.perf_counter()
+    d = g.makeValenceRho(occ)
+    _log(f"  makeValenceRho {1e3 * (time.perf_counter() - t0):.2f} ms")
+    total = occ.sum()
+    assert abs(d["charge"][0] - total) < 1e-9 * total
+    e_sum = np.asarray(s.pot_coeffs) @ d["pot_rho"][0] + d["kinetic"][0] + d["nuclear"][0]
+    assert abs(e_sum - d["electron_energy"][0]) < 1e-9 * abs(d["electron_energy"][0])
+    g.close()
+
+
 def test_diamond_c8_full_parity(gpu):
     """First workload at full size: H(k), S(k) and all eigenvalues."""
     _log("diamond_c8 full")

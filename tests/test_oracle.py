@@ -510,3 +510,55 @@ def test_solve_raises_on_indefinite_overlap():
     S = np.diag([1.0, -1.0]).astype(complex)
     with pytest.raises(RuntimeError):
         oracle.solve(np.eye(2, dtype=complex), S)
+
+
+# --------------------------------------------------------------------------
+# valence density traces (makeValenceRho, valeCharge.F90:34-526)
+def test_matrix_element_mult_is_trace_of_product():
+    """matrixElementMult on packed upper triangles equals Tr[M rho] of the
+    full Hermitian (or symmetric) matrices."""
+    rng = np.random.default_rng(12)
+    n = 7
+    for cplx in (True, False):
+        A = rng.normal(size=(n, n)) + (1j * rng.normal(size=(n, n)) if cplx else 0)
+        B = rng.normal(size=(n, n)) + (1j * rng.normal(size=(n, n)) if cplx else 0)
+        A, B = A + np.conj(A.T), B + np.conj(B.T)
+        t = oracle.matrix_element_mult(oracle.pack_upper(np.triu(A)), oracle.pack_upper(np.triu(B)))
+        assert abs(t - np.trace(A @ B).real) < 1e-12 * np.abs(A).sum() * np.abs(B).max()
+
+
+@pytest.mark.parametrize("case", [dict(lmax=1, kmesh=(2, 2, 1)),
+                                  dict(core=True, gamma=True, two_types=True),
+                                  dict(lmax=1, rel=1, spin=2, kmesh=(2, 1, 1))])
+def test_vale_charge_invariants(case):
+    """Tr[S rho] is the summed population (the eigenvectors are
+    S-orthonormal) and sum_j c_j potRho_j + KE + NP (+ MV) traces equal
+    sum f * eigenvalue per spin (Tr[H rho] = sum f e), from the oracle's
+    own Mode A terms and solve."""
+    from tests.conftest import fermi_occupancy, mode_a_hamiltonian, mode_a_terms
+    s = configs.small_test(**case)
+    o = oracle.Oracle(s)
+    t = mode_a_terms(o, s)
+    coefs = np.asarray(s.pot_coeffs).reshape(s.spin, -1)
+    ws, vs = [], []
+    for q in range(s.spin):
+        w, v = oracle.solve(mode_a_hamiltonian(t, coefs[q]), t["S"])
+        ws.append(np.asarray(w).reshape(s.nk, -1))
+        vs.append(np.asarray(v).reshape(s.nk, s.vale_dim, -1))
+    n_el = sum(s.pot_types[p].Z for p in s.pot_type) - 2 * s.core_dim
+    occ = fermi_occupancy(ws, s.kweights, n_el, s.spin)
+    ee, tr, pr = oracle.vale_charge(ws, vs, occ,
+                                    t["S"], t["NP"], t["KE"], t["V"], t["MV"])
+    total = occ.sum(axis=(0, 2))
+    assert abs(tr[0, 0] - total.sum()) < 1e-9 * total.sum()
+    if s.spin == 2:
+        assert abs(tr[1, 0] - (total[0] - total[1])) < 1e-9 * total.sum()
+    for q in range(s.spin):
+        if s.spin == 1:
+            r_tr, r_pr = tr[0], pr[0]
+        else:   # the spin-q density from the total and difference traces
+            sg = 1.0 if q == 0 else -1.0
+            r_tr, r_pr = 0.5 * (tr[0] + sg * tr[1]), 0.5 * (pr[0] + sg * pr[1])
+        e_sum = coefs[q] @ r_pr + r_tr[1] + r_tr[2] + r_tr[3]
+        assert abs(e_sum - ee[q]) < 1e-9 * max(1.0, np.abs(ee).max())
+    assert s.rel == 0 or abs(tr[0, 3]) > 0.0
