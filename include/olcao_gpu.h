@@ -200,8 +200,10 @@ olcao_status olcao_scf_iteration(olcao_ctx *ctx, const double *potCoeffs,
  * runs one cell per process through OLCAO.F90:533-770; each cell's result is
  * what olcao_scf_iteration gives for it.  nWorkers host threads (1..16) take
  * the cells heaviest first, each cell on its own streams, so the device runs
- * several cells at once: first every cell's integral build, then the solves
- * (those of the tridiagonal path, a cooperative launch, one at a time).
+ * several cells at once: every cell's integral build, followed by its solve
+ * where that is the one-workgroup LDS solver; the other solves (the
+ * tridiagonal path, a cooperative launch, and rocSOLVER's) run last, one at
+ * a time, with nothing else on the device.
  * potCoeffs[i], numStates[i], evals[i] as olcao_scf_iteration for cell i.
  * cellMs (may be NULL): [nCells] each cell's time from its first to its last
  * event within the batch (its latency while sharing the device).  On failure

@@ -1,6 +1,6 @@
 """Seeded synthetic systems for the integral -> eigenvalue path.
 
-Four of the five workloads, in order:
+The five workloads:
   diamond_c8   diamond-cubic C, 8-atom cell, 1s 2s 2p from 20 even-tempered
                exponents 0.12..1e6, 16 potential Gaussians 0.1..1e5,
                4x4x4 Monkhorst-Pack k-mesh (64 k)                    n = 40
@@ -10,7 +10,9 @@ Four of the five workloads, in order:
                8x8x8 k-mesh (512 k)                                  n = 57
   asio2_1000   1000-atom amorphous SiO2 cube at 2.2 g/cm^3, Si 1s 2s 2p 3s 3p,
                O 2s 2p + 1s core, Gamma only (real)            n = 5665
-(the fourth, 64 uneven cells in one launch, is not generated here)
+  uneven_batch 64 independent cells of 8..216 C/Si/O atoms (diamond or
+               zincblende, 0.1 A jitter, 2.2..3.5 g/cm^3), minimal or
+               extended basis, 1..64 k-points each              n = 40..1296
 
 plus small variants used by the tests.  Exponents are even-tempered,
 alpha_i = amin (amax/amin)^(i/(N-1)), the rule of makePotAlphas

@@ -182,6 +182,13 @@ struct olcao_ctx {
     // trace partial sums, results and the term-matrix pointer list
     DBuf<double> Rho[2], Ysc, dOcc, dTrPart, dTrOut;
     DBuf<const double *> dTrPtr;
+    // early Cholesky of S(k) on a stream of its own, overlapping the
+    // Hamiltonian sweep (tridiagonal solver path, single-cell iterations)
+    hipStream_t cholStream = nullptr;
+    rocblas_handle cholBlas = nullptr;
+    hipEvent_t evSDone = nullptr, evCholDone = nullptr;
+    DBuf<int> dInfoChol;
+    bool cholPending = false;
     DBuf<double> H[2], CV_H[2], CC_H[2];   // per spin
     DBuf<int> dInfo;
     DBuf<unsigned long long> dCounters;
@@ -1231,10 +1238,14 @@ bool trd_fits(int n, int nb, bool cplx, TrdPlan &plan) { return trd_plan(n, nb, 
 // reduction fits (blocked Cholesky, GEMM triangular solves, trd_eig); else,
 // and always when library is set, rocSOLVER (potrf, trsm, heevd).
 void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *A,
-             double *B, double *D, double *E, int *dinfo, std::vector<int> &info, bool library)
+             double *B, double *D, double *E, int *dinfo, std::vector<int> &info, bool library,
+             hipEvent_t cholDone = nullptr, const int *dinfoChol = nullptr)
 {
     const rocblas_stride st = (rocblas_stride)n * n;
     info.assign(nb, 0);
+    TrdPlan plan;
+    if (cholDone && (library || small_hegv_fits(n, cplx) || !trd_fits(n, nb, cplx, plan)))
+        fail(OLCAO_ERR_STATE, "an early Cholesky factor reached a solver path without it");
     if (!library && small_hegv_fits(n, cplx)) {
         // one workgroup per matrix, everything in LDS (ksolve.hip)
         launch_small_hegv(cplx, n, nb, A, B, D, dinfo, s);
@@ -1245,7 +1256,6 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
             if (info[k] < 0) info[k] = 1;   // Jacobi did not converge: reported as zhegv info in 1..n
         return;
     }
-    TrdPlan plan;
     if (!library && trd_fits(n, nb, cplx, plan)) {
         // blocked Cholesky, GEMM triangular solves, cooperative reduction,
         // divide and conquer, blocked back-transforms (all batched)
@@ -1253,8 +1263,14 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
         ew.Dinv.alloc((size_t)n * n * el * nb);
         ew.Y.alloc((size_t)n * n * el * nb);
         ew.Uc.alloc((size_t)n * n * el * nb);
-        chol_blocked(h, s, cplx, n, nb, B, ew.Uc.p, ew.Dinv.p, dinfo);
-        HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        if (cholDone) {
+            // the factor was started early on another stream (early_cholesky)
+            HIPCHK(hipStreamWaitEvent(s, cholDone, 0));
+            HIPCHK(hipMemcpyAsync(info.data(), dinfoChol, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        } else {
+            chol_blocked(h, s, cplx, n, nb, B, ew.Uc.p, ew.Dinv.p, dinfo);
+            HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        }
         HIPCHK(hipStreamSynchronize(s));
         for (int k = 0; k < nb; ++k)
             if (info[k] != 0) {
@@ -1346,10 +1362,11 @@ void solve_device(olcao_ctx &c, int q, int numStates, const double *Hsrc, const 
     c.E.alloc((size_t)n * nk);
     c.dInfo.alloc(nk);
     HIPCHK(hipMemcpyAsync(Aq.p, Hsrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
+    const bool pre = c.cholPending && Ssrc == c.S.p;
+    if (!pre) HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
     std::vector<int> info;
     gen_eig(c.eig, c.blas, s, !c.gamma, n, nk, Aq.p, c.Bw.p, c.D.p, c.E.p, c.dInfo.p, info,
-            c.eigLibrary);
+            c.eigLibrary, pre ? c.evCholDone : nullptr, pre ? c.dInfoChol.p : nullptr);
     std::vector<double> w((size_t)n * nk);
     HIPCHK(hipMemcpyAsync(w.data(), c.D.p, w.size() * 8, hipMemcpyDeviceToHost, s));
     if (evecs) {
@@ -1412,24 +1429,25 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
         c->eigLibrary = opt && opt->eigensolver == OLCAO_EIG_LIBRARY;
         if (c->precision != OLCAO_PREC_FP64 && c->precision != OLCAO_PREC_MIXED)
             fail(OLCAO_ERR_INPUT, "precision must be OLCAO_PREC_FP64 or OLCAO_PREC_MIXED");
-        // Non-blocking streams: a synchronous copy on the null stream
-        // elsewhere in the process (libraries issue them) neither waits for
-        // nor stalls this context's work.  The runtime spreads a process's
-        // streams over a few hardware queues in creation order, so the main
-        // stream is created after a rotating number of side streams: the
-        // main streams of successive contexts (where their solves run) land
-        // on different hardware queues and can run at the same time.
+        // The runtime spreads a process's streams over a few hardware
+        // queues in creation order, so the main stream is created after a
+        // rotating number of side streams: the main streams of successive
+        // contexts land on different hardware queues and their work can run
+        // at the same time (with four streams per context they all shared
+        // one queue, and a batch of cells ran no faster than one by one).
         const int rot = g_streamRotation.fetch_add(1) % (olcao_ctx::kSide + 1);
-        for (int q = 0; q < rot; ++q) HIPCHK(hipStreamCreateWithFlags(&c->side[q], hipStreamNonBlocking));
-        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        for (int q = rot; q < olcao_ctx::kSide; ++q)
-            HIPCHK(hipStreamCreateWithFlags(&c->side[q], hipStreamNonBlocking));
+        for (int q = 0; q < rot; ++q) HIPCHK(hipStreamCreate(&c->side[q]));
+        HIPCHK(hipStreamCreate(&c->stream));
+        for (int q = rot; q < olcao_ctx::kSide; ++q) HIPCHK(hipStreamCreate(&c->side[q]));
         for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
         HIPCHK(hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming));
         for (auto &e : c->evJoin) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         if (rocblas_create_handle(&c->blas) != rocblas_status_success)
             fail(OLCAO_ERR_HIP, "rocblas_create_handle failed");
         rocblas_set_stream(c->blas, c->stream);
+        // no atomics in the library's GEMMs and in rocSOLVER's BLAS calls:
+        // the same inputs give the same bits whatever else shares the device
+        rocblas_set_atomics_mode(c->blas, rocblas_atomics_not_allowed);
         setup_system(*c, *sys);
         build_lists(*c);
         upload(*c);
@@ -1450,7 +1468,12 @@ void olcao_ctx_destroy(olcao_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto q : c->side)
         if (q) (void)hipStreamSynchronize(q);
+    if (c->cholStream) (void)hipStreamSynchronize(c->cholStream);
     if (c->blas) rocblas_destroy_handle(c->blas);
+    if (c->cholBlas) rocblas_destroy_handle(c->cholBlas);
+    if (c->evSDone) (void)hipEventDestroy(c->evSDone);
+    if (c->evCholDone) (void)hipEventDestroy(c->evCholDone);
+    if (c->cholStream) (void)hipStreamDestroy(c->cholStream);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->evFork) (void)hipEventDestroy(c->evFork);
@@ -1559,6 +1582,7 @@ static void solve_spins(olcao_ctx *c, int numStates, double *evals, double *evec
     for (int q = 0; q < c->spin; ++q)
         solve_device(*c, q, numStates, c->H[q].p, c->S.p, evals ? evals + q * ev : nullptr,
                      evecs ? evecs + q * vec : nullptr);
+    c->cholPending = false;
 }
 
 olcao_status olcao_secular_eqn_pscf(olcao_ctx *c, int numStates, double *evals, double *evecs)
@@ -1694,10 +1718,46 @@ olcao_status olcao_vale_charge(olcao_ctx *c, int numStates, const double *occupa
     });
 }
 
+// Where the solve takes the tridiagonal path, S(k)'s blocked Cholesky
+// factor (and its inverted diagonal blocks) only needs the overlap sweep: it
+// starts on a stream of its own once S(k) is complete and runs beside the
+// Hamiltonian sweep; the solve waits for it (gen_eig) and every spin reuses
+// it.
+static void early_cholesky(olcao_ctx &c)
+{
+    c.cholPending = false;
+    TrdPlan plan;
+    const bool cplx = !c.gamma;
+    const int n = c.valeDim, nb = c.nk, el = cplx ? 2 : 1;
+    if (c.eigLibrary || small_hegv_fits(n, cplx) || !trd_fits(n, nb, cplx, plan)) return;
+    if (!c.cholStream) {
+        HIPCHK(hipStreamCreate(&c.cholStream));
+        if (rocblas_create_handle(&c.cholBlas) != rocblas_status_success)
+            fail(OLCAO_ERR_HIP, "rocblas_create_handle failed");
+        rocblas_set_stream(c.cholBlas, c.cholStream);
+        rocblas_set_atomics_mode(c.cholBlas, rocblas_atomics_not_allowed);
+        HIPCHK(hipEventCreateWithFlags(&c.evSDone, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c.evCholDone, hipEventDisableTiming));
+    }
+    EigWork &ew = c.eig;
+    ew.Dinv.alloc((size_t)n * n * el * nb);
+    ew.Y.alloc((size_t)n * n * el * nb);
+    ew.Uc.alloc((size_t)n * n * el * nb);
+    c.Bw.alloc(c.matDoubles());
+    c.dInfoChol.alloc(nb);
+    HIPCHK(hipEventRecord(c.evSDone, c.stream));
+    HIPCHK(hipStreamWaitEvent(c.cholStream, c.evSDone, 0));
+    HIPCHK(hipMemcpyAsync(c.Bw.p, c.S.p, c.matDoubles() * 8, hipMemcpyDeviceToDevice, c.cholStream));
+    chol_blocked(c.cholBlas, c.cholStream, cplx, n, nb, c.Bw.p, ew.Uc.p, ew.Dinv.p, c.dInfoChol.p);
+    HIPCHK(hipEventRecord(c.evCholDone, c.cholStream));
+    c.cholPending = true;
+}
+
 // First half of a Mode-B iteration: potential coefficients in, overlap
 // sweep, fused Hamiltonian sweep per spin, contraction, Bloch sums and
-// orthogonalisation, all queued on the context's streams.
-static void iteration_build(olcao_ctx &c, const double *coef)
+// orthogonalisation, all queued on the context's streams.  early: start
+// S(k)'s Cholesky factor beside the Hamiltonian sweep (early_cholesky).
+static void iteration_build(olcao_ctx &c, const double *coef, bool early)
 {
     ensure_mats(c);
     hipStream_t s = c.stream;
@@ -1705,6 +1765,7 @@ static void iteration_build(olcao_ctx &c, const double *coef)
     HIPCHK(hipEventRecord(c.ev[0], s));
     overlap_sweep(c);
     HIPCHK(hipEventRecord(c.ev[1], s));
+    if (early) early_cholesky(c);
     // events: 0 start, 1 overlap done; per spin q: 2+3q before the
     // three-centre sweep, 3+3q after it, 4+3q after contraction, Bloch
     // sum and orthogonalisation; 8 after the solve
@@ -1755,19 +1816,17 @@ olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates
 {
     if (!c || !coef) return OLCAO_ERR_INPUT;
     return guard(c, [&] {
-        iteration_build(*c, coef);
+        iteration_build(*c, coef, true);
         iteration_solve(*c, numStates, evals);
     });
 }
 
-// Whether the context's solve takes the tridiagonal path, whose reduction
-// kernel is a cooperative launch that needs all of its workgroups resident
-// at once (ktrd.hip).
-static bool cooperative_solve(const olcao_ctx &c)
+// Whether the context's solve is the one-workgroup LDS solver (one kernel
+// of this library per solve), the only solve the batched call runs
+// alongside other cells' work.
+static bool concurrent_solve(const olcao_ctx &c)
 {
-    TrdPlan plan;
-    return !c.eigLibrary && !small_hegv_fits(c.valeDim, !c.gamma) &&
-           trd_fits(c.valeDim, c.nk, !c.gamma, plan);
+    return !c.eigLibrary && small_hegv_fits(c.valeDim, !c.gamma);
 }
 
 olcao_status olcao_scf_iteration_batch(int nCells, olcao_ctx *const *ctxs,
@@ -1812,22 +1871,29 @@ olcao_status olcao_scf_iteration_batch(int nCells, olcao_ctx *const *ctxs,
         work();
         for (auto &th : pool) th.join();
     };
-    // 1. every cell's integral build, concurrently
-    run(order, nWorkers, [&](int i) { iteration_build(*ctxs[i], potCoeffs[i]); });
-    // 2. the solves that are ordinary launches, concurrently; then the
-    // cooperative ones one at a time with nothing else on the device
-    std::vector<int> plain, coop;
-    for (int i : order) (cooperative_solve(*ctxs[i]) ? coop : plain).push_back(i);
-    if (!failed.load())
-        run(plain, nWorkers, [&](int i) { iteration_solve(*ctxs[i], numStates[i], evals[i]); });
-    // every other cell's work has ended (each plain solve waits for its own
-    // stream); the cooperative cells' builds finish before the first of them
-    for (int i : coop)
-        if (st[i] == OLCAO_OK &&
-            (st[i] = guard(ctxs[i], [&] { HIPCHK(hipStreamSynchronize(ctxs[i]->stream)); })) != OLCAO_OK)
-            failed.store(true);
-    if (!failed.load())
-        run(coop, 1, [&](int i) { iteration_solve(*ctxs[i], numStates[i], evals[i]); });
+    // 1. every cell's integral build, concurrently, each followed by its
+    // solve when that is the LDS solver
+    std::vector<char> alone(nCells);
+    std::vector<int> serial;
+    for (int i : order)
+        if ((alone[i] = !concurrent_solve(*ctxs[i]))) serial.push_back(i);
+    run(order, nWorkers, [&](int i) {
+        iteration_build(*ctxs[i], potCoeffs[i], false);
+        if (!alone[i]) iteration_solve(*ctxs[i], numStates[i], evals[i]);
+    });
+    // 2. the other solves one at a time with nothing else on the device.
+    // The tridiagonal path's reduction is a cooperative launch that needs
+    // all of its workgroups resident.  rocSOLVER's solves gave wrong
+    // eigenvalues for some cells (and once an S(k) it took for indefinite)
+    // when they overlapped other cells' kernels from other host threads,
+    // while the same cells solved alone were right every time and no
+    // library buffer was written outside its bounds (guard pads around
+    // every allocation stayed intact); the cause was not isolated.
+    if (!failed.load() && !serial.empty()) {
+        const olcao_status sync = guard(nullptr, [&] { HIPCHK(hipDeviceSynchronize()); });
+        if (sync != OLCAO_OK) return sync;
+        run(serial, 1, [&](int i) { iteration_solve(*ctxs[i], numStates[i], evals[i]); });
+    }
     for (int t = 0; t < nCells; ++t) {
         const int i = order[t];
         if (st[i] != OLCAO_OK) {
@@ -1869,6 +1935,7 @@ static olcao_status solve_host(int n, int nb, double *A, double *B, double *w, b
         rocblas_handle h;
         rocblas_create_handle(&h);
         rocblas_set_stream(h, s);
+        rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed);
         const size_t per = (size_t)n * n * (cplx ? 2 : 1);
         DBuf<double> dA, dB, dD, dE;
         DBuf<int> dI;

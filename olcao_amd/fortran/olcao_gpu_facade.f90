@@ -105,6 +105,11 @@ module O_GPUFacade
          type(c_ptr), value :: c, coef, ev, evec
          integer(c_int), value :: ns
       end function
+      integer(c_int) function c_vale_charge(c, ns, occ, ee, tr, prho) bind(C, name="olcao_vale_charge")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: c, occ, ee, tr, prho
+         integer(c_int), value :: ns
+      end function
       integer(c_int) function c_zhegv(n, nb, A, B, w) bind(C, name="olcao_solve_zhegv")
          import :: c_int, c_ptr
          integer(c_int), value :: n, nb
@@ -305,6 +310,23 @@ contains
             & c_loc(energyEigenValues), c_null_ptr)
       if (st /= 0) call gpuStop(st, 'secularEqnSCF')
    end subroutine secularEqnSCF
+
+   ! valeCharge.F90:34 (makeValenceRho, inSCF = 1) after setupSCF and a
+   ! solve: electronPopulation (numStates*spin*nk, its order), then
+   ! electronEnergy(spin), the traces(4, spin) = charge, nuclear, kinetic,
+   ! mass velocity, and potRho(potDim, spin); spin index 1 = total,
+   ! 2 = up - down.
+   subroutine makeValenceRho(numStates, electronPopulation, electronEnergy, traces, potRho)
+      integer, intent(in) :: numStates
+      real(double), intent(in), target :: electronPopulation(:)
+      real(double), intent(out), target :: electronEnergy(:)
+      real(double), intent(out), target :: traces(:,:)
+      real(double), intent(out), target :: potRho(:,:)
+      integer(c_int) :: st
+      st = c_vale_charge(ctx, int(numStates, c_int), c_loc(electronPopulation), &
+            & c_loc(electronEnergy), c_loc(traces), c_loc(potRho))
+      if (st /= 0) call gpuStop(st, 'makeValenceRho')
+   end subroutine makeValenceRho
 
    ! interfaces.F90:73-120, same dummy arguments: A (upper) in, eigenvectors
    ! out; B (upper) in; the first resultDim eigenvalues returned.

@@ -50,5 +50,5 @@ def test_fortran_facade_source_names_reference_procedures():
     src = open(os.path.join(ROOT, "olcao_amd", "fortran", "olcao_gpu_facade.f90")).read()
     for proc in ("gaussOverlapOL", "gaussOverlapKE", "gaussOverlapNP",
                  "elecPotGaussOverlap", "gaussOverlapHamPSCF", "secularEqnPSCF",
-                 "secularEqnSCF", "solveZHEGV", "solveDSYGV"):
+                 "secularEqnSCF", "solveZHEGV", "solveDSYGV", "makeValenceRho"):
         assert re.search(r"subroutine\s+" + proc + r"\b", src, re.I), proc

@@ -286,8 +286,10 @@ def test_uneven_batch_cells_match_oracle(gpu):
 def test_uneven_batch_equals_single_cells(gpu):
     """All 64 cells of the uneven batch (8..216 atoms, n = 40..1296, 1..64
     k-points; every eigensolver path) in one batched call, twice in a row:
-    each cell's eigenvalues equal its single-context Mode-B iteration's, and
-    they are finite and ascending."""
+    each cell's eigenvalues equal its single-context Mode-B iteration's to
+    1e-12 of the cell's spectrum scale and 1e-6 eV (the library's BLAS
+    calls use no atomics, so they are normally bit-identical), and they are
+    finite and ascending."""
     cells = configs.uneven_batch()
     _log("uneven batch: creating 64 contexts")
     ctxs = [gpu.OlcaoGPU(s) for s in cells]
@@ -298,15 +300,23 @@ def test_uneven_batch_equals_single_cells(gpu):
         _log(f"  cell {i} n={n} nk={g.nk}: {g.timings()['total_ms']:.2f} ms")
     for rep in range(2):
         wb, ms = gpu.scf_iteration_batch(ctxs, ns, 8)
-        worst = 0.0
-        for w1, w2 in zip(single, wb):
+        worst, worst_rel, exact = 0.0, 0.0, 0
+        for i, (w1, w2) in enumerate(zip(single, wb)):
             w2 = np.asarray(w2)
             assert w2.shape == w1.shape
             assert np.all(np.isfinite(w2)) and np.all(np.diff(w2, axis=-1) >= 0.0)
-            worst = max(worst, float(np.abs(w2 - w1).max()))
-        _log(f"  batch pass {rep}: max |batch - single| {worst * HARTREE_EV:.2e} eV, "
-             f"latency p50 {np.percentile(ms, 50):.1f} ms p99 {np.percentile(ms, 99):.1f} ms")
-        assert worst * HARTREE_EV < 1e-9
+            d = float(np.abs(w2 - w1).max())
+            if d != 0.0:
+                _log(f"    cell {i} (n={ns[i]}, nk={ctxs[i].nk}, gamma={ctxs[i].gamma}) "
+                     f"differs by {d * HARTREE_EV:.2e} eV")
+            worst = max(worst, d)
+            worst_rel = max(worst_rel, d / np.abs(w1).max())
+            exact += int(d == 0.0)
+        _log(f"  batch pass {rep}: max |batch - single| {worst * HARTREE_EV:.2e} eV "
+             f"({worst_rel:.1e} of the spectrum scale), {exact} of {len(single)} cells "
+             f"bit-identical, latency p50 {np.percentile(ms, 50):.1f} ms "
+             f"p99 {np.percentile(ms, 99):.1f} ms")
+        assert worst_rel < 1e-12 and worst * HARTREE_EV < 1e-6
     for g in ctxs:
         g.close()
 
