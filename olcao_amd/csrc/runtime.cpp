@@ -182,13 +182,6 @@ struct olcao_ctx {
     // trace partial sums, results and the term-matrix pointer list
     DBuf<double> Rho[2], Ysc, dOcc, dTrPart, dTrOut;
     DBuf<const double *> dTrPtr;
-    // early Cholesky of S(k) on a stream of its own, overlapping the
-    // Hamiltonian sweep (tridiagonal solver path, single-cell iterations)
-    hipStream_t cholStream = nullptr;
-    rocblas_handle cholBlas = nullptr;
-    hipEvent_t evSDone = nullptr, evCholDone = nullptr;
-    DBuf<int> dInfoChol;
-    bool cholPending = false;
     DBuf<double> H[2], CV_H[2], CC_H[2];   // per spin
     DBuf<int> dInfo;
     DBuf<unsigned long long> dCounters;
@@ -1238,14 +1231,10 @@ bool trd_fits(int n, int nb, bool cplx, TrdPlan &plan) { return trd_plan(n, nb, 
 // reduction fits (blocked Cholesky, GEMM triangular solves, trd_eig); else,
 // and always when library is set, rocSOLVER (potrf, trsm, heevd).
 void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *A,
-             double *B, double *D, double *E, int *dinfo, std::vector<int> &info, bool library,
-             hipEvent_t cholDone = nullptr, const int *dinfoChol = nullptr)
+             double *B, double *D, double *E, int *dinfo, std::vector<int> &info, bool library)
 {
     const rocblas_stride st = (rocblas_stride)n * n;
     info.assign(nb, 0);
-    TrdPlan plan;
-    if (cholDone && (library || small_hegv_fits(n, cplx) || !trd_fits(n, nb, cplx, plan)))
-        fail(OLCAO_ERR_STATE, "an early Cholesky factor reached a solver path without it");
     if (!library && small_hegv_fits(n, cplx)) {
         // one workgroup per matrix, everything in LDS (ksolve.hip)
         launch_small_hegv(cplx, n, nb, A, B, D, dinfo, s);
@@ -1256,6 +1245,7 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
             if (info[k] < 0) info[k] = 1;   // Jacobi did not converge: reported as zhegv info in 1..n
         return;
     }
+    TrdPlan plan;
     if (!library && trd_fits(n, nb, cplx, plan)) {
         // blocked Cholesky, GEMM triangular solves, cooperative reduction,
         // divide and conquer, blocked back-transforms (all batched)
@@ -1263,14 +1253,8 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
         ew.Dinv.alloc((size_t)n * n * el * nb);
         ew.Y.alloc((size_t)n * n * el * nb);
         ew.Uc.alloc((size_t)n * n * el * nb);
-        if (cholDone) {
-            // the factor was started early on another stream (early_cholesky)
-            HIPCHK(hipStreamWaitEvent(s, cholDone, 0));
-            HIPCHK(hipMemcpyAsync(info.data(), dinfoChol, nb * sizeof(int), hipMemcpyDeviceToHost, s));
-        } else {
-            chol_blocked(h, s, cplx, n, nb, B, ew.Uc.p, ew.Dinv.p, dinfo);
-            HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
-        }
+        chol_blocked(h, s, cplx, n, nb, B, ew.Uc.p, ew.Dinv.p, dinfo);
+        HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         for (int k = 0; k < nb; ++k)
             if (info[k] != 0) {
@@ -1362,11 +1346,10 @@ void solve_device(olcao_ctx &c, int q, int numStates, const double *Hsrc, const 
     c.E.alloc((size_t)n * nk);
     c.dInfo.alloc(nk);
     HIPCHK(hipMemcpyAsync(Aq.p, Hsrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
-    const bool pre = c.cholPending && Ssrc == c.S.p;
-    if (!pre) HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
     std::vector<int> info;
     gen_eig(c.eig, c.blas, s, !c.gamma, n, nk, Aq.p, c.Bw.p, c.D.p, c.E.p, c.dInfo.p, info,
-            c.eigLibrary, pre ? c.evCholDone : nullptr, pre ? c.dInfoChol.p : nullptr);
+            c.eigLibrary);
     std::vector<double> w((size_t)n * nk);
     HIPCHK(hipMemcpyAsync(w.data(), c.D.p, w.size() * 8, hipMemcpyDeviceToHost, s));
     if (evecs) {
@@ -1468,12 +1451,7 @@ void olcao_ctx_destroy(olcao_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto q : c->side)
         if (q) (void)hipStreamSynchronize(q);
-    if (c->cholStream) (void)hipStreamSynchronize(c->cholStream);
     if (c->blas) rocblas_destroy_handle(c->blas);
-    if (c->cholBlas) rocblas_destroy_handle(c->cholBlas);
-    if (c->evSDone) (void)hipEventDestroy(c->evSDone);
-    if (c->evCholDone) (void)hipEventDestroy(c->evCholDone);
-    if (c->cholStream) (void)hipStreamDestroy(c->cholStream);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->evFork) (void)hipEventDestroy(c->evFork);
@@ -1582,7 +1560,6 @@ static void solve_spins(olcao_ctx *c, int numStates, double *evals, double *evec
     for (int q = 0; q < c->spin; ++q)
         solve_device(*c, q, numStates, c->H[q].p, c->S.p, evals ? evals + q * ev : nullptr,
                      evecs ? evecs + q * vec : nullptr);
-    c->cholPending = false;
 }
 
 olcao_status olcao_secular_eqn_pscf(olcao_ctx *c, int numStates, double *evals, double *evecs)
@@ -1718,46 +1695,12 @@ olcao_status olcao_vale_charge(olcao_ctx *c, int numStates, const double *occupa
     });
 }
 
-// Where the solve takes the tridiagonal path, S(k)'s blocked Cholesky
-// factor (and its inverted diagonal blocks) only needs the overlap sweep: it
-// starts on a stream of its own once S(k) is complete and runs beside the
-// Hamiltonian sweep; the solve waits for it (gen_eig) and every spin reuses
-// it.
-static void early_cholesky(olcao_ctx &c)
-{
-    c.cholPending = false;
-    TrdPlan plan;
-    const bool cplx = !c.gamma;
-    const int n = c.valeDim, nb = c.nk, el = cplx ? 2 : 1;
-    if (c.eigLibrary || small_hegv_fits(n, cplx) || !trd_fits(n, nb, cplx, plan)) return;
-    if (!c.cholStream) {
-        HIPCHK(hipStreamCreate(&c.cholStream));
-        if (rocblas_create_handle(&c.cholBlas) != rocblas_status_success)
-            fail(OLCAO_ERR_HIP, "rocblas_create_handle failed");
-        rocblas_set_stream(c.cholBlas, c.cholStream);
-        rocblas_set_atomics_mode(c.cholBlas, rocblas_atomics_not_allowed);
-        HIPCHK(hipEventCreateWithFlags(&c.evSDone, hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&c.evCholDone, hipEventDisableTiming));
-    }
-    EigWork &ew = c.eig;
-    ew.Dinv.alloc((size_t)n * n * el * nb);
-    ew.Y.alloc((size_t)n * n * el * nb);
-    ew.Uc.alloc((size_t)n * n * el * nb);
-    c.Bw.alloc(c.matDoubles());
-    c.dInfoChol.alloc(nb);
-    HIPCHK(hipEventRecord(c.evSDone, c.stream));
-    HIPCHK(hipStreamWaitEvent(c.cholStream, c.evSDone, 0));
-    HIPCHK(hipMemcpyAsync(c.Bw.p, c.S.p, c.matDoubles() * 8, hipMemcpyDeviceToDevice, c.cholStream));
-    chol_blocked(c.cholBlas, c.cholStream, cplx, n, nb, c.Bw.p, ew.Uc.p, ew.Dinv.p, c.dInfoChol.p);
-    HIPCHK(hipEventRecord(c.evCholDone, c.cholStream));
-    c.cholPending = true;
-}
-
 // First half of a Mode-B iteration: potential coefficients in, overlap
 // sweep, fused Hamiltonian sweep per spin, contraction, Bloch sums and
-// orthogonalisation, all queued on the context's streams.  early: start
-// S(k)'s Cholesky factor beside the Hamiltonian sweep (early_cholesky).
-static void iteration_build(olcao_ctx &c, const double *coef, bool early)
+// orthogonalisation, all queued on the context's streams.  (Starting S(k)'s
+// Cholesky factor on a stream of its own beside the Hamiltonian sweep was
+// measured: the sweep slowed by as much as the solve gained.)
+static void iteration_build(olcao_ctx &c, const double *coef)
 {
     ensure_mats(c);
     hipStream_t s = c.stream;
@@ -1765,7 +1708,6 @@ static void iteration_build(olcao_ctx &c, const double *coef, bool early)
     HIPCHK(hipEventRecord(c.ev[0], s));
     overlap_sweep(c);
     HIPCHK(hipEventRecord(c.ev[1], s));
-    if (early) early_cholesky(c);
     // events: 0 start, 1 overlap done; per spin q: 2+3q before the
     // three-centre sweep, 3+3q after it, 4+3q after contraction, Bloch
     // sum and orthogonalisation; 8 after the solve
@@ -1816,7 +1758,7 @@ olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates
 {
     if (!c || !coef) return OLCAO_ERR_INPUT;
     return guard(c, [&] {
-        iteration_build(*c, coef, true);
+        iteration_build(*c, coef);
         iteration_solve(*c, numStates, evals);
     });
 }
@@ -1878,7 +1820,7 @@ olcao_status olcao_scf_iteration_batch(int nCells, olcao_ctx *const *ctxs,
     for (int i : order)
         if ((alone[i] = !concurrent_solve(*ctxs[i]))) serial.push_back(i);
     run(order, nWorkers, [&](int i) {
-        iteration_build(*ctxs[i], potCoeffs[i], false);
+        iteration_build(*ctxs[i], potCoeffs[i]);
         if (!alone[i]) iteration_solve(*ctxs[i], numStates[i], evals[i]);
     });
     // 2. the other solves one at a time with nothing else on the device.
