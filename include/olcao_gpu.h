@@ -202,8 +202,9 @@ olcao_status olcao_scf_iteration(olcao_ctx *ctx, const double *potCoeffs,
  * the cells heaviest first, each cell on its own streams, so the device runs
  * several cells at once: every cell's integral build, followed by its solve
  * where that is the one-workgroup LDS solver; the other solves (the
- * tridiagonal path, a cooperative launch, and rocSOLVER's) run last, one at
- * a time, with nothing else on the device.
+ * tridiagonal path, a cooperative launch, and rocSOLVER's) run last with
+ * nothing else on the device, the k-points of all cells with matrices of
+ * the same size and kind in one batched solve.
  * potCoeffs[i], numStates[i], evals[i] as olcao_scf_iteration for cell i.
  * cellMs (may be NULL): [nCells] each cell's time from its first to its last
  * event within the batch (its latency while sharing the device).  On failure

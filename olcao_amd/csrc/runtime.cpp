@@ -24,7 +24,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <thread>
 #include <vector>
 
@@ -182,6 +184,9 @@ struct olcao_ctx {
     // trace partial sums, results and the term-matrix pointer list
     DBuf<double> Rho[2], Ysc, dOcc, dTrPart, dTrOut;
     DBuf<const double *> dTrPtr;
+    // the batched call's grouped solves: several cells' matrices in one batch
+    DBuf<double> gA, gB, gD, gE;
+    DBuf<int> gInfo;
     DBuf<double> H[2], CV_H[2], CC_H[2];   // per spin
     DBuf<int> dInfo;
     DBuf<unsigned long long> dCounters;
@@ -1729,12 +1734,10 @@ static void iteration_build(olcao_ctx &c, const double *coef)
 }
 
 // Second half: the solve for all k and spins, then the stage times.
-static void iteration_solve(olcao_ctx &c, int numStates, double *evals)
+// Stage times from the iteration's events; solveMs < 0: the solve ran on
+// this context's stream and ev[8] marks its end.
+static void record_timings(olcao_ctx &c, double solveMs)
 {
-    hipStream_t s = c.stream;
-    solve_spins(&c, numStates, evals, nullptr);
-    HIPCHK(hipEventRecord(c.ev[8], s));
-    HIPCHK(hipEventSynchronize(c.ev[8]));
     auto el = [&](int a, int b) {
         float v = 0.f;
         HIPCHK(hipEventElapsedTime(&v, c.ev[a], c.ev[b]));
@@ -1749,9 +1752,103 @@ static void iteration_solve(olcao_ctx &c, int numStates, double *evals)
     c.timings[0] = el(0, 1);
     c.timings[1] = ham;
     c.timings[2] = cb;
-    c.timings[3] = el(1 + 3 * c.spin, 8);
-    c.timings[4] = el(0, 8);
+    c.timings[3] = solveMs < 0 ? el(1 + 3 * c.spin, 8) : solveMs;
+    c.timings[4] = solveMs < 0 ? el(0, 8) : el(0, 1 + 3 * c.spin) + solveMs;
     c.timings[5] = t3c;
+}
+
+static void iteration_solve(olcao_ctx &c, int numStates, double *evals)
+{
+    hipStream_t s = c.stream;
+    solve_spins(&c, numStates, evals, nullptr);
+    HIPCHK(hipEventRecord(c.ev[8], s));
+    HIPCHK(hipEventSynchronize(c.ev[8]));
+    record_timings(c, -1.0);
+}
+
+// One generalised solve over several cells' matrices of the same size and
+// kind (the batched call): every member's H(k) and S(k) are copied into one
+// strided batch on the first member's stream, solved in one gen_eig call
+// (the batched device path or rocSOLVER's strided-batched one), and the
+// eigenvalues, eigenvectors (kept for the valence density) and stage times
+// go back to each member.  A member is a (cell, spin).
+struct SolveMember {
+    int cell, spin;
+};
+
+static void grouped_solve(const std::vector<SolveMember> &mem, olcao_ctx *const *ctxs,
+                          const int *numStates, double *const *evals, std::vector<olcao_status> &st)
+{
+    olcao_ctx &h = *ctxs[mem[0].cell];
+    const int n = h.valeDim;
+    const bool cplx = !h.gamma;
+    const size_t per = (size_t)n * n * (cplx ? 2 : 1);
+    int nbTot = 0;
+    for (const auto &m : mem) nbTot += ctxs[m.cell]->nk;
+    hipStream_t s = h.stream;
+    h.gA.alloc(per * nbTot);
+    h.gB.alloc(per * nbTot);
+    h.gD.alloc((size_t)n * nbTot);
+    h.gE.alloc((size_t)n * nbTot);
+    h.gInfo.alloc(nbTot);
+    size_t off = 0;
+    for (const auto &m : mem) {
+        const olcao_ctx &c = *ctxs[m.cell];
+        HIPCHK(hipMemcpyAsync(h.gA.p + off * per, c.H[m.spin].p, per * c.nk * 8, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.gB.p + off * per, c.S.p, per * c.nk * 8, hipMemcpyDeviceToDevice, s));
+        off += c.nk;
+    }
+    HIPCHK(hipEventRecord(h.ev[9], s));
+    std::vector<int> info;
+    gen_eig(h.eig, h.blas, s, cplx, n, nbTot, h.gA.p, h.gB.p, h.gD.p, h.gE.p, h.gInfo.p, info,
+            h.eigLibrary);
+    HIPCHK(hipEventRecord(h.ev[10], s));
+    std::vector<double> w((size_t)n * nbTot);
+    HIPCHK(hipMemcpyAsync(w.data(), h.gD.p, w.size() * 8, hipMemcpyDeviceToHost, s));
+    off = 0;
+    for (const auto &m : mem) {
+        olcao_ctx &c = *ctxs[m.cell];
+        DBuf<double> &X = m.spin ? c.Aw2 : c.Aw;
+        X.alloc(c.matDoubles());
+        HIPCHK(hipMemcpyAsync(X.p, h.gA.p + off * per, per * c.nk * 8, hipMemcpyDeviceToDevice, s));
+        off += c.nk;
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h.ev[9], h.ev[10]));
+    off = 0;
+    for (const auto &m : mem) {
+        olcao_ctx &c = *ctxs[m.cell];
+        const int ns = numStates[m.cell];
+        olcao_status bad = OLCAO_OK;
+        for (int k = 0; k < c.nk && bad == OLCAO_OK; ++k) {
+            const int v = info[off + k];
+            if (v > n) {
+                c.err = "S(k) not positive definite at k=" + std::to_string(k) + " (info=" + std::to_string(v) + ")";
+                bad = OLCAO_ERR_NOT_SPD;
+            } else if (v != 0) {
+                c.err = "eigensolver did not converge at k=" + std::to_string(k) + " (info=" + std::to_string(v) + ")";
+                bad = OLCAO_ERR_SOLVER;
+            }
+        }
+        if (bad == OLCAO_OK && (ns <= 0 || ns > n)) {
+            c.err = "numStates must lie in 1..";
+            c.err += std::to_string(n);
+            bad = OLCAO_ERR_INPUT;
+        }
+        if (bad != OLCAO_OK) {
+            st[m.cell] = bad;
+            off += c.nk;
+            continue;
+        }
+        double *ev = evals[m.cell] + (size_t)m.spin * c.nk * ns;
+        for (int k = 0; k < c.nk; ++k)
+            std::memcpy(ev + (size_t)k * ns, w.data() + (off + k) * n, ns * sizeof(double));
+        c.hEv[m.spin].assign(w.begin() + off * n, w.begin() + (off + c.nk) * n);
+        c.vecsValid[m.spin] = true;
+        record_timings(c, ms);
+        off += c.nk;
+    }
 }
 
 olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates, double *evals)
@@ -1831,10 +1928,26 @@ olcao_status olcao_scf_iteration_batch(int nCells, olcao_ctx *const *ctxs,
     // while the same cells solved alone were right every time and no
     // library buffer was written outside its bounds (guard pads around
     // every allocation stayed intact); the cause was not isolated.
+    // Cells whose matrices have the same size and kind are solved together
+    // in one batched call (grouped_solve).
     if (!failed.load() && !serial.empty()) {
         const olcao_status sync = guard(nullptr, [&] { HIPCHK(hipDeviceSynchronize()); });
         if (sync != OLCAO_OK) return sync;
-        run(serial, 1, [&](int i) { iteration_solve(*ctxs[i], numStates[i], evals[i]); });
+        std::map<std::tuple<int, bool, bool>, std::vector<SolveMember>> groups;
+        for (int i : serial)
+            for (int q = 0; q < ctxs[i]->spin; ++q)
+                groups[{ctxs[i]->valeDim, ctxs[i]->gamma, ctxs[i]->eigLibrary}].push_back({i, q});
+        for (auto &g : groups) {
+            const int first = g.second[0].cell;
+            const olcao_status r = guard(ctxs[first], [&] {
+                grouped_solve(g.second, ctxs, numStates, evals, st);
+            });
+            if (r != OLCAO_OK)
+                for (const auto &m : g.second) {
+                    st[m.cell] = r;
+                    if (m.cell != first) ctxs[m.cell]->err = ctxs[first]->err;
+                }
+        }
     }
     for (int t = 0; t < nCells; ++t) {
         const int i = order[t];
