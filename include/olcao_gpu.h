@@ -113,6 +113,9 @@ const char *olcao_last_error(const olcao_ctx *ctx);
  * nPrimitivePairs, nAtomPairs. */
 olcao_status olcao_get_dims(const olcao_ctx *ctx, long long dims[8]);
 
+/* flags[0..3] = gamma, rel, spin, precision of the context. */
+olcao_status olcao_get_flags(const olcao_ctx *ctx, int flags[4]);
+
 /* Derived setup for inspection: the sorted cell list (cellDimsReal,
  * cellSizesReal, lattice.f90:571-695) and the Cartesian k-points
  * (kpoints.f90:424-454).  Any pointer may be NULL. */
@@ -164,6 +167,14 @@ olcao_status olcao_secular_eqn_pscf(olcao_ctx *ctx, int numStates, double *evals
  * the device, each orthogonalised when coreDim > 0 (setupSCF,
  * OLCAO.F90:248-279). */
 olcao_status olcao_setup_scf(olcao_ctx *ctx);
+
+/* The device-resident matrices olcao_setup_scf built, copied to host, in the
+ * layout of olcao_gauss_overlap's vv (cv for which = 5): which 0 overlap
+ * (atomOverlap), 1 kinetic (atomKEOverlap), 2 mass velocity (atomMVOverlap,
+ * rel = 1), 3 nuclear (atomNPOverlap), 4 potential term `term`
+ * (atomPotOverlap), 5 raw coreVale overlap (atomOverlapCV) -- the
+ * matrices setupSCF leaves in its integral file (hdf5SCFIntg.F90:188-460). */
+olcao_status olcao_get_scf_term(const olcao_ctx *ctx, int which, int term, double *out);
 
 /* Mode A iteration: H = NP + KE (+ MV) + sum_j c_j V_j per spin (MV and a
  * coefficient of exactly 0 add their term unscaled, matrixSubs.F90:369-373),

@@ -1,0 +1,43 @@
+/* olcao_h5.h -- the setup's integral file in the reference's HDF5 layout.
+ *
+ * libolcao_h5.so (host code over the HDF5 C library) writes the matrices
+ * that olcao_setup_scf keeps on the device into the groups and datasets that
+ * setupSCF writes (src/uolcao/hdf5SCFIntg.F90:105-460, written by
+ * integrals.F90's gaussOverlap* / elecPotGaussOverlap and ortho*), so a
+ * reference reader (secularEqnSCF's readPackedMatrix, valeCharge) finds them
+ * where it looks:
+ *
+ *   /atomIntgGroup/atomOverlap/<k>          packed S(k)      (1 or 2, n(n+1)/2)
+ *   /atomIntgGroup/atomKEOverlap/<k>        packed KE(k)
+ *   /atomIntgGroup/atomMVOverlap/<k>        packed MV(k)     (rel = 1)
+ *   /atomIntgGroup/atomNPOverlap/<k>        packed NP(k)
+ *   /atomIntgGroup/atomPotOverlap/<j>/<k>   packed V_j(k)
+ *   /atomIntgGroup/atomOverlapCV/real<k>, imag<k>   coreVale (coreDim, valeDim)
+ *
+ * <k> and <j> are 1-based, written "%07d".  Dataset dimensions are given in
+ * the reference's Fortran order; components are (re, im), one at Gamma.
+ * Packed index i(i-1)/2 + j for column i, row j <= i (matrixSubs.F90:506).
+ * Datasets are chunked (at most 250M elements per chunk, hdf5SCFIntg.F90:
+ * 144-175) and deflated at level 1.  Every group that holds a term gets the
+ * integer attribute "status" = 1 once written, as the reference records a
+ * finished calculation (integrals.F90:3728-3731).  Groups of operators
+ * outside this path (atomDMOverlap, atomMMOverlap, atomKOverlap...) are not
+ * written. */
+#ifndef OLCAO_H5_H
+#define OLCAO_H5_H
+
+#include "olcao_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Write the file at `path` (truncating it) from a context after
+ * olcao_setup_scf.  Returns OLCAO_ERR_STATE before the setup and
+ * OLCAO_ERR_INPUT when the file cannot be written. */
+olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char *path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

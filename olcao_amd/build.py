@@ -68,6 +68,7 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
     if verbose:
         print(f"built {LIB} in {time.time() - t0:.1f}s", flush=True)
     build_fortran(verbose)
+    build_h5(verbose)
     return LIB
 
 
@@ -85,6 +86,34 @@ def build_fortran(verbose: bool = True) -> str:
     if verbose:
         print(f"built {FLIB}", flush=True)
     return FLIB
+
+
+H5LIB = os.path.join(HERE, "libolcao_h5.so")
+H5_PREFIX = "/opt/conda"
+
+
+def build_h5(verbose: bool = True):
+    """The setup's integral file in the reference's HDF5 layout
+    (csrc/h5scf.cpp), host code over the HDF5 C library, linked to the GPU
+    library next to it.  Skipped where no HDF5 C library is installed."""
+    inc = os.path.join(H5_PREFIX, "include", "hdf5.h")
+    if not os.path.exists(inc):
+        if verbose:
+            print(f"skipped {H5LIB}: no HDF5 C library under {H5_PREFIX}", flush=True)
+        return None
+    tmp = H5LIB + ".tmp"
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", f"-I{H5_PREFIX}/include",
+           os.path.join(SRC, "h5scf.cpp"), "-o", tmp, f"-L{HERE}", "-lolcao_gpu",
+           f"-L{H5_PREFIX}/lib", "-lhdf5", "-Wl,--enable-new-dtags", "-Wl,-rpath,$ORIGIN",
+           # the system C++ runtime before the HDF5 prefix's own, older one
+           "-Wl,-rpath,/usr/lib/x86_64-linux-gnu", f"-Wl,-rpath,{H5_PREFIX}/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for h5scf.cpp:\n{r.stderr[-4000:]}")
+    os.replace(tmp, H5LIB)
+    if verbose:
+        print(f"built {H5LIB}", flush=True)
+    return H5LIB
 
 
 if __name__ == "__main__":

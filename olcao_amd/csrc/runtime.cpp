@@ -1609,6 +1609,42 @@ olcao_status olcao_setup_scf(olcao_ctx *c)
     });
 }
 
+olcao_status olcao_get_flags(const olcao_ctx *c, int flags[4])
+{
+    if (!c || !flags) return OLCAO_ERR_INPUT;
+    flags[0] = c->gamma ? 1 : 0;
+    flags[1] = c->rel;
+    flags[2] = c->spin;
+    flags[3] = c->precision;
+    return OLCAO_OK;
+}
+
+olcao_status olcao_get_scf_term(const olcao_ctx *c, int which, int term, double *out)
+{
+    if (!c || !out) return OLCAO_ERR_INPUT;
+    olcao_ctx *m = const_cast<olcao_ctx *>(c);
+    return guard(m, [&] {
+        if (!c->haveScf) fail(OLCAO_ERR_STATE, "call olcao_setup_scf first");
+        const double *src = nullptr;
+        size_t count = c->matDoubles();
+        switch (which) {
+        case 0: src = c->S.p; break;
+        case 1: src = c->KEm.p; break;
+        case 2: src = c->rel ? c->MVm.p : nullptr; break;
+        case 3: src = c->NPm.p; break;
+        case 4:
+            if (term < 0 || term >= c->potDim) fail(OLCAO_ERR_INPUT, "term must lie in 0..%d", c->potDim - 1);
+            src = c->Vm[term].p;
+            break;
+        case 5: src = c->CV_S.p; count = c->cvDoubles(); break;
+        default: fail(OLCAO_ERR_INPUT, "which must lie in 0..5");
+        }
+        if (!src || count == 0) fail(OLCAO_ERR_STATE, "that matrix is not part of this system");
+        HIPCHK(hipMemcpyAsync(out, src, count * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
 olcao_status olcao_secular_eqn_scf(olcao_ctx *c, const double *coef, int numStates, double *evals, double *evecs)
 {
     if (!c || !coef) return OLCAO_ERR_INPUT;

@@ -69,6 +69,7 @@ FUNCTIONS = {
     "olcao_ctx_destroy": [C.c_void_p],
     "olcao_last_error": [C.c_void_p],
     "olcao_get_dims": [C.c_void_p, C.c_void_p],
+    "olcao_get_flags": [C.c_void_p, C.c_void_p],
     "olcao_get_setup": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_gauss_overlap": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_gauss_overlap_np": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
@@ -76,6 +77,7 @@ FUNCTIONS = {
     "olcao_gauss_overlap_ham_pscf": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_secular_eqn_pscf": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
     "olcao_setup_scf": [C.c_void_p],
+    "olcao_get_scf_term": [C.c_void_p, C.c_int, C.c_int, C.c_void_p],
     "olcao_secular_eqn_scf": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
     "olcao_scf_iteration": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p],
     "olcao_vale_charge": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],

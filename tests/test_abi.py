@@ -52,3 +52,20 @@ def test_fortran_facade_source_names_reference_procedures():
                  "elecPotGaussOverlap", "gaussOverlapHamPSCF", "secularEqnPSCF",
                  "secularEqnSCF", "solveZHEGV", "solveDSYGV", "makeValenceRho"):
         assert re.search(r"subroutine\s+" + proc + r"\b", src, re.I), proc
+
+
+def test_hdf5_writer_library_exports_its_symbol():
+    """libolcao_h5.so (built where the HDF5 C library is installed) loads and
+    exports what include/olcao_h5.h declares."""
+    path = os.path.join(ROOT, "olcao_amd", "libolcao_h5.so")
+    if not os.path.exists("/opt/conda/include/hdf5.h"):
+        pytest.skip("no HDF5 C library on this host")
+    lib = C.CDLL(path)
+    src = open(os.path.join(ROOT, "include", "olcao_h5.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = set(re.findall(r"\b(olcao_[a-z_0-9]+)\s*\(", src))
+    assert names == {"olcao_write_scf_integrals_h5"}
+    for name in names:
+        assert hasattr(lib, name), name
+    lib.olcao_write_scf_integrals_h5.argtypes = [C.c_void_p, C.c_char_p]
+    assert lib.olcao_write_scf_integrals_h5(None, b"/nonexistent") == 3

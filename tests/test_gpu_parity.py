@@ -381,6 +381,55 @@ def test_vale_charge_si64_invariants(gpu):
     g.close()
 
 
+@pytest.mark.parametrize("case", [dict(core=True, kmesh=(2, 1, 1)),
+                                  dict(core=True, gamma=True, rel=1, two_types=True)])
+def test_scf_integral_file(gpu, tmp_path, case):
+    """The setup's integral file in the reference's HDF5 layout: every term
+    group and "%07d" dataset, packed (components, n(n+1)/2) datasets equal to
+    the oracle's packed terms to 1e-10, coreVale real/imag (coreDim,
+    valeDim) datasets, chunked and deflated, status = 1."""
+    import ctypes as C
+    import os
+
+    from tests import h5read
+    from tests.conftest import ROOT, mode_a_terms
+    if not h5read.available():
+        pytest.skip("no HDF5 C library on this host")
+    _log(f"integral file {case}")
+    s = configs.small_test(**case)
+    g, o = _pair(gpu, s)
+    t = mode_a_terms(o, s)
+    cv_raw = o.integral(oracle.OL)[1]
+    g.setupSCF()
+    path = str(tmp_path / "scf-integrals.hdf5")
+    lib = C.CDLL(os.path.join(ROOT, "olcao_amd", "libolcao_h5.so"))
+    lib.olcao_write_scf_integrals_h5.argtypes = [C.c_void_p, C.c_char_p]
+    assert lib.olcao_write_scf_integrals_h5(g.ctx, path.encode()) == 0
+    f = h5read.File(path)
+    n, nk, comps = g.vale_dim, g.nk, 1 if s.gamma else 2
+    groups = [("atomOverlap", t["S"]), ("atomKEOverlap", t["KE"]), ("atomNPOverlap", t["NP"])]
+    groups += [("atomMVOverlap", t["MV"])] if s.rel else []
+    groups += [(f"atomPotOverlap/{j + 1:07d}", v) for j, v in enumerate(t["V"])]
+    for group, ref in groups:
+        assert f.status("/atomIntgGroup/" + group) == 1, group
+        for k in range(nk):
+            data, chunked, nfilt = f.read(f"/atomIntgGroup/{group}/{k + 1:07d}")
+            assert data.shape == (n * (n + 1) // 2, comps) and chunked and nfilt >= 1
+            p = oracle.pack_upper(ref if s.gamma else ref[k])
+            exp = p[:, None] if s.gamma else np.stack([p.real, p.imag], axis=1)
+            assert _ortho_close(data, exp, 1e-10), (group, k)
+    if not s.rel:
+        assert not f.exists("/atomIntgGroup/atomMVOverlap/0000001")
+    for k in range(nk):
+        cvk = cv_raw if s.gamma else cv_raw[k]
+        for part, val in (("real", np.real(cvk)), ("imag", np.imag(cvk)))[:comps]:
+            data, chunked, nfilt = f.read(f"/atomIntgGroup/atomOverlapCV/{part}{k + 1:07d}")
+            assert data.shape == (n, s.core_dim) and chunked and nfilt >= 1
+            assert block_rel_err(data, val.T) < 1e-10
+    f.close()
+    g.close()
+
+
 def test_diamond_c8_full_parity(gpu):
     """First workload at full size: H(k), S(k) and all eigenvalues."""
     _log("diamond_c8 full")
