@@ -113,6 +113,13 @@ const char *olcao_last_error(const olcao_ctx *ctx);
  * nPrimitivePairs, nAtomPairs. */
 olcao_status olcao_get_dims(const olcao_ctx *ctx, long long dims[8]);
 
+/* Eigenpairs of the last solve for one spin (0-based), as the eigenvalue
+ * and eigenvector datasets of secularEqnSCF hold them (secularEqn.F90:
+ * 259-313): evals [nk][numStates] ascending, evecs [nk][numStates][valeDim]
+ * complex (real at Gamma).  Either pointer may be NULL. */
+olcao_status olcao_get_eigen(const olcao_ctx *ctx, int spin, int numStates, double *evals,
+                             double *evecs);
+
 /* flags[0..3] = gamma, rel, spin, precision of the context. */
 olcao_status olcao_get_flags(const olcao_ctx *ctx, int flags[4]);
 

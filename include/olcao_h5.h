@@ -37,6 +37,21 @@ extern "C" {
  * OLCAO_ERR_INPUT when the file cannot be written. */
 olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char *path);
 
+/* Add the eigenpairs of the last solve to the file at `path` (created when
+ * absent; groups of an earlier call are replaced) as secularEqnSCF writes
+ * them (hdf5SCFEigVal.f90:60-95, hdf5SCFEigVec.F90:70-160,
+ * secularEqn.F90:259-313):
+ *
+ *   /eigenValues/<k><s>                      (numStates), ascending, Hartree
+ *   /eigenVectors/real<k><s>, imag<k><s>     (valeDim, numStates)
+ *   /eigenVectors/<k><s>                     the same at Gamma (real)
+ *
+ * <k><s> = k-point and spin, each "%07d" and 1-based.  Chunked (at most
+ * 250M elements per chunk), deflate level 1; each first eigenvector dataset
+ * carries the integer attribute "status", 0 as the reference leaves it once
+ * all k-points of a spin are done (secularEqn.F90:307-313). */
+olcao_status olcao_write_scf_eigen_h5(olcao_ctx *ctx, int numStates, const char *path);
+
 #ifdef __cplusplus
 }
 #endif

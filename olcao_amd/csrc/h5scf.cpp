@@ -48,30 +48,34 @@ std::string name7(int i)
     return b;
 }
 
-void set_status(hid_t group)
+void set_status(hid_t group, int value = 1)
 {
     const hsize_t one = 1;
     Handle space{check(H5Screate_simple(1, &one, nullptr), "attribute space"), H5Sclose};
     Handle attr{check(H5Acreate2(group, "status", H5T_NATIVE_INT, space.id, H5P_DEFAULT, H5P_DEFAULT),
                       "status attribute"),
                 H5Aclose};
-    const int done = 1;
-    check(H5Awrite(attr.id, H5T_NATIVE_INT, &done), "status write", 0);
+    check(H5Awrite(attr.id, H5T_NATIVE_INT, &value), "status write", 0);
 }
 
-// A chunked, deflated 2-D dataset of doubles; dims in C order.
-void write_dataset(hid_t group, const std::string &name, const hsize_t dims[2],
-                   const hsize_t chunk[2], const double *data)
+// A chunked, deflated dataset of doubles (rank 1 or 2); dims in C order.
+// Returns the open dataset when keep is set (for an attribute), else -1.
+hid_t write_dataset(hid_t group, const std::string &name, const hsize_t dims[2],
+                    const hsize_t chunk[2], const double *data, int rank = 2, bool keep = false)
 {
-    Handle space{check(H5Screate_simple(2, dims, nullptr), "dataspace"), H5Sclose};
+    Handle space{check(H5Screate_simple(rank, dims, nullptr), "dataspace"), H5Sclose};
     Handle dcpl{check(H5Pcreate(H5P_DATASET_CREATE), "dataset properties"), H5Pclose};
-    check(H5Pset_chunk(dcpl.id, 2, chunk), "chunk", 0);
+    check(H5Pset_chunk(dcpl.id, rank, chunk), "chunk", 0);
     check(H5Pset_deflate(dcpl.id, 1), "deflate", 0);
     Handle ds{check(H5Dcreate2(group, name.c_str(), H5T_NATIVE_DOUBLE, space.id, H5P_DEFAULT, dcpl.id,
                                H5P_DEFAULT),
                     "dataset"),
               H5Dclose};
     check(H5Dwrite(ds.id, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, data), "dataset write", 0);
+    if (!keep) return -1;
+    const hid_t id = ds.id;
+    ds.id = -1;
+    return id;
 }
 
 }  // namespace
@@ -166,6 +170,68 @@ extern "C" olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char 
         }
     } catch (const H5Error &e) {
         std::fprintf(stderr, "olcao_write_scf_integrals_h5: %s failed for %s\n", e.what.c_str(), path);
+        return OLCAO_ERR_INPUT;
+    }
+    return OLCAO_OK;
+}
+
+extern "C" olcao_status olcao_write_scf_eigen_h5(olcao_ctx *ctx, int numStates, const char *path)
+{
+    if (!ctx || !path) return OLCAO_ERR_INPUT;
+    long long dims[8];
+    int flags[4];
+    olcao_status st = olcao_get_dims(ctx, dims);
+    if (st == OLCAO_OK) st = olcao_get_flags(ctx, flags);
+    if (st != OLCAO_OK) return st;
+    const int n = (int)dims[0], nk = (int)dims[2];
+    const bool gamma = flags[0] != 0;
+    const int spin = flags[2], comps = gamma ? 1 : 2;
+    std::vector<std::vector<double>> w(spin), v(spin);
+    for (int q = 0; q < spin; ++q) {
+        w[q].resize((size_t)nk * numStates);
+        v[q].resize((size_t)nk * numStates * n * comps);
+        if ((st = olcao_get_eigen(ctx, q, numStates, w[q].data(), v[q].data())) != OLCAO_OK) return st;
+    }
+    try {
+        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+        hid_t fid = H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT);
+        if (fid < 0) fid = check(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), "file create");
+        Handle file{fid, H5Fclose};
+        for (const char *g : {"eigenValues", "eigenVectors"})
+            if (H5Lexists(file.id, g, H5P_DEFAULT) > 0) check(H5Ldelete(file.id, g, H5P_DEFAULT), "old group", 0);
+        Handle gv{check(H5Gcreate2(file.id, "eigenValues", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                        "eigenValues"),
+                  H5Gclose};
+        Handle ge{check(H5Gcreate2(file.id, "eigenVectors", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                        "eigenVectors"),
+                  H5Gclose};
+        // (valeDim, numStates) in Fortran order; chunks as
+        // hdf5SCFEigVec.F90:82-89
+        const hsize_t vDims[2] = {(hsize_t)numStates, (hsize_t)n};
+        hsize_t vChunk[2] = {(hsize_t)numStates, (hsize_t)n};
+        if ((double)n * numStates > 250000000.0) {
+            vChunk[0] = std::max<hsize_t>(1, (hsize_t)(std::sqrt(250000000.0 / ((double)n * numStates)) * numStates));
+            vChunk[1] = std::max<hsize_t>(1, std::min<hsize_t>(n, 250000000 / vChunk[0]));
+        }
+        const hsize_t sDims[2] = {(hsize_t)numStates, 1}, sChunk[2] = {(hsize_t)numStates, 1};
+        std::vector<double> part((size_t)numStates * n);
+        for (int q = 0; q < spin; ++q)
+            for (int k = 0; k < nk; ++k) {
+                const std::string kq = name7(k + 1) + name7(q + 1);
+                write_dataset(gv.id, kq, sDims, sChunk, w[q].data() + (size_t)k * numStates, 1);
+                const double *src = v[q].data() + (size_t)k * numStates * n * comps;
+                for (int c = 0; c < comps; ++c) {
+                    for (size_t e = 0; e < part.size(); ++e) part[e] = src[e * comps + c];
+                    const std::string name = gamma ? kq : (c == 0 ? "real" : "imag") + kq;
+                    const hid_t ds = write_dataset(ge.id, name, vDims, vChunk, part.data(), 2, c == 0);
+                    if (c == 0) {
+                        Handle keep{ds, H5Dclose};
+                        set_status(keep.id, 0);
+                    }
+                }
+            }
+    } catch (const H5Error &e) {
+        std::fprintf(stderr, "olcao_write_scf_eigen_h5: %s failed for %s\n", e.what.c_str(), path);
         return OLCAO_ERR_INPUT;
     }
     return OLCAO_OK;

@@ -1609,6 +1609,31 @@ olcao_status olcao_setup_scf(olcao_ctx *c)
     });
 }
 
+olcao_status olcao_get_eigen(const olcao_ctx *c, int spin, int numStates, double *evals,
+                             double *evecs)
+{
+    if (!c) return OLCAO_ERR_INPUT;
+    olcao_ctx *m = const_cast<olcao_ctx *>(c);
+    return guard(m, [&] {
+        const int n = c->valeDim, nk = c->nk, cp = c->gamma ? 1 : 2;
+        if (spin < 0 || spin >= c->spin) fail(OLCAO_ERR_INPUT, "spin must lie in 0..%d", c->spin - 1);
+        if (numStates <= 0 || numStates > n) fail(OLCAO_ERR_INPUT, "numStates must lie in 1..%d", n);
+        if (!c->vecsValid[spin]) fail(OLCAO_ERR_STATE, "solve first");
+        if (evals)
+            for (int k = 0; k < nk; ++k)
+                std::memcpy(evals + (size_t)k * numStates, c->hEv[spin].data() + (size_t)k * n,
+                            numStates * sizeof(double));
+        if (evecs) {
+            DBuf<double> tmp;
+            tmp.alloc((size_t)nk * numStates * n * cp);
+            launch_extract_evecs((spin ? c->Aw2 : c->Aw).p, n, nk, numStates, cp, tmp.p, c->stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(evecs, tmp.p, tmp.n * 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+    });
+}
+
 olcao_status olcao_get_flags(const olcao_ctx *c, int flags[4])
 {
     if (!c || !flags) return OLCAO_ERR_INPUT;

@@ -70,6 +70,7 @@ FUNCTIONS = {
     "olcao_last_error": [C.c_void_p],
     "olcao_get_dims": [C.c_void_p, C.c_void_p],
     "olcao_get_flags": [C.c_void_p, C.c_void_p],
+    "olcao_get_eigen": [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p],
     "olcao_get_setup": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_gauss_overlap": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_gauss_overlap_np": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],

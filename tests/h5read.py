@@ -85,3 +85,16 @@ class File:
         h.H5Aclose(a)
         h.H5Gclose(g)
         return v.value
+
+    def dataset_status(self, name):
+        h = self.h
+        d = h.H5Dopen2(self.fid, name.encode(), 0)
+        assert d >= 0, name
+        a = h.H5Aopen(d, b"status", 0)
+        assert a >= 0, name
+        v = C.c_int(-1)
+        it = C.c_int64.in_dll(h, "H5T_NATIVE_INT_g").value
+        assert h.H5Aread(a, it, C.byref(v)) >= 0
+        h.H5Aclose(a)
+        h.H5Dclose(d)
+        return v.value

@@ -381,13 +381,16 @@ def test_vale_charge_si64_invariants(gpu):
     g.close()
 
 
-@pytest.mark.parametrize("case", [dict(core=True, kmesh=(2, 1, 1)),
+@pytest.mark.parametrize("case", [dict(core=True, kmesh=(2, 1, 1), spin=2),
                                   dict(core=True, gamma=True, rel=1, two_types=True)])
 def test_scf_integral_file(gpu, tmp_path, case):
     """The setup's integral file in the reference's HDF5 layout: every term
     group and "%07d" dataset, packed (components, n(n+1)/2) datasets equal to
     the oracle's packed terms to 1e-10, coreVale real/imag (coreDim,
-    valeDim) datasets, chunked and deflated, status = 1."""
+    valeDim) datasets, chunked and deflated, status = 1; then the eigenpairs
+    of a Mode A solve added to the same file (eigenValues/<k><s>,
+    eigenVectors/{real,imag}<k><s>, status 0) equal to what the solve
+    returned."""
     import ctypes as C
     import os
 
@@ -426,6 +429,28 @@ def test_scf_integral_file(gpu, tmp_path, case):
             data, chunked, nfilt = f.read(f"/atomIntgGroup/atomOverlapCV/{part}{k + 1:07d}")
             assert data.shape == (n, s.core_dim) and chunked and nfilt >= 1
             assert block_rel_err(data, val.T) < 1e-10
+    f.close()
+    ns = n - 1
+    w, v = g.secularEqnSCF(s.pot_coeffs, ns, vectors=True)
+    w = np.asarray(w).reshape(s.spin, nk, ns)
+    v = np.asarray(v).reshape(s.spin, nk, n, ns)
+    lib.olcao_write_scf_eigen_h5.argtypes = [C.c_void_p, C.c_int, C.c_char_p]
+    assert lib.olcao_write_scf_eigen_h5(g.ctx, ns, path.encode()) == 0
+    f = h5read.File(path)
+    assert f.exists("/atomIntgGroup/atomOverlap/0000001")   # the integral groups stay
+    for q in range(s.spin):
+        for k in range(nk):
+            kq = f"{k + 1:07d}{q + 1:07d}"
+            ev, chunked, nfilt = f.read(f"/eigenValues/{kq}")
+            assert ev.shape == (ns,) and chunked and nfilt >= 1
+            assert np.array_equal(ev, w[q, k])
+            parts = [("", np.real(v[q, k]))] if s.gamma else \
+                [("real", np.real(v[q, k])), ("imag", np.imag(v[q, k]))]
+            for part, val in parts:
+                data, chunked, nfilt = f.read(f"/eigenVectors/{part}{kq}")
+                assert data.shape == (ns, n) and chunked and nfilt >= 1
+                assert np.array_equal(data, val.T)
+            assert f.dataset_status(f"/eigenVectors/{parts[0][0]}{kq}") == 0
     f.close()
     g.close()
 
