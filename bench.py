@@ -57,9 +57,10 @@ FP64_PEAK_TFLOPS = 78.6
 SWEEP_HBM_BYTES = {"si64": 6.5101e9}
 
 
-def cpu_baseline(sysin, total_triples, stride):
-    """Oracle on one core over every stride-th atom pair, scaled by the
-    three-centre evaluation ratio; LAPACK solve for all k on one thread."""
+def cpu_baseline(sysin, total_triples, stride, threads=1):
+    """Oracle on `threads` host cores (OpenMP over atom pairs) over every
+    stride-th atom pair, scaled by the three-centre evaluation ratio;
+    LAPACK solve for all k on as many threads."""
     import numpy as np
     from threadpoolctl import threadpool_limits
 
@@ -68,9 +69,9 @@ def cpu_baseline(sysin, total_triples, stride):
     oracle.Oracle.set_pair_sample(stride, 0)
     try:
         t0 = time.perf_counter()
-        S, _, _ = o.integral(oracle.OL, nthreads=1)
+        S, _, _ = o.integral(oracle.OL, nthreads=threads)
         t1 = time.perf_counter()
-        H, _, _ = o.integral(oracle.HAM, coef=sysin.pot_coeffs, nthreads=1)
+        H, _, _ = o.integral(oracle.HAM, coef=sysin.pot_coeffs, nthreads=threads)
         t2 = time.perf_counter()
         _, _, sample_tr = o.last_counts()
     finally:
@@ -81,7 +82,7 @@ def cpu_baseline(sysin, total_triples, stride):
     Z = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
     Sm = Z @ Z.conj().T / n + np.eye(n)
     Hm = (Z + Z.conj().T) / 2
-    with threadpool_limits(limits=1):
+    with threadpool_limits(limits=threads):
         t3 = time.perf_counter()
         for _ in range(o.nk):
             oracle.solve(Hm, Sm)
@@ -324,6 +325,11 @@ def main():
         est, sample = cpu_baseline(s, tri, args.cpu_stride)
         out["cpu_baseline"] = {"value": est, "unit": "s", "cores": 1, "kind": "port",
                                "sample": sample}
+        # the same port on all host cores this process may use (at most 16)
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        est, sample = cpu_baseline(s, tri, args.cpu_stride, threads=cores)
+        out["cpu_baseline_all_cores"] = {"value": est, "unit": "s", "cores": cores,
+                                         "kind": "port", "sample": sample}
     print(json.dumps(out), flush=True)
 
 
