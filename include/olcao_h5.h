@@ -37,8 +37,9 @@ extern "C" {
  * OLCAO_ERR_INPUT when the file cannot be written. */
 olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char *path);
 
-/* Add the eigenpairs of the last solve to the file at `path` (created when
- * absent; groups of an earlier call are replaced) as secularEqnSCF writes
+/* Add the eigenpairs of the last solve to the HDF5 file at `path` (created
+ * when absent, OLCAO_ERR_INPUT when the path holds another kind of file;
+ * groups of an earlier call are replaced) as secularEqnSCF writes
  * them (hdf5SCFEigVal.f90:60-95, hdf5SCFEigVec.F90:70-160,
  * secularEqn.F90:259-313):
  *

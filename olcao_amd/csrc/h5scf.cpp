@@ -194,8 +194,17 @@ extern "C" olcao_status olcao_write_scf_eigen_h5(olcao_ctx *ctx, int numStates, 
     }
     try {
         H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
-        hid_t fid = H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT);
-        if (fid < 0) fid = check(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), "file create");
+        // an existing HDF5 file gets the groups added; a path that exists
+        // but is not one is left alone
+        std::FILE *probe = std::fopen(path, "rb");
+        hid_t fid;
+        if (probe) {
+            std::fclose(probe);
+            if (H5Fis_hdf5(path) <= 0) throw H5Error{"opening a file that is not HDF5"};
+            fid = check(H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT), "file open");
+        } else {
+            fid = check(H5Fcreate(path, H5F_ACC_EXCL, H5P_DEFAULT, H5P_DEFAULT), "file create");
+        }
         Handle file{fid, H5Fclose};
         for (const char *g : {"eigenValues", "eigenVectors"})
             if (H5Lexists(file.id, g, H5P_DEFAULT) > 0) check(H5Ldelete(file.id, g, H5P_DEFAULT), "old group", 0);
