@@ -59,7 +59,10 @@ void set_status(hid_t group, int value = 1)
 }
 
 // A chunked, deflated dataset of doubles (rank 1 or 2); dims in C order.
-// Returns the open dataset when keep is set (for an attribute), else -1.
+// data == nullptr creates it without writing (the reference creates every
+// dataset of the file at set-up, also those of terms it does not compute,
+// which then read as the fill value 0).  Returns the open dataset when keep
+// is set (for an attribute), else -1.
 hid_t write_dataset(hid_t group, const std::string &name, const hsize_t dims[2],
                     const hsize_t chunk[2], const double *data, int rank = 2, bool keep = false)
 {
@@ -71,7 +74,9 @@ hid_t write_dataset(hid_t group, const std::string &name, const hsize_t dims[2],
                                H5P_DEFAULT),
                     "dataset"),
               H5Dclose};
-    check(H5Dwrite(ds.id, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, data), "dataset write", 0);
+    if (data)
+        check(H5Dwrite(ds.id, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, data), "dataset write",
+              0);
     if (!keep) return -1;
     const hid_t id = ds.id;
     ds.id = -1;
@@ -122,6 +127,12 @@ extern "C" olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char 
             }
             set_status(group);
         };
+        // the packed datasets of a term this path does not compute, and its
+        // status 0 (the state initSCFIntegralHDF5 leaves them in,
+        // hdf5SCFIntg.F90:380-530)
+        auto empty_term = [&](hid_t group) {
+            for (int k = 0; k < nk; ++k) write_dataset(group, name7(k + 1), vvDims, vvChunk, nullptr);
+        };
         const struct {
             const char *name;
             int which;
@@ -129,7 +140,42 @@ extern "C" olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char 
         for (const auto &t : terms) {
             Handle g{check(H5Gcreate2(top.id, t.name, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), t.name),
                      H5Gclose};
-            if (t.which != 2 || rel) write_term(g.id, t.which, 0);
+            if (t.which != 2 || rel) {
+                write_term(g.id, t.which, 0);
+            } else {   // no mass-velocity term without rel
+                empty_term(g.id);
+                set_status(g.id, 0);
+            }
+        }
+        // Operators outside this path, created empty with status 0 so that
+        // the reference's reader (accessSCFIntegralHDF5, :563-760) finds
+        // every group, dataset and attribute it opens: dipole and momentum
+        // matrices (packed, one subgroup per Cartesian direction) and the
+        // k-overlaps (full valeDim x valeDim, "real"/"imag" per k).
+        hsize_t fullChunk[2] = {(hsize_t)n, (hsize_t)n};
+        if ((double)n * n > 250000000.0) {
+            const double f = std::sqrt((double)n * n / 250000000.0);
+            fullChunk[0] = fullChunk[1] = std::max<hsize_t>(1, (hsize_t)(n / f));
+        }
+        const hsize_t fullDims[2] = {(hsize_t)n, (hsize_t)n};
+        for (const char *name : {"atomDMOverlap", "atomMMOverlap", "atomKOverlap", "atomKOverlapPlusG"}) {
+            Handle g{check(H5Gcreate2(top.id, name, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), name),
+                     H5Gclose};
+            const bool kov = name[4] == 'K';
+            for (int x = 1; x <= 3; ++x) {
+                Handle sub{check(H5Gcreate2(g.id, name7(x).c_str(), H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                                 "xyz group"),
+                           H5Gclose};
+                for (int k = 0; k < nk; ++k) {
+                    if (kov) {
+                        write_dataset(sub.id, "real" + name7(k + 1), fullDims, fullChunk, nullptr);
+                        write_dataset(sub.id, "imag" + name7(k + 1), fullDims, fullChunk, nullptr);
+                    } else {
+                        write_dataset(sub.id, name7(k + 1), vvDims, vvChunk, nullptr);
+                    }
+                }
+            }
+            set_status(g.id, 0);
         }
         {
             Handle pot{check(H5Gcreate2(top.id, "atomPotOverlap", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),

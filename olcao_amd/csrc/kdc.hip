@@ -138,6 +138,11 @@ struct MergeLds {
     int K, Nd, Nr, close, unsorted;
     double rho;
 };
+// The merge kernels hold MergeLds plus up to three double and two int
+// arrays of kDcMax entries in LDS; all of it must fit one workgroup's
+// 160 KiB, for whatever kDcMaxN is set to.
+constexpr size_t kDcMergeLdsBytes = sizeof(MergeLds) + (size_t)kDcMax * (3 * 8 + 2 * 4);
+static_assert(kDcMergeLdsBytes <= 160 * 1024, "divide-and-conquer merge state exceeds LDS");
 
 // Sort the children's poles, form z, deflate (dlaed2's rules: small rho z,
 // then Givens rotations of poles closer than the tolerance, in one ascending
@@ -783,11 +788,13 @@ __global__ void k_dc_copy(const int4 *tiles, const int4 *nodes, int n, const dou
 
 }  // namespace
 
-void dc_plan(int n, DcPlan &p)
+bool dc_plan(int n, DcPlan &p)
 {
     p = DcPlan{};
+    // every merge keeps its poles, z and rotations in LDS arrays of kDcMax
+    if (n < 1 || n > kDcMaxN) return false;
     p.n = n;
-    if (n < 2) return;
+    if (n < 2) return true;
     // merges by height above the 1 x 1 leaves
     std::vector<std::vector<int4>> byH;
     struct Rec {
@@ -823,15 +830,17 @@ void dc_plan(int n, DcPlan &p)
         L.nCol = (int)p.colTiles.size() - L.colBase;
         p.levels.push_back(L);
     }
+    return true;
 }
 
 size_t dc_state_doubles(int n, int nb) { return (size_t)7 * n * nb; }
 size_t dc_state_ints(int n, int nb) { return (size_t)7 * n * nb; }
 
-void launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double *E, double *Q,
+bool launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double *E, double *Q,
                double *Qt, double *W, int *ord, hipStream_t s)
 {
     const int n = p.n;
+    if (n < 1 || n > kDcMaxN) return false;
     const size_t nn = (size_t)n * nb;
     DcState st;
     st.dk = dv.stD;
@@ -868,6 +877,7 @@ void launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double
         hipLaunchKernelGGL(k_dc_copy, dim3(L.nTiles, nb), dim3(256), 0, s, dv.tiles + L.tileBase,
                            dv.nodes, n, Qt, Q);
     }
+    return true;
 }
 
 }  // namespace olcao

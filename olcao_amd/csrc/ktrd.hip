@@ -40,6 +40,9 @@ namespace {
 constexpr int kTrdThreads = 256;
 constexpr int kTrdWaves = kTrdThreads / 64;
 constexpr int kTrdK = 10;        // 64-column chunks of a row: n <= 640
+// the divide and conquer that follows the reduction keeps per-merge arrays
+// of kDcMaxN entries in LDS: the reduction may not take larger matrices
+static_assert(64 * kTrdK <= kDcMaxN, "tridiagonal path larger than the divide-and-conquer LDS");
 constexpr int kTrdRpw = 5;       // owned rows per wave: ceil(n / G) <= 20
 constexpr size_t kTrdLdsMax = 160 * 1024 - 256;
 constexpr long long kSpinLimit = 1ll << 22;

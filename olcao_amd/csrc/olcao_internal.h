@@ -284,10 +284,11 @@ struct DcDev {
     int4 *cnt;        // nodes x nb
     double *rho;      // nodes x nb
 };
-void dc_plan(int n, DcPlan &p);
+// false (nothing planned or launched) when n is outside 1..kDcMaxN
+bool dc_plan(int n, DcPlan &p);
 size_t dc_state_doubles(int n, int nb);
 size_t dc_state_ints(int n, int nb);
-void launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double *E, double *Q,
+bool launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double *E, double *Q,
                double *Qt, double *W, int *ord, hipStream_t s);
 
 }  // namespace olcao

@@ -480,9 +480,13 @@ OHD void acc_e3c_herm(const PairConst &q, double a3, const double *C, double w,
 // As acc_e3c_herm with the term's pair constants precomputed:
 // mu = zeta a3 / (zeta + a3), pref0 = w (pi/(zeta + a3))^(3/2) (the same
 // operations in the same order, so the sums are identical).  MIX evaluates
-// exp(-mu X^2) from the FP64 argument with a base-2 range reduction in FP64
-// and the hardware FP32 exponential of the fractional part (relative error
-// ~1e-7), the polynomial part and the accumulation staying in FP64.
+// exp(-mu X^2) of the distant images, where it is below e^-kMixedExpArg =
+// 1e-4, from the FP64 argument with a base-2 range reduction in FP64 and the
+// hardware FP32 exponential of the fractional part (relative error ~1e-7,
+// so at most ~1e-11 of the item's prefactor); nearer images, which carry
+// the largest terms, keep the FP64 exponential.  The polynomial part and
+// the accumulation stay in FP64.
+constexpr double kMixedExpArg = 9.210340371976184;   // -ln(1e-4)
 OHD double exp_neg_mixed(double arg)   // exp(-arg), arg >= 0
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -529,7 +533,9 @@ OHD void acc_e3c_herm_tab(const PairConst &q, double mu, double pref0, const dou
         X[x] = q.P[x] - C[x];
         X2 += X[x] * X[x];
     }
-    const double pref = pref0 * (MIX ? exp_neg_mixed(mu * X2) : exp_neg_tab(mu * X2, expTab));
+    const double arg = mu * X2;
+    const double pref = pref0 * (MIX && arg > kMixedExpArg ? exp_neg_mixed(arg)
+                                                             : exp_neg_tab(arg, expTab));
     double h[3][L + 1];
 #pragma unroll
     for (int x = 0; x < 3; ++x) {

@@ -64,10 +64,21 @@ template <class T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    // one owner per allocation: copies would free it twice
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n)
+    {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DBuf &operator=(DBuf &&) = delete;
     void alloc(size_t count)
     {
         if (count == n && p) return;
-        release();
+        const hipError_t fe = release();
+        if (fe != hipSuccess) fail(OLCAO_ERR_HIP, "hipFree: %s", hipGetErrorString(fe));
         if (count == 0) return;
         if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
             p = nullptr;
@@ -81,13 +92,21 @@ struct DBuf {
         if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T),
                                               hipMemcpyHostToDevice, s));
     }
-    void release()
+    hipError_t release()
     {
-        if (p) (void)hipFree(p);
+        const hipError_t e = p ? hipFree(p) : hipSuccess;
         p = nullptr;
         n = 0;
+        return e;
     }
-    ~DBuf() { release(); }
+    ~DBuf()
+    {
+        // a destructor cannot report through olcao_status: a failed free
+        // (which leaves a sticky device error the next call reports) is
+        // written to stderr
+        const hipError_t e = release();
+        if (e != hipSuccess) std::fprintf(stderr, "olcao: hipFree failed: %s\n", hipGetErrorString(e));
+    }
 };
 
 // Buffers of the tridiagonal eigensolver path, kept between calls.
@@ -1076,7 +1095,8 @@ void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, 
     ew.Wd.alloc(per * nb);
     ew.ord.alloc((size_t)2 * n * nb);   // per matrix: sort order, then its inverse
     if (ew.dc.n != n) {
-        dc_plan(n, ew.dc);
+        if (!dc_plan(n, ew.dc))
+            fail(OLCAO_ERR_UNSUPPORTED, "divide and conquer takes n <= %d, not %d", kDcMaxN, n);
         ew.dcNodes.upload(ew.dc.nodes, s);
         ew.dcTiles.upload(ew.dc.tiles, s);
         ew.dcRoot.upload(ew.dc.rootTiles, s);
@@ -1090,7 +1110,8 @@ void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, 
     ew.dcRho.alloc(ew.dc.nodes.size() * nb);
     const DcDev dv{ew.dcNodes.p, ew.dcTiles.p, ew.dcRoot.p, ew.dcCol.p, ew.dcStD.p, ew.dcStI.p,
                    ew.dcCnt.p, ew.dcRho.p};
-    launch_dc(ew.dc, dv, nb, D, E, ew.Z.p, ew.Zt.p, ew.Wd.p, ew.ord.p, s);
+    if (!launch_dc(ew.dc, dv, nb, D, E, ew.Z.p, ew.Zt.p, ew.Wd.p, ew.ord.p, s))
+        fail(OLCAO_ERR_UNSUPPORTED, "divide and conquer takes n <= %d, not %d", kDcMaxN, n);
     HIPCHK(hipGetLastError());
     ew.Tw.alloc((size_t)((n - 1 + kTrdBlock - 1) / kTrdBlock) * kTrdBlock * kTrdBlock * el * nb);
     launch_trd_tfactor(cplx, n, nb, ew.V.p, ew.tau.p, ew.Tw.p, ew.U.p, s);
@@ -2046,12 +2067,27 @@ static olcao_status solve_host(int n, int nb, double *A, double *B, double *w, b
         if (n <= 0 || nb <= 0) fail(OLCAO_ERR_INPUT, "n and nbatch must be positive");
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) fail(OLCAO_ERR_NO_DEVICE, "no GPU");
-        hipStream_t s;
-        HIPCHK(hipStreamCreate(&s));
-        rocblas_handle h;
-        rocblas_create_handle(&h);
-        rocblas_set_stream(h, s);
-        rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed);
+        // the stream and the handle are released on every path, the
+        // failures thrown below included
+        struct Res {
+            hipStream_t s = nullptr;
+            rocblas_handle h = nullptr;
+            ~Res()
+            {
+                if (h) rocblas_destroy_handle(h);
+                if (s) (void)hipStreamDestroy(s);
+            }
+        } res;
+        HIPCHK(hipStreamCreate(&res.s));
+        if (rocblas_create_handle(&res.h) != rocblas_status_success) {
+            res.h = nullptr;
+            fail(OLCAO_ERR_HIP, "rocblas_create_handle failed");
+        }
+        hipStream_t s = res.s;
+        rocblas_handle h = res.h;
+        if (rocblas_set_stream(h, s) != rocblas_status_success ||
+            rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed) != rocblas_status_success)
+            fail(OLCAO_ERR_HIP, "rocblas handle set-up failed");
         const size_t per = (size_t)n * n * (cplx ? 2 : 1);
         DBuf<double> dA, dB, dD, dE;
         DBuf<int> dI;
@@ -2065,8 +2101,6 @@ static olcao_status solve_host(int n, int nb, double *A, double *B, double *w, b
         HIPCHK(hipMemcpyAsync(A, dA.p, per * nb * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(w, dD.p, (size_t)n * nb * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        rocblas_destroy_handle(h);
-        (void)hipStreamDestroy(s);
         for (int k = 0; k < nb; ++k) {
             if (info[k] > n) fail(OLCAO_ERR_NOT_SPD, "B not positive definite in batch %d", k);
             if (info[k] != 0) fail(OLCAO_ERR_SOLVER, "no convergence in batch %d", k);

@@ -56,6 +56,7 @@ def lib():
         _lib.orc_integral.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp, C.c_int]
         _lib.orc_last_counts.argtypes = [C.POINTER(C.c_longlong)] * 3
         _lib.orc_set_pair_sample.argtypes = [C.c_int, C.c_int]
+        _lib.orc_set_three_center_range_scale.argtypes = [C.c_double]
     return _lib
 
 
@@ -269,6 +270,12 @@ class Oracle:
     def set_pair_sample(stride, offset=0):
         """Sample every stride-th atom pair in later sweeps (1 = all)."""
         lib().orc_set_pair_sample(int(stride), int(offset))
+
+    @staticmethod
+    def set_three_center_range_scale(f):
+        """Test-only: scale the three-centre image range (threeAlphaDist
+        and its twins) by f in later sweeps; 1 is the reference's range."""
+        lib().orc_set_three_center_range_scale(float(f))
 
     def last_counts(self):
         v = [C.c_longlong() for _ in range(3)]

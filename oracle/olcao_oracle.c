@@ -633,6 +633,12 @@ void orc_nearest_lattice_vector(const double *vec, const double *v, double *out)
 /* Integral drivers. */
 static long long cnt_pc, cnt_pp, cnt_tr;
 static int sample_stride = 1, sample_offset = 0;
+/* Test-only factor on the three-centre image range (threeAlphaDist and its
+ * nuclear and fused-sweep twins); 1 everywhere except in the test that
+ * shows the brute-force lattice sums detect a range that drops images. */
+static double three_range_scale = 1.0;
+
+void orc_set_three_center_range_scale(double f) { three_range_scale = f > 0.0 ? f : 1.0; }
 
 void orc_set_pair_sample(int stride, int offset)
 {
@@ -747,7 +753,7 @@ static void pair_block(const drv_ctx *c, const op_args *o, int ti, int tj,
             if (Z < SMALL_THRESH) continue;
             double nuc = s->potNucAlpha[pt];
             double nucDist = tau * (ea1 + ea2 + nuc) / (ea1 + ea2) / nuc;
-            double tad = (1 - s2 / adist) * nucDist;
+            double tad = (1 - s2 / adist) * nucDist * three_range_scale;
             const double *pp = s->potPos + 3 * ps;
             double dv[3] = {P[0] - pp[0], P[1] - pp[1], P[2] - pp[2]};
             double lv2[3];
@@ -780,7 +786,7 @@ static void pair_block(const drv_ctx *c, const op_args *o, int ti, int tj,
         int n = o->term - c->potCumul[t];
         double a3 = s->potAlpha[t * s->maxPotAlphas + n];
         double potDist = tau * (ea1 + ea2 + a3) / (ea1 + ea2) / a3;
-        double tadv = (1 - s2 / adist) * potDist;
+        double tadv = (1 - s2 / adist) * potDist * three_range_scale;
         int any = 0;
         SITE_LOOP(pt == t, a3, tadv, {
             any = 1;
@@ -801,7 +807,8 @@ static void pair_block(const drv_ctx *c, const op_args *o, int ti, int tj,
                     if (Z < SMALL_THRESH) continue;
                     a3 = s->potNucAlpha[t];
                 }
-                double tadv = tau * (1 - s2 / adist) * (a1 + a2 + a3) / (a1 + a2) / a3;
+                double tadv = tau * (1 - s2 / adist) * (a1 + a2 + a3) / (a1 + a2) / a3 *
+                              three_range_scale;
                 if (n < na) {
                     SITE_LOOP(pt == t, a3, tadv, {
                         orc_prim(2, a1, a2, a3, A, B, Cp, sw, one);

@@ -88,6 +88,8 @@ int orc_integral(const orc_sys *s, int op, int term, const double *coef,
 /* Restrict the next sweeps to atom pairs with (pairIndex - offset) %
  * stride == 0 (bounded CPU-baseline samples); stride 1 restores all. */
 void orc_set_pair_sample(int stride, int offset);
+/* test-only: scale the three-centre image range (1 = the reference's) */
+void orc_set_three_center_range_scale(double f);
 
 /* Work counters of the last orc_integral call (pair-cells, primitive
  * pairs, three-centre triples) for reporting. */

@@ -421,8 +421,27 @@ def test_scf_integral_file(gpu, tmp_path, case):
             p = oracle.pack_upper(ref if s.gamma else ref[k])
             exp = p[:, None] if s.gamma else np.stack([p.real, p.imag], axis=1)
             assert _ortho_close(data, exp, 1e-10), (group, k)
-    if not s.rel:
-        assert not f.exists("/atomIntgGroup/atomMVOverlap/0000001")
+    # every group, dataset and status attribute the reference's reader opens
+    # (accessSCFIntegralHDF5, hdf5SCFIntg.F90:563-760), the terms outside
+    # this path (and MV without rel) present with status 0
+    top = "/atomIntgGroup/"
+    for k in range(nk):
+        kn = f"{k + 1:07d}"
+        paths = [f"{top}{g}/{kn}" for g in ("atomOverlap", "atomKEOverlap", "atomMVOverlap",
+                                             "atomNPOverlap")]
+        paths += [f"{top}atomPotOverlap/{j + 1:07d}/{kn}" for j in range(s.pot_dim)]
+        for x in range(1, 4):
+            paths += [f"{top}{g}/{x:07d}/{kn}" for g in ("atomDMOverlap", "atomMMOverlap")]
+            paths += [f"{top}{g}/{x:07d}/{part}{kn}" for g in ("atomKOverlap", "atomKOverlapPlusG")
+                      for part in ("real", "imag")]
+        paths += [f"{top}atomOverlapCV/{part}{kn}" for part in ("real", "imag")[:comps]]
+        for path in paths:
+            assert f.exists(path), path
+    for group in ("atomDMOverlap", "atomMMOverlap", "atomKOverlap", "atomKOverlapPlusG"):
+        assert f.status(top + group) == 0, group
+    data, _, _ = f.read(f"{top}atomKOverlap/0000001/real0000001")
+    assert data.shape == (n, n) and not data.any()
+    assert f.status(top + "atomMVOverlap") == (1 if s.rel else 0)
     for k in range(nk):
         cvk = cv_raw if s.gamma else cv_raw[k]
         for part, val in (("real", np.real(cvk)), ("imag", np.imag(cvk)))[:comps]:
@@ -645,8 +664,9 @@ def test_error_paths(gpu):
 
 def test_si64_full_size_properties(gpu):
     """Benchmark workload at full size: S(k) positive definite (the solve
-    succeeds), H(-k) = conj H(k) over the shifted 2x2x2 mesh, Mode A equals
-    Mode B on the same coefficients."""
+    succeeds), H(-k) = conj H(k) over the shifted 2x2x2 mesh, Mode A
+    (setup terms, then H = NP + KE + sum_j c_j V_j) equals Mode B (the fused
+    rebuild) on the same coefficients to 1e-6 eV over all states."""
     _log("si64 full")
     s = configs.si64()
     g = gpu.OlcaoGPU(s)
@@ -657,10 +677,60 @@ def test_si64_full_size_properties(gpu):
         b = np.where(np.all(np.abs(k + k[a]) < 1e-12, axis=1))[0][0]
         assert block_rel_err(H[b], np.conj(H[a])) < 1e-11
         assert block_rel_err(S[b], np.conj(S[a])) < 1e-11
-    w = g.scf_iteration(s.pot_coeffs, 64)
-    assert np.all(np.isfinite(w))
+    n = g.vale_dim
+    wb = g.scf_iteration(s.pot_coeffs, n)
+    assert np.all(np.isfinite(wb))
     _log(f"  timings {g.timings()}")
+    g.setupSCF()
+    wa = g.secularEqnSCF(s.pot_coeffs, n)
+    err = np.abs(wa - wb).max() * HARTREE_EV
+    _log(f"  Mode A vs Mode B max |dE| {err:.2e} eV")
+    assert err < 1e-6
     g.close()
+
+
+def _pair_blocks(s, pidx):
+    """(row slice, column slice) of the atom pairs i <= j numbered row-major
+    over i, as the oracle's pair sampling numbers them."""
+    off = np.cumsum([0] + [s.types[t].num_vale_states for t in s.atom_type])
+    pairs = [(i, j) for i in range(s.n_atoms) for j in range(i, s.n_atoms)]
+    return [(slice(off[pairs[p][0]], off[pairs[p][0] + 1]),
+             slice(off[pairs[p][1]], off[pairs[p][1] + 1])) for p in pidx]
+
+
+@pytest.mark.parametrize("make,stride,offset", [("si64", 97, 5), ("quartz", 4, 1)])
+def test_full_size_sampled_pairs_match_oracle(gpu, make, stride, offset):
+    """The benchmark (Si64, 576 orbitals, 8 k) and quartz (512 k) at full
+    size against the oracle on a sample of atom pairs (every stride-th pair
+    i <= j from offset; the oracle leaves the other blocks zero): S(k) and
+    the fused H(k) blocks of those pairs for every k, and the mixed form's H
+    blocks, within 1e-10 (FP64) and 1e-6 (mixed) of the sampled scale."""
+    _log(f"{make} sampled pairs, stride {stride}")
+    s = getattr(configs, make)()
+    g = gpu.OlcaoGPU(s)
+    Sg = g.gaussOverlapOL()[0]
+    Hg = g.gaussOverlapHamPSCF(s.pot_coeffs)[0]
+    g.close()
+    gm = gpu.OlcaoGPU(s, precision=gpu.PREC_MIXED)
+    Hm = gm.gaussOverlapHamPSCF(s.pot_coeffs)[0]
+    gm.close()
+    o = oracle.Oracle(s)
+    oracle.Oracle.set_pair_sample(stride, offset)
+    try:
+        So = o.integral(oracle.OL)[0]
+        Ho = o.integral(oracle.HAM, coef=s.pot_coeffs)[0]
+    finally:
+        oracle.Oracle.set_pair_sample(1, 0)
+    npairs = s.n_atoms * (s.n_atoms + 1) // 2
+    blocks = _pair_blocks(s, range(offset, npairs, stride))
+
+    def take(M):
+        return np.concatenate([M[:, r, c].reshape(M.shape[0], -1) for r, c in blocks], axis=1)
+    es, eh, em = (block_rel_err(take(Sg), take(So)), block_rel_err(take(Hg), take(Ho)),
+                  block_rel_err(take(Hm), take(Ho)))
+    _log(f"  {len(blocks)} pairs: S {es:.2e}, H {eh:.2e}, mixed H {em:.2e}")
+    assert es < 1e-10 and eh < 1e-10
+    assert em < 1e-6
 
 
 def band_energy(w, kweights, n_occ):
@@ -669,7 +739,7 @@ def band_energy(w, kweights, n_occ):
     return float(np.sum(np.asarray(kweights)[:, None] * np.asarray(w)[:, :n_occ]))
 
 
-@pytest.mark.parametrize("make", ["si64", "diamond_c8"])
+@pytest.mark.parametrize("make", ["si64", "diamond_c8", "quartz", "asio2_1000"])
 def test_mixed_precision_form_tolerances(gpu, make):
     """The chosen (mixed-precision) form against the FP64 form on the full
     workloads: occupied-band eigenvalues within 1 meV, band energy within
@@ -699,6 +769,32 @@ def test_mixed_precision_form_tolerances(gpu, make):
     assert eh < 1e-6
     g64.close()
     gmx.close()
+
+
+def test_mixed_precision_uneven_batch_tolerances(gpu):
+    """The uneven batch (64 cells) in the mixed form against the FP64 form,
+    both through the batched call: every cell's occupied-band eigenvalues
+    within 1 meV and its band energy within 1e-5 Ha/atom."""
+    cells = configs.uneven_batch()
+    _log("uneven batch, mixed vs FP64")
+    g64 = [gpu.OlcaoGPU(s) for s in cells]
+    gmx = [gpu.OlcaoGPU(s, precision=gpu.PREC_MIXED) for s in cells]
+    ns = [g.vale_dim for g in g64]
+    w64, _ = gpu.scf_iteration_batch(g64, ns, 8)
+    wmx, _ = gpu.scf_iteration_batch(gmx, ns, 8)
+    worst_ev, worst_e = 0.0, 0.0
+    for i, s in enumerate(cells):
+        a = np.asarray(w64[i]).reshape(s.nk, -1)
+        b = np.asarray(wmx[i]).reshape(s.nk, -1)
+        n_el = sum(s.pot_types[t].Z for t in s.pot_type)
+        n_occ = min(ns[i], int(round(n_el / 2)))
+        ev = np.abs(a[:, :n_occ] - b[:, :n_occ]).max() * HARTREE_EV * 1e3
+        de = abs(band_energy(a, s.kweights, n_occ) - band_energy(b, s.kweights, n_occ)) / s.n_atoms
+        worst_ev, worst_e = max(worst_ev, ev), max(worst_e, de)
+        assert ev < 1.0 and de < 1e-5, (i, ev, de)
+    _log(f"  worst cell: occupied bands {worst_ev:.3e} meV, band energy {worst_e:.2e} Ha/atom")
+    for g in g64 + gmx:
+        g.close()
 
 
 @pytest.mark.parametrize("make", ["diamond_c8", "quartz", "si64"])

@@ -1,0 +1,205 @@
+"""Brute-force pins of the oracle's three-centre lattice sums.
+
+The nuclear (gaussOverlapNP + nuclearPE, integrals.F90:1531-1942,
+:2533-2697), potential-term (gaussOverlapEP + electronicPE, :1946-2354,
+:2700-2856) and fused Hamiltonian (gaussOverlapHamPSCF, :2860-3605) sweeps of
+the oracle are compared with direct sums over every atom pair, pair cell,
+exponent pair, potential site and potential image inside a range TWICE the
+reference's exponent cutoff (the reference keeps a term when
+xi s^2 + zeta a3 / (zeta + a3) |P - C|^2 <= tau; this sum keeps it up to
+2 tau).  Nothing of the reference's screening is used: no alphaDist table,
+no nearest-image origin, no site walk, no cell-list ordering.  The
+primitives are the oracle's, themselves pinned by the reference's golden
+vectors.  Contraction, Bloch phase exp(i k.L) and placement are explicit.
+
+A range that drops images (the oracle's three-centre range halved, planted
+on the CPU through its test-only knob) moves the sweeps by far more than the
+tolerance, so the check detects it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from olcao_amd import configs
+from tests.conftest import block_rel_err
+
+LCLASS = {1: 0, 4: 1, 9: 2, 16: 3}
+
+
+def _full(M):
+    return np.triu(M) + np.conj(np.triu(M, 1)).swapaxes(-1, -2)
+
+
+def _lattice_points(cell, radius):
+    """All lattice vectors n @ cell inside a box that covers the sphere of
+    the given radius about any point of the cell."""
+    cell = np.asarray(cell, float)
+    vol = abs(np.dot(cell[0], np.cross(cell[1], cell[2])))
+    # distance between lattice planes of each family: V / |a_j x a_k|
+    h = [vol / np.linalg.norm(np.cross(cell[(i + 1) % 3], cell[(i + 2) % 3])) for i in range(3)]
+    N = [int(math.ceil(radius / hi)) + 2 for i, hi in enumerate(h)]
+    g = np.array(np.meshgrid(*[np.arange(-n, n + 1) for n in N], indexing="ij")).reshape(3, -1).T
+    return g @ cell
+
+
+class _Brute:
+    """Direct lattice sums for one small system."""
+
+    def __init__(self, s, o, range_factor=2.0):
+        self.s, self.o = s, o
+        self.tau = -math.log(s.basis_cutoff)
+        self.lim = range_factor * self.tau
+        self.kc = o.kcart
+        self.cell = np.asarray(s.cell, float)
+        self.off = np.cumsum([0] + [s.types[t].num_vale_states for t in s.atom_type])
+        self.n = int(self.off[-1])
+        amin = min(float(np.min(t.alphas)) for t in s.types)
+        pmin = min(float(np.min(p.alphas)) for p in s.pot_types)
+        # widest pair separation and widest potential distance in range
+        self.Lpair = _lattice_points(self.cell, math.sqrt(self.lim / (amin / 2)) + 1.0)
+        self.Lpot = _lattice_points(self.cell, math.sqrt(self.lim / (pmin * 2 * amin / (2 * amin + pmin))) + 1.0)
+
+    def _terms(self, kind, term):
+        """(site positions, a3, factor) of the three-centre terms of a sweep:
+        kind 'ep' one potential term, 'np' the nuclei (-Z, alpha_nuc)."""
+        s = self.s
+        out = []
+        if kind == "np":
+            for ps in range(len(s.pot_pos)):
+                pt = s.pot_type[ps]
+                Z = s.pot_types[pt].Z
+                if Z >= oracle.SMALL_THRESH:
+                    out.append((s.pot_pos[ps], s.pot_types[pt].nuc_alpha, -Z, 3))
+        else:
+            first = 0
+            for t, p in enumerate(s.pot_types):
+                if first <= term < first + len(p.alphas):
+                    a3 = p.alphas[term - first]
+                    for ps in range(len(s.pot_pos)):
+                        if s.pot_type[ps] == t:
+                            out.append((s.pot_pos[ps], a3, 1.0, 2))
+                first += len(p.alphas)
+        return out
+
+    def matrix(self, kind, term=0):
+        """Full [nk, n, n] matrix of the sweep, all atom pairs (i, j)."""
+        s, o = self.s, self.o
+        nk = len(self.kc)
+        ref = np.zeros((nk, self.n, self.n), complex)
+        terms = self._terms(kind, term) if kind in ("ep", "np") else []
+        for i in range(s.n_atoms):
+            for j in range(s.n_atoms):
+                ti, tj = s.atom_type[i], s.atom_type[j]
+                t1, t2 = s.types[ti], s.types[tj]
+                ns1, ns2 = t1.num_vale_states, t2.num_vale_states
+                A = np.asarray(s.atom_pos[i], float)
+                ins = dict(a=[], A=[], B=[], C=[], sw=[], a1=[], a2=[], L=[], fac=[])
+                for a1i, a1 in enumerate(t1.alphas):
+                    for a2i, a2 in enumerate(t2.alphas):
+                        la, lb = LCLASS[o.lmai[ti, a1i]], LCLASS[o.lmai[tj, a2i]]
+                        sw = (1 << la) + 16 * (1 << lb)
+                        zeta, xi = a1 + a2, a1 * a2 / (a1 + a2)
+                        B = s.atom_pos[j] + self.Lpair
+                        s2 = np.sum((A - B) ** 2, axis=1)
+                        keep = xi * s2 <= self.lim
+                        B, L, s2 = B[keep], self.Lpair[keep], s2[keep]
+                        if kind in ("ol", "ke"):
+                            m = len(B)
+                            ins["a"].append(np.tile([a1, a2, 0.0], (m, 1)))
+                            ins["A"].append(np.tile(A, (m, 1)))
+                            ins["B"].append(B)
+                            ins["C"].append(np.tile(A, (m, 1)))
+                            ins["sw"].append(np.full(m, sw))
+                            ins["a1"].append(np.full(m, a1i))
+                            ins["a2"].append(np.full(m, a2i))
+                            ins["L"].append(L)
+                            ins["fac"].append(np.ones(m))
+                            continue
+                        P = (a1 * A + a2 * B) / zeta
+                        rest = self.lim - xi * s2
+                        for site, a3, fac, _op in terms:
+                            mu = zeta * a3 / (zeta + a3)
+                            Cs = site + self.Lpot                       # [nC, 3]
+                            d2 = np.sum((P[:, None, :] - Cs[None]) ** 2, axis=2)
+                            pi_, ci = np.nonzero(mu * d2 <= rest[:, None])
+                            m = len(pi_)
+                            if m == 0:
+                                continue
+                            ins["a"].append(np.tile([a1, a2, a3], (m, 1)))
+                            ins["A"].append(np.tile(A, (m, 1)))
+                            ins["B"].append(B[pi_])
+                            ins["C"].append(Cs[ci])
+                            ins["sw"].append(np.full(m, sw))
+                            ins["a1"].append(np.full(m, a1i))
+                            ins["a2"].append(np.full(m, a2i))
+                            ins["L"].append(L[pi_])
+                            ins["fac"].append(np.full(m, fac))
+                if not ins["a"]:
+                    continue
+                cat = {k: np.concatenate(v) for k, v in ins.items()}
+                op = {"ol": 0, "ke": 1, "ep": 2, "np": 3}[kind]
+                sh = oracle.prim_batch(op, cat["a"], cat["A"], cat["B"], cat["C"], cat["sw"])
+                lm1 = o.lmi[ti, :ns1] - 1
+                lm2 = o.lmi[tj, :ns2] - 1
+                G = sh[:, lm1][:, :, lm2]                                  # [E, ns1, ns2]
+                G = G * o.basis[ti, :ns1, :][:, cat["a1"]].T[:, :, None]
+                G = G * o.basis[tj, :ns2, :][:, cat["a2"]].T[:, None, :]
+                G = G * cat["fac"][:, None, None]
+                ph = np.exp(1j * (cat["L"] @ self.kc.T))                  # [E, nk]
+                ref[:, self.off[i]:self.off[i + 1], self.off[j]:self.off[j + 1]] += \
+                    np.einsum("ek,epq->kpq", ph, G)
+        return ref
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    s = configs.small_test(kmesh=(2, 1, 1), n_alpha=3, lmax=1, n_pot=3, two_types=True)
+    o = oracle.Oracle(s)
+    return s, o, _Brute(s, o)
+
+
+def test_nuclear_sweep_equals_unscreened_sum(tiny):
+    s, o, b = tiny
+    M = o.integral(oracle.NP)[0]
+    assert block_rel_err(_full(M), b.matrix("np")) < 1e-11
+
+
+@pytest.mark.parametrize("term", [0, 2, 3])
+def test_potential_term_sweep_equals_unscreened_sum(tiny, term):
+    """Terms 0 and 2 are the first type's most diffuse and tightest
+    Gaussians, term 3 the second type's most diffuse one."""
+    s, o, b = tiny
+    M = o.integral(oracle.EP, term=term)[0]
+    assert block_rel_err(_full(M), b.matrix("ep", term)) < 1e-11
+
+
+def test_fused_hamiltonian_equals_unscreened_sum(tiny):
+    """The fused sweep's own range test (integrals.F90:3297-3305) against
+    KE + sum_j c_j V_j + NP summed directly."""
+    s, o, b = tiny
+    H = o.integral(oracle.HAM, coef=s.pot_coeffs)[0]
+    ref = b.matrix("ke") + b.matrix("np")
+    for j, c in enumerate(s.pot_coeffs):
+        ref = ref + c * b.matrix("ep", j)
+    assert block_rel_err(_full(H), ref) < 1e-11
+
+
+def test_dropped_images_are_detected(tiny):
+    """Planted mistake: the oracle's three-centre range halved (images the
+    reference keeps are dropped).  Every sweep then leaves the direct sum by
+    far more than the 1e-11 tolerance."""
+    s, o, b = tiny
+    ref_np, ref_ep = b.matrix("np"), b.matrix("ep", 0)
+    try:
+        oracle.Oracle.set_three_center_range_scale(0.5)
+        M_np = o.integral(oracle.NP)[0]
+        M_ep = o.integral(oracle.EP, term=0)[0]
+        M_h = o.integral(oracle.HAM, coef=s.pot_coeffs)[0]
+    finally:
+        oracle.Oracle.set_three_center_range_scale(1.0)
+    assert block_rel_err(_full(M_np), ref_np) > 1e-8
+    assert block_rel_err(_full(M_ep), ref_ep) > 1e-8
+    H = o.integral(oracle.HAM, coef=s.pot_coeffs)[0]
+    assert block_rel_err(_full(M_h), _full(H)) > 1e-8
