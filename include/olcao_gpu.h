@@ -47,7 +47,12 @@ typedef enum {
     OLCAO_ERR_NO_DEVICE = 9     /* no GPU visible */
 } olcao_status;
 
-/* Precision of the contraction + Bloch sum (north of the FP64 integrals). */
+/* Arithmetic of the integral build.  OLCAO_PREC_FP64: everything in FP64
+ * (the drop-in form).  OLCAO_PREC_MIXED: the three-Gaussian exponentials of
+ * distant potential images (those below 1e-4) by the FP32 hardware
+ * exponential after an FP64 range reduction; everything else as FP64.  The
+ * screening exponents, the Cholesky factorisation and the eigensolve are
+ * FP64 in both forms. */
 enum { OLCAO_PREC_FP64 = 0, OLCAO_PREC_MIXED = 1 };
 
 /* The system as the reference holds it after parseInput

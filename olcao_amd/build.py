@@ -85,7 +85,30 @@ def build_fortran(verbose: bool = True) -> str:
     os.replace(tmp, FLIB)
     if verbose:
         print(f"built {FLIB}", flush=True)
+    build_fortran_driver(verbose)
     return FLIB
+
+
+DRIVER_SRC = os.path.join(HERE, "..", "tests", "fortran", "facade_driver.f90")
+DRIVER = os.path.join(HERE, "..", "tests", "fortran", "facade_driver")
+
+
+def build_fortran_driver(verbose: bool = True):
+    """The Fortran test program that uses the facade module at the
+    reference driver's call sites (tests/fortran/facade_driver.f90), linked
+    to the facade and GPU libraries."""
+    if not os.path.exists(DRIVER_SRC):
+        return None
+    tmp = DRIVER + ".tmp"
+    cmd = [FC, "-O2", "-module-dir", BUILD, "-I", BUILD, DRIVER_SRC, "-o", tmp, f"-L{HERE}",
+           "-lolcao_fortran", "-lolcao_gpu", "-Wl,-rpath,$ORIGIN/../../olcao_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed for the facade driver:\n{r.stderr[-4000:]}")
+    os.replace(tmp, DRIVER)
+    if verbose:
+        print(f"built {DRIVER}", flush=True)
+    return DRIVER
 
 
 H5LIB = os.path.join(HERE, "libolcao_h5.so")

@@ -1,10 +1,35 @@
 ! Fortran host facade of the MI355X integral -> eigenvalue path.
 !
-! Keeps the reference's procedure names and argument lists so that the call
-! sites in src/uolcao/OLCAO.F90 (setupSCF :248-279, mainSCF :482,
-! intgPSCF :645-662, bandPSCF :758) and the solver call sites in
-! src/uolcao/secularEqn.F90 (:227, :509) keep their shape; each procedure
-! forwards to the C-ABI of include/olcao_gpu.h through ISO_C_BINDING.
+! Keeps the reference's procedure NAMES, so that the call sites in
+! src/uolcao/OLCAO.F90 (setupSCF :248-279, mainSCF :482, intgPSCF :645-662,
+! bandPSCF :758) and the solver call sites in src/uolcao/secularEqn.F90
+! (:227, :509) call a procedure of the same name; each forwards to the C-ABI
+! of include/olcao_gpu.h through ISO_C_BINDING.  The argument lists differ
+! from the reference's where the reference passes HDF5 dataset ids and reads
+! module globals; the facade takes and returns arrays instead:
+!   gaussOverlapOL/KE/MV/NP(valeVale, coreVale, coreCore)
+!       reference (numComponents, fullCVDims, packedVVDims, did, CVdid, aid)
+!       integrals.F90:151, :656, :1091, :1531: the matrices come back as
+!       arrays (valeVale orthogonalised when coreDim > 0); the caller packs
+!       and writes them (or calls olcao_write_scf_integrals_h5)
+!   elecPotGaussOverlap(term, valeVale, coreVale, coreCore)
+!       reference (packedVVDims, did(nk,potDim), aid(potDim)) :2357: one
+!       potential term per call instead of all of them
+!   gaussOverlapHamPSCF(potCoeffs, valeVale, coreVale, coreCore)
+!       reference (did(nk,spin), aid) :2860: the coefficients come in as an
+!       argument instead of O_Potential's potCoeffs
+!   secularEqnSCF(potCoeffs, numStates, energyEigenValues)
+!       reference (spinDirection, numStates) secularEqn.F90:37: all spins in
+!       one call, the terms from the device instead of the HDF5 file
+!   secularEqnPSCF(numStates, energyEigenValues)
+!       reference (spin, numStates, numComponents, ol_did, ham_did, eVal_did,
+!       eVec_did, eVec_aid) :331: the matrices stay on the device
+!   setupSCF(), makeValenceRho(...): no single reference procedure; they
+!       stand for the setup's term set and valeCharge.F90:34
+!   solveZHEGV, solveDSYGV: the reference's dummy arguments unchanged
+!       (interfaces.F90:73, :152)
+! tests/fortran/facade_driver.f90 is a Fortran program that calls these at
+! the reference driver's call sites.
 !
 ! Errors follow the reference: the library's message is written to unit 20
 ! and execution stops (integrals.F90:377-384, interfaces.F90:107-110).
