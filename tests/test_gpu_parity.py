@@ -435,8 +435,8 @@ def test_scf_integral_file(gpu, tmp_path, case):
             paths += [f"{top}{g}/{x:07d}/{part}{kn}" for g in ("atomKOverlap", "atomKOverlapPlusG")
                       for part in ("real", "imag")]
         paths += [f"{top}atomOverlapCV/{part}{kn}" for part in ("real", "imag")[:comps]]
-        for path in paths:
-            assert f.exists(path), path
+        for name in paths:
+            assert f.exists(name), name
     for group in ("atomDMOverlap", "atomMMOverlap", "atomKOverlap", "atomKOverlapPlusG"):
         assert f.status(top + group) == 0, group
     data, _, _ = f.read(f"{top}atomKOverlap/0000001/real0000001")
