@@ -48,11 +48,13 @@ typedef enum {
 } olcao_status;
 
 /* Arithmetic of the integral build.  OLCAO_PREC_FP64: everything in FP64
- * (the drop-in form).  OLCAO_PREC_MIXED: the three-Gaussian exponentials of
- * distant potential images (those below 1e-4) by the FP32 hardware
- * exponential after an FP64 range reduction; everything else as FP64.  The
- * screening exponents, the Cholesky factorisation and the eigensolve are
- * FP64 in both forms. */
+ * (the drop-in form).  OLCAO_PREC_MIXED: the Bloch sums as exact products
+ * of BF16 slices on the matrix cores with FP32 accumulation (a relative
+ * error of about 2^-40 of each block column's largest entry), and the
+ * three-Gaussian exponentials of distant potential images (those below
+ * 1e-4) by the FP32 hardware exponential after an FP64 range reduction;
+ * everything else as FP64.  The screening exponents, the Cholesky
+ * factorisation and the eigensolve are FP64 in both forms. */
 enum { OLCAO_PREC_FP64 = 0, OLCAO_PREC_MIXED = 1 };
 
 /* The system as the reference holds it after parseInput
@@ -99,10 +101,16 @@ typedef struct {
  * OLCAO_EIG_LIBRARY always uses rocSOLVER. */
 enum { OLCAO_EIG_AUTO = 0, OLCAO_EIG_LIBRARY = 1 };
 
+/* Bloch-sum kernel: OLCAO_BLOCH_AUTO takes the matrix-core kernel (FP64
+ * MFMA in the FP64 form, exact split BF16 products in the mixed form);
+ * OLCAO_BLOCH_VALU the vector-ALU kernel (FP64 in both forms). */
+enum { OLCAO_BLOCH_AUTO = 0, OLCAO_BLOCH_VALU = 1, OLCAO_BLOCH_MFMA = 2 };
+
 typedef struct {
     int precision;                /* OLCAO_PREC_* */
     int eigensolver;              /* OLCAO_EIG_* */
-    int reserved[6];
+    int bloch;                    /* OLCAO_BLOCH_* */
+    int reserved[5];
 } olcao_options;
 
 /* Context lifetime.  Replaces the derived setup of OLCAO.F90:setupSCF

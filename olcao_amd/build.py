@@ -36,6 +36,7 @@ UNITS = [("kernels.hip", "kernels.o", []),
          ("ktrd.hip", "ktrd.o", []),
          ("kdc.hip", "kdc.o", []),
          ("kchol.hip", "kchol.o", []),
+         ("kbloch.hip", "kbloch.o", []),
          ("runtime.cpp", "runtime.o", ["-x", "hip"])]
 
 

@@ -148,6 +148,9 @@ int nshh(int l) { return (l + 1) * (l + 1); }
 struct olcao_ctx {
     std::string err;
     int precision = 0;
+    bool blochMfma = true;        // matrix-core Bloch sum (else the vector-ALU one)
+    std::vector<unsigned short> hPslice;   // phase slices of the split Bloch sum
+    DBuf<unsigned short> dPslice;
     bool gamma = false;
     int rel = 0;                 // 1: scalar-relativistic (mass-velocity term)
     int spin = 1;                // 1 or 2 potential-coefficient sets
@@ -618,6 +621,8 @@ void upload(olcao_ctx &c)
     c.dPotSiteType.upload(c.hPotSiteType, s);
     c.dCellDims.upload(c.hCellDims, s); c.dCellSizes.upload(c.hCellSizes, s);
     c.dKcart.upload(c.hKcart, s); c.dPhase.upload(c.hPhase, s);
+    phase_slices_host(c.hPhase, (int)c.hCellSizes.size(), c.nk, c.gamma, c.hPslice);
+    c.dPslice.upload(c.hPslice, s);
     c.dPairI.upload(c.hPairI, s); c.dPairJ.upload(c.hPairJ, s);
     c.dPairLv.upload(c.hPairLv, s); c.dPairPc.upload(c.hPairPc, s);
     c.dPcPair.upload(c.hPcPair, s); c.dPcCell.upload(c.hPcCell, s);
@@ -967,7 +972,11 @@ void contract_scatter(olcao_ctx &c, double *vv, double *cv, double *cc)
         launch_fill(cv, (long long)c.cvDoubles(), 0.0, s);
         launch_fill(cc, (long long)c.ccDoubles(), 0.0, s);
     }
-    launch_bloch_scatter(c.dev, c.dX.p, vv, cv, cc, c.gamma ? 1 : 0, s);
+    if (c.blochMfma)
+        launch_bloch_mfma(c.dev, c.dX.p, c.dPslice.p, vv, cv, cc, c.gamma ? 1 : 0,
+                          c.precision == OLCAO_PREC_MIXED, s);
+    else
+        launch_bloch_scatter(c.dev, c.dX.p, vv, cv, cc, c.gamma ? 1 : 0, s);
     HIPCHK(hipGetLastError());
 }
 
@@ -1436,6 +1445,9 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
     olcao_status st = guard(c, [&] {
         c->precision = opt ? opt->precision : 0;
         c->eigLibrary = opt && opt->eigensolver == OLCAO_EIG_LIBRARY;
+        c->blochMfma = !(opt && opt->bloch == OLCAO_BLOCH_VALU);
+        if (opt && (opt->bloch < OLCAO_BLOCH_AUTO || opt->bloch > OLCAO_BLOCH_MFMA))
+            fail(OLCAO_ERR_INPUT, "bloch must be OLCAO_BLOCH_AUTO, _VALU or _MFMA");
         if (c->precision != OLCAO_PREC_FP64 && c->precision != OLCAO_PREC_MIXED)
             fail(OLCAO_ERR_INPUT, "precision must be OLCAO_PREC_FP64 or OLCAO_PREC_MIXED");
         // The runtime spreads a process's streams over a few hardware

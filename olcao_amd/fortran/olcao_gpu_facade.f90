@@ -73,7 +73,8 @@ module O_GPUFacade
    type, bind(C) :: olcao_options_t
       integer(c_int) :: precision
       integer(c_int) :: eigensolver
-      integer(c_int) :: reserved(6)
+      integer(c_int) :: bloch
+      integer(c_int) :: reserved(5)
    end type olcao_options_t
 
    type(c_ptr), save :: ctx = c_null_ptr
@@ -239,6 +240,7 @@ contains
       if (present(spin)) s%spin = spin
       o%precision = 0
       o%eigensolver = 0
+      o%bloch = 0
       o%reserved = 0
       st = c_ctx_create(s, o, ctx)
       if (st /= 0) call gpuStop(st, 'gpuInit')

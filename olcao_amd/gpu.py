@@ -57,10 +57,12 @@ class _System(C.Structure):
 
 
 class _Options(C.Structure):
-    _fields_ = [("precision", C.c_int), ("eigensolver", C.c_int), ("reserved", C.c_int * 6)]
+    _fields_ = [("precision", C.c_int), ("eigensolver", C.c_int), ("bloch", C.c_int),
+                ("reserved", C.c_int * 5)]
 
 
 EIG_AUTO, EIG_LIBRARY = 0, 1
+BLOCH_AUTO, BLOCH_VALU, BLOCH_MFMA = 0, 1, 2
 
 
 _lib = None
@@ -176,7 +178,7 @@ def boys_batch(T):
 class OlcaoGPU:
     """One device context for one system (setupSCF's derived state)."""
 
-    def __init__(self, sysin, precision=PREC_FP64, eigensolver=EIG_AUTO):
+    def __init__(self, sysin, precision=PREC_FP64, eigensolver=EIG_AUTO, bloch=BLOCH_AUTO):
         self.sysin = sysin
         f = sysin.flat()
         self._keep = f
@@ -200,6 +202,7 @@ class OlcaoGPU:
         opt = _Options()
         opt.precision = precision
         opt.eigensolver = eigensolver
+        opt.bloch = bloch
         ctx = C.c_void_p()
         code = lib().olcao_ctx_create(C.byref(s), C.byref(opt), C.byref(ctx))
         _check(code, None)

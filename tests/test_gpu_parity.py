@@ -771,6 +771,34 @@ def test_mixed_precision_form_tolerances(gpu, make):
     gmx.close()
 
 
+@pytest.mark.parametrize("case", ["core", "core_gamma_rel", "quartz"])
+def test_bloch_kernels_agree(gpu, case):
+    """The Bloch sum on the matrix cores against the vector-ALU kernel on
+    the same device blocks: FP64 MFMA within 1e-14 of the matrix scale (the
+    FP64 MFMA adds its four products per step in cell order, as the vector
+    kernel's FMAs do, so the two agree bit for bit); the mixed form's exact
+    BF16 slice products within the integral parity bar 1e-10 (their
+    truncation is about 2^-40 of each block column's largest entry per cell;
+    1e-11 to 3e-11 measured); overlap, fused Hamiltonian and the core
+    blocks, complex and Gamma builds, and quartz's 512 k."""
+    s = configs.quartz() if case == "quartz" else configs.small_test(**SMALL[case])
+    res = {}
+    for name, prec, bloch in (("valu", gpu.PREC_FP64, gpu.BLOCH_VALU),
+                              ("mfma", gpu.PREC_FP64, gpu.BLOCH_MFMA),
+                              ("split", gpu.PREC_MIXED, gpu.BLOCH_MFMA)):
+        g = gpu.OlcaoGPU(s, precision=prec, bloch=bloch)
+        res[name] = (g.gaussOverlapOL(), g.gaussOverlapHamPSCF(s.pot_coeffs))
+        g.close()
+    for name, tol in (("mfma", 1e-14), ("split", 1e-10)):
+        for a, b in zip(res[name], res["valu"]):
+            for x, y in zip(a, b):
+                if y is None:
+                    continue
+                e = block_rel_err(x, y)
+                _log(f"  {case} {name}: {e:.2e}")
+                assert _ortho_close(x, y, tol), (case, name, e)
+
+
 def test_mixed_precision_uneven_batch_tolerances(gpu):
     """The uneven batch (64 cells) in the mixed form against the FP64 form,
     both through the batched call: every cell's occupied-band eigenvalues
