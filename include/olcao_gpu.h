@@ -50,7 +50,7 @@ typedef enum {
 /* Arithmetic of the integral build.  OLCAO_PREC_FP64: everything in FP64
  * (the drop-in form).  OLCAO_PREC_MIXED: the Bloch sums as exact products
  * of BF16 slices on the matrix cores with FP32 accumulation (a relative
- * error of about 2^-40 of each block column's largest entry), and the
+ * error of about 2^-48 of each block column's largest entry per cell), and the
  * three-Gaussian exponentials of distant potential images (those below
  * 1e-4) by the FP32 hardware exponential after an FP64 range reduction;
  * everything else as FP64.  The screening exponents, the Cholesky
@@ -110,7 +110,11 @@ typedef struct {
     int precision;                /* OLCAO_PREC_* */
     int eigensolver;              /* OLCAO_EIG_* */
     int bloch;                    /* OLCAO_BLOCH_* */
-    int reserved[5];
+    int graph;                    /* 1: olcao_scf_iteration replays a captured graph
+                                     of its device work (build and solve) after its
+                                     first call; the library eigensolver (n > 640)
+                                     is not captured and such contexts stay eager */
+    int reserved[4];
 } olcao_options;
 
 /* Context lifetime.  Replaces the derived setup of OLCAO.F90:setupSCF
@@ -250,8 +254,16 @@ olcao_status olcao_scf_iteration_batch(int nCells, olcao_ctx *const *ctxs,
  * t[0] overlap sweep (incl. its contraction and Bloch sum), t[1] Hamiltonian
  * primitive blocks, t[2] Hamiltonian contraction + Bloch sum (+ ortho),
  * t[3] solve, t[4] total; t[5] three-centre kernels alone (all summed over
- * spins). */
+ * spins); t[6] 1 when the iteration was a graph replay (then only t[4] is
+ * measured, from the coefficient upload to the graph's end). */
 olcao_status olcao_get_timings(const olcao_ctx *ctx, double t[8]);
+
+/* Graph mode of a context (olcao_options.graph): *state 0 off, 1 a captured
+ * iteration graph is current, 2 the next olcao_scf_iteration runs eagerly
+ * and captures (first call, or a buffer was reallocated since the capture),
+ * 3 capture refused; *reason (may be NULL) then points to the context's
+ * message, valid until the context is destroyed. */
+olcao_status olcao_get_graph_state(const olcao_ctx *ctx, int *state, const char **reason);
 
 /* Work counters of the last three-centre sweep: counts[2*(4*la+lb)] three-
  * Gaussian overlap evaluations and counts[2*(4*la+lb)+1] nuclear

@@ -24,9 +24,9 @@
 //               quantum 2^(E-14-8d), so an FP32 accumulator per order sums
 //               them without rounding as long as it stays below 2^24; it is
 //               flushed into FP64 every kFlushChunks chunks of 32 cells.
-//               Orders d < kSlices are kept (15 BF16 products per chunk and
-//               component); the truncation leaves a relative error of about
-//               2^-40 of the column's largest block entry.
+//               Orders d < kSlices = 6 are kept (21 BF16 products per chunk
+//               and component); the truncation leaves a relative error of
+//               about 2^-48 of the column's largest block entry per cell.
 #include <cstring>
 #include <vector>
 
@@ -86,7 +86,10 @@ __device__ __forceinline__ void bloch_store(const DevCtx &c, int i, int j, int k
     }
 }
 
-constexpr int kFlushChunks = 4;   // 4 x 32 cells x 5 products x 2^14 < 2^24
+constexpr int kFlushChunks = 4;   // 4 chunks x 32 cells x 6 products x 2^14 < 2^24
+static_assert(kFlushChunks * 32 * kBlochSlices * (1 << 14) < (1 << 24),
+              "an order's FP32 accumulator must stay exact between flushes");
+static_assert(kBlochSlices <= 8, "the phase-slice table holds 8 slices");
 
 // BF16 bits of a small integer held in a float (exact: <= 8 significant bits)
 __device__ __forceinline__ unsigned short bf16_bits(float v)

@@ -305,6 +305,13 @@ __global__ __launch_bounds__(64) void k_bloch(DevCtx c, const double *X, double 
     }
 }
 
+__global__ void k_zero_words(unsigned *p, long long n)
+{
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x)
+        p[t] = 0u;
+}
+
 __global__ void k_fill(double *p, long long n, double v)
 {
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
@@ -608,6 +615,15 @@ void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv, double *
     const size_t lds = ((size_t)kBlochCells * per + 2 * (size_t)kBlochCells * KT) * sizeof(double);
     hipLaunchKernelGGL(k_bloch, dim3(c.nPairs * nTiles), dim3(64), lds, s, c, X, vv, cv, cc,
                        gamma, KT, nTiles);
+}
+
+void launch_zero_words(void *p, long long words, hipStream_t s)
+{
+    if (words <= 0) return;
+    long long blocks = (words + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)blocks), dim3(256), 0, s,
+                       static_cast<unsigned *>(p), words);
 }
 
 void launch_fill(double *p, long long n, double v, hipStream_t s)

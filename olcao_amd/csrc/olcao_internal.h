@@ -205,12 +205,16 @@ void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv,
 // over kBlochSlices fixed-point slices per operand.  pslice: the phase
 // slices [nCells][nk (1 at Gamma)][re, im][8] as BF16 bit patterns, from
 // phase_slices_host.
-constexpr int kBlochSlices = 5;
+constexpr int kBlochSlices = 6;
 void launch_bloch_mfma(const DevCtx &c, const double *X, const unsigned short *pslice, double *vv,
                        double *cv, double *cc, int gamma, bool split, hipStream_t s);
 void phase_slices_host(const std::vector<double> &phase, int nCells, int nk, int gamma,
                        std::vector<unsigned short> &out);
 void launch_fill(double *p, long long n, double v, hipStream_t s);
+// Zeroes `words` 4-byte words by a kernel: clears counters and status
+// arrays in sequences that may be captured in a graph, where memset nodes
+// are not used.
+void launch_zero_words(void *p, long long words, hipStream_t s);
 void launch_assemble(long long n2nk, int n, const double *NP, const double *KE,
                      const double *MV, const double *const *V, const double *coef, int nTerms,
                      double *H, int cplx, hipStream_t s);

@@ -60,6 +60,15 @@ struct Fail {
         if (e_ != hipSuccess) fail(OLCAO_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
     } while (0)
 
+// Counts device allocations and frees of the library's buffers: a captured
+// iteration graph holds device addresses and is replayed only while its
+// context's count is the one it was captured at.  Inside an entry point the
+// count is the context's (set by guard for the calling thread); elsewhere
+// the process-wide one.
+std::atomic<unsigned long long> g_allocEpoch{1};
+thread_local std::atomic<unsigned long long> *tl_allocEpoch = nullptr;
+inline void bump_alloc_epoch() { (tl_allocEpoch ? *tl_allocEpoch : g_allocEpoch).fetch_add(1); }
+
 template <class T>
 struct DBuf {
     T *p = nullptr;
@@ -80,6 +89,7 @@ struct DBuf {
         const hipError_t fe = release();
         if (fe != hipSuccess) fail(OLCAO_ERR_HIP, "hipFree: %s", hipGetErrorString(fe));
         if (count == 0) return;
+        bump_alloc_epoch();
         if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
             p = nullptr;
             fail(OLCAO_ERR_ALLOC, "hipMalloc of %zu bytes failed", count * sizeof(T));
@@ -94,6 +104,7 @@ struct DBuf {
     }
     hipError_t release()
     {
+        if (p) bump_alloc_epoch();
         const hipError_t e = p ? hipFree(p) : hipSuccess;
         p = nullptr;
         n = 0;
@@ -121,6 +132,7 @@ struct EigWork {
     DBuf<int2> dcRoot, dcCol;
     DBuf<double> dcStD, dcRho;
     DBuf<int> dcStI;
+    int lastPath = -1;     // path of the last gen_eig: 0 LDS solver, 1 tridiagonal, 2 library
 };
 
 double angnorm(int lm)   // lm = 1..16, basis.f90:86-101
@@ -148,6 +160,14 @@ int nshh(int l) { return (l + 1) * (l + 1); }
 struct olcao_ctx {
     std::string err;
     int precision = 0;
+    // the Mode-B iteration as a captured graph: replayed while no buffer of
+    // the library was (re)allocated since the capture
+    bool useGraph = false, graphFailed = false, lastReplay = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    std::atomic<unsigned long long> allocEpoch{1};
+    unsigned long long graphEpoch = 0;
+    std::string graphErr;
     bool blochMfma = true;        // matrix-core Bloch sum (else the vector-ALU one)
     std::vector<unsigned short> hPslice;   // phase slices of the split Bloch sum
     DBuf<unsigned short> dPslice;
@@ -196,7 +216,10 @@ struct olcao_ctx {
     long long imageEntries = 0;
     double listMs = 0.0;
     DBuf<double> dBlocks, dX, dCoef;
-    DBuf<double> S, Aw, Bw, D, E, CV_S, CC_S, Tmp;
+    DBuf<double> S, Aw, Bw, CV_S, CC_S, Tmp;
+    // per spin: eigenvalues, tridiagonal work, solver and reduction status
+    DBuf<double> D[2], E[2];
+    DBuf<int> dInfo[2], tInfo[2];
     // eigenvectors of spin 2 (spin 1's stay in Aw) and every spin's
     // eigenvalues [nk][n] from the last solve, for the valence density
     DBuf<double> Aw2;
@@ -208,9 +231,8 @@ struct olcao_ctx {
     DBuf<const double *> dTrPtr;
     // the batched call's grouped solves: several cells' matrices in one batch
     DBuf<double> gA, gB, gD, gE;
-    DBuf<int> gInfo;
+    DBuf<int> gInfo, gTInfo;
     DBuf<double> H[2], CV_H[2], CC_H[2];   // per spin
-    DBuf<int> dInfo;
     DBuf<unsigned long long> dCounters;
     // Mode A term matrices (orthogonalised when coreDim > 0)
     DBuf<double> KEm, NPm, MVm;
@@ -1111,7 +1133,7 @@ void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, 
         ew.dcRoot.upload(ew.dc.rootTiles, s);
         ew.dcCol.upload(ew.dc.colTiles, s);
     }
-    HIPCHK(hipMemsetAsync(ew.cnt.p, 0, plan.counters * sizeof(unsigned), s));
+    launch_zero_words(ew.cnt.p, (long long)plan.counters, s);
     HIPCHK(launch_trd(cplx, plan, n, nb, A, ew.V.p, D, E, ew.tau.p, ew.pub.p, ew.cnt.p, dinfo, s));
     ew.dcStD.alloc(dc_state_doubles(n, nb));
     ew.dcStI.alloc(dc_state_ints(n, nb));
@@ -1190,7 +1212,7 @@ void chol_blocked(rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, dou
                   double *Dinv, int *dinfo)
 {
     const int el = cplx ? 2 : 1;
-    HIPCHK(hipMemsetAsync(dinfo, 0, nb * sizeof(int), s));
+    launch_zero_words(dinfo, nb, s);
     for (int j0 = 0; j0 < n; j0 += kCholBlock) {
         const int jb = std::min(kCholBlock, n - j0), rest = n - j0 - jb;
         if (j0 > 0)
@@ -1256,6 +1278,29 @@ void solve_back(rocblas_handle h, bool cplx, int n, int nb, const double *U, con
 // Whether the tridiagonal path takes (n, nb), and its launch plan.
 bool trd_fits(int n, int nb, bool cplx, TrdPlan &plan) { return trd_plan(n, nb, cplx, plan); }
 
+// zhegv-style status of the last gen_eig from its device info arrays
+// (dinfo: the LDS solver's or the Cholesky's or rocSOLVER's; tinfo: the
+// tridiagonal reduction's): 0 ok, > n not positive definite, 1..n no
+// convergence.
+void read_info(const EigWork &ew, const int *dinfo, const int *tinfo, int n, int nb,
+               hipStream_t s, std::vector<int> &info)
+{
+    std::vector<int> a(nb, 0), b(nb, 0);
+    HIPCHK(hipMemcpyAsync(a.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (ew.lastPath == 1)
+        HIPCHK(hipMemcpyAsync(b.data(), tinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    info.assign(nb, 0);
+    for (int k = 0; k < nb; ++k) {
+        if (ew.lastPath == 0)
+            info[k] = a[k] < 0 ? 1 : a[k];   // Jacobi did not converge: reported in 1..n
+        else if (ew.lastPath == 1)
+            info[k] = a[k] != 0 ? a[k] + n : (b[k] == -2 ? 1 : 0);   // -2: a group barrier timed out
+        else
+            info[k] = a[k];
+    }
+}
+
 // Batched generalised eigensolve A x = w B x from the upper triangles
 // (what zhegv/dsygv with itype 1, 'V', 'U' compute, interfaces.F90:73-200):
 // B = U^H U, C = U^-H A U^-1 (triangular solves on the Hermitian-filled A),
@@ -1265,50 +1310,58 @@ bool trd_fits(int n, int nb, bool cplx, TrdPlan &plan) { return trd_plan(n, nb, 
 // small n (everything in ksolve.hip); the device path where the tridiagonal
 // reduction fits (blocked Cholesky, GEMM triangular solves, trd_eig); else,
 // and always when library is set, rocSOLVER (potrf, trsm, heevd).
+// defer: enqueue only, with no host synchronisation (graph capture); the
+// caller reads the status with read_info once the work has run.  The
+// library path synchronises inside and is refused then.
 void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *A,
-             double *B, double *D, double *E, int *dinfo, std::vector<int> &info, bool library)
+             double *B, double *D, double *E, int *dinfo, int *tinfo, std::vector<int> &info,
+             bool library, bool defer = false)
 {
     const rocblas_stride st = (rocblas_stride)n * n;
     info.assign(nb, 0);
     if (!library && small_hegv_fits(n, cplx)) {
         // one workgroup per matrix, everything in LDS (ksolve.hip)
+        ew.lastPath = 0;
         launch_small_hegv(cplx, n, nb, A, B, D, dinfo, s);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        for (int k = 0; k < nb; ++k)
-            if (info[k] < 0) info[k] = 1;   // Jacobi did not converge: reported as zhegv info in 1..n
+        if (!defer) read_info(ew, dinfo, tinfo, n, nb, s, info);
         return;
     }
     TrdPlan plan;
     if (!library && trd_fits(n, nb, cplx, plan)) {
         // blocked Cholesky, GEMM triangular solves, cooperative reduction,
         // divide and conquer, blocked back-transforms (all batched)
+        ew.lastPath = 1;
         const int el = cplx ? 2 : 1;
         ew.Dinv.alloc((size_t)n * n * el * nb);
         ew.Y.alloc((size_t)n * n * el * nb);
         ew.Uc.alloc((size_t)n * n * el * nb);
         chol_blocked(h, s, cplx, n, nb, B, ew.Uc.p, ew.Dinv.p, dinfo);
-        HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        for (int k = 0; k < nb; ++k)
-            if (info[k] != 0) {
-                info[k] += n;   // zhegv's convention: B not positive definite
-                return;
-            }
+        if (!defer) {
+            // stop before the reduction when some S(k) is not positive
+            // definite, as zhegv does
+            HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (int k = 0; k < nb; ++k)
+                if (info[k] != 0) {
+                    for (int q = 0; q < nb; ++q) info[q] = info[q] != 0 ? info[q] + n : 0;
+                    return;
+                }
+        }
         launch_herm_fill(A, n, nb, el, s);
         solve_left_h(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
         solve_right(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, ew.Y.p, A);
-        trd_eig(ew, plan, h, s, cplx, n, nb, A, D, E, dinfo);
+        trd_eig(ew, plan, h, s, cplx, n, nb, A, D, E, tinfo);
         solve_back(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
         HIPCHK(hipMemcpyAsync(A, ew.Y.p, (size_t)n * n * el * nb * sizeof(double),
                               hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(info.data(), dinfo, nb * sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        for (int k = 0; k < nb; ++k)
-            if (info[k] == -2) info[k] = 1;   // a group barrier timed out
+        if (!defer) read_info(ew, dinfo, tinfo, n, nb, s, info);
         return;
     }
+    if (defer)
+        fail(OLCAO_ERR_UNSUPPORTED, "the library eigensolver (n = %d) synchronises inside and is "
+                                    "not captured in a graph", n);
+    ew.lastPath = 2;
     rocblas_status r;
     if (cplx)
         r = rocsolver_zpotrf_strided_batched(h, rocblas_fill_upper, n, (rocblas_double_complex *)B, n,
@@ -1365,28 +1418,38 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
     HIPCHK(hipStreamSynchronize(s));
 }
 
-// Batched generalised eigensolve of H, S (upper) on the device.
-void solve_device(olcao_ctx &c, int q, int numStates, const double *Hsrc, const double *Ssrc,
-                  double *evals, double *evecs)
+// Batched generalised eigensolve of H, S (upper) on the device, spin q:
+// solve_enqueue queues the copies and the solve (defer: with no host
+// synchronisation, for graph capture; info is then read by the caller with
+// read_info), solve_finish checks the status and returns the eigenpairs.
+void solve_enqueue(olcao_ctx &c, int q, const double *Hsrc, const double *Ssrc, bool defer,
+                   std::vector<int> &info)
 {
     hipStream_t s = c.stream;
     const int n = c.valeDim, nk = c.nk;
-    if (numStates <= 0 || numStates > n) fail(OLCAO_ERR_INPUT, "numStates must lie in 1..%d", n);
     // spin q's eigenvectors stay in Aw (q = 0) or Aw2 (q = 1) for the density
     DBuf<double> &Aq = q ? c.Aw2 : c.Aw;
     c.vecsValid[q] = false;
     Aq.alloc(c.matDoubles());
     c.Bw.alloc(c.matDoubles());
-    c.D.alloc((size_t)n * nk);
-    c.E.alloc((size_t)n * nk);
-    c.dInfo.alloc(nk);
+    c.D[q].alloc((size_t)n * nk);
+    c.E[q].alloc((size_t)n * nk);
+    c.dInfo[q].alloc(nk);
+    c.tInfo[q].alloc(nk);
     HIPCHK(hipMemcpyAsync(Aq.p, Hsrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
-    std::vector<int> info;
-    gen_eig(c.eig, c.blas, s, !c.gamma, n, nk, Aq.p, c.Bw.p, c.D.p, c.E.p, c.dInfo.p, info,
-            c.eigLibrary);
+    gen_eig(c.eig, c.blas, s, !c.gamma, n, nk, Aq.p, c.Bw.p, c.D[q].p, c.E[q].p, c.dInfo[q].p,
+            c.tInfo[q].p, info, c.eigLibrary, defer);
+}
+
+void solve_finish(olcao_ctx &c, int q, int numStates, const std::vector<int> &info,
+                  double *evals, double *evecs)
+{
+    hipStream_t s = c.stream;
+    const int n = c.valeDim, nk = c.nk;
+    DBuf<double> &Aq = q ? c.Aw2 : c.Aw;
     std::vector<double> w((size_t)n * nk);
-    HIPCHK(hipMemcpyAsync(w.data(), c.D.p, w.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(w.data(), c.D[q].p, w.size() * 8, hipMemcpyDeviceToHost, s));
     if (evecs) {
         DBuf<double> tmp;
         const int cp = c.gamma ? 1 : 2;
@@ -1407,6 +1470,16 @@ void solve_device(olcao_ctx &c, int q, int numStates, const double *Hsrc, const 
     c.vecsValid[q] = true;
 }
 
+void solve_device(olcao_ctx &c, int q, int numStates, const double *Hsrc, const double *Ssrc,
+                  double *evals, double *evecs)
+{
+    if (numStates <= 0 || numStates > c.valeDim)
+        fail(OLCAO_ERR_INPUT, "numStates must lie in 1..%d", c.valeDim);
+    std::vector<int> info;
+    solve_enqueue(c, q, Hsrc, Ssrc, false, info);
+    solve_finish(c, q, numStates, info, evals, evecs);
+}
+
 thread_local std::string g_noctx_err;   // message of context-free calls
 std::atomic<int> g_streamRotation{0};   // contexts created so far (stream placement)
 
@@ -1414,6 +1487,12 @@ template <class F>
 olcao_status guard(olcao_ctx *c, F &&f)
 {
     std::string &msg = c ? c->err : g_noctx_err;
+    // allocations inside this entry point count against c's graph
+    struct EpochScope {
+        std::atomic<unsigned long long> *prev;
+        explicit EpochScope(olcao_ctx *c) : prev(tl_allocEpoch) { tl_allocEpoch = c ? &c->allocEpoch : nullptr; }
+        ~EpochScope() { tl_allocEpoch = prev; }
+    } scope(c);
     try {
         f();
         msg.clear();
@@ -1446,6 +1525,7 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
         c->precision = opt ? opt->precision : 0;
         c->eigLibrary = opt && opt->eigensolver == OLCAO_EIG_LIBRARY;
         c->blochMfma = !(opt && opt->bloch == OLCAO_BLOCH_VALU);
+        c->useGraph = opt && opt->graph != 0;
         if (opt && (opt->bloch < OLCAO_BLOCH_AUTO || opt->bloch > OLCAO_BLOCH_MFMA))
             fail(OLCAO_ERR_INPUT, "bloch must be OLCAO_BLOCH_AUTO, _VALU or _MFMA");
         if (c->precision != OLCAO_PREC_FP64 && c->precision != OLCAO_PREC_MIXED)
@@ -1489,6 +1569,8 @@ void olcao_ctx_destroy(olcao_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto q : c->side)
         if (q) (void)hipStreamSynchronize(q);
+    if (c->graphExec) (void)hipGraphExecDestroy(c->graphExec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->blas) rocblas_destroy_handle(c->blas);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1799,14 +1881,16 @@ olcao_status olcao_vale_charge(olcao_ctx *c, int numStates, const double *occupa
 // orthogonalisation, all queued on the context's streams.  (Starting S(k)'s
 // Cholesky factor on a stream of its own beside the Hamiltonian sweep was
 // measured: the sweep slowed by as much as the solve gained.)
-static void iteration_build(olcao_ctx &c, const double *coef)
+// The device work of the build, everything after the coefficient upload;
+// events: record the stage events (not inside a graph capture).
+static void iteration_device(olcao_ctx &c, bool events)
 {
-    ensure_mats(c);
     hipStream_t s = c.stream;
-    HIPCHK(hipMemcpyAsync(c.dCoef.p, coef, (size_t)c.potDim * c.spin * 8, hipMemcpyHostToDevice, s));
-    HIPCHK(hipEventRecord(c.ev[0], s));
+    auto mark = [&](int e) {
+        if (events) HIPCHK(hipEventRecord(c.ev[e], s));
+    };
     overlap_sweep(c);
-    HIPCHK(hipEventRecord(c.ev[1], s));
+    mark(1);
     // events: 0 start, 1 overlap done; per spin q: 2+3q before the
     // three-centre sweep, 3+3q after it, 4+3q after contraction, Bloch
     // sum and orthogonalisation; 8 after the solve
@@ -1817,14 +1901,28 @@ static void iteration_build(olcao_ctx &c, const double *coef)
         two_center_forked(c, OP_KE, 0, c.rel ? 0xffffu : ~kFusedKEClasses & 0xffffu);
         if (c.rel) two_center_forked(c, OP_MV, 1);
         launch_fill(reinterpret_cast<double *>(c.dCounters.p), 32, 0.0, s);
-        HIPCHK(hipEventRecord(c.ev[2 + 3 * q], s));
+        mark(2 + 3 * q);
         three_center_forked(c, K3_HAM, 0, 0, c.dCoef.p + (size_t)q * c.potDim, c.rel ? 1 : 1 | kAccAddKE);
-        HIPCHK(hipEventRecord(c.ev[3 + 3 * q], s));
+        mark(3 + 3 * q);
         contract_scatter(c, c.H[q].p, c.CV_H[q].p, c.CC_H[q].p);
         ortho(c, c.H[q].p, c.CV_H[q].p, c.CC_H[q].p, false);
-        HIPCHK(hipEventRecord(c.ev[4 + 3 * q], s));
+        mark(4 + 3 * q);
     }
     c.haveH = true;
+}
+
+static void upload_coef(olcao_ctx &c, const double *coef)
+{
+    ensure_mats(c);
+    HIPCHK(hipMemcpyAsync(c.dCoef.p, coef, (size_t)c.potDim * c.spin * 8, hipMemcpyHostToDevice,
+                          c.stream));
+}
+
+static void iteration_build(olcao_ctx &c, const double *coef)
+{
+    upload_coef(c, coef);
+    HIPCHK(hipEventRecord(c.ev[0], c.stream));
+    iteration_device(c, true);
 }
 
 // Second half: the solve for all k and spins, then the stage times.
@@ -1849,6 +1947,7 @@ static void record_timings(olcao_ctx &c, double solveMs)
     c.timings[3] = solveMs < 0 ? el(1 + 3 * c.spin, 8) : solveMs;
     c.timings[4] = solveMs < 0 ? el(0, 8) : el(0, 1 + 3 * c.spin) + solveMs;
     c.timings[5] = t3c;
+    c.timings[6] = 0.0;
 }
 
 static void iteration_solve(olcao_ctx &c, int numStates, double *evals)
@@ -1858,6 +1957,80 @@ static void iteration_solve(olcao_ctx &c, int numStates, double *evals)
     HIPCHK(hipEventRecord(c.ev[8], s));
     HIPCHK(hipEventSynchronize(c.ev[8]));
     record_timings(c, -1.0);
+    c.lastReplay = false;
+}
+
+// Capture the device work of one Mode-B iteration (build and the solve of
+// every spin, with no host synchronisation) into a graph, right after an
+// eager iteration allocated every buffer it uses.  The coefficients are
+// uploaded before each replay and the status and eigenvalues read after
+// it.  A graph with a memset node is not kept (such nodes have given wrong
+// results on replays); a capture that fails leaves the context eager and
+// the reason in graphErr.
+static void capture_iteration(olcao_ctx &c)
+{
+    hipStream_t s = c.stream;
+    hipGraph_t g = nullptr;
+    try {
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            iteration_device(c, false);
+            std::vector<int> info;
+            for (int q = 0; q < c.spin; ++q) solve_enqueue(c, q, c.H[q].p, c.S.p, true, info);
+        } catch (...) {
+            hipGraph_t junk = nullptr;
+            (void)hipStreamEndCapture(s, &junk);
+            if (junk) (void)hipGraphDestroy(junk);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(s, &g));
+        size_t nn = 0;
+        HIPCHK(hipGraphGetNodes(g, nullptr, &nn));
+        std::vector<hipGraphNode_t> nodes(nn);
+        HIPCHK(hipGraphGetNodes(g, nodes.data(), &nn));
+        for (hipGraphNode_t nd : nodes) {
+            hipGraphNodeType t;
+            HIPCHK(hipGraphNodeGetType(nd, &t));
+            if (t == hipGraphNodeTypeMemset) fail(OLCAO_ERR_UNSUPPORTED, "the captured iteration holds a memset node");
+        }
+        hipGraphExec_t ex = nullptr;
+        HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        if (c.graphExec) (void)hipGraphExecDestroy(c.graphExec);
+        if (c.graph) (void)hipGraphDestroy(c.graph);
+        c.graph = g;
+        c.graphExec = ex;
+        c.graphEpoch = c.allocEpoch.load();
+    } catch (const Fail &e) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        c.graphFailed = true;
+        c.graphErr = e.msg;
+    }
+}
+
+// One Mode-B iteration by replaying the captured graph.
+static void iteration_replay(olcao_ctx &c, const double *coef, int numStates, double *evals)
+{
+    hipStream_t s = c.stream;
+    upload_coef(c, coef);
+    HIPCHK(hipEventRecord(c.ev[0], s));
+    HIPCHK(hipGraphLaunch(c.graphExec, s));
+    HIPCHK(hipEventRecord(c.ev[8], s));
+    const size_t ev = (size_t)c.nk * numStates;
+    for (int q = 0; q < c.spin; ++q) {
+        std::vector<int> info;
+        read_info(c.eig, c.dInfo[q].p, c.tInfo[q].p, c.valeDim, c.nk, s, info);
+        solve_finish(c, q, numStates, info, evals ? evals + q * ev : nullptr, nullptr);
+    }
+    HIPCHK(hipEventSynchronize(c.ev[8]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev[0], c.ev[8]));
+    for (double &t : c.timings) t = 0.0;
+    c.timings[4] = ms;
+    c.timings[6] = 1.0;
+    c.haveS = c.haveH = true;
+    c.lastReplay = true;
 }
 
 // One generalised solve over several cells' matrices of the same size and
@@ -1885,6 +2058,7 @@ static void grouped_solve(const std::vector<SolveMember> &mem, olcao_ctx *const 
     h.gD.alloc((size_t)n * nbTot);
     h.gE.alloc((size_t)n * nbTot);
     h.gInfo.alloc(nbTot);
+    h.gTInfo.alloc(nbTot);
     size_t off = 0;
     for (const auto &m : mem) {
         const olcao_ctx &c = *ctxs[m.cell];
@@ -1894,7 +2068,7 @@ static void grouped_solve(const std::vector<SolveMember> &mem, olcao_ctx *const 
     }
     HIPCHK(hipEventRecord(h.ev[9], s));
     std::vector<int> info;
-    gen_eig(h.eig, h.blas, s, cplx, n, nbTot, h.gA.p, h.gB.p, h.gD.p, h.gE.p, h.gInfo.p, info,
+    gen_eig(h.eig, h.blas, s, cplx, n, nbTot, h.gA.p, h.gB.p, h.gD.p, h.gE.p, h.gInfo.p, h.gTInfo.p, info,
             h.eigLibrary);
     HIPCHK(hipEventRecord(h.ev[10], s));
     std::vector<double> w((size_t)n * nbTot);
@@ -1949,8 +2123,15 @@ olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates
 {
     if (!c || !coef) return OLCAO_ERR_INPUT;
     return guard(c, [&] {
+        if (numStates <= 0 || numStates > c->valeDim)
+            fail(OLCAO_ERR_INPUT, "numStates must lie in 1..%d", c->valeDim);
+        if (c->useGraph && c->graphExec && c->graphEpoch == c->allocEpoch.load()) {
+            iteration_replay(*c, coef, numStates, evals);
+            return;
+        }
         iteration_build(*c, coef);
         iteration_solve(*c, numStates, evals);
+        if (c->useGraph && !c->graphFailed) capture_iteration(*c);
     });
 }
 
@@ -1975,6 +2156,9 @@ olcao_status olcao_scf_iteration_batch(int nCells, olcao_ctx *const *ctxs,
             return OLCAO_ERR_INPUT;
         }
     nWorkers = std::max(1, std::min(nWorkers, 16));
+    // the grouped solves below allocate on behalf of several contexts: a
+    // captured iteration graph of any of them is captured again later
+    for (int i = 0; i < nCells; ++i) ctxs[i]->graphEpoch = 0;
     // heaviest cells first: three-centre image entries plus the solve's n^3
     std::vector<int> order(nCells);
     std::vector<double> weight(nCells);
@@ -2066,6 +2250,19 @@ olcao_status olcao_get_work_counts(olcao_ctx *c, unsigned long long counts[32])
     });
 }
 
+olcao_status olcao_get_graph_state(const olcao_ctx *c, int *state, const char **reason)
+{
+    if (!c || !state) return OLCAO_ERR_INPUT;
+    if (!c->useGraph)
+        *state = 0;
+    else if (c->graphFailed)
+        *state = 3;
+    else
+        *state = c->graphExec && c->graphEpoch == c->allocEpoch.load() ? 1 : 2;
+    if (reason) *reason = c->graphErr.c_str();
+    return OLCAO_OK;
+}
+
 olcao_status olcao_get_timings(const olcao_ctx *c, double t[8])
 {
     if (!c || !t) return OLCAO_ERR_INPUT;
@@ -2102,14 +2299,15 @@ static olcao_status solve_host(int n, int nb, double *A, double *B, double *w, b
             fail(OLCAO_ERR_HIP, "rocblas handle set-up failed");
         const size_t per = (size_t)n * n * (cplx ? 2 : 1);
         DBuf<double> dA, dB, dD, dE;
-        DBuf<int> dI;
+        DBuf<int> dI, dTI;
         dA.alloc(per * nb); dB.alloc(per * nb); dD.alloc((size_t)n * nb); dE.alloc((size_t)n * nb);
         dI.alloc(nb);
+        dTI.alloc(nb);
         HIPCHK(hipMemcpyAsync(dA.p, A, per * nb * 8, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(dB.p, B, per * nb * 8, hipMemcpyHostToDevice, s));
         std::vector<int> info;
         EigWork ew;
-        gen_eig(ew, h, s, cplx, n, nb, dA.p, dB.p, dD.p, dE.p, dI.p, info, false);
+        gen_eig(ew, h, s, cplx, n, nb, dA.p, dB.p, dD.p, dE.p, dI.p, dTI.p, info, false);
         HIPCHK(hipMemcpyAsync(A, dA.p, per * nb * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(w, dD.p, (size_t)n * nb * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));

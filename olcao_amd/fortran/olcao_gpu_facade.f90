@@ -74,7 +74,8 @@ module O_GPUFacade
       integer(c_int) :: precision
       integer(c_int) :: eigensolver
       integer(c_int) :: bloch
-      integer(c_int) :: reserved(5)
+      integer(c_int) :: graph
+      integer(c_int) :: reserved(4)
    end type olcao_options_t
 
    type(c_ptr), save :: ctx = c_null_ptr
@@ -241,6 +242,7 @@ contains
       o%precision = 0
       o%eigensolver = 0
       o%bloch = 0
+      o%graph = 0
       o%reserved = 0
       st = c_ctx_create(s, o, ctx)
       if (st /= 0) call gpuStop(st, 'gpuInit')

@@ -58,7 +58,7 @@ class _System(C.Structure):
 
 class _Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("eigensolver", C.c_int), ("bloch", C.c_int),
-                ("reserved", C.c_int * 5)]
+                ("graph", C.c_int), ("reserved", C.c_int * 4)]
 
 
 EIG_AUTO, EIG_LIBRARY = 0, 1
@@ -87,6 +87,7 @@ FUNCTIONS = {
     "olcao_scf_iteration_batch": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_int, C.c_void_p, C.c_void_p],
     "olcao_get_timings": [C.c_void_p, C.c_void_p],
+    "olcao_get_graph_state": [C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_get_work_counts": [C.c_void_p, C.c_void_p],
     "olcao_solve_zhegv": [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_solve_dsygv": [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
@@ -178,7 +179,8 @@ def boys_batch(T):
 class OlcaoGPU:
     """One device context for one system (setupSCF's derived state)."""
 
-    def __init__(self, sysin, precision=PREC_FP64, eigensolver=EIG_AUTO, bloch=BLOCH_AUTO):
+    def __init__(self, sysin, precision=PREC_FP64, eigensolver=EIG_AUTO, bloch=BLOCH_AUTO,
+                 graph=False):
         self.sysin = sysin
         f = sysin.flat()
         self._keep = f
@@ -203,6 +205,7 @@ class OlcaoGPU:
         opt.precision = precision
         opt.eigensolver = eigensolver
         opt.bloch = bloch
+        opt.graph = 1 if graph else 0
         ctx = C.c_void_p()
         code = lib().olcao_ctx_create(C.byref(s), C.byref(opt), C.byref(ctx))
         _check(code, None)
@@ -377,7 +380,16 @@ class OlcaoGPU:
         t = np.zeros(8)
         _check(lib().olcao_get_timings(self.ctx, _p(t)), self.ctx)
         return dict(overlap_ms=t[0], hamiltonian_ms=t[1], contract_bloch_ms=t[2],
-                    solve_ms=t[3], total_ms=t[4], three_center_ms=t[5])
+                    solve_ms=t[3], total_ms=t[4], three_center_ms=t[5], graph_replay=t[6] > 0)
+
+    def graph_state(self):
+        """(state, reason): 0 graph mode off, 1 a captured graph is current,
+        2 waiting for the next iteration to capture, 3 capture refused
+        (reason says why)."""
+        st = C.c_int(0)
+        why = C.c_char_p()
+        _check(lib().olcao_get_graph_state(self.ctx, C.byref(st), C.byref(why)), self.ctx)
+        return st.value, (why.value or b"").decode()
 
 
 def scf_iteration_batch(cells, num_states, workers=8):
