@@ -95,7 +95,7 @@ typedef struct {
 
 /* Generalised eigensolve: OLCAO_EIG_AUTO uses the batched in-LDS solver
  * (one workgroup per k-point: Cholesky, reduction, Jacobi) where the matrices
- * fit (complex n <= 69, real n <= 98), else the device tridiagonal path
+ * fit (complex n <= 70, real n <= 99), else the device tridiagonal path
  * (blocked Cholesky, cooperative Householder reduction, divide and conquer,
  * blocked back-transform) up to n = 640, and rocSOLVER beyond;
  * OLCAO_EIG_LIBRARY always uses rocSOLVER. */
@@ -276,6 +276,19 @@ olcao_status olcao_get_work_counts(olcao_ctx *ctx, unsigned long long counts[32]
 olcao_status olcao_solve_zhegv(int n, int nbatch, double *A, double *B, double *w);
 /* Real twin, replaces solveDSYGV (interfaces.F90:152-200). */
 olcao_status olcao_solve_dsygv(int n, int nbatch, double *A, double *B, double *w);
+
+/* olcao_solve_zhegv without device allocation.  It has no counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as olcao_solve_zhegv (host buffers), for n <= 70, the batched
+ * in-LDS solver's range; the batch is staged in a fixed 64 MiB device array
+ * per GPU (at most 64 MiB / ((4 n^2 + n + 1) * 8 B) matrices per call,
+ * e.g. 645 at n = 57), larger batches are refused with OLCAO_ERR_INPUT.
+ * The array is shared by every call on a GPU and keeps the last call's
+ * contents: calls that overlap on one GPU (from several host threads)
+ * race.  Runs on the null stream and synchronises with the host before it
+ * returns; the message of a failure is olcao_last_error(NULL). */
+olcao_status olcao_solve_zhegv_noalloc(int n, int nbatch, double *A, double *B, double *w);
 
 /* Primitive integral blocks on the device, for parity tests of the kernels'
  * device functions against the reference subroutines overlap2CIntg

@@ -37,6 +37,7 @@ UNITS = [("kernels.hip", "kernels.o", []),
          ("kdc.hip", "kdc.o", []),
          ("kchol.hip", "kchol.o", []),
          ("kbloch.hip", "kbloch.o", []),
+         ("noalloc.hip", "noalloc.o", []),
          ("runtime.cpp", "runtime.o", ["-x", "hip"])]
 
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <vector>
 #include <stdint.h>
 
@@ -117,6 +118,9 @@ inline __host__ __device__ int img_nham(unsigned long long e) { return (int)((e 
 inline __host__ __device__ int img_nep(unsigned long long e) { return (int)((e >> 48) & 63); }
 inline __host__ __device__ int img_nucham(unsigned long long e) { return (int)((e >> 54) & 1); }
 inline __host__ __device__ int img_nucnp(unsigned long long e) { return (int)((e >> 55) & 1); }
+
+// The message olcao_last_error(NULL) returns to the calling thread.
+void set_noctx_error(const std::string &msg);
 
 // Kernel launchers (kernels.hip).  All asynchronous on `s`.
 // The image list of a record is cut into chunks of about kChunkItems

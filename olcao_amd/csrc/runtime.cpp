@@ -1508,6 +1508,8 @@ olcao_status guard(olcao_ctx *c, F &&f)
 
 }  // namespace
 
+void olcao::set_noctx_error(const std::string &msg) { g_noctx_err = msg; }
+
 // ===========================================================================
 extern "C" {
 

@@ -91,6 +91,7 @@ FUNCTIONS = {
     "olcao_get_work_counts": [C.c_void_p, C.c_void_p],
     "olcao_solve_zhegv": [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_solve_dsygv": [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
+    "olcao_solve_zhegv_noalloc": [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_prim_batch": [C.c_int, C.c_int] + [C.c_void_p] * 6,
     "olcao_boys_batch": [C.c_int, C.c_void_p, C.c_void_p],
 }
