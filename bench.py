@@ -8,9 +8,14 @@ generalised eigensolve for every k-point.  Inputs are resident on the
 device before the timed region (the system and screened work lists are
 built once, as the reference's setup does).
 
-Both forms run: the plain FP64 drop-in and the chosen mixed form (the
-three-Gaussian exponentials by an FP64 range reduction and the FP32 hardware
-exponential); `value` is the mixed form, the FP64 time and the mixed form's
+Two forms run in the same process:
+  plain   the FP64 drop-in with the default tile shapes, each iteration
+          launched kernel by kernel;
+  chosen  the mixed form (split BF16 matrix-core Bloch sums, FP32
+          exponentials of distant images) with the autotuned tile shapes
+          (olcao_amd/tuning.json), each iteration replayed from a captured
+          graph.
+`value` is the chosen form; the plain form's time and the chosen form's
 eigenvalue and band-energy errors against it are reported beside it.
 
 Workload: si64 -- Si64 2x2x2 diamond supercell (a = 5.431 A, 0.05 A seeded
@@ -18,10 +23,13 @@ jitter), s+p+d per Si from 24 even-tempered exponents 0.05..1e7, 16
 potential Gaussians 0.1..1e5 per site, 2x2x2 Monkhorst-Pack mesh (8 k),
 576 orbitals; synthetic seeded data.
 
-The JSON line carries the roofline of the dominant kernel (the three-centre
-sweep: algorithmic FP64 operations per launch from olcao_amd.workmodel over
-the device's own evaluation counters, divided by its HIP-event time) and a
-CPU baseline: the oracle (a C restatement of the reference algorithm) on one
+The JSON line carries the roofline of the largest kernel of the iteration,
+the tridiagonal reduction k_trd ((16/3) n^3 real flops per complex matrix,
+timed by HIP events around its launch on the context stream), the
+three-centre sweep's beside it (FP64 operations of the kernels' own
+McMurchie-Davidson formulation, olcao_amd.workmodel, over the device's
+evaluation counters, divided by the sweep's HIP-event time), and a CPU
+baseline: the oracle (a C restatement of the reference algorithm) on one
 host core over a bounded sample of atom pairs, extrapolated by the
 three-centre work ratio, plus LAPACK zhegv for all k.
 """
@@ -55,6 +63,9 @@ FP64_PEAK_TFLOPS = 78.6
 # inside a timed run, so the figure is carried here; other workloads report
 # null.
 SWEEP_HBM_BYTES = {"si64": 6.5101e9}
+# HBM bytes of the tridiagonal reductions of one Si64 iteration (8 matrices),
+# measured the same way; null where not measured.
+TRD_HBM_BYTES = {}
 
 
 def cpu_baseline(sysin, total_triples, stride, threads=1):
@@ -96,29 +107,57 @@ def cpu_baseline(sysin, total_triples, stride, threads=1):
     return est, sample
 
 
-def run_form(s, precision, steps, warmup, label):
-    """Time `steps` Mode-B iterations of one form after `warmup` untimed ones."""
+def _form_args(chosen):
+    from olcao_amd import gpu
+    return dict(precision=gpu.PREC_MIXED if chosen else gpu.PREC_FP64,
+                tuning="table" if chosen else None)
+
+
+def run_form(s, steps, warmup, label, chosen):
+    """Time `steps` Mode-B iterations of one form after `warmup` untimed
+    ones (the chosen form as graph replays).  Nothing but the iterations
+    runs in the timed region."""
     import numpy as np
 
     from olcao_amd import gpu
-    g = gpu.OlcaoGPU(s, precision=precision)
+    g = gpu.OlcaoGPU(s, graph=chosen, **_form_args(chosen))
     ns = g.vale_dim
     for w in range(warmup):
         g.scf_iteration(s.pot_coeffs, ns)
-        print(f"[bench] {label} warmup {w + 1}/{warmup} {g.timings()}", flush=True)
-    stage = []
+        print(f"[bench] {label} warmup {w + 1}/{warmup}", flush=True)
+    gstate = g.graph_state()
     t0 = time.perf_counter()
     for k in range(steps):
         evals = g.scf_iteration(s.pot_coeffs, ns)
-        stage.append(g.timings())
-        print(f"[bench] {label} step {k + 1}/{steps} {stage[-1]}", flush=True)
     t1 = time.perf_counter()
-    counts = g.work_counts()
-    info = dict(sec=(t1 - t0) / steps, evals=np.asarray(evals), counts=counts,
-                stage={k: float(np.mean([x[k] for x in stage])) for k in stage[0]},
+    last = g.timings()
+    print(f"[bench] {label}: {steps} steps, {(t1 - t0) / steps * 1e3:.3f} ms per step, graph "
+          f"state {gstate[0]} {gstate[1]}, last step {last['total_ms']:.3f} ms on the device",
+          flush=True)
+    info = dict(sec=(t1 - t0) / steps, evals=np.asarray(evals), tuning=g.tuning,
+                graph=bool(last["graph_replay"]),
                 dims=(g.vale_dim, g.nk, g.n_pair_cells, g.n_prim_pairs))
     g.close()
     return info
+
+
+def time_stages(s, steps, chosen):
+    """Stage and kernel times (HIP events on the context stream) of one form
+    over `steps` eager iterations after one: the mean of each."""
+    import numpy as np
+
+    from olcao_amd import gpu
+    g = gpu.OlcaoGPU(s, **_form_args(chosen))
+    g.scf_iteration(s.pot_coeffs, g.vale_dim)
+    st = []
+    for _ in range(steps):
+        g.scf_iteration(s.pot_coeffs, g.vale_dim)
+        st.append(g.timings())
+    out = {k: float(np.mean([x[k] for x in st])) for k in st[0] if k != "graph_replay"}
+    out["counts"] = g.work_counts()
+    out["class_pairs"] = g.class_pairs()
+    g.close()
+    return out
 
 
 def run_batch_form(cells, precision, steps, warmup, label, workers):
@@ -130,7 +169,8 @@ def run_batch_form(cells, precision, steps, warmup, label, workers):
     ctxs = []
     t0 = time.perf_counter()
     for s in cells:
-        ctxs.append(gpu.OlcaoGPU(s, precision=precision))
+        ctxs.append(gpu.OlcaoGPU(s, precision=precision,
+                                 tuning="table" if precision == gpu.PREC_MIXED else None))
     print(f"[bench] {label}: {len(ctxs)} contexts set up in {time.perf_counter() - t0:.1f} s",
           flush=True)
     ns = [g.vale_dim for g in ctxs]
@@ -140,7 +180,7 @@ def run_batch_form(cells, precision, steps, warmup, label, workers):
         g.scf_iteration(g.sysin.pot_coeffs, n)
         t = g.timings()
         single.append(t["total_ms"])
-        flops += workmodel.sweep_flops(g.work_counts())
+        flops += workmodel.sweep_flops(g.work_counts(), g.class_pairs())
         t3c += t["three_center_ms"]
     print(f"[bench] {label}: isolated per-cell ms p50 {np.percentile(single, 50):.2f} "
           f"p99 {np.percentile(single, 99):.2f} sum {sum(single):.1f}", flush=True)
@@ -187,7 +227,8 @@ def main_batch(args):
         "metric": METRIC, "value": mixed["sec"], "unit": "s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": mixed["sec"] * 1e3,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 (f32 exponential in the three-centre items)", "data": "synthetic (seeded)",
+        "dtype": "f64 (bf16-split Bloch sums with f32 accumulation, f32 exponentials of "
+                 "distant images)", "data": "synthetic (seeded)",
         "config": {"workload": WORKLOADS["uneven_batch"], "cells": len(cells),
                    "form": "mixed (value) beside plain FP64", "mode": "B, all cells per step",
                    "workers": args.workers,
@@ -266,60 +307,85 @@ def main():
 
     s = getattr(configs, args.workload)()
     print(f"[bench] setting up {s.name}: valeDim={s.vale_dim} nk={s.nk}", flush=True)
-    if args.mixed_only:
-        mixed = run_form(s, gpu.PREC_MIXED, args.steps, args.warmup, "mixed")
-        print(json.dumps({"workload": args.workload, "mixed_s_per_iter": mixed["sec"],
-                          "stage_ms_mixed": mixed["stage"]}), flush=True)
+    if args.mixed_only or args.fp64_only:   # counter runs: one form, eager
+        st = time_stages(s, args.steps, args.mixed_only)
+        print(json.dumps({"workload": args.workload, "form": "chosen" if args.mixed_only else "plain",
+                          "stage_ms": {k: v for k, v in st.items() if k.endswith("_ms")}}),
+              flush=True)
         return
-    fp64 = run_form(s, gpu.PREC_FP64, args.steps, args.warmup, "fp64")
-    if args.fp64_only:
-        print(json.dumps({"workload": args.workload, "fp64_s_per_iter": fp64["sec"],
-                          "stage_ms_fp64": fp64["stage"]}), flush=True)
-        return
-    mixed = run_form(s, gpu.PREC_MIXED, args.steps, args.warmup, "mixed")
+    plain = run_form(s, args.steps, args.warmup, "plain", False)
+    chosen = run_form(s, args.steps, args.warmup, "chosen", True)
+    st_plain = time_stages(s, 3, False)
+    st_chosen = time_stages(s, 3, True)
     HARTREE_EV = 27.211386245988
     n_el = sum(s.pot_types[t].Z for t in s.pot_type) - 2 * s.core_dim
     n_occ = min(s.vale_dim, int(round(n_el / 2)))
-    eig_err_mev = float(np.abs(mixed["evals"][:, :n_occ] - fp64["evals"][:, :n_occ]).max()) \
+    ev_c = chosen["evals"]
+    eig_err_mev = float(np.abs(ev_c[..., :n_occ] - plain["evals"][..., :n_occ]).max()) \
         * HARTREE_EV * 1e3
-    eig_err_all_mev = float(np.abs(mixed["evals"] - fp64["evals"]).max()) * HARTREE_EV * 1e3
-    de = abs(band_energy(mixed["evals"], s.kweights, n_occ) -
-             band_energy(fp64["evals"], s.kweights, n_occ)) / s.n_atoms
-    counts = mixed["counts"]
-    flops = workmodel.sweep_flops(counts)
-    tri = int(counts.sum())
-    t3c = mixed["stage"]["three_center_ms"] * 1e-3
-    achieved = flops / t3c / 1e12
-    vdim, nk, npc, npp = mixed["dims"]
+    eig_err_all_mev = float(np.abs(ev_c - plain["evals"]).max()) * HARTREE_EV * 1e3
+    de = abs(band_energy(ev_c, s.kweights, n_occ) -
+             band_energy(plain["evals"], s.kweights, n_occ)) / s.n_atoms
+    vdim, nk, npc, npp = plain["dims"]
+    # roofline of the largest kernel: the tridiagonal reduction
+    cplx = not s.gamma
+    trd_ms = st_plain["trd_ms"]
+    trd_flops = workmodel.hetrd_flops(vdim, nk * s.spin, cplx)
+    # the three-centre sweep, by the kernels' own formulation
+    counts = st_plain["counts"]
+    sweep_flops = workmodel.sweep_flops(counts, st_plain["class_pairs"])
+    t3c = st_plain["three_center_ms"]
+    tri = int(np.asarray(counts).sum())
+    if trd_ms > 0:
+        roof = {"bound": "mfma", "achieved": trd_flops / (trd_ms * 1e-3) / 1e12,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": trd_flops / (trd_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                "traffic": TRD_HBM_BYTES.get(args.workload),
+                "kernel": f"k_trd (tridiagonal reduction, {nk * s.spin} x n={vdim} "
+                          f"{'complex' if cplx else 'real'}; FP64 vector work, whose peak equals "
+                          "the FP64 matrix peak)",
+                "flops_per_launch": trd_flops, "launch_ms": trd_ms}
+    else:
+        roof = None
+    roof_3c = {"bound": "mfma", "achieved": sweep_flops / (t3c * 1e-3) / 1e12,
+               "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+               "frac": sweep_flops / (t3c * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+               "traffic": SWEEP_HBM_BYTES.get(args.workload),
+               "kernel": "k_three_center + k_herm_finish (all classes, one Hamiltonian sweep, "
+                         "plain form; McMurchie-Davidson operation counts)",
+               "flops_per_launch": sweep_flops, "launch_ms": t3c,
+               "special_evals": workmodel.sweep_specials(counts)}
     out = {
         "metric": METRIC,
-        "value": mixed["sec"],
+        "value": chosen["sec"],
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": mixed["sec"] * 1e3,
+        "ms_per_step": chosen["sec"] * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (f32 exponential in the three-centre items)",
+        "dtype": "f64 (bf16-split Bloch sums with f32 accumulation, f32 exponentials of "
+                 "distant images)",
         "data": "synthetic (seeded)",
         "config": {"workload": WORKLOADS[args.workload],
-                   "form": "mixed (value) beside plain FP64", "mode": "B (full rebuild per iteration)",
+                   "form": "chosen (mixed, autotuned, graph replay) beside plain FP64",
+                   "mode": "B (full rebuild per iteration)",
                    "valeDim": vdim, "nk": nk, "pair_cells": npc, "primitive_pairs": npp,
                    "three_center_evals": tri,
-                   "fp64_s_per_iter": fp64["sec"],
-                   "mixed_speedup_vs_fp64": fp64["sec"] / mixed["sec"],
-                   "mixed_vs_fp64_max_eig_err_meV_occupied": eig_err_mev,
-                   "mixed_vs_fp64_max_eig_err_meV_all_states": eig_err_all_mev,
+                   "plain_fp64_s_per_iter": plain["sec"],
+                   "chosen_speedup_vs_plain": plain["sec"] / chosen["sec"],
+                   "chosen_tuning": chosen["tuning"], "chosen_graph_replay": chosen["graph"],
+                   "chosen_vs_plain_max_eig_err_meV_occupied": eig_err_mev,
+                   "chosen_vs_plain_max_eig_err_meV_all_states": eig_err_all_mev,
                    "occupied_states": n_occ,
-                   "mixed_vs_fp64_band_energy_err_Ha_per_atom": de,
-                   "stage_ms_mixed": mixed["stage"], "stage_ms_fp64": fp64["stage"]},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": SWEEP_HBM_BYTES.get(args.workload),
-                     "kernel": "k_three_center (all classes, one sweep, mixed form)",
-                     "flops_per_launch": flops, "launch_ms": t3c * 1e3},
+                   "chosen_vs_plain_band_energy_err_Ha_per_atom": de,
+                   "stage_ms_plain": {k: v for k, v in st_plain.items() if k.endswith("_ms")},
+                   "stage_ms_chosen_eager": {k: v for k, v in st_chosen.items()
+                                             if k.endswith("_ms")}},
+        "roofline": roof if roof is not None else roof_3c,
+        "roofline_three_center": roof_3c,
     }
     if not args.no_cpu:
         est, sample = cpu_baseline(s, tri, args.cpu_stride)

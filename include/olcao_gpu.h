@@ -101,7 +101,9 @@ typedef struct {
  * OLCAO_EIG_LIBRARY always uses rocSOLVER. */
 enum { OLCAO_EIG_AUTO = 0, OLCAO_EIG_LIBRARY = 1 };
 
-/* Bloch-sum kernel: OLCAO_BLOCH_AUTO takes the matrix-core kernel (FP64
+/* The tile shapes chunkItems, segPairs and bloch below are what the
+ * project's autotuner (olcao_amd/autotune.py) chooses per workload.
+ * Bloch-sum kernel: OLCAO_BLOCH_AUTO takes the matrix-core kernel (FP64
  * MFMA in the FP64 form, exact split BF16 products in the mixed form);
  * OLCAO_BLOCH_VALU the vector-ALU kernel (FP64 in both forms). */
 enum { OLCAO_BLOCH_AUTO = 0, OLCAO_BLOCH_VALU = 1, OLCAO_BLOCH_MFMA = 2 };
@@ -114,7 +116,11 @@ typedef struct {
                                      of its device work (build and solve) after its
                                      first call; the library eigensolver (n > 640)
                                      is not captured and such contexts stay eager */
-    int reserved[4];
+    int chunkItems;               /* three-centre work per lane: image-list chunks of
+                                     about this many items (16..1048576; 0: 192) */
+    int segPairs;                 /* contraction segment length in primitive pairs
+                                     (1..64; 0: 32) */
+    int reserved[2];
 } olcao_options;
 
 /* Context lifetime.  Replaces the derived setup of OLCAO.F90:setupSCF
@@ -255,7 +261,9 @@ olcao_status olcao_scf_iteration_batch(int nCells, olcao_ctx *const *ctxs,
  * primitive blocks, t[2] Hamiltonian contraction + Bloch sum (+ ortho),
  * t[3] solve, t[4] total; t[5] three-centre kernels alone (all summed over
  * spins); t[6] 1 when the iteration was a graph replay (then only t[4] is
- * measured, from the coefficient upload to the graph's end). */
+ * measured, from the coefficient upload to the graph's end); t[7] the
+ * tridiagonal reduction kernel (k_trd) alone, summed over spins (0 when the
+ * solve took another path). */
 olcao_status olcao_get_timings(const olcao_ctx *ctx, double t[8]);
 
 /* Graph mode of a context (olcao_options.graph): *state 0 off, 1 a captured
@@ -264,6 +272,10 @@ olcao_status olcao_get_timings(const olcao_ctx *ctx, double t[8]);
  * 3 capture refused; *reason (may be NULL) then points to the context's
  * message, valid until the context is destroyed. */
 olcao_status olcao_get_graph_state(const olcao_ctx *ctx, int *state, const char **reason);
+
+/* Screened primitive pairs per angular class k = 4 la + lb (the finish
+ * kernels' work units). */
+olcao_status olcao_get_class_pairs(const olcao_ctx *ctx, long long counts[16]);
 
 /* Work counters of the last three-centre sweep: counts[2*(4*la+lb)] three-
  * Gaussian overlap evaluations and counts[2*(4*la+lb)+1] nuclear

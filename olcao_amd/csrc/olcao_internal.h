@@ -181,7 +181,8 @@ void launch_image_fill(const DevCtx &c, const PPRec *recs, int n, const long lon
 void launch_nuc_compact(const unsigned long long *img, const long long *beg, const long long *end,
                         int nc, int *cnt, const long long *nOff, unsigned long long *outN,
                         hipStream_t s);
-// Contraction work: segments of at most kSegPP primitive pairs, and the
+// Contraction work: segments of at most kSegPP (the default; a context may
+// take 1..64, a wavefront's lanes) primitive pairs, and the
 // pair-cells that have several segments {pair-cell, first partial, segment
 // count}.
 constexpr int kSegPP = 32;

@@ -133,6 +133,10 @@ struct EigWork {
     DBuf<double> dcStD, dcRho;
     DBuf<int> dcStI;
     int lastPath = -1;     // path of the last gen_eig: 0 LDS solver, 1 tridiagonal, 2 library
+    // events around the tridiagonal reduction (set by a context; the
+    // reduction is timed when they exist and the solve is not captured)
+    hipEvent_t trdEv[2] = {nullptr, nullptr};
+    bool trdTimed = false;
 };
 
 double angnorm(int lm)   // lm = 1..16, basis.f90:86-101
@@ -169,6 +173,8 @@ struct olcao_ctx {
     unsigned long long graphEpoch = 0;
     std::string graphErr;
     bool blochMfma = true;        // matrix-core Bloch sum (else the vector-ALU one)
+    int chunkItems = kChunkItems; // three-centre items per image-list chunk
+    int segPairs = kSegPP;        // primitive pairs per contraction segment
     std::vector<unsigned short> hPslice;   // phase slices of the split Bloch sum
     DBuf<unsigned short> dPslice;
     bool gamma = false;
@@ -240,6 +246,7 @@ struct olcao_ctx {
     DBuf<double *> dVptr;
     bool haveS = false, haveH = false, haveScf = false;
     double timings[8] = {};
+    double trdMs = 0.0;           // tridiagonal reductions of the current iteration
 
     DevCtx dev{};
     ClassLists cls{};
@@ -667,7 +674,7 @@ void upload(olcao_ctx &c)
     }
     c.dBlocks.alloc(std::max<long long>(c.blockDoubles, 1));
     c.dX.alloc(std::max<size_t>(c.hPcPair.size() * (size_t)c.maxS * c.maxS, 1));
-    {   // contraction segments (at most kSegPP primitive pairs each)
+    {   // contraction segments (at most segPairs primitive pairs each)
         if (2 * (size_t)c.maxS * c.maxA * sizeof(double) > 64 * 1024)
             fail(OLCAO_ERR_INPUT, "basis too large for the contraction kernel (states x exponents)");
         std::vector<ContractSeg> segs;
@@ -677,7 +684,7 @@ void upload(olcao_ctx &c)
         for (size_t pc = 0; pc < c.hPcPair.size(); ++pc) {
             const int p0 = c.hPcPp[pc], p1 = c.hPcPp[pc + 1];
             cut.clear();
-            for (int q = p0; q < p1; q += kSegPP) cut.emplace_back(q, std::min(kSegPP, p1 - q));
+            for (int q = p0; q < p1; q += c.segPairs) cut.emplace_back(q, std::min(c.segPairs, p1 - q));
             const int pr = c.hPcPair[pc];
             const int ti = c.hAtomType[c.hPairI[pr]], tj = c.hAtomType[c.hPairJ[pr]];
             auto seg = [&](int q0, int nq, int part) {
@@ -782,14 +789,14 @@ void build_image_lists(olcao_ctx &c)
         c.cls.img[k] = c.dImg[k].p;
         c.imageEntries += off[n];
         launch_image_fill(c.dev, c.cls.rec[k], n, c.cls.ioff[k], c.dImg[k].p, dOverflow.p, s);
-        // chunks of about kChunkItems items, entries split evenly within a record
+        // chunks of about chunkItems items, entries split evenly within a record
         struct Chunk { int rec; long long beg, end; double est; };
         std::vector<Chunk> ch;
         ch.reserve(n);
         for (int q = 0; q < n; ++q) {
             const int o = order[q];
             const long long ne = cnt[o];
-            const int nch = (int)std::max<long long>(1, std::min<long long>(ne, (items[o] + kChunkItems - 1) / kChunkItems));
+            const int nch = (int)std::max<long long>(1, std::min<long long>(ne, (items[o] + c.chunkItems - 1) / c.chunkItems));
             for (int u = 0; u < nch; ++u) {
                 const long long b = off[q] + ne * u / nch, e = off[q] + ne * (u + 1) / nch;
                 ch.push_back({q, b, e, ne ? (double)items[o] * (double)(e - b) / (double)ne : 0.0});
@@ -1111,7 +1118,7 @@ void ensure_mats(olcao_ctx &c)
 // X <- X - U_b (V_b^H X), with batched GEMMs.  D receives w, A receives Y,
 // dinfo the reduction's status.
 void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, bool cplx, int n,
-             int nb, double *A, double *D, double *E, int *dinfo)
+             int nb, double *A, double *D, double *E, int *dinfo, bool defer)
 {
     const int el = cplx ? 2 : 1;
     const size_t per = (size_t)n * n;
@@ -1134,7 +1141,13 @@ void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, 
         ew.dcCol.upload(ew.dc.colTiles, s);
     }
     launch_zero_words(ew.cnt.p, (long long)plan.counters, s);
+    const bool timed = ew.trdEv[0] && !defer;
+    if (timed) HIPCHK(hipEventRecord(ew.trdEv[0], s));
     HIPCHK(launch_trd(cplx, plan, n, nb, A, ew.V.p, D, E, ew.tau.p, ew.pub.p, ew.cnt.p, dinfo, s));
+    if (timed) {
+        HIPCHK(hipEventRecord(ew.trdEv[1], s));
+        ew.trdTimed = true;
+    }
     ew.dcStD.alloc(dc_state_doubles(n, nb));
     ew.dcStI.alloc(dc_state_ints(n, nb));
     ew.dcCnt.alloc(ew.dc.nodes.size() * nb);
@@ -1351,7 +1364,7 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
         launch_herm_fill(A, n, nb, el, s);
         solve_left_h(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
         solve_right(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, ew.Y.p, A);
-        trd_eig(ew, plan, h, s, cplx, n, nb, A, D, E, tinfo);
+        trd_eig(ew, plan, h, s, cplx, n, nb, A, D, E, tinfo, defer);
         solve_back(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
         HIPCHK(hipMemcpyAsync(A, ew.Y.p, (size_t)n * n * el * nb * sizeof(double),
                               hipMemcpyDeviceToDevice, s));
@@ -1459,6 +1472,12 @@ void solve_finish(olcao_ctx &c, int q, int numStates, const std::vector<int> &in
         HIPCHK(hipStreamSynchronize(s));
     }
     HIPCHK(hipStreamSynchronize(s));
+    if (c.eig.trdTimed) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c.eig.trdEv[0], c.eig.trdEv[1]));
+        c.trdMs += ms;
+        c.eig.trdTimed = false;
+    }
     for (int k = 0; k < nk; ++k) {
         if (info[k] > n) fail(OLCAO_ERR_NOT_SPD, "S(k) not positive definite at k=%d (info=%d)", k, info[k]);
         if (info[k] != 0) fail(OLCAO_ERR_SOLVER, "eigensolver did not converge at k=%d (info=%d)", k, info[k]);
@@ -1528,6 +1547,16 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
         c->eigLibrary = opt && opt->eigensolver == OLCAO_EIG_LIBRARY;
         c->blochMfma = !(opt && opt->bloch == OLCAO_BLOCH_VALU);
         c->useGraph = opt && opt->graph != 0;
+        if (opt && opt->chunkItems) {
+            if (opt->chunkItems < 16 || opt->chunkItems > (1 << 20))
+                fail(OLCAO_ERR_INPUT, "chunkItems must lie in 16..1048576 (0: default)");
+            c->chunkItems = opt->chunkItems;
+        }
+        if (opt && opt->segPairs) {
+            if (opt->segPairs < 1 || opt->segPairs > 64)
+                fail(OLCAO_ERR_INPUT, "segPairs must lie in 1..64 (0: default)");
+            c->segPairs = opt->segPairs;
+        }
         if (opt && (opt->bloch < OLCAO_BLOCH_AUTO || opt->bloch > OLCAO_BLOCH_MFMA))
             fail(OLCAO_ERR_INPUT, "bloch must be OLCAO_BLOCH_AUTO, _VALU or _MFMA");
         if (c->precision != OLCAO_PREC_FP64 && c->precision != OLCAO_PREC_MIXED)
@@ -1544,6 +1573,7 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
         for (int q = rot; q < olcao_ctx::kSide; ++q) HIPCHK(hipStreamCreate(&c->side[q]));
         for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
         HIPCHK(hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming));
+        for (auto &e : c->eig.trdEv) HIPCHK(hipEventCreate(&e));
         for (auto &e : c->evJoin) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         if (rocblas_create_handle(&c->blas) != rocblas_status_success)
             fail(OLCAO_ERR_HIP, "rocblas_create_handle failed");
@@ -1577,6 +1607,8 @@ void olcao_ctx_destroy(olcao_ctx *c)
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->evFork) (void)hipEventDestroy(c->evFork);
+    for (auto e : c->eig.trdEv)
+        if (e) (void)hipEventDestroy(e);
     for (auto e : c->evJoin)
         if (e) (void)hipEventDestroy(e);
     for (auto q : c->side)
@@ -1922,6 +1954,7 @@ static void upload_coef(olcao_ctx &c, const double *coef)
 
 static void iteration_build(olcao_ctx &c, const double *coef)
 {
+    c.trdMs = 0.0;
     upload_coef(c, coef);
     HIPCHK(hipEventRecord(c.ev[0], c.stream));
     iteration_device(c, true);
@@ -1950,6 +1983,7 @@ static void record_timings(olcao_ctx &c, double solveMs)
     c.timings[4] = solveMs < 0 ? el(0, 8) : el(0, 1 + 3 * c.spin) + solveMs;
     c.timings[5] = t3c;
     c.timings[6] = 0.0;
+    c.timings[7] = c.trdMs;
 }
 
 static void iteration_solve(olcao_ctx &c, int numStates, double *evals)
@@ -2084,6 +2118,7 @@ static void grouped_solve(const std::vector<SolveMember> &mem, olcao_ctx *const 
         off += c.nk;
     }
     HIPCHK(hipStreamSynchronize(s));
+    h.eig.trdTimed = false;
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, h.ev[9], h.ev[10]));
     off = 0;
@@ -2262,6 +2297,13 @@ olcao_status olcao_get_graph_state(const olcao_ctx *c, int *state, const char **
     else
         *state = c->graphExec && c->graphEpoch == c->allocEpoch.load() ? 1 : 2;
     if (reason) *reason = c->graphErr.c_str();
+    return OLCAO_OK;
+}
+
+olcao_status olcao_get_class_pairs(const olcao_ctx *c, long long counts[16])
+{
+    if (!c || !counts) return OLCAO_ERR_INPUT;
+    for (int k = 0; k < 16; ++k) counts[k] = (long long)c->classIdx[k].size();
     return OLCAO_OK;
 }
 

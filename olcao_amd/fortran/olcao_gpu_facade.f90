@@ -75,7 +75,9 @@ module O_GPUFacade
       integer(c_int) :: eigensolver
       integer(c_int) :: bloch
       integer(c_int) :: graph
-      integer(c_int) :: reserved(4)
+      integer(c_int) :: chunkItems
+      integer(c_int) :: segPairs
+      integer(c_int) :: reserved(2)
    end type olcao_options_t
 
    type(c_ptr), save :: ctx = c_null_ptr
@@ -243,6 +245,8 @@ contains
       o%eigensolver = 0
       o%bloch = 0
       o%graph = 0
+      o%chunkItems = 0
+      o%segPairs = 0
       o%reserved = 0
       st = c_ctx_create(s, o, ctx)
       if (st /= 0) call gpuStop(st, 'gpuInit')

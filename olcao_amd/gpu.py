@@ -58,7 +58,8 @@ class _System(C.Structure):
 
 class _Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("eigensolver", C.c_int), ("bloch", C.c_int),
-                ("graph", C.c_int), ("reserved", C.c_int * 4)]
+                ("graph", C.c_int), ("chunkItems", C.c_int), ("segPairs", C.c_int),
+                ("reserved", C.c_int * 2)]
 
 
 EIG_AUTO, EIG_LIBRARY = 0, 1
@@ -88,6 +89,7 @@ FUNCTIONS = {
                                   C.c_int, C.c_void_p, C.c_void_p],
     "olcao_get_timings": [C.c_void_p, C.c_void_p],
     "olcao_get_graph_state": [C.c_void_p, C.c_void_p, C.c_void_p],
+    "olcao_get_class_pairs": [C.c_void_p, C.c_void_p],
     "olcao_get_work_counts": [C.c_void_p, C.c_void_p],
     "olcao_solve_zhegv": [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_solve_dsygv": [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
@@ -181,7 +183,19 @@ class OlcaoGPU:
     """One device context for one system (setupSCF's derived state)."""
 
     def __init__(self, sysin, precision=PREC_FP64, eigensolver=EIG_AUTO, bloch=BLOCH_AUTO,
-                 graph=False):
+                 graph=False, chunk_items=0, seg_pairs=0, tuning=None):
+        """tuning: a dict of autotuned tile shapes (chunk_items, seg_pairs,
+        bloch), or "table" for the entry of olcao_amd/tuning.json nearest to
+        this system (olcao_amd.autotune.lookup); it overrides chunk_items,
+        seg_pairs and bloch."""
+        if tuning is not None:
+            from olcao_amd import autotune
+            t = autotune.lookup(sysin, precision) if tuning == "table" else tuning
+            if t:
+                chunk_items = t.get("chunk_items", chunk_items)
+                seg_pairs = t.get("seg_pairs", seg_pairs)
+                bloch = t.get("bloch", bloch)
+        self.tuning = dict(chunk_items=chunk_items, seg_pairs=seg_pairs, bloch=bloch)
         self.sysin = sysin
         f = sysin.flat()
         self._keep = f
@@ -207,6 +221,8 @@ class OlcaoGPU:
         opt.eigensolver = eigensolver
         opt.bloch = bloch
         opt.graph = 1 if graph else 0
+        opt.chunkItems = int(chunk_items)
+        opt.segPairs = int(seg_pairs)
         ctx = C.c_void_p()
         code = lib().olcao_ctx_create(C.byref(s), C.byref(opt), C.byref(ctx))
         _check(code, None)
@@ -381,7 +397,14 @@ class OlcaoGPU:
         t = np.zeros(8)
         _check(lib().olcao_get_timings(self.ctx, _p(t)), self.ctx)
         return dict(overlap_ms=t[0], hamiltonian_ms=t[1], contract_bloch_ms=t[2],
-                    solve_ms=t[3], total_ms=t[4], three_center_ms=t[5], graph_replay=t[6] > 0)
+                    solve_ms=t[3], total_ms=t[4], three_center_ms=t[5], graph_replay=t[6] > 0,
+                    trd_ms=t[7])
+
+    def class_pairs(self):
+        """Screened primitive pairs per class, [4, 4] (la, lb)."""
+        v = np.zeros(16, np.int64)
+        _check(lib().olcao_get_class_pairs(self.ctx, _p(v)), self.ctx)
+        return v.reshape(4, 4)
 
     def graph_state(self):
         """(state, reason): 0 graph mode off, 1 a captured graph is current,
