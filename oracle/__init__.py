@@ -33,7 +33,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libolcao_ref_prims.so")
 
-OL, KE, NP, EP, HAM, MV = 0, 1, 2, 3, 4, 5
+OL, KE, NP, EP, HAM, MV, DM, MM = 0, 1, 2, 3, 4, 5, 6, 7
 SMALL_THRESH = 1.0e-8
 
 _lib = None

@@ -392,6 +392,80 @@ static void os_massvel(int la, int lb, double a1, double a2, double pref,
         }
 }
 
+/* Dipole (op 5, dipole3CIntg, gaussIntegrals.f90:49404) and momentum
+ * (op 6, momentum2CIntg, :23041) integrals, three Cartesian components each.
+ * Along the component's axis, with g_i = (x-A)^i exp(-a (x-A)^2) and the
+ * ket g_j = (x-B)^j exp(-b (x-B)^2):
+ *   dipole    <g_i| x - C |g_j> = E(i, j+1) + (B - C) E(i, j)
+ *   momentum  <g_i| d/dx  |g_j> = j E(i, j-1) - 2 b E(i, j+1)
+ *                               = -(i E(i-1, j) - 2 a E(i+1, j))
+ * (integration by parts), the derivative taken on the more diffuse
+ * primitive: on the tight one the two terms are larger than the result and
+ * cancel.  The overlap table E on the other two axes, extended by one
+ * along the side that needs it. */
+static void os_vec3(int op, int la, int lb, double a1, double a2, double pref,
+                    const double PA[3], const double PB[3], const double BC[3], double h,
+                    double pc[3][20][20])
+{
+    double E[3][MVMAX][MVMAX];
+    const int bra = op == 6 && a1 < a2;   /* momentum: differentiate the bra */
+    for (int x = 0; x < 3; x++) {
+        if (bra)
+            os1d_table(la + 1, lb, PA[x], PB[x], h, E[x]);
+        else
+            os1d_table(la, lb + 1, PA[x], PB[x], h, E[x]);
+    }
+    int na = NCART[la], nb = NCART[lb];
+    for (int a = 0; a < na; a++)
+        for (int b = 0; b < nb; b++)
+            for (int c = 0; c < 3; c++) {
+                double v = pref;
+                for (int x = 0; x < 3; x++) {
+                    int i = CART[a][x], j = CART[b][x];
+                    if (x != c)
+                        v *= E[x][i][j];
+                    else if (op == 5)
+                        v *= E[x][i][j + 1] + BC[x] * E[x][i][j];
+                    else if (bra)
+                        v *= 2.0 * a1 * E[x][i + 1][j] - (i > 0 ? i * E[x][i - 1][j] : 0.0);
+                    else
+                        v *= (j > 0 ? j * E[x][i][j - 1] : 0.0) - 2.0 * a2 * E[x][i][j + 1];
+                }
+                pc[c][a][b] = v;
+            }
+}
+
+void orc_prim3(int op, double a1, double a2, const double *A, const double *B, const double *C,
+               int sw, double *sh)
+{
+    init_tables();
+    int la, lb;
+    decode_switch(sw, &la, &lb);
+    double zeta = a1 + a2, xi = a1 * a2 / zeta, d2 = 0.0;
+    double PA[3], PB[3], BC[3];
+    for (int x = 0; x < 3; x++) {
+        double P = (a1 * A[x] + a2 * B[x]) / zeta;
+        PA[x] = P - A[x];
+        PB[x] = P - B[x];
+        BC[x] = B[x] - C[x];
+        d2 += (A[x] - B[x]) * (A[x] - B[x]);
+    }
+    static __thread double pc[3][20][20];
+    os_vec3(op, la, lb, a1, a2, pow(PI / zeta, 1.5) * exp(-xi * d2), PA, PB, BC,
+            1.0 / (2.0 * zeta), pc);
+    for (int c = 0; c < 3; c++) cart_to_sh(la, lb, pc[c], sh + 256 * c);
+}
+
+void orc_prim3_batch(int op, int n, const double *a, const double *A, const double *B,
+                     const double *C, const int *sw, double *sh)
+{
+    init_tables();
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++)
+        orc_prim3(op, a[2 * i], a[2 * i + 1], A + 3 * i, B + 3 * i, C + 3 * i, sw[i],
+                  sh + (long)768 * i);
+}
+
 void orc_prim(int op, double a1, double a2, double a3, const double *A,
               const double *B, const double *C, int sw, double *sh)
 {
@@ -700,6 +774,15 @@ static void pair_block(const drv_ctx *c, const op_args *o, int ti, int tj,
     *contrib = 1;
     if (o->op == ORC_OL || o->op == ORC_KE) {
         orc_prim(o->op == ORC_OL ? 0 : 1, a1, a2, 0.0, A, B, A, sw, blk);
+        return;
+    }
+    if (o->op == ORC_DM || o->op == ORC_MM) {
+        /* one Cartesian component (term) of the dipole about coef[0..2]
+         * (gaussOverlapDM, integrals3Terms.F90:64) or of the gradient
+         * (gaussOverlapMM, :511) */
+        double three[768];
+        orc_prim3(o->op == ORC_DM ? 5 : 6, a1, a2, A, B, o->op == ORC_DM ? o->coef : A, sw, three);
+        for (int q = 0; q < 256; q++) blk[q] = three[256 * o->term + q];
         return;
     }
     if (o->op == ORC_MV) {   /* accumulated with a minus sign, integrals.F90:1434-1447 */
@@ -1042,9 +1125,21 @@ int orc_integral(const orc_sys *s, int op, int term, const double *coef,
                             sum += s->basis[(ti * ms + a) * ma + al] * Y[(b * ma + al) * 16 + lm1];
                         X[a * nt2 + b] = sum;
                     }
-                /* applyPhaseFactors / applyPhaseFactorsGamma */
+                /* applyPhaseFactors / applyPhaseFactorsGamma; the momentum
+                 * matrix takes a factor -i (runCode 1, intgSaving.F90:
+                 * 178-187, the Gamma build keeping -X, :727-731) */
                 if (s->gamma) {
-                    for (int q = 0; q < nt1 * nt2; q++) cur[q] += X[q];
+                    const double sg = op == ORC_MM ? -1.0 : 1.0;
+                    for (int q = 0; q < nt1 * nt2; q++) cur[q] += sg * X[q];
+                } else if (op == ORC_MM) {
+                    for (int kk = 0; kk < nk; kk++) {
+                        double pr = c.phase[2 * (k * nk + kk)], pi = c.phase[2 * (k * nk + kk) + 1];
+                        double *ck = cur + (long)kk * nt1 * nt2 * 2;
+                        for (int q = 0; q < nt1 * nt2; q++) {
+                            ck[2 * q] += pi * X[q];
+                            ck[2 * q + 1] -= pr * X[q];
+                        }
+                    }
                 } else {
                     for (int kk = 0; kk < nk; kk++) {
                         double pr = c.phase[2 * (k * nk + kk)], pi = c.phase[2 * (k * nk + kk) + 1];

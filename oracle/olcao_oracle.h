@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-enum { ORC_OL = 0, ORC_KE = 1, ORC_NP = 2, ORC_EP = 3, ORC_HAM = 4, ORC_MV = 5 };
+enum { ORC_OL = 0, ORC_KE = 1, ORC_NP = 2, ORC_EP = 3, ORC_HAM = 4, ORC_MV = 5, ORC_DM = 6, ORC_MM = 7 };
 enum { ORC_OK = 0, ORC_ERR_CELLS = 1, ORC_ERR_INPUT = 2 };
 
 typedef struct {
@@ -70,6 +70,13 @@ void orc_prim_batch(int op, int n, const double *a, const double *A,
                     double *sh);
 
 /* Lattice (lattice.f90). */
+/* Three-component primitives: op 5 dipole about C (dipole3CIntg), op 6
+ * gradient on the ket (momentum2CIntg); sh: [3][16][16] column-major per
+ * component; a: [n][2], A/B/C: [n][3]. */
+void orc_prim3(int op, double a1, double a2, const double *A, const double *B, const double *C,
+               int sw, double *sh);
+void orc_prim3_batch(int op, int n, const double *a, const double *A, const double *B,
+                     const double *C, const int *sw, double *sh);
 double orc_neglig_limit(const double *vec, double logThresh,
                         double minAtomicAlpha, double minPotAlpha);
 int orc_make_lattice(const double *vec, double negligLimit, int maxCells,
@@ -80,7 +87,9 @@ void orc_nearest_lattice_vector(const double *vec, const double *v,
 /* Integral drivers (integrals.F90 + intgSaving.F90).  vv is valeVale
  * (complex, or real when s->gamma), cv coreVale, cc coreCore; all zeroed
  * by the call.  term: global potential term index for ORC_EP.  coef:
- * potential coefficients [potDim] of one spin for ORC_HAM.  ORC_MV is the
+ * potential coefficients [potDim] of one spin for ORC_HAM.  ORC_DM, ORC_MM:
+ * Cartesian component `term` (0..2) of the dipole about coef[0..2] and of
+ * the momentum (-i times the gradient).  ORC_MV is the
  * negated mass-velocity sweep (gaussOverlapMV).  nthreads <= 0: default. */
 int orc_integral(const orc_sys *s, int op, int term, const double *coef,
                  double *vv, double *cv, double *cc, int nthreads);

@@ -31,6 +31,22 @@ contains
       end select
    end subroutine ref_prim
 
+   ! Three-component operators: op 5 dipole3CIntg (dipole about C), op 6
+   ! momentum2CIntg (gradient on the ket).  sh is column-major (16,16,3).
+   subroutine ref_prim3(op, a1, a2, A, B, C, sw, sh) bind(C, name="ref_prim3")
+      integer(c_int), value :: op, sw
+      real(c_double), value :: a1, a2
+      real(c_double), intent(in) :: A(3), B(3), C(3)
+      real(c_double), intent(inout) :: sh(16,16,3)
+      sh(:,:,:) = 0.0d0
+      select case (op)
+      case (5)
+         call dipole3CIntg(a1, a2, A, B, C, sw, sh)
+      case (6)
+         call momentum2CIntg(a1, a2, A, B, sw, sh)
+      end select
+   end subroutine ref_prim3
+
    subroutine ref_boys(T, F) bind(C, name="ref_boys")
       real(c_double), value :: T
       real(c_double), intent(out) :: F(7)

@@ -54,6 +54,7 @@ class _Brute:
         self.kc = o.kcart
         self.cell = np.asarray(s.cell, float)
         self.off = np.cumsum([0] + [s.types[t].num_vale_states for t in s.atom_type])
+        self.dipole = np.array([0.7, -1.3, 2.1])
         self.n = int(self.off[-1])
         amin = min(float(np.min(t.alphas)) for t in s.types)
         pmin = min(float(np.min(p.alphas)) for p in s.pot_types)
@@ -105,12 +106,12 @@ class _Brute:
                         s2 = np.sum((A - B) ** 2, axis=1)
                         keep = xi * s2 <= self.lim
                         B, L, s2 = B[keep], self.Lpair[keep], s2[keep]
-                        if kind in ("ol", "ke"):
+                        if kind in ("ol", "ke", "dm", "mm"):
                             m = len(B)
                             ins["a"].append(np.tile([a1, a2, 0.0], (m, 1)))
                             ins["A"].append(np.tile(A, (m, 1)))
                             ins["B"].append(B)
-                            ins["C"].append(np.tile(A, (m, 1)))
+                            ins["C"].append(np.tile(self.dipole if kind == "dm" else A, (m, 1)))
                             ins["sw"].append(np.full(m, sw))
                             ins["a1"].append(np.full(m, a1i))
                             ins["a2"].append(np.full(m, a2i))
@@ -139,8 +140,15 @@ class _Brute:
                 if not ins["a"]:
                     continue
                 cat = {k: np.concatenate(v) for k, v in ins.items()}
-                op = {"ol": 0, "ke": 1, "ep": 2, "np": 3}[kind]
-                sh = oracle.prim_batch(op, cat["a"], cat["A"], cat["B"], cat["C"], cat["sw"])
+                if kind in ("dm", "mm"):
+                    from tests.test_oracle import _prim3
+                    inp = np.concatenate([cat["a"][:, :2], cat["A"], cat["B"], cat["C"]], axis=1)
+                    cls = np.zeros((len(inp), 3), np.int32)
+                    cls[:, 2] = cat["sw"]
+                    sh = _prim3(5 if kind == "dm" else 6, inp, cls)[:, term]
+                else:
+                    op = {"ol": 0, "ke": 1, "ep": 2, "np": 3}[kind]
+                    sh = oracle.prim_batch(op, cat["a"], cat["A"], cat["B"], cat["C"], cat["sw"])
                 lm1 = o.lmi[ti, :ns1] - 1
                 lm2 = o.lmi[tj, :ns2] - 1
                 G = sh[:, lm1][:, :, lm2]                                  # [E, ns1, ns2]
@@ -148,6 +156,8 @@ class _Brute:
                 G = G * o.basis[tj, :ns2, :][:, cat["a2"]].T[:, None, :]
                 G = G * cat["fac"][:, None, None]
                 ph = np.exp(1j * (cat["L"] @ self.kc.T))                  # [E, nk]
+                if kind == "mm":   # the momentum matrix: -i times the gradient's
+                    ph = -1j * ph
                 ref[:, self.off[i]:self.off[i + 1], self.off[j]:self.off[j + 1]] += \
                     np.einsum("ek,epq->kpq", ph, G)
         return ref
@@ -203,3 +213,21 @@ def test_dropped_images_are_detected(tiny):
     assert block_rel_err(_full(M_ep), ref_ep) > 1e-8
     H = o.integral(oracle.HAM, coef=s.pot_coeffs)[0]
     assert block_rel_err(_full(M_h), _full(H)) > 1e-8
+
+
+@pytest.mark.parametrize("kind,op", [("dm", oracle.DM), ("mm", oracle.MM)])
+@pytest.mark.parametrize("comp", [0, 2])
+def test_dipole_and_momentum_sweeps_equal_unscreened_sums(tiny, kind, op, comp):
+    """gaussOverlapDM / gaussOverlapMM (integrals3Terms.F90:64, :511): one
+    Cartesian component of the dipole about a fixed centre and of the
+    momentum (-i times the gradient, applyPhaseFactors runCode 1) against
+    the direct lattice sum, on the upper triangle the reference stores (the
+    dipole about a fixed centre is not translation invariant, so its
+    Bloch-summed matrix is not Hermitian); the momentum matrix is."""
+    s, o, b = tiny
+    M = o.integral(op, term=comp, coef=b.dipole)[0]
+    ref = b.matrix(kind, comp)
+    assert block_rel_err(np.triu(M), np.triu(ref)) < 1e-11
+    if kind == "mm":
+        herm = np.abs(ref - np.conj(ref.swapaxes(-1, -2))).max() / np.abs(ref).max()
+        assert herm < 1e-11
