@@ -176,6 +176,24 @@ olcao_status olcao_gauss_overlap_np(olcao_ctx *ctx, double *vv, double *cv,
 olcao_status olcao_elec_pot_gauss_overlap(olcao_ctx *ctx, int term, double *vv,
                                           double *cv, double *cc);
 
+/* Dipole-moment matrices (gaussOverlapDM, integrals3Terms.F90:64, with
+ * dipole3CIntg gaussIntegrals.f90:49404): the three Cartesian components of
+ * <phi_i| r - center |phi_j(. - L)> Bloch-summed like the overlap, each
+ * orthogonalised against the core (ortho3Terms opCode 6, :1388) and stored
+ * upper-triangular as valeVale.  center: the dipole centre (O_Input
+ * dipoleCenter), bohr.  vv, cv, cc: three blocks of the layout of
+ * olcao_gauss_overlap's, component-major (x, y, z). */
+olcao_status olcao_gauss_overlap_dm(olcao_ctx *ctx, const double *center, double *vv,
+                                    double *cv, double *cc);
+
+/* Momentum matrices (gaussOverlapMM, integrals3Terms.F90:511, with
+ * momentum2CIntg gaussIntegrals.f90:23041): the gradient integrals
+ * <phi_i| d/dr |phi_j(. - L)> Bloch-summed with the factor -i
+ * (applyPhaseFactors runCode 1, intgSaving.F90:178-187; -X at Gamma,
+ * :727-731), i.e. the Hermitian matrices of p = -i nabla; orthogonalised
+ * (opCode 7) and laid out as olcao_gauss_overlap_dm. */
+olcao_status olcao_gauss_overlap_mm(olcao_ctx *ctx, double *vv, double *cv, double *cc);
+
 /* Fused Hamiltonian sweep with the potential coefficients folded in
  * (gaussOverlapHamPSCF, integrals.F90:2860): KE (- MV when rel = 1)
  * + sum_j c_j V_j - sum Z N, orthogonalised when coreDim > 0

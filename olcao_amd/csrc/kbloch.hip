@@ -44,9 +44,14 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ void bloch_store(const DevCtx &c, int i, int j, int k, int s1, int s2,
                                             int nv1, int nv2, int v1, int v2, int c1, int c2,
                                             const double *lv, double vre, double vim, int gamma,
-                                            double *vv, double *cv, double *cc)
+                                            int negI, double *vv, double *cv, double *cc)
 {
     const long long n = c.valeDim, nc = c.coreDim;
+    if (negI) {   // times -i (at Gamma the kept part is -X)
+        const double t = vre;
+        vre = gamma ? -t : vim;
+        vim = -t;
+    }
     if (!gamma) {
         const double *kp = c.kcart + 3 * k;
         const double dot = kp[0] * lv[0] + kp[1] * lv[1] + kp[2] * lv[2];
@@ -100,7 +105,8 @@ __device__ __forceinline__ unsigned short bf16_bits(float v)
 template <bool SPLIT>
 __global__ __launch_bounds__(64) void k_bloch_mfma(DevCtx c, const double *X,
                                                    const unsigned short *pslice, double *vv,
-                                                   double *cv, double *cc, int gamma, int nTiles)
+                                                   double *cv, double *cc, int gamma, int negI,
+                                                   int nTiles)
 {
     const int lane = threadIdx.x;
     const int pr = blockIdx.x / nTiles, tile = blockIdx.x - pr * nTiles;
@@ -238,8 +244,8 @@ __global__ __launch_bounds__(64) void k_bloch_mfma(DevCtx c, const double *X,
             // result rows: FP64 MFMA 4 r + grp, BF16 MFMA 4 grp + r
             const int k = k0 + (SPLIT ? 4 * grp + r : 4 * r + grp);
             if (k >= nk) continue;
-            bloch_store(c, i, j, k, s1, s2, nv1, nv2, v1, v2, c1, c2, lv, re[r], im[r], gamma, vv,
-                        cv, cc);
+            bloch_store(c, i, j, k, s1, s2, nv1, nv2, v1, v2, c1, c2, lv, re[r], im[r], gamma, negI,
+                        vv, cv, cc);
         }
     }
 }
@@ -247,17 +253,17 @@ __global__ __launch_bounds__(64) void k_bloch_mfma(DevCtx c, const double *X,
 }  // namespace
 
 void launch_bloch_mfma(const DevCtx &c, const double *X, const unsigned short *pslice, double *vv,
-                       double *cv, double *cc, int gamma, bool split, hipStream_t s)
+                       double *cv, double *cc, int gamma, int negI, bool split, hipStream_t s)
 {
     if (c.nPairs == 0) return;
     const int nk = gamma ? 1 : c.nk;
     const int nTiles = (nk + 15) / 16;
     if (split)
         hipLaunchKernelGGL(k_bloch_mfma<true>, dim3(c.nPairs * nTiles), dim3(64), 0, s, c, X, pslice,
-                           vv, cv, cc, gamma, nTiles);
+                           vv, cv, cc, gamma, negI, nTiles);
     else
         hipLaunchKernelGGL(k_bloch_mfma<false>, dim3(c.nPairs * nTiles), dim3(64), 0, s, c, X, pslice,
-                           vv, cv, cc, gamma, nTiles);
+                           vv, cv, cc, gamma, negI, nTiles);
 }
 
 void phase_slices_host(const std::vector<double> &phase, int nCells, int nk, int gamma,

@@ -49,6 +49,10 @@ __global__ __launch_bounds__(64) void k_two_center(DevCtx c, const int *list, in
         two_center_cart<LA, LB, true>(q, pc);
     else if (op == OP_MV)
         massvel_cart<LA, LB>(q, pc);
+    else if (op >= OP_MM)
+        vec_cart<LA, LB, false>(q, op - OP_MM, c.dipC, pc);
+    else if (op >= OP_DM)
+        vec_cart<LA, LB, true>(q, op - OP_DM, c.dipC, pc);
     else
         two_center_cart<LA, LB, false>(q, pc);
     double sh[SA][SB];
@@ -189,8 +193,8 @@ constexpr int kBlochCells = 16;
 constexpr int kBlochOut = 8;
 
 __global__ __launch_bounds__(64) void k_bloch(DevCtx c, const double *X, double *vv,
-                                              double *cv, double *cc, int gamma, int KT,
-                                              int nTiles)
+                                              double *cv, double *cc, int gamma, int negI,
+                                              int KT, int nTiles)
 {
     extern __shared__ double lds[];
     const int lane = threadIdx.x;
@@ -260,6 +264,11 @@ __global__ __launch_bounds__(64) void k_bloch(DevCtx c, const double *X, double 
             const int s2 = e / nt1, s1 = e - s2 * nt1;
             const int k = k0 + kk;
             double vre = re[r], vim = im[r];
+            if (negI) {   // times -i (at Gamma the kept part is -X)
+                const double t = vre;
+                vre = gamma ? -t : vim;
+                vim = -t;
+            }
             if (!gamma) {
                 const double *kp = c.kcart + 3 * k;
                 const double dot = kp[0] * lv[0] + kp[1] * lv[1] + kp[2] * lv[2];
@@ -603,7 +612,7 @@ void launch_contract(const DevCtx &c, const ContractPlan &cp, const double *bloc
 }
 
 void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv, double *cv,
-                          double *cc, int gamma, hipStream_t s)
+                          double *cc, int gamma, int negI, hipStream_t s)
 {
     if (c.nPairs == 0) return;
     // k tile: about 512 outputs per wavefront, and at least ~1024 wavefronts
@@ -614,7 +623,7 @@ void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv, double *
     const int nTiles = (nk + KT - 1) / KT;
     const size_t lds = ((size_t)kBlochCells * per + 2 * (size_t)kBlochCells * KT) * sizeof(double);
     hipLaunchKernelGGL(k_bloch, dim3(c.nPairs * nTiles), dim3(64), lds, s, c, X, vv, cv, cc,
-                       gamma, KT, nTiles);
+                       gamma, negI, KT, nTiles);
 }
 
 void launch_zero_words(void *p, long long words, hipStream_t s)

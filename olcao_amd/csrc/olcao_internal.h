@@ -17,7 +17,9 @@ constexpr int kMaxPotSites = 1024;        // sites staged in LDS (28 B each)
 // Three-centre sweep kinds.
 enum ThreeKind { K3_HAM = 0, K3_EP = 1, K3_NP = 2 };
 // Two-centre operators.
-enum TwoOp { OP_OL = 0, OP_KE = 1, OP_MV = 2 };
+// OP_DM + c, OP_MM + c: Cartesian component c of the dipole about
+// DevCtx::dipC and of the gradient (the momentum matrix's blocks).
+enum TwoOp { OP_OL = 0, OP_KE = 1, OP_MV = 2, OP_DM = 3, OP_MM = 6 };
 
 // Device view of the system and the screened work lists.  Passed by value
 // to every kernel; all pointers are device memory.
@@ -75,6 +77,7 @@ struct DevCtx {
     // nuclear evaluations of the last three-centre sweep
     unsigned long long *counters;
     int anyNuc;                 // some potential type has Z >= smallThresh
+    double dipC[3];             // dipole centre of the OP_DM sweeps
 };
 
 inline __host__ __device__ int pp_a1(int ab) { return ab & 255; }
@@ -203,8 +206,10 @@ struct ContractPlan {
 };
 void launch_contract(const DevCtx &c, const ContractPlan &cp, const double *blocks, double *X,
                      hipStream_t s);
+// negI: multiply the Bloch sum by -i (the momentum matrix, applyPhaseFactors
+// runCode 1, intgSaving.F90:178-187; -X at Gamma, :727-731).
 void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv,
-                          double *cv, double *cc, int gamma, hipStream_t s);
+                          double *cv, double *cc, int gamma, int negI, hipStream_t s);
 // Bloch sum, origin shift and scatter on the matrix cores (kbloch.hip):
 // FP64 MFMA, or (split) exact BF16 slice products with FP32 accumulation
 // over kBlochSlices fixed-point slices per operand.  pslice: the phase
@@ -212,7 +217,7 @@ void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv,
 // phase_slices_host.
 constexpr int kBlochSlices = 6;
 void launch_bloch_mfma(const DevCtx &c, const double *X, const unsigned short *pslice, double *vv,
-                       double *cv, double *cc, int gamma, bool split, hipStream_t s);
+                       double *cv, double *cc, int gamma, int negI, bool split, hipStream_t s);
 void phase_slices_host(const std::vector<double> &phase, int nCells, int nk, int gamma,
                        std::vector<unsigned short> &out);
 void launch_fill(double *p, long long n, double v, hipStream_t s);

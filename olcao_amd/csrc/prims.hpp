@@ -407,6 +407,60 @@ OHD void massvel_cart(const PairConst &q, double (&pc)[ncart(LA)][ncart(LB)])
         }
 }
 
+// One Cartesian component `comp` of the dipole integrals <a| (r - C) |b>
+// (DIP, dipole3CIntg, gaussIntegrals.f90:49404) or of the gradient
+// integrals <a| d/dr |b> (momentum2CIntg, :23041).  Along the component's
+// axis, g_i = (x-A)^i e^{-a (x-A)^2}, g_j = (x-B)^j e^{-b (x-B)^2}:
+//   dipole    <g_i| x - C |g_j> = E(i, j+1) + (B - C) E(i, j)
+//   gradient  <g_i| d/dx |g_j>  = j E(i, j-1) - 2 b E(i, j+1)
+//                               = 2 a E(i+1, j) - i E(i-1, j)
+// the derivative taken on the more diffuse primitive (on the tight one the
+// two terms cancel); the overlap table on the other two axes.
+template <int LA, int LB, bool DIP>
+OHD void vec_cart(const PairConst &q, int comp, const double *C,
+                  double (&pc)[ncart(LA)][ncart(LB)])
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB);
+    const double t = kPi / q.zeta;
+    const double pref = t * sqrt(t) * exp(-q.xid2);
+    const double h = 1.0 / (2.0 * q.zeta);
+    const bool bra = !DIP && q.a1 < q.a2;
+    // Ek: ket-extended table [i][j <= LB+1]; Eb: bra-extended [j][i <= LA+1]
+    double Ek[3][LA + 1][LB + 2], Eb[3][LB + 1][LA + 2];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        if (bra)
+            os1d<LB, LA + 1>(q.P[x] - q.B[x], q.P[x] - q.A[x], h, Eb[x]);
+        else
+            os1d<LA, LB + 1>(q.P[x] - q.A[x], q.P[x] - q.B[x], h, Ek[x]);
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            double v = pref;
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                const int i = cexp(a, x), j = cexp(b, x);
+                const double e0 = bra ? Eb[x][j][i] : Ek[x][i][j];
+                double f;
+                if (x != comp) {
+                    f = e0;
+                } else if (DIP) {
+                    f = Ek[x][i][j + 1] + (q.B[x] - C[x]) * e0;
+                } else if (bra) {
+                    f = 2.0 * q.a1 * Eb[x][j][i + 1];
+                    if (i > 0) f -= i * Eb[x][j][i > 0 ? i - 1 : 0];
+                } else {
+                    f = -2.0 * q.a2 * Ek[x][i][j + 1];
+                    if (j > 0) f += j * Ek[x][i][j > 0 ? j - 1 : 0];
+                }
+                v *= f;
+            }
+            pc[a][b] = v;
+        }
+}
+
 // ---------------------------------------------------------------------
 // Three-centre integrals in McMurchie-Davidson form about the pair centre P.
 //

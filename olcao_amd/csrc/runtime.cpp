@@ -955,7 +955,7 @@ void three_center(olcao_ctx &c, int kind, int tt, int ti, const double *coef, in
 
 // ---------------------------------------------------------------------------
 // One integral sweep into (vv, cv, cc) device matrices.
-enum Sweep { SW_OL, SW_KE, SW_NP, SW_EP, SW_HAM, SW_MV };
+enum Sweep { SW_OL, SW_KE, SW_NP, SW_EP, SW_HAM, SW_MV, SW_DM, SW_MM };
 
 // Primitive-pair blocks of one sweep.  SW_MV stores the negated
 // mass-velocity blocks, the term as it enters H (gaussOverlapMV accumulates
@@ -971,6 +971,8 @@ void sweep_blocks(olcao_ctx &c, Sweep kind, int term, int spinIdx)
     case SW_OL: two_center_forked(c, OP_OL, 0); break;
     case SW_KE: two_center_forked(c, OP_KE, 0); break;
     case SW_MV: two_center_forked(c, OP_MV, 0); break;
+    case SW_DM: two_center_forked(c, OP_DM + term, 0); break;   // centre in c.dev.dipC
+    case SW_MM: two_center_forked(c, OP_MM + term, 0); break;
     case SW_NP: three_center(c, K3_NP, 0, 0, nullptr, 0); break;
     case SW_EP: {
         int t = 0;
@@ -991,8 +993,9 @@ void sweep_blocks(olcao_ctx &c, Sweep kind, int term, int spinIdx)
     }
 }
 
-// Contraction, Bloch sum and scatter of the current blocks.
-void contract_scatter(olcao_ctx &c, double *vv, double *cv, double *cc)
+// Contraction, Bloch sum and scatter of the current blocks; negI: the
+// Bloch sum times -i (the momentum matrix).
+void contract_scatter(olcao_ctx &c, double *vv, double *cv, double *cc, int negI = 0)
 {
     hipStream_t s = c.stream;
     launch_contract(c.dev, c.cplan, c.dBlocks.p, c.dX.p, s);
@@ -1002,17 +1005,17 @@ void contract_scatter(olcao_ctx &c, double *vv, double *cv, double *cc)
         launch_fill(cc, (long long)c.ccDoubles(), 0.0, s);
     }
     if (c.blochMfma)
-        launch_bloch_mfma(c.dev, c.dX.p, c.dPslice.p, vv, cv, cc, c.gamma ? 1 : 0,
+        launch_bloch_mfma(c.dev, c.dX.p, c.dPslice.p, vv, cv, cc, c.gamma ? 1 : 0, negI,
                           c.precision == OLCAO_PREC_MIXED, s);
     else
-        launch_bloch_scatter(c.dev, c.dX.p, vv, cv, cc, c.gamma ? 1 : 0, s);
+        launch_bloch_scatter(c.dev, c.dX.p, vv, cv, cc, c.gamma ? 1 : 0, negI, s);
     HIPCHK(hipGetLastError());
 }
 
 void sweep(olcao_ctx &c, Sweep kind, int term, int spinIdx, double *vv, double *cv, double *cc)
 {
     sweep_blocks(c, kind, term, spinIdx);
-    contract_scatter(c, vv, cv, cc);
+    contract_scatter(c, vv, cv, cc, kind == SW_MM ? 1 : 0);
 }
 
 // Orthogonalisation of valence against core (intgOrtho.F90, integrals.F90
@@ -1674,6 +1677,41 @@ olcao_status olcao_gauss_overlap_np(olcao_ctx *c, double *vv, double *cv, double
         copy_out_spin(*c, 0, vv, cv, cc);
         HIPCHK(hipStreamSynchronize(c->stream));
     });
+}
+
+// The three Cartesian components of the dipole (about `center`) or
+// momentum matrices, each as an operator sweep (orthogonalised against the
+// core with its own core blocks, ortho3Terms integrals3Terms.F90:1388),
+// copied out component-major.
+static void vector_sweeps(olcao_ctx &c, Sweep kind, const double *center, double *vv, double *cv,
+                          double *cc)
+{
+    ensure_mats(c);
+    if (center)
+        for (int x = 0; x < 3; ++x) c.dev.dipC[x] = center[x];
+    for (int comp = 0; comp < 3; ++comp) {
+        operator_sweep(c, kind, comp, 0, c.H[0].p, c.CV_H[0].p, c.CC_H[0].p);
+        copy_out(c, c.H[0].p, vv ? vv + (size_t)comp * c.matDoubles() : nullptr, c.matDoubles());
+        if (c.coreDim) {
+            copy_out(c, c.CV_H[0].p, cv ? cv + (size_t)comp * c.cvDoubles() : nullptr, c.cvDoubles());
+            copy_out(c, c.CC_H[0].p, cc ? cc + (size_t)comp * c.ccDoubles() : nullptr, c.ccDoubles());
+        }
+        HIPCHK(hipStreamSynchronize(c.stream));
+    }
+    c.haveH = false;
+}
+
+olcao_status olcao_gauss_overlap_dm(olcao_ctx *c, const double *center, double *vv, double *cv,
+                                    double *cc)
+{
+    if (!c || !center) return OLCAO_ERR_INPUT;
+    return guard(c, [&] { vector_sweeps(*c, SW_DM, center, vv, cv, cc); });
+}
+
+olcao_status olcao_gauss_overlap_mm(olcao_ctx *c, double *vv, double *cv, double *cc)
+{
+    if (!c) return OLCAO_ERR_INPUT;
+    return guard(c, [&] { vector_sweeps(*c, SW_MM, nullptr, vv, cv, cc); });
 }
 
 olcao_status olcao_elec_pot_gauss_overlap(olcao_ctx *c, int term, double *vv, double *cv, double *cc)

@@ -78,6 +78,8 @@ FUNCTIONS = {
     "olcao_gauss_overlap": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_gauss_overlap_np": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_elec_pot_gauss_overlap": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
+    "olcao_gauss_overlap_dm": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "olcao_gauss_overlap_mm": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_gauss_overlap_ham_pscf": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_secular_eqn_pscf": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
     "olcao_setup_scf": [C.c_void_p],
@@ -317,6 +319,21 @@ class OlcaoGPU:
         _check(lib().olcao_elec_pot_gauss_overlap(self.ctx, int(term), _p(vv), _p(cv), _p(cc)),
                self.ctx)
         return self._shape(vv, cv, cc)
+
+    def gaussOverlapDM(self, center):
+        """Dipole-moment matrices about `center`: three (vv, cv, cc) tuples,
+        x, y, z (gaussOverlapDM, integrals3Terms.F90:64)."""
+        vv, cv, cc = self._bufs(3)
+        ctr = np.ascontiguousarray(center, np.float64).reshape(3)
+        _check(lib().olcao_gauss_overlap_dm(self.ctx, _p(ctr), _p(vv), _p(cv), _p(cc)), self.ctx)
+        return self._shape(vv, cv, cc, 3)
+
+    def gaussOverlapMM(self):
+        """Momentum matrices (-i nabla): three (vv, cv, cc) tuples
+        (gaussOverlapMM, integrals3Terms.F90:511)."""
+        vv, cv, cc = self._bufs(3)
+        _check(lib().olcao_gauss_overlap_mm(self.ctx, _p(vv), _p(cv), _p(cc)), self.ctx)
+        return self._shape(vv, cv, cc, 3)
 
     def gaussOverlapHamPSCF(self, pot_coeffs):
         """One (vv, cv, cc) per spin; a single tuple when spin = 1."""

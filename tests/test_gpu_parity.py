@@ -151,6 +151,30 @@ def test_integral_sweeps_match_oracle(gpu, case):
     g.close()
 
 
+@pytest.mark.parametrize("case", ["two_types", "core", "core_gamma_rel", "hex_spd"])
+def test_dipole_and_momentum_sweeps_match_oracle(gpu, case):
+    """The dipole (about a fixed centre) and momentum matrices, every
+    Cartesian component: valeVale orthogonalised against the core, coreVale
+    and coreCore raw, against the oracle to 1e-10 of each matrix's scale;
+    complex and Gamma builds, core states, s-p-d bases."""
+    s = configs.small_test(**SMALL[case])
+    g, o = _pair(gpu, s)
+    center = np.array([0.7, -1.3, 2.1])
+    dm, mm = g.gaussOverlapDM(center), g.gaussOverlapMM()
+    S_cv = o.integral(oracle.OL)[1]
+    for name, dev, op in (("DM", dm, oracle.DM), ("MM", mm, oracle.MM)):
+        for comp in range(3):
+            vv, cv, cc = o.integral(op, term=comp, coef=center)
+            ref = oracle.ortho_dataset(vv, cv, cc, S_cv)
+            e = block_rel_err(dev[comp][0], ref)
+            _log(f"  {case} {name}[{comp}]: {e:.2e}")
+            assert _ortho_close(dev[comp][0], ref, 1e-10), (case, name, comp, e)
+            if cv is not None:
+                assert block_rel_err(dev[comp][1], cv) < 1e-10
+                assert block_rel_err(dev[comp][2], cc) < 1e-10
+    g.close()
+
+
 @pytest.mark.parametrize("case", ["core", "rel_spin2", "core_gamma_rel"])
 def test_solve_core_rel_spin_matches_oracle(gpu, case):
     """Eigenvalues of every spin (Mode B) and of Mode A after the core
