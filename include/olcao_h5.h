@@ -37,6 +37,19 @@ extern "C" {
  * OLCAO_ERR_INPUT when the file cannot be written. */
 olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char *path);
 
+/* Fill the dipole-moment (which bit 0, about `dipoleCenter`) and momentum
+ * (which bit 1) groups of an integral file written by
+ * olcao_write_scf_integrals_h5: computes them on the device
+ * (olcao_gauss_overlap_dm / _mm), writes each component's packed matrices
+ * into /atomIntgGroup/atomDMOverlap/<x>/<k> (atomMMOverlap likewise, x =
+ * 0000001..0000003) and sets the group's status to 1, as gaussOverlapDM /
+ * gaussOverlapMM with ortho3Terms leave the file when setupSCF computes them
+ * for properties (OLCAO.F90:290-302, integrals3Terms.F90:1388-1608).
+ * dipoleCenter may be NULL when bit 0 is clear.  OLCAO_ERR_INPUT when the
+ * file is not such an integral file. */
+olcao_status olcao_write_scf_vector_terms_h5(olcao_ctx *ctx, int which,
+                                             const double *dipoleCenter, const char *path);
+
 /* Add the eigenpairs of the last solve to the HDF5 file at `path` (created
  * when absent, OLCAO_ERR_INPUT when the path holds another kind of file;
  * groups of an earlier call are replaced) as secularEqnSCF writes

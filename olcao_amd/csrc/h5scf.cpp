@@ -221,6 +221,59 @@ extern "C" olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char 
     return OLCAO_OK;
 }
 
+extern "C" olcao_status olcao_write_scf_vector_terms_h5(olcao_ctx *ctx, int which,
+                                                        const double *dipoleCenter,
+                                                        const char *path)
+{
+    if (!ctx || !path || (which & ~3) || ((which & 1) && !dipoleCenter)) return OLCAO_ERR_INPUT;
+    long long dims[8];
+    int flags[4];
+    olcao_status st = olcao_get_dims(ctx, dims);
+    if (st == OLCAO_OK) st = olcao_get_flags(ctx, flags);
+    if (st != OLCAO_OK) return st;
+    const int n = (int)dims[0], nc = (int)dims[1], nk = (int)dims[2];
+    const int comps = flags[0] ? 1 : 2;
+    const size_t full = (size_t)comps * n * n * nk;
+    const size_t npack = (size_t)n * (n + 1) / 2;
+    std::vector<double> mat(3 * full), packed(npack * comps);
+    std::vector<double> cv(std::max<size_t>(3 * (size_t)comps * nc * n * nk, 1)),
+        cc(std::max<size_t>(3 * (size_t)comps * nc * nc * nk, 1));
+    try {
+        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+        if (H5Fis_hdf5(path) <= 0) throw H5Error{"opening a file that is not HDF5"};
+        Handle file{check(H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT), "file open"), H5Fclose};
+        for (int op = 0; op < 2; ++op) {
+            if (!(which >> op & 1)) continue;
+            const olcao_status r = op == 0
+                ? olcao_gauss_overlap_dm(ctx, dipoleCenter, mat.data(), cv.data(), cc.data())
+                : olcao_gauss_overlap_mm(ctx, mat.data(), cv.data(), cc.data());
+            if (r != OLCAO_OK) return r;
+            const std::string grp = std::string("/atomIntgGroup/") + (op == 0 ? "atomDMOverlap" : "atomMMOverlap");
+            Handle g{check(H5Gopen2(file.id, grp.c_str(), H5P_DEFAULT), "term group"), H5Gclose};
+            for (int x = 0; x < 3; ++x)
+                for (int k = 0; k < nk; ++k) {
+                    const double *m = mat.data() + x * full + (size_t)comps * n * n * k;
+                    size_t p = 0;
+                    for (int i = 0; i < n; ++i)            // column i, rows 0..i
+                        for (int j = 0; j <= i; ++j, ++p)
+                            for (int q = 0; q < comps; ++q)
+                                packed[p * comps + q] = m[((size_t)j + (size_t)i * n) * comps + q];
+                    const std::string ds = grp + "/" + name7(x + 1) + "/" + name7(k + 1);
+                    Handle d{check(H5Dopen2(file.id, ds.c_str(), H5P_DEFAULT), "term dataset"), H5Dclose};
+                    check(H5Dwrite(d.id, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, packed.data()),
+                          "term write", 0);
+                }
+            Handle a{check(H5Aopen(g.id, "status", H5P_DEFAULT), "status attribute"), H5Aclose};
+            const int one = 1;
+            check(H5Awrite(a.id, H5T_NATIVE_INT, &one), "status write", 0);
+        }
+    } catch (const H5Error &e) {
+        std::fprintf(stderr, "olcao_write_scf_vector_terms_h5: %s failed for %s\n", e.what.c_str(), path);
+        return OLCAO_ERR_INPUT;
+    }
+    return OLCAO_OK;
+}
+
 extern "C" olcao_status olcao_write_scf_eigen_h5(olcao_ctx *ctx, int numStates, const char *path)
 {
     if (!ctx || !path) return OLCAO_ERR_INPUT;

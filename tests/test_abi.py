@@ -64,7 +64,8 @@ def test_hdf5_writer_library_exports_its_symbol():
     src = open(os.path.join(ROOT, "include", "olcao_h5.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     names = set(re.findall(r"\b(olcao_[a-z_0-9]+)\s*\(", src))
-    assert names == {"olcao_write_scf_integrals_h5", "olcao_write_scf_eigen_h5"}
+    assert names == {"olcao_write_scf_integrals_h5", "olcao_write_scf_eigen_h5",
+                     "olcao_write_scf_vector_terms_h5"}
     for name in names:
         assert hasattr(lib, name), name
     lib.olcao_write_scf_integrals_h5.argtypes = [C.c_void_p, C.c_char_p]

@@ -466,6 +466,24 @@ def test_scf_integral_file(gpu, tmp_path, case):
     data, _, _ = f.read(f"{top}atomKOverlap/0000001/real0000001")
     assert data.shape == (n, n) and not data.any()
     assert f.status(top + "atomMVOverlap") == (1 if s.rel else 0)
+    f.close()
+    # the dipole and momentum groups filled in, as setupSCF does for
+    # properties (OLCAO.F90:290-302)
+    center = np.array([0.3, 0.2, -0.4])
+    lib.olcao_write_scf_vector_terms_h5.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_char_p]
+    assert lib.olcao_write_scf_vector_terms_h5(g.ctx, 3, center.ctypes.data, path.encode()) == 0
+    f = h5read.File(path)
+    S_cv = o.integral(oracle.OL)[1]
+    for group, op in (("atomDMOverlap", oracle.DM), ("atomMMOverlap", oracle.MM)):
+        assert f.status(top + group) == 1, group
+        for x in range(3):
+            ref = oracle.ortho_dataset(*o.integral(op, term=x, coef=center), S_cv)
+            for k in range(nk):
+                data, chunked, nfilt = f.read(f"{top}{group}/{x + 1:07d}/{k + 1:07d}")
+                p = oracle.pack_upper(ref if s.gamma else ref[k])
+                exp = p[:, None] if s.gamma else np.stack([p.real, p.imag], axis=1)
+                assert data.shape == exp.shape and chunked and nfilt >= 1
+                assert _ortho_close(data, exp, 1e-10), (group, x, k)
     for k in range(nk):
         cvk = cv_raw if s.gamma else cv_raw[k]
         for part, val in (("real", np.real(cvk)), ("imag", np.imag(cvk)))[:comps]:
