@@ -6,8 +6,8 @@
 //
 // Reduction.  One group of G workgroups per matrix, the whole grid launched
 // cooperatively; groups walk through the batch.  Workgroup b owns the rows
-// r = b, b+G, ... of the full Hermitian matrix: columns 0..63 in registers,
-// columns 64..n-1 in LDS, so the matrix is read from global memory once and
+// r = b, b+G, ... of the full Hermitian matrix: columns 0..127 in registers,
+// columns 128..n-1 in LDS, so the matrix is read from global memory once and
 // never written back.  Step i builds H_i = I - tau_i v_i v_i^H with
 // v_i(i+1) = 1, zeroing A(i+2:n, i) (zhetd2's order), in one pass and one
 // group barrier:
@@ -16,9 +16,10 @@
 //            p(r) = tau_i A(r,:) v_i; p(r), A(r, i+1) and the wave's share of
 //            p^H v_i are published
 //   barrier  the group's arrival counter
-//   vectors  every wave reads p and column i+1 and forms, redundantly,
-//            w = p - tau_i/2 (p^H v_i) v_i, the updated column i+1, d(i+1)
-//            and the next reflector (zlarfg)
+//   vectors  the workgroup copies p and column i+1 into LDS, each entry
+//            read from the publication buffer once per workgroup; every
+//            wave then forms, redundantly, w = p - tau_i/2 (p^H v_i) v_i,
+//            the updated column i+1, d(i+1) and the next reflector (zlarfg)
 // v, w and the column are register vectors, lane l holding entries l, l+64, ...
 //
 // Back-transform.  Q = H_0 H_1 ... H_{n-2}; for blocks of kTrdBlock
@@ -40,6 +41,7 @@ namespace {
 constexpr int kTrdThreads = 256;
 constexpr int kTrdWaves = kTrdThreads / 64;
 constexpr int kTrdK = 10;        // 64-column chunks of a row: n <= 640
+constexpr int kTrdKR = 2;        // chunks held in registers (columns 0..127)
 // the divide and conquer that follows the reduction keeps per-merge arrays
 // of kDcMaxN entries in LDS: the reduction may not take larger matrices
 static_assert(64 * kTrdK <= kDcMaxN, "tridiagonal path larger than the divide-and-conquer LDS");
@@ -215,11 +217,14 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
     // compiler treat every row-dependent branch as a scalar branch
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g = blockIdx.x % nGroups, b = blockIdx.x / nGroups;
-    const int Lr = n > 64 ? n - 64 : 0;
+    constexpr int KR = KC < kTrdKR ? KC : kTrdKR;
+    const int Lr = n > 64 * KR ? n - 64 * KR : 0;
     const int rowsPer = (n + G - 1) / G;
     // wave-uniform per-row scalars of this wave's rows, after the rows:
     // [row j][RV, RVP, RWP, PR, COLR, RCN]
     T *rsc = rows + rowsPer * Lr + wv * kTrdRpw * kRs;
+    // the step's p and column i+1 as the workgroup read them: [p (n) | column (n)]
+    T *stP = rows + rowsPer * Lr + kTrdWaves * kTrdRpw * kRs, *stC = stP + n;
     unsigned *myCnt = cnt + 64 * g;
     // two publication buffers per group: [p (n) | column (n) | wave dots (4G)]
     const long long pubLen = (long long)(2 * n + kTrdWaves * G) * E;
@@ -236,31 +241,34 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
         double *tauG = tauOut + (long long)E * n * mat;
 
         // owned rows of the full Hermitian matrix from the lower triangle;
-        // row j of this wave is r = b + (wv + 4 j) G; columns 0..63 to
+        // row j of this wave is r = b + (wv + 4 j) G; columns 0..127 to
         // registers, the rest to LDS (one row at a time)
-        T rreg[kTrdRpw];
+        T rreg[KR][kTrdRpw];
 #pragma unroll
-        for (int j = 0; j < kTrdRpw; ++j) {
-            const int r = b + (wv + kTrdWaves * j) * G, c = lane;
-            T a = N::zero();
-            if (r < n && c < n) {
-                a = c <= r ? N::load(A, r + (long long)c * n) : N::conj(N::load(A, c + (long long)r * n));
-                if (c == r) a = N::mk(N::re(a), 0.0);
+        for (int k = 0; k < KR; ++k)
+#pragma unroll
+            for (int j = 0; j < kTrdRpw; ++j) {
+                const int r = b + (wv + kTrdWaves * j) * G, c = 64 * k + lane;
+                T a = N::zero();
+                if (r < n && c < n) {
+                    a = c <= r ? N::load(A, r + (long long)c * n)
+                               : N::conj(N::load(A, c + (long long)r * n));
+                    if (c == r) a = N::mk(N::re(a), 0.0);
+                }
+                rreg[k][j] = a;
             }
-            rreg[j] = a;
-        }
 #pragma unroll 1
         for (int j = 0; j < kTrdRpw; ++j) {
             const int s = wv + kTrdWaves * j, r = b + s * G;
             if (r >= n) break;
 #pragma unroll
-            for (int k = 1; k < KC; ++k) {
+            for (int k = KR; k < KC; ++k) {
                 const int c = 64 * k + lane;
                 if (c < n) {
                     T a = c <= r ? N::load(A, r + (long long)c * n)
                                  : N::conj(N::load(A, c + (long long)r * n));
                     if (c == r) a = N::mk(N::re(a), 0.0);
-                    rows[s * Lr + c - 64] = a;
+                    rows[s * Lr + c - 64 * KR] = a;
                 }
             }
         }
@@ -324,15 +332,17 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
                     T a[kTrdRpw];
 #pragma unroll
                     for (int j = 0; j < kTrdRpw; ++j)
-                        if (live[j]) a[j] = k == 0 ? rreg[j] : rows[(wv + kTrdWaves * j) * Lr + c - 64];
+                        if (live[j])
+                            a[j] = k < KR ? rreg[k < KR ? k : 0][j]
+                                          : rows[(wv + kTrdWaves * j) * Lr + c - 64 * KR];
 #pragma unroll
                     for (int j = 0; j < kTrdRpw; ++j)
                         if (live[j]) {
                             a[j] = N::sub(N::sub(a[j], N::cmul(wp[k], vr[j])), N::cmul(vp[k], wr[j]));
-                            if (k == 0)
-                                rreg[j] = a[j];
+                            if (k < KR)
+                                rreg[k < KR ? k : 0][j] = a[j];
                             else
-                                rows[(wv + kTrdWaves * j) * Lr + c - 64] = a[j];
+                                rows[(wv + kTrdWaves * j) * Lr + c - 64 * KR] = a[j];
                             acc[j] = N::add(acc[j], N::mul(a[j], vc[k]));
                             if (k == kLive) colj[j] = wshfl<CPLX>(a[j], ci);
                         }
@@ -359,20 +369,34 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
                 return;
             }
             // ---- vectors: w, column i+1, next reflector ----
-            // every publication load is issued before any of them is used, so
-            // the step pays one memory round trip
-            T pk[KC], ck[KC];
+            // p and column i+1 (entries i+1..n-1) go through LDS, each entry
+            // loaded once per workgroup; all loads are issued before any
+            // store, so the step pays one memory round trip
+            {
+                constexpr int kPer = (64 * KC + kTrdThreads - 1) / kTrdThreads;
+                T lp[kPer], lc[kPer];
 #pragma unroll
-            for (int k = 0; k < KC; ++k) {
-                const int c = 64 * k + lane;
-                const int cc = c < n && c > i ? c : i + 1;   // in range, value unused
-                pk[k] = pldT<CPLX>(pP, cc);
-                ck[k] = pldT<CPLX>(pC, cc);
+                for (int t = 0; t < kPer; ++t) {
+                    const int c = i + 1 + (int)threadIdx.x + kTrdThreads * t;
+                    if (c < n) {
+                        lp[t] = pldT<CPLX>(pP, c);
+                        lc[t] = pldT<CPLX>(pC, c);
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < kPer; ++t) {
+                    const int c = i + 1 + (int)threadIdx.x + kTrdThreads * t;
+                    if (c < n) {
+                        stP[c] = lp[t];
+                        stC[c] = lc[t];
+                    }
+                }
             }
-            const T p1 = pldT<CPLX>(pP, i + 1);
             T part = N::zero();
             for (int q = lane; q < kTrdWaves * G; q += 64) part = N::add(part, pldT<CPLX>(pPart, q));
             const T dot = wsum<CPLX>(part);
+            __syncthreads();
+            const T p1 = stP[i + 1];
             const T alpha = N::scale(N::mul(tau, dot), -0.5);
             // w(i+1) = p(i+1) + alpha since v_i(i+1) = 1
             const T w1 = N::add(p1, alpha);
@@ -380,8 +404,9 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
             for (int k = 0; k < KC; ++k) {
                 const int c = 64 * k + lane;
                 const bool on = c < n && c > i;
-                const T wk = N::add(pk[k], N::mul(alpha, vc[k]));
-                const T cv = N::sub(N::sub(ck[k], N::cmul(w1, vc[k])), wk);
+                const T pk = on ? stP[c] : N::zero(), ck = on ? stC[c] : N::zero();
+                const T wk = N::add(pk, N::mul(alpha, vc[k]));
+                const T cv = N::sub(N::sub(ck, N::cmul(w1, vc[k])), wk);
                 vp[k] = vc[k];
                 wp[k] = N::sel(on, wk, N::zero());
                 cn[k] = N::sel(on, cv, N::zero());
@@ -586,14 +611,17 @@ bool trd_plan(int n, int nb, bool cplx, TrdPlan &p)
     p = TrdPlan{};
     if (n < 2 || nb < 1 || n > 64 * kTrdK) return false;
     const size_t el = cplx ? 16 : 8;
-    const size_t Lr = n > 64 ? (size_t)n - 64 : 0;
+    // LDS: the owned rows past the register columns, the per-row scalars,
+    // the staged p and column
+    const size_t Lr = n > 64 * kTrdKR ? (size_t)n - 64 * kTrdKR : 0;
+    const size_t fixed = ((size_t)kTrdWaves * kTrdRpw * kRs + 2 * (size_t)n) * el;
     int G = (n + kTrdWaves * kTrdRpw - 1) / (kTrdWaves * kTrdRpw);
     for (; G <= 64; ++G) {
         const size_t rowsPer = ((size_t)n + G - 1) / G;
-        if (rowsPer * Lr * el + (size_t)kTrdWaves * kTrdRpw * kRs * el <= kTrdLdsMax) break;
+        if (rowsPer * Lr * el + fixed <= kTrdLdsMax) break;
     }
     if (G > 64) return false;
-    const size_t lds = (((size_t)n + G - 1) / G) * Lr * el + (size_t)kTrdWaves * kTrdRpw * kRs * el;
+    const size_t lds = (((size_t)n + G - 1) / G) * Lr * el + fixed;
     int dev = 0, cus = 0, perCU = 0;
     if (hipGetDevice(&dev) != hipSuccess) return false;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
