@@ -126,7 +126,14 @@ typedef struct {
 /* Context lifetime.  Replaces the derived setup of OLCAO.F90:setupSCF
  * (:196-244): initializeLattice (lattice.f90:310), initializeFindVec (:892),
  * initializeKPoints (kpoints.f90:937), renormalizeBasis (basis.f90:55),
- * makeAlphaDist (gaussRelations.f90:64), plus the screened work lists. */
+ * makeAlphaDist (gaussRelations.f90:64), plus the screened work lists.
+ *
+ * Threads: one context is used by one host thread at a time.  Several
+ * contexts are not to be iterated from concurrent host threads outside
+ * olcao_scf_iteration_batch: with rocSOLVER solves (n > 640) overlapping
+ * other contexts' kernels, wrong eigenvalues were observed once for one or
+ * two of 19 such cells, with the cause not found; the batched call runs
+ * those solves alone. */
 olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
                               olcao_ctx **out);
 void olcao_ctx_destroy(olcao_ctx *ctx);
