@@ -281,6 +281,24 @@ olcao_status olcao_scf_iteration_batch(int nCells, olcao_ctx *const *ctxs,
                                        double *const *evals, int nWorkers, double *cellMs,
                                        int *failedCell);
 
+/* The batched iteration with a choice of how the cells' integral builds
+ * share the device (no counterpart in the reference).
+ * OLCAO_BATCH_CELLS: as olcao_scf_iteration_batch, every cell's whole build
+ * on its own streams, concurrently from the worker threads.
+ * OLCAO_BATCH_SHARED: per spin, the cells' own stages (overlap sweep,
+ * kinetic blocks; afterwards contraction, Bloch sums, orthogonalisation)
+ * from the worker threads, and between them the fused Hamiltonian sweep's
+ * three-centre work of all cells in shared launches, one per angular class
+ * and group of cells, waves taking 64 chunks of one cell from a task list
+ * that interleaves the cells' chunk lists heaviest first.
+ * Each cell's result is the same in both modes; other arguments as
+ * olcao_scf_iteration_batch. */
+enum { OLCAO_BATCH_CELLS = 0, OLCAO_BATCH_SHARED = 1 };
+olcao_status olcao_scf_iteration_batch_mode(int nCells, olcao_ctx *const *ctxs,
+                                            const double *const *potCoeffs, const int *numStates,
+                                            double *const *evals, int nWorkers, int mode,
+                                            double *cellMs, int *failedCell);
+
 /* Stage times (ms, HIP events) of the last olcao_scf_iteration:
  * t[0] overlap sweep (incl. its contraction and Bloch sum), t[1] Hamiltonian
  * primitive blocks, t[2] Hamiltonian contraction + Bloch sum (+ ortho),

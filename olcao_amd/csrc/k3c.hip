@@ -34,14 +34,15 @@ namespace olcao {
 // pair and the potential exponent only), which takes the division and the
 // square root out of the item loop.  MIX: mixed-precision exponential.
 // One lane per chunk of a record's image list (chRec, chBeg, chEnd; n
-// chunks); the chunk's Hermite sums go to W[e][chunk].
+// chunks); the chunk's Hermite sums go to W[e][chunk].  The wave takes the
+// chunks i0 .. i0 + 63.
 template <int LA, int LB, bool NUC, bool TAB, bool MIX>
-__global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs, int n,
-                                                     const int *chRec, const long long *chBeg,
-                                                     const long long *chEnd,
-                                                     const unsigned long long *img, int kind,
-                                                     int termType, int termIdx,
-                                                     const double *coef, int addW, double *W)
+__device__ __forceinline__ void three_center_wave(const DevCtx &c, const PPRec *recs, int n, int i0,
+                                                  const int *chRec, const long long *chBeg,
+                                                  const long long *chEnd,
+                                                  const unsigned long long *img, int kind,
+                                                  int termType, int termIdx, const double *coef,
+                                                  int addW, double *W)
 {
     constexpr int L = LA + LB, NH = nherm(L);
     __shared__ double tAlpha[kMaxPotTypes * kMaxPotAlphas];
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
     for (int j = lane; j < 256; j += 64) sExp2[j] = exp2(j * (1.0 / 256.0));
     __syncthreads();
 
-    const int i = blockIdx.x * 64 + lane;
+    const int i = i0 + lane;
     const bool active = i < n;
     const int ri = active ? i : 0;
     const PPRec rec = recs[chRec[ri]];
@@ -176,6 +177,38 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
     }
 }
 
+template <int LA, int LB, bool NUC, bool TAB, bool MIX>
+__global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs, int n,
+                                                     const int *chRec, const long long *chBeg,
+                                                     const long long *chEnd,
+                                                     const unsigned long long *img, int kind,
+                                                     int termType, int termIdx,
+                                                     const double *coef, int addW, double *W)
+{
+    three_center_wave<LA, LB, NUC, TAB, MIX>(c, recs, n, (int)blockIdx.x * 64, chRec, chBeg, chEnd,
+                                             img, kind, termType, termIdx, coef, addW, W);
+}
+
+// The fused Hamiltonian sweep of many cells in one launch: wave b takes the
+// 64 chunks of task b (a cell and its first chunk) with that cell's system,
+// lists and coefficients.  NUC: the nuclear pass over the compacted
+// entries, added onto the three-Gaussian pass's W.
+template <int LA, int LB, bool NUC, bool TAB, bool MIX>
+__global__ __launch_bounds__(64) void k_three_center_multi(const DevCtx *ctxs, const Cell3c *cells,
+                                                           const Task3c *tasks)
+{
+    const Task3c tk = tasks[blockIdx.x];
+    const Cell3c a = cells[tk.cell];
+    const DevCtx c = ctxs[a.ctx];
+    if (NUC)
+        three_center_wave<LA, LB, true, false, false>(c, a.recs, a.nChunk, tk.first, a.chRec,
+                                                      a.nOff, a.nOff + 1, a.imgN, K3_HAM, 0, 0,
+                                                      a.coef, 1, a.W);
+    else
+        three_center_wave<LA, LB, false, TAB, MIX>(c, a.recs, a.nChunk, tk.first, a.chRec, a.chBeg,
+                                                   a.chEnd, a.img, K3_HAM, 0, 0, a.coef, 0, a.W);
+}
+
 // W -> Cartesian block -> solid harmonics, one thread per primitive pair:
 //   <a|O|b> = K_AB sum_tuv Ex^{ax bx}_t Ey^{ay by}_u Ez^{az bz}_v W_tuv,
 // then the reference's Cartesian -> solid-harmonic map, one ket harmonic
@@ -188,12 +221,12 @@ __global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs
 constexpr int kFinishStage = 81;
 
 // W of record i: the sum of its chunks' W in entry order (rcOff, rcList;
-// nChunk columns).
+// nChunk columns).  The wave takes the records finOrder[li0 .. li0 + 63].
 template <int LA, int LB>
-__global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const double *W, int nrec,
-                                                    const int *finOrder, const int *rcOff,
-                                                    const int *rcList, int nChunk,
-                                                    int accumulate, int addKE, double *blocks)
+__device__ __forceinline__ void herm_finish_wave(const PPRec *recs, const double *W, int nrec,
+                                                 int li0, const int *finOrder, const int *rcOff,
+                                                 const int *rcList, int nChunk, int accumulate,
+                                                 int addKE, double *blocks)
 {
     constexpr int L = LA + LB, NH = nherm(L);
     constexpr int NA = ncart(LA), SA = nsh(LA), SB = nsh(LB);
@@ -201,7 +234,7 @@ __global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const dou
     __shared__ double sOut[STAGE ? 64 * SA * SB + 1 : 1];
     __shared__ long long sOff[64];
     const int lane = threadIdx.x;
-    const int li = blockIdx.x * 64 + lane;
+    const int li = li0 + lane;
     const bool active = li < nrec;
     const int i = finOrder[active ? li : nrec - 1];
     const PPRec rec = recs[i];
@@ -262,22 +295,32 @@ __global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const dou
         }
         sOff[lane] = active ? rec.off : -1;
         __syncthreads();
-        store_staged_blocks<SA * SB>(sOut, sOff, min(64, nrec - (int)blockIdx.x * 64), accumulate,
-                                     blocks);
+        store_staged_blocks<SA * SB>(sOut, sOff, min(64, nrec - li0), accumulate, blocks);
     }
 }
 
-// Dynamic LDS of the three-centre kernels: site positions and types, then
-// (TAB) the per-lane term table.
-static size_t site_bytes(const DevCtx &c)
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void k_herm_finish(const PPRec *recs, const double *W, int nrec,
+                                                    const int *finOrder, const int *rcOff,
+                                                    const int *rcList, int nChunk,
+                                                    int accumulate, int addKE, double *blocks)
 {
-    return (size_t)(((3 * c.nPotSites + (c.nPotSites + 1) / 2 + 1) & ~1)) * sizeof(double);
+    herm_finish_wave<LA, LB>(recs, W, nrec, (int)blockIdx.x * 64, finOrder, rcOff, rcList, nChunk,
+                             accumulate, addKE, blocks);
 }
-static size_t table_bytes(const DevCtx &c)
+
+// The finish of many cells in one launch: wave b takes 64 records of task b.
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void k_herm_finish_multi(const Cell3c *cells, const Task3c *tasks)
 {
-    return (size_t)c.nPotTypes * c.maxPA * 64 * 2 * sizeof(double);
+    const Task3c tk = tasks[blockIdx.x];
+    const Cell3c a = cells[tk.cell];
+    herm_finish_wave<LA, LB>(a.recs, a.W, a.nRec, tk.first, a.finOrder, a.rcOff, a.rcList,
+                             a.nChunk, a.finAcc, a.finKE, a.blocks);
 }
-constexpr size_t kTableLdsMax = 48 * 1024;
+
+static size_t site_bytes(const DevCtx &c) { return k3c_site_bytes(c); }
+static size_t table_bytes(const DevCtx &c) { return k3c_table_bytes(c); }
 
 template <int LA, int LB>
 static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, int ti,
@@ -324,6 +367,32 @@ static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, i
                        addKE ? 0 : (accumulate & 1), addKE ? 1 : 0, blocks);
 }
 
+// The fused Hamiltonian sweep of class (LA, LB) over the cells of a batch:
+// the three-Gaussian pass per kernel form, the nuclear pass, the finish.
+template <int LA, int LB>
+static void three_multi(const Multi3c &m, hipStream_t s)
+{
+    for (int v = 0; v < 3; ++v) {
+        if (m.nE3c[v] == 0) continue;
+        const dim3 g(m.nE3c[v]);
+        if (v == 0)
+            hipLaunchKernelGGL((k_three_center_multi<LA, LB, false, true, true>), g, dim3(64),
+                               m.ldsE3c[v], s, m.ctxs, m.cells, m.e3c[v]);
+        else if (v == 1)
+            hipLaunchKernelGGL((k_three_center_multi<LA, LB, false, true, false>), g, dim3(64),
+                               m.ldsE3c[v], s, m.ctxs, m.cells, m.e3c[v]);
+        else
+            hipLaunchKernelGGL((k_three_center_multi<LA, LB, false, false, false>), g, dim3(64),
+                               m.ldsE3c[v], s, m.ctxs, m.cells, m.e3c[v]);
+    }
+    if (m.nNuc)
+        hipLaunchKernelGGL((k_three_center_multi<LA, LB, true, false, false>), dim3(m.nNuc),
+                           dim3(64), m.ldsNuc, s, m.ctxs, m.cells, m.nuc);
+    if (m.nFin)
+        hipLaunchKernelGGL((k_herm_finish_multi<LA, LB>), dim3(m.nFin), dim3(64), 0, s, m.cells,
+                           m.fin);
+}
+
 #define OLCAO_CAT2(a, b) a##b
 #define OLCAO_CAT(a, b) OLCAO_CAT2(a, b)
 // The classes (OLCAO_LA, lb) whose bit 4 OLCAO_LA + lb is set in mask.
@@ -336,6 +405,18 @@ void OLCAO_CAT(launch_3c_la, OLCAO_LA)(const DevCtx &c, const ClassLists &cl, in
     if (mask >> (k0 + 1) & 1u) three_one<OLCAO_LA, 1>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
     if (mask >> (k0 + 2) & 1u) three_one<OLCAO_LA, 2>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
     if (mask >> (k0 + 3) & 1u) three_one<OLCAO_LA, 3>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+}
+
+// The batched class m.k = 4 OLCAO_LA + lb.
+void OLCAO_CAT(launch_3c_multi_la, OLCAO_LA)(const Multi3c &m, hipStream_t s)
+{
+    switch (m.k - 4 * OLCAO_LA) {
+    case 0: three_multi<OLCAO_LA, 0>(m, s); break;
+    case 1: three_multi<OLCAO_LA, 1>(m, s); break;
+    case 2: three_multi<OLCAO_LA, 2>(m, s); break;
+    case 3: three_multi<OLCAO_LA, 3>(m, s); break;
+    default: break;
+    }
 }
 
 }  // namespace olcao

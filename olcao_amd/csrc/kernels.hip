@@ -420,6 +420,21 @@ void launch_3c_la2(const DevCtx &, const ClassLists &, int, int, int, const doub
 void launch_3c_la3(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
                    double *, int, unsigned, hipStream_t);
 
+void launch_3c_multi_la0(const Multi3c &, hipStream_t);
+void launch_3c_multi_la1(const Multi3c &, hipStream_t);
+void launch_3c_multi_la2(const Multi3c &, hipStream_t);
+void launch_3c_multi_la3(const Multi3c &, hipStream_t);
+
+void launch_three_center_multi(const Multi3c &m, hipStream_t s)
+{
+    switch (m.k / 4) {
+    case 0: launch_3c_multi_la0(m, s); break;
+    case 1: launch_3c_multi_la1(m, s); break;
+    case 2: launch_3c_multi_la2(m, s); break;
+    default: launch_3c_multi_la3(m, s); break;
+    }
+}
+
 void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind, int tt, int ti,
                          const double *coef, int accumulate, double *blocks, int mixed,
                          unsigned mask, hipStream_t s)

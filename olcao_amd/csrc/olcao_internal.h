@@ -172,6 +172,59 @@ constexpr unsigned kFusedKEClasses = 0x37ffu;
 void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind, int termType,
                          int termIdx, const double *coef, int accumulate, double *blocks,
                          int mixed, unsigned mask, hipStream_t s);
+// Dynamic LDS of the three-centre kernels: site positions and types, then
+// (the tabulated form, used when the table fits kTableLdsMax) the per-lane
+// term table of mu and the prefactor.
+inline size_t k3c_site_bytes(const DevCtx &c)
+{
+    return (size_t)(((3 * c.nPotSites + (c.nPotSites + 1) / 2 + 1) & ~1)) * sizeof(double);
+}
+inline size_t k3c_table_bytes(const DevCtx &c)
+{
+    return (size_t)c.nPotTypes * c.maxPA * 64 * 2 * sizeof(double);
+}
+constexpr size_t kTableLdsMax = 48 * 1024;
+
+// The fused Hamiltonian sweep (K3_HAM) of many cells, one launch per class
+// and kernel form over all of them.  Cell3c: one cell's lists and buffers
+// for one class, as launch_three_center takes them from its context; the
+// finish flags as launch_three_center derives them from `accumulate`.
+struct Cell3c {
+    int ctx;                    // the cell's entry in Multi3c::ctxs
+    int nChunk, nRec;
+    const PPRec *recs;
+    const int *chRec;
+    const long long *chBeg, *chEnd, *nOff;
+    const unsigned long long *img, *imgN;
+    const double *coef;         // the spin's potential coefficients
+    double *W;
+    const int *finOrder, *rcOff, *rcList;
+    double *blocks;
+    int finAcc, finKE;          // finish: add onto the store / add the kinetic block
+};
+// One wave of a batched launch: 64 chunks (or finish records) of one cell
+// from `first` on.
+struct Task3c {
+    int cell, first;
+};
+// The launches of class k over a batch (device pointers).  e3c[v]: the
+// three-Gaussian pass in kernel form v (0 tabulated + mixed-precision
+// exponential, 1 tabulated, 2 untabulated); nuc: the nuclear pass of the
+// cells with nuclear terms; fin: the finish.  LDS: the largest of the
+// cells' in each launch.
+struct Multi3c {
+    int k;
+    const DevCtx *ctxs;
+    const Cell3c *cells;
+    const Task3c *e3c[3];
+    int nE3c[3];
+    const Task3c *nuc;
+    int nNuc;
+    const Task3c *fin;
+    int nFin;
+    size_t ldsE3c[3], ldsNuc;
+};
+void launch_three_center_multi(const Multi3c &m, hipStream_t s);
 // Image lists: pass 1 counts entries and fused-sweep items per record, pass 2
 // writes the entries at off[i]; *overflow is set if a lattice index does not
 // fit the entry.

@@ -238,6 +238,9 @@ struct olcao_ctx {
     // the batched call's grouped solves: several cells' matrices in one batch
     DBuf<double> gA, gB, gD, gE;
     DBuf<int> gInfo, gTInfo;
+    // the batched call's three-centre launch arguments (DevCtx, Cell3c and
+    // Task3c arrays of all cells), held by the batch's first cell
+    DBuf<unsigned char> dBatch;
     DBuf<double> H[2], CV_H[2], CC_H[2];   // per spin
     DBuf<unsigned long long> dCounters;
     // Mode A term matrices (orthogonalised when coreDim > 0)
@@ -2113,6 +2116,207 @@ static void iteration_replay(olcao_ctx &c, const double *coef, int numStates, do
 // (the batched device path or rocSOLVER's strided-batched one), and the
 // eigenvalues, eigenvectors (kept for the valence density) and stage times
 // go back to each member.  A member is a (cell, spin).
+// ---------------------------------------------------------------------------
+// The batched call's build.  Per spin q, three phases:
+//   A  on each cell's streams, from the worker threads: the coefficients,
+//      the overlap sweep (q = 0), the kinetic (and mass-velocity) blocks;
+//   B  the fused Hamiltonian three-centre work of every cell, one launch per
+//      class and kernel form over all cells (batch_three_center);
+//   C  again per cell: contraction, Bloch sums, orthogonalisation.
+// Each cell's blocks, W and matrices are its own, computed by the same
+// kernels in the same order as in its single-context iteration.
+static void batch_phase_a(olcao_ctx &c, const double *coef, int q)
+{
+    hipStream_t s = c.stream;
+    if (q == 0) {
+        c.trdMs = 0.0;
+        upload_coef(c, coef);
+        HIPCHK(hipEventRecord(c.ev[0], s));
+        overlap_sweep(c);
+        HIPCHK(hipEventRecord(c.ev[1], s));
+    }
+    two_center_forked(c, OP_KE, 0, c.rel ? 0xffffu : ~kFusedKEClasses & 0xffffu);
+    if (c.rel) two_center_forked(c, OP_MV, 1);
+    launch_fill(reinterpret_cast<double *>(c.dCounters.p), 32, 0.0, s);
+    HIPCHK(hipEventRecord(c.ev[2 + 3 * q], s));
+}
+
+static void batch_phase_c(olcao_ctx &c, int q, hipEvent_t built)
+{
+    hipStream_t s = c.stream;
+    HIPCHK(hipStreamWaitEvent(s, built, 0));
+    HIPCHK(hipEventRecord(c.ev[3 + 3 * q], s));
+    contract_scatter(c, c.H[q].p, c.CV_H[q].p, c.CC_H[q].p);
+    ortho(c, c.H[q].p, c.CV_H[q].p, c.CC_H[q].p, false);
+    HIPCHK(hipEventRecord(c.ev[4 + 3 * q], s));
+    if (q == c.spin - 1) c.haveH = true;
+}
+
+// Phase B of spin q over `cells` (every cell's phase A complete): the
+// launch arguments go to the device in one copy, then the launches run on
+// own's streams.  A launch unit is one class over one group of cells: the
+// three-Gaussian pass per kernel form, the nuclear pass, the finish, in
+// that order on one stream; units are spread over the streams by work,
+// heaviest first.  Cells are grouped by the dynamic LDS their kernels take
+// (a launch reserves the largest of its cells') and each such set is split
+// in two, alternating down the weight order, so that the heavy classes have
+// independent launches for the other streams.  Waves take 64 chunks of one
+// cell; the cells' chunk lists are interleaved task by task (each list runs
+// heaviest first).  Leaves own.ev[11] recorded after all of it.
+static void batch_three_center(olcao_ctx *const *ctxs, const std::vector<int> &cells, int q,
+                               olcao_ctx &own)
+{
+    const int nc = (int)cells.size();
+    std::vector<DevCtx> dcs(nc);
+    for (int j = 0; j < nc; ++j) dcs[j] = ctxs[cells[j]]->dev;
+    auto tabbed = [](const olcao_ctx &c) { return k3c_table_bytes(c.dev) <= kTableLdsMax; };
+    constexpr int kSplit = 2;
+    std::vector<int> group(nc);
+    int nGroups = 0;
+    {
+        std::map<size_t, std::vector<int>> byLds;
+        for (int j = 0; j < nc; ++j) {
+            const olcao_ctx &c = *ctxs[cells[j]];
+            byLds[k3c_site_bytes(c.dev) + (tabbed(c) ? k3c_table_bytes(c.dev) : 0)].push_back(j);
+        }
+        for (auto &kv : byLds) {
+            for (size_t t = 0; t < kv.second.size(); ++t) group[kv.second[t]] = nGroups + (int)(t % kSplit);
+            nGroups += (int)std::min<size_t>(kSplit, kv.second.size());
+        }
+    }
+    std::vector<Cell3c> args;
+    struct Unit {
+        int k = 0;
+        std::vector<Task3c> lists[5];   // e3c forms 0..2, nuclear, finish
+        size_t ldsE3c[3] = {0, 0, 0}, ldsNuc = 0;
+        double weight = 0.0;
+        size_t off[5] = {};
+    };
+    std::vector<Unit> units;
+    for (int k = 0; k < 16; ++k) {
+        const int L = k / 4 + k % 4;
+        const bool fuse = (kFusedKEClasses >> k) & 1u;
+        for (int g = 0; g < nGroups; ++g) {
+            Unit U;
+            U.k = k;
+            std::vector<int> members, form;
+            for (int j = 0; j < nc; ++j) {
+                const olcao_ctx &c = *ctxs[cells[j]];
+                if (group[j] != g || c.cls.count[k] == 0) continue;
+                Cell3c a{};
+                a.ctx = j;
+                a.nChunk = c.cls.nChunk[k];
+                a.nRec = c.cls.count[k];
+                a.recs = c.cls.rec[k];
+                a.chRec = c.cls.chRec[k];
+                a.chBeg = c.cls.chBeg[k];
+                a.chEnd = c.cls.chEnd[k];
+                a.nOff = c.cls.nOff[k];
+                a.img = c.cls.img[k];
+                a.imgN = c.cls.imgN[k];
+                a.coef = c.dCoef.p + (size_t)q * c.potDim;
+                a.W = c.cls.W[k];
+                a.finOrder = c.cls.finOrder[k];
+                a.rcOff = c.cls.rcOff[k];
+                a.rcList = c.cls.rcList[k];
+                a.blocks = c.dBlocks.p;
+                // as launch_three_center with accumulate = 1 (| kAccAddKE when rel = 0)
+                const bool addKE = fuse && !c.rel;
+                a.finAcc = addKE ? 0 : 1;
+                a.finKE = addKE ? 1 : 0;
+                const bool tab = tabbed(c);
+                const int v = tab ? (c.precision == OLCAO_PREC_MIXED ? 0 : 1) : 2;
+                U.ldsE3c[v] = std::max(U.ldsE3c[v], k3c_site_bytes(c.dev) +
+                                                        (tab ? k3c_table_bytes(c.dev) : 0) + 16);
+                if (c.dev.anyNuc) U.ldsNuc = std::max(U.ldsNuc, k3c_site_bytes(c.dev) + 16);
+                U.weight += (double)a.nChunk * ((L + 1) * (L + 2) * (L + 3) / 6);
+                members.push_back((int)args.size());
+                form.push_back(v);
+                args.push_back(a);
+            }
+            if (members.empty()) continue;
+            // tasks interleaved across the cells: every cell's t-th wave, then t + 1
+            for (int t = 0;; ++t) {
+                bool any = false;
+                for (size_t m = 0; m < members.size(); ++m) {
+                    const Cell3c &a = args[members[m]];
+                    if (64 * t < a.nChunk) {
+                        U.lists[form[m]].push_back({members[m], 64 * t});
+                        if (dcs[a.ctx].anyNuc) U.lists[3].push_back({members[m], 64 * t});
+                        any = true;
+                    }
+                    if (64 * t < a.nRec) {
+                        U.lists[4].push_back({members[m], 64 * t});
+                        any = true;
+                    }
+                }
+                if (!any) break;
+            }
+            units.push_back(std::move(U));
+        }
+    }
+    // one buffer: [DevCtx x nc][Cell3c x args][every unit's task lists]
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t off = up(sizeof(DevCtx) * nc);
+    const size_t offCells = off;
+    off = up(off + sizeof(Cell3c) * std::max<size_t>(1, args.size()));
+    for (Unit &U : units)
+        for (int l = 0; l < 5; ++l) {
+            U.off[l] = off;
+            off = up(off + sizeof(Task3c) * U.lists[l].size());
+        }
+    std::vector<unsigned char> host(off);
+    std::memcpy(host.data(), dcs.data(), sizeof(DevCtx) * nc);
+    if (!args.empty()) std::memcpy(host.data() + offCells, args.data(), sizeof(Cell3c) * args.size());
+    for (const Unit &U : units)
+        for (int l = 0; l < 5; ++l)
+            if (!U.lists[l].empty())
+                std::memcpy(host.data() + U.off[l], U.lists[l].data(), sizeof(Task3c) * U.lists[l].size());
+    own.dBatch.alloc(off);
+    HIPCHK(hipMemcpy(own.dBatch.p, host.data(), off, hipMemcpyHostToDevice));
+    unsigned char *base = own.dBatch.p;
+    // units over own's streams, heaviest first onto the least-loaded one
+    constexpr int nS = 1 + olcao_ctx::kSide;
+    hipStream_t st[nS] = {own.stream};
+    for (int i = 1; i < nS; ++i) st[i] = own.side[i - 1];
+    std::vector<int> order(units.size());
+    for (size_t u = 0; u < units.size(); ++u) order[u] = (int)u;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return units[a].weight > units[b].weight; });
+    double load[nS] = {};
+    bool used[nS] = {};
+    HIPCHK(hipEventRecord(own.evFork, own.stream));
+    for (int u : order) {
+        const Unit &U = units[u];
+        const int i = (int)(std::min_element(load, load + nS) - load);
+        load[i] += U.weight;
+        if (i > 0 && !used[i]) HIPCHK(hipStreamWaitEvent(st[i], own.evFork, 0));
+        used[i] = true;
+        Multi3c m{};
+        m.k = U.k;
+        m.ctxs = reinterpret_cast<const DevCtx *>(base);
+        m.cells = reinterpret_cast<const Cell3c *>(base + offCells);
+        for (int v = 0; v < 3; ++v) {
+            m.e3c[v] = reinterpret_cast<const Task3c *>(base + U.off[v]);
+            m.nE3c[v] = (int)U.lists[v].size();
+            m.ldsE3c[v] = U.ldsE3c[v];
+        }
+        m.nuc = reinterpret_cast<const Task3c *>(base + U.off[3]);
+        m.nNuc = (int)U.lists[3].size();
+        m.ldsNuc = U.ldsNuc;
+        m.fin = reinterpret_cast<const Task3c *>(base + U.off[4]);
+        m.nFin = (int)U.lists[4].size();
+        launch_three_center_multi(m, st[i]);
+    }
+    HIPCHK(hipGetLastError());
+    for (int i = 1; i < nS; ++i)
+        if (used[i]) {
+            HIPCHK(hipEventRecord(own.evJoin[i - 1], st[i]));
+            HIPCHK(hipStreamWaitEvent(own.stream, own.evJoin[i - 1], 0));
+        }
+    HIPCHK(hipEventRecord(own.ev[11], own.stream));
+}
+
 struct SolveMember {
     int cell, spin;
 };
@@ -2223,8 +2427,18 @@ olcao_status olcao_scf_iteration_batch(int nCells, olcao_ctx *const *ctxs,
                                        double *const *evals, int nWorkers, double *cellMs,
                                        int *failedCell)
 {
+    return olcao_scf_iteration_batch_mode(nCells, ctxs, potCoeffs, numStates, evals, nWorkers,
+                                          OLCAO_BATCH_CELLS, cellMs, failedCell);
+}
+
+olcao_status olcao_scf_iteration_batch_mode(int nCells, olcao_ctx *const *ctxs,
+                                            const double *const *potCoeffs, const int *numStates,
+                                            double *const *evals, int nWorkers, int mode,
+                                            double *cellMs, int *failedCell)
+{
     if (failedCell) *failedCell = -1;
     if (nCells <= 0 || !ctxs || !potCoeffs || !numStates || !evals) return OLCAO_ERR_INPUT;
+    if (mode != OLCAO_BATCH_CELLS && mode != OLCAO_BATCH_SHARED) return OLCAO_ERR_INPUT;
     for (int i = 0; i < nCells; ++i)
         if (!ctxs[i] || !potCoeffs[i] || !evals[i]) {
             if (failedCell) *failedCell = i;
@@ -2263,16 +2477,45 @@ olcao_status olcao_scf_iteration_batch(int nCells, olcao_ctx *const *ctxs,
         work();
         for (auto &th : pool) th.join();
     };
-    // 1. every cell's integral build, concurrently, each followed by its
-    // solve when that is the LDS solver
+    // 1. every cell's integral build, each cell followed by its solve when
+    // that is the LDS solver.  OLCAO_BATCH_CELLS: each cell's whole build on
+    // its own streams from the worker threads, concurrently.
+    // OLCAO_BATCH_SHARED: per spin, the cells' own stages from the worker
+    // threads and their three-centre work in shared launches over all cells.
     std::vector<char> alone(nCells);
     std::vector<int> serial;
-    for (int i : order)
+    int maxSpin = 1;
+    for (int i : order) {
         if ((alone[i] = !concurrent_solve(*ctxs[i]))) serial.push_back(i);
-    run(order, nWorkers, [&](int i) {
-        iteration_build(*ctxs[i], potCoeffs[i]);
-        if (!alone[i]) iteration_solve(*ctxs[i], numStates[i], evals[i]);
-    });
+        maxSpin = std::max(maxSpin, ctxs[i]->spin);
+    }
+    if (mode == OLCAO_BATCH_CELLS)
+        run(order, nWorkers, [&](int i) {
+            iteration_build(*ctxs[i], potCoeffs[i]);
+            if (!alone[i]) iteration_solve(*ctxs[i], numStates[i], evals[i]);
+        });
+    for (int q = 0; q < maxSpin && mode == OLCAO_BATCH_SHARED && !failed.load(); ++q) {
+        std::vector<int> list;
+        for (int i : order)
+            if (ctxs[i]->spin > q) list.push_back(i);
+        run(list, nWorkers, [&](int i) { batch_phase_a(*ctxs[i], potCoeffs[i], q); });
+        if (failed.load()) break;
+        olcao_ctx &own = *ctxs[list[0]];
+        const olcao_status r = guard(&own, [&] {
+            HIPCHK(hipDeviceSynchronize());
+            batch_three_center(ctxs, list, q, own);
+        });
+        if (r != OLCAO_OK) {
+            st[list[0]] = r;
+            failed.store(true);
+            break;
+        }
+        run(list, nWorkers, [&](int i) {
+            olcao_ctx &c = *ctxs[i];
+            batch_phase_c(c, q, own.ev[11]);
+            if (q == c.spin - 1 && !alone[i]) iteration_solve(c, numStates[i], evals[i]);
+        });
+    }
     // 2. the other solves one at a time with nothing else on the device.
     // The tridiagonal path's reduction is a cooperative launch that needs
     // all of its workgroups resident.  rocSOLVER's solves gave wrong

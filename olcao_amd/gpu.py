@@ -31,6 +31,7 @@ LIB_PATH = os.path.join(HERE, "libolcao_gpu.so")
 STATUS = {0: "OK", 1: "HIP", 2: "ALLOC", 3: "INPUT", 4: "CELLS", 5: "NOT_SPD",
           6: "SOLVER", 7: "UNSUPPORTED", 8: "STATE", 9: "NO_DEVICE"}
 PREC_FP64, PREC_MIXED = 0, 1
+BATCH_CELLS, BATCH_SHARED = 0, 1
 
 
 class OlcaoError(RuntimeError):
@@ -89,6 +90,8 @@ FUNCTIONS = {
     "olcao_vale_charge": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_scf_iteration_batch": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_int, C.c_void_p, C.c_void_p],
+    "olcao_scf_iteration_batch_mode": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_int, C.c_int, C.c_void_p, C.c_void_p],
     "olcao_get_timings": [C.c_void_p, C.c_void_p],
     "olcao_get_graph_state": [C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_get_class_pairs": [C.c_void_p, C.c_void_p],
@@ -433,11 +436,13 @@ class OlcaoGPU:
         return st.value, (why.value or b"").decode()
 
 
-def scf_iteration_batch(cells, num_states, workers=8):
+def scf_iteration_batch(cells, num_states, workers=8, mode=BATCH_CELLS):
     """Batched Mode-B iteration over independent cells (OlcaoGPU contexts,
-    one per cell) in one call: olcao_scf_iteration_batch.  num_states: one
-    count per cell.  Returns (eigenvalues per cell as scf_iteration gives
-    them, each cell's latency in ms within the batch)."""
+    one per cell) in one call: olcao_scf_iteration_batch_mode.  num_states:
+    one count per cell; mode: BATCH_CELLS (each cell's build on its own
+    streams) or BATCH_SHARED (the cells' three-centre work in shared
+    launches).  Returns (eigenvalues per cell as scf_iteration gives them,
+    each cell's latency in ms within the batch)."""
     n = len(cells)
     coefs = [c._coeffs(c.sysin.pot_coeffs) for c in cells]
     ws = [np.zeros((c.spin, c.nk, ns)) for c, ns in zip(cells, num_states)]
@@ -447,9 +452,10 @@ def scf_iteration_batch(cells, num_states, workers=8):
     ns = np.asarray(num_states, np.int32)
     ms = np.zeros(n)
     bad = np.full(1, -1, np.int32)
-    code = lib().olcao_scf_iteration_batch(n, C.cast(ctxs, C.c_void_p), C.cast(cp, C.c_void_p),
-                                           _p(ns), C.cast(wp, C.c_void_p), workers, _p(ms),
-                                           _p(bad))
+    code = lib().olcao_scf_iteration_batch_mode(n, C.cast(ctxs, C.c_void_p),
+                                                C.cast(cp, C.c_void_p), _p(ns),
+                                                C.cast(wp, C.c_void_p), workers, mode, _p(ms),
+                                                _p(bad))
     if code != 0:
         msg = lib().olcao_last_error(None).decode()
         raise OlcaoError(code, f"cell {int(bad[0])}: {msg}")

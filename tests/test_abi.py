@@ -44,6 +44,8 @@ def test_null_arguments_are_rejected():
     bad = C.c_int(7)
     assert lib.olcao_scf_iteration_batch(0, None, None, None, None, 4, None, C.byref(bad)) == 3
     assert bad.value == -1
+    assert lib.olcao_scf_iteration_batch_mode(0, None, None, None, None, 4, 1, None,
+                                              C.byref(bad)) == 3
 
 
 def test_fortran_facade_source_names_reference_procedures():

@@ -309,11 +309,12 @@ def test_uneven_batch_cells_match_oracle(gpu):
 
 def test_uneven_batch_equals_single_cells(gpu):
     """All 64 cells of the uneven batch (8..216 atoms, n = 40..1296, 1..64
-    k-points; every eigensolver path) in one batched call, twice in a row:
-    each cell's eigenvalues equal its single-context Mode-B iteration's to
-    1e-12 of the cell's spectrum scale and 1e-6 eV (the library's BLAS
-    calls use no atomics, so they are normally bit-identical), and they are
-    finite and ascending."""
+    k-points; every eigensolver path) in one batched call, twice in a row in
+    each build mode (every cell on its own streams; the three-centre work of
+    all cells in shared launches): each cell's eigenvalues equal its
+    single-context Mode-B iteration's to 1e-12 of the cell's spectrum scale
+    and 1e-6 eV (the library's BLAS calls use no atomics, so they are
+    normally bit-identical), and they are finite and ascending."""
     cells = configs.uneven_batch()
     _log("uneven batch: creating 64 contexts")
     ctxs = [gpu.OlcaoGPU(s) for s in cells]
@@ -322,8 +323,9 @@ def test_uneven_batch_equals_single_cells(gpu):
     for i, (g, n) in enumerate(zip(ctxs, ns)):
         single.append(np.asarray(g.scf_iteration(g.sysin.pot_coeffs, n)))
         _log(f"  cell {i} n={n} nk={g.nk}: {g.timings()['total_ms']:.2f} ms")
-    for rep in range(2):
-        wb, ms = gpu.scf_iteration_batch(ctxs, ns, 8)
+    for rep, mode in enumerate((gpu.BATCH_CELLS, gpu.BATCH_CELLS, gpu.BATCH_SHARED,
+                                gpu.BATCH_SHARED)):
+        wb, ms = gpu.scf_iteration_batch(ctxs, ns, 8, mode)
         worst, worst_rel, exact = 0.0, 0.0, 0
         for i, (w1, w2) in enumerate(zip(single, wb)):
             w2 = np.asarray(w2)
@@ -336,11 +338,40 @@ def test_uneven_batch_equals_single_cells(gpu):
             worst = max(worst, d)
             worst_rel = max(worst_rel, d / np.abs(w1).max())
             exact += int(d == 0.0)
-        _log(f"  batch pass {rep}: max |batch - single| {worst * HARTREE_EV:.2e} eV "
+        _log(f"  batch pass {rep} (mode {mode}): max |batch - single| {worst * HARTREE_EV:.2e} eV "
              f"({worst_rel:.1e} of the spectrum scale), {exact} of {len(single)} cells "
              f"bit-identical, latency p50 {np.percentile(ms, 50):.1f} ms "
              f"p99 {np.percentile(ms, 99):.1f} ms")
         assert worst_rel < 1e-12 and worst * HARTREE_EV < 1e-6
+    for g in ctxs:
+        g.close()
+
+
+def test_batch_of_mixed_kinds_equals_single_cells(gpu):
+    """The batched call over cells of every kind its build distinguishes:
+    spin 1 and 2 (a second three-centre launch over the spin-2 cells), rel
+    0 and 1 (kinetic blocks added by the finish or beforehand), core states,
+    Gamma, f shells, untabulated and tabulated three-centre kernel forms,
+    FP64 and mixed precision, in one call with the three-centre work in
+    shared launches (twice in a row) and with every cell on its own streams;
+    each cell's eigenvalues equal its single-context iteration's (the same
+    kernels on the same data, in the same order) to 1e-12 of its spectrum
+    scale."""
+    specs = [(SMALL["rel_spin2"], gpu.PREC_FP64), (SMALL["core_gamma_rel"], gpu.PREC_FP64),
+             (SMALL["spdf"], gpu.PREC_MIXED), (SMALL["core"], gpu.PREC_MIXED),
+             (dict(core=True, spin=2, lmax=2, kmesh=(2, 2, 1)), gpu.PREC_FP64),
+             (SMALL["hex_spd"], gpu.PREC_FP64), (SMALL["two_types"], gpu.PREC_MIXED)]
+    ctxs = [gpu.OlcaoGPU(configs.small_test(**kw), precision=p) for kw, p in specs]
+    ns = [g.vale_dim for g in ctxs]
+    single = [np.asarray(g.scf_iteration(g.sysin.pot_coeffs, n)) for g, n in zip(ctxs, ns)]
+    for rep, mode in enumerate((gpu.BATCH_SHARED, gpu.BATCH_SHARED, gpu.BATCH_CELLS)):
+        wb, _ = gpu.scf_iteration_batch(ctxs, ns, 4, mode)
+        for i, (w1, w2) in enumerate(zip(single, wb)):
+            w2 = np.asarray(w2)
+            d = float(np.abs(w2 - w1).max())
+            _log(f"  batch {rep} (mode {mode}) cell {i} (spin {ctxs[i].spin}, n={ns[i]}): {d:.2e} Ha")
+            assert w2.shape == w1.shape
+            assert d <= 1e-12 * np.abs(w1).max(), (i, d)
     for g in ctxs:
         g.close()
 
