@@ -66,6 +66,31 @@ olcao_status olcao_write_scf_vector_terms_h5(olcao_ctx *ctx, int which,
  * all k-points of a spin are done (secularEqn.F90:307-313). */
 olcao_status olcao_write_scf_eigen_h5(olcao_ctx *ctx, int numStates, const char *path);
 
+/* The PSCF file of one k-point set (hdf5PSCF.F90:60-190 with
+ * hdf5PSCFIntg.F90:95-452, hdf5PSCFEigVal.f90, hdf5PSCFEigVec.F90), as
+ * intgPSCF and bandPSCF leave it (OLCAO.F90:533-770), under the group
+ * kPointName (the reference names it from the k-point mesh, e.g.
+ * "00004_00004_00004"; an existing group of that name is replaced, the
+ * file is created when absent, OLCAO_ERR_INPUT when the path holds another
+ * kind of file):
+ *
+ *   <kPointName>/atomIntgGroup/atomOverlap/<k>      S(k), packed, status 1
+ *   <kPointName>/atomIntgGroup/atomHamOverlap/<k>   H(k) of potCoeffs, status 1
+ *   <kPointName>/atomIntgGroup/atomOverlapCV/{real,imag}<k>   (coreDim > 0)
+ *   <kPointName>/atomIntgGroup/atom{DM,MM,K,KPlusG}Overlap/...  empty, status 0
+ *   <kPointName>/eigenValues/<k><s>, eigenVectors/...  the last solve's
+ *
+ * S and H are rebuilt by the sweeps (orthogonalised when coreDim > 0, as
+ * gaussOverlapOL and gaussOverlapHamPSCF write them); the eigenpairs are
+ * those of the last solve (olcao_secular_eqn_pscf or olcao_scf_iteration),
+ * OLCAO_ERR_STATE without one.  The file-level mesh groups of
+ * hdf5PSCFField.F90 (wavGroup, rhoGroup, potGroup, mesh) are not written.
+ * Spin 2 is refused (OLCAO_ERR_UNSUPPORTED): the reference creates both
+ * spins' Hamiltonian datasets under one name and stops at the second
+ * (hdf5PSCFIntg.F90:338-344). */
+olcao_status olcao_write_pscf_h5(olcao_ctx *ctx, const double *potCoeffs, int numStates,
+                                 const char *kPointName, const char *path);
+
 #ifdef __cplusplus
 }
 #endif

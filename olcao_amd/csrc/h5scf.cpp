@@ -83,6 +83,162 @@ hid_t write_dataset(hid_t group, const std::string &name, const hsize_t dims[2],
     return id;
 }
 
+// Column k of a [nk][n][n] (complex: interleaved) matrix set packed as
+// matrixSubs.F90:506-534 does: column i, rows 0..i, components innermost.
+void pack_upper(const double *mat, int n, int comps, int k, double *packed)
+{
+    const double *m = mat + (size_t)comps * n * n * k;
+    size_t p = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j, ++p)
+            for (int q = 0; q < comps; ++q)
+                packed[p * comps + q] = m[((size_t)j + (size_t)i * n) * comps + q];
+}
+
+// The packed vale-vale dataspace (numComponents, n(n+1)/2) in Fortran order
+// and its chunks of at most 250M elements (hdf5SCFIntg.F90:144-153,
+// hdf5PSCFIntg.F90:123-133).
+struct PackedSpace {
+    hsize_t dims[2], chunk[2];
+    PackedSpace(int n, int comps)
+    {
+        const hsize_t npack = (hsize_t)n * (n + 1) / 2;
+        dims[0] = npack;
+        dims[1] = (hsize_t)comps;
+        chunk[0] = std::min<hsize_t>(npack, 250000000 / comps);
+        chunk[1] = (hsize_t)comps;
+    }
+};
+
+// One packed dataset per k-point of a term group, then its status 1.
+void write_packed_term(hid_t group, const double *mat, int n, int comps, int nk)
+{
+    const PackedSpace sp(n, comps);
+    std::vector<double> packed((size_t)n * (n + 1) / 2 * comps);
+    for (int k = 0; k < nk; ++k) {
+        pack_upper(mat, n, comps, k, packed.data());
+        write_dataset(group, name7(k + 1), sp.dims, sp.chunk, packed.data());
+    }
+    set_status(group);
+}
+
+// The groups of operators outside this path, created with their datasets
+// and status 0 so that the reference's readers find every group, dataset
+// and attribute they open: dipole and momentum matrices (packed, one
+// subgroup per Cartesian direction) and the k-overlaps (full valeDim x
+// valeDim, "real"/"imag" per k).
+void empty_operator_groups(hid_t top, int n, int comps, int nk)
+{
+    const PackedSpace sp(n, comps);
+    hsize_t fullChunk[2] = {(hsize_t)n, (hsize_t)n};
+    if ((double)n * n > 250000000.0) {
+        const double f = std::sqrt((double)n * n / 250000000.0);
+        fullChunk[0] = fullChunk[1] = std::max<hsize_t>(1, (hsize_t)(n / f));
+    }
+    const hsize_t fullDims[2] = {(hsize_t)n, (hsize_t)n};
+    for (const char *name : {"atomDMOverlap", "atomMMOverlap", "atomKOverlap", "atomKOverlapPlusG"}) {
+        Handle g{check(H5Gcreate2(top, name, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), name), H5Gclose};
+        const bool kov = name[4] == 'K';
+        for (int x = 1; x <= 3; ++x) {
+            Handle sub{check(H5Gcreate2(g.id, name7(x).c_str(), H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                             "xyz group"),
+                       H5Gclose};
+            for (int k = 0; k < nk; ++k) {
+                if (kov) {
+                    write_dataset(sub.id, "real" + name7(k + 1), fullDims, fullChunk, nullptr);
+                    write_dataset(sub.id, "imag" + name7(k + 1), fullDims, fullChunk, nullptr);
+                } else {
+                    write_dataset(sub.id, name7(k + 1), sp.dims, sp.chunk, nullptr);
+                }
+            }
+        }
+        set_status(g.id, 0);
+    }
+}
+
+// coreVale (coreDim, valeDim) per component, "real<k>" and "imag<k>"
+// (hdf5SCFIntg.F90:438-446), chunks as :165-175; cv: coreVale(coreDim,
+// valeDim, nk) (complex interleaved) as olcao_gauss_overlap returns it.
+void write_core_vale(hid_t top, const double *cv, int n, int nc, int comps, int nk)
+{
+    Handle g{check(H5Gcreate2(top, "atomOverlapCV", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                   "atomOverlapCV"),
+             H5Gclose};
+    std::vector<double> part((size_t)nc * n);
+    const hsize_t cvDims[2] = {(hsize_t)n, (hsize_t)nc};
+    hsize_t cvChunk[2] = {(hsize_t)n, (hsize_t)nc};
+    const double prod = (double)nc * n;
+    if (prod > 250000000.0) {
+        const double f = std::sqrt(prod / 250000000.0);
+        cvChunk[0] = std::max<hsize_t>(1, (hsize_t)(n / f));
+        cvChunk[1] = std::max<hsize_t>(1, (hsize_t)(nc / f));
+    }
+    for (int k = 0; k < nk; ++k)
+        for (int q = 0; q < comps; ++q) {
+            const double *src = cv + (size_t)comps * nc * n * k;
+            for (size_t e = 0; e < (size_t)nc * n; ++e) part[e] = src[e * comps + q];
+            write_dataset(g.id, (q == 0 ? "real" : "imag") + name7(k + 1), cvDims, cvChunk,
+                          part.data());
+        }
+}
+
+// eigenValues/<k><s> and eigenVectors/{real,imag}<k><s> (one real dataset
+// at Gamma) under `parent`, each first eigenvector dataset with status 0
+// (hdf5SCFEigVal.f90:60-95, hdf5SCFEigVec.F90:70-160, and the same layout
+// in hdf5PSCFEigVal.f90 and hdf5PSCFEigVec.F90).  w[q]: [nk][numStates];
+// v[q]: [nk][numStates][n] (complex interleaved).
+void write_eigen_groups(hid_t parent, const std::vector<std::vector<double>> &w,
+                        const std::vector<std::vector<double>> &v, int n, int nk, int numStates,
+                        bool gamma)
+{
+    const int comps = gamma ? 1 : 2;
+    for (const char *g : {"eigenValues", "eigenVectors"})
+        if (H5Lexists(parent, g, H5P_DEFAULT) > 0) check(H5Ldelete(parent, g, H5P_DEFAULT), "old group", 0);
+    Handle gv{check(H5Gcreate2(parent, "eigenValues", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                    "eigenValues"),
+              H5Gclose};
+    Handle ge{check(H5Gcreate2(parent, "eigenVectors", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                    "eigenVectors"),
+              H5Gclose};
+    // (valeDim, numStates) in Fortran order; chunks as hdf5SCFEigVec.F90:82-89
+    const hsize_t vDims[2] = {(hsize_t)numStates, (hsize_t)n};
+    hsize_t vChunk[2] = {(hsize_t)numStates, (hsize_t)n};
+    if ((double)n * numStates > 250000000.0) {
+        vChunk[0] = std::max<hsize_t>(1, (hsize_t)(std::sqrt(250000000.0 / ((double)n * numStates)) * numStates));
+        vChunk[1] = std::max<hsize_t>(1, std::min<hsize_t>(n, 250000000 / vChunk[0]));
+    }
+    const hsize_t sDims[2] = {(hsize_t)numStates, 1}, sChunk[2] = {(hsize_t)numStates, 1};
+    std::vector<double> part((size_t)numStates * n);
+    for (size_t q = 0; q < w.size(); ++q)
+        for (int k = 0; k < nk; ++k) {
+            const std::string kq = name7(k + 1) + name7((int)q + 1);
+            write_dataset(gv.id, kq, sDims, sChunk, w[q].data() + (size_t)k * numStates, 1);
+            const double *src = v[q].data() + (size_t)k * numStates * n * comps;
+            for (int c = 0; c < comps; ++c) {
+                for (size_t e = 0; e < part.size(); ++e) part[e] = src[e * comps + c];
+                const std::string name = gamma ? kq : (c == 0 ? "real" : "imag") + kq;
+                const hid_t ds = write_dataset(ge.id, name, vDims, vChunk, part.data(), 2, c == 0);
+                if (c == 0) {
+                    Handle keep{ds, H5Dclose};
+                    set_status(keep.id, 0);
+                }
+            }
+        }
+}
+
+// An existing HDF5 file opened for writing, or a new one; a path that
+// exists but holds another kind of file is left alone.
+hid_t open_or_create(const char *path)
+{
+    std::FILE *probe = std::fopen(path, "rb");
+    if (probe) {
+        std::fclose(probe);
+        if (H5Fis_hdf5(path) <= 0) throw H5Error{"opening a file that is not HDF5"};
+        return check(H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT), "file open");
+    }
+    return check(H5Fcreate(path, H5F_ACC_EXCL, H5P_DEFAULT, H5P_DEFAULT), "file create");
+}
+
 }  // namespace
 
 extern "C" olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char *path)
@@ -98,8 +254,7 @@ extern "C" olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char 
     const int rel = flags[1];
     const int comps = gamma ? 1 : 2;                      // numComponents
     const size_t full = (size_t)comps * n * n * nk;
-    const size_t npack = (size_t)n * (n + 1) / 2;
-    std::vector<double> mat(full), packed(npack * comps);
+    std::vector<double> mat(full);
     // a first read checks the setup has run before the file is touched
     if ((st = olcao_get_scf_term(ctx, 0, 0, mat.data())) != OLCAO_OK) return st;
     try {
@@ -109,29 +264,17 @@ extern "C" olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char 
         Handle top{check(H5Gcreate2(file.id, "atomIntgGroup", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
                          "atomIntgGroup"),
                    H5Gclose};
-        // packed vale-vale dataspace (numComponents, n(n+1)/2) in Fortran
-        // order; chunks of at most 250M elements (hdf5SCFIntg.F90:144-153)
-        const hsize_t vvDims[2] = {npack, (hsize_t)comps};
-        const hsize_t vvChunk[2] = {std::min<hsize_t>(npack, 250000000 / comps), (hsize_t)comps};
+        const PackedSpace sp(n, comps);
         auto write_term = [&](hid_t group, int which, int term) {
             if (olcao_get_scf_term(ctx, which, term, mat.data()) != OLCAO_OK)
                 throw H5Error{"reading a term matrix from the device"};
-            for (int k = 0; k < nk; ++k) {
-                const double *m = mat.data() + (size_t)comps * n * n * k;
-                size_t p = 0;
-                for (int i = 0; i < n; ++i)            // column i, rows 0..i
-                    for (int j = 0; j <= i; ++j, ++p)
-                        for (int q = 0; q < comps; ++q)
-                            packed[p * comps + q] = m[((size_t)j + (size_t)i * n) * comps + q];
-                write_dataset(group, name7(k + 1), vvDims, vvChunk, packed.data());
-            }
-            set_status(group);
+            write_packed_term(group, mat.data(), n, comps, nk);
         };
         // the packed datasets of a term this path does not compute, and its
         // status 0 (the state initSCFIntegralHDF5 leaves them in,
         // hdf5SCFIntg.F90:380-530)
         auto empty_term = [&](hid_t group) {
-            for (int k = 0; k < nk; ++k) write_dataset(group, name7(k + 1), vvDims, vvChunk, nullptr);
+            for (int k = 0; k < nk; ++k) write_dataset(group, name7(k + 1), sp.dims, sp.chunk, nullptr);
         };
         const struct {
             const char *name;
@@ -147,36 +290,8 @@ extern "C" olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char 
                 set_status(g.id, 0);
             }
         }
-        // Operators outside this path, created empty with status 0 so that
-        // the reference's reader (accessSCFIntegralHDF5, :563-760) finds
-        // every group, dataset and attribute it opens: dipole and momentum
-        // matrices (packed, one subgroup per Cartesian direction) and the
-        // k-overlaps (full valeDim x valeDim, "real"/"imag" per k).
-        hsize_t fullChunk[2] = {(hsize_t)n, (hsize_t)n};
-        if ((double)n * n > 250000000.0) {
-            const double f = std::sqrt((double)n * n / 250000000.0);
-            fullChunk[0] = fullChunk[1] = std::max<hsize_t>(1, (hsize_t)(n / f));
-        }
-        const hsize_t fullDims[2] = {(hsize_t)n, (hsize_t)n};
-        for (const char *name : {"atomDMOverlap", "atomMMOverlap", "atomKOverlap", "atomKOverlapPlusG"}) {
-            Handle g{check(H5Gcreate2(top.id, name, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), name),
-                     H5Gclose};
-            const bool kov = name[4] == 'K';
-            for (int x = 1; x <= 3; ++x) {
-                Handle sub{check(H5Gcreate2(g.id, name7(x).c_str(), H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
-                                 "xyz group"),
-                           H5Gclose};
-                for (int k = 0; k < nk; ++k) {
-                    if (kov) {
-                        write_dataset(sub.id, "real" + name7(k + 1), fullDims, fullChunk, nullptr);
-                        write_dataset(sub.id, "imag" + name7(k + 1), fullDims, fullChunk, nullptr);
-                    } else {
-                        write_dataset(sub.id, name7(k + 1), vvDims, vvChunk, nullptr);
-                    }
-                }
-            }
-            set_status(g.id, 0);
-        }
+        // operators outside this path (hdf5SCFIntg.F90:563-760 opens them)
+        empty_operator_groups(top.id, n, comps, nk);
         {
             Handle pot{check(H5Gcreate2(top.id, "atomPotOverlap", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
                              "atomPotOverlap"),
@@ -190,29 +305,10 @@ extern "C" olcao_status olcao_write_scf_integrals_h5(olcao_ctx *ctx, const char 
             }
         }
         if (nc > 0) {
-            // coreVale (coreDim, valeDim) per component, "real<k>" and
-            // "imag<k>" (hdf5SCFIntg.F90:438-446); chunks as :165-175
-            Handle g{check(H5Gcreate2(top.id, "atomOverlapCV", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
-                           "atomOverlapCV"),
-                     H5Gclose};
-            std::vector<double> cv((size_t)comps * nc * n * nk), part((size_t)nc * n);
+            std::vector<double> cv((size_t)comps * nc * n * nk);
             if (olcao_get_scf_term(ctx, 5, 0, cv.data()) != OLCAO_OK)
                 throw H5Error{"reading coreVale from the device"};
-            const hsize_t cvDims[2] = {(hsize_t)n, (hsize_t)nc};
-            hsize_t cvChunk[2] = {(hsize_t)n, (hsize_t)nc};
-            const double prod = (double)nc * n;
-            if (prod > 250000000.0) {
-                const double f = std::sqrt(prod / 250000000.0);
-                cvChunk[0] = std::max<hsize_t>(1, (hsize_t)(n / f));
-                cvChunk[1] = std::max<hsize_t>(1, (hsize_t)(nc / f));
-            }
-            for (int k = 0; k < nk; ++k)
-                for (int q = 0; q < comps; ++q) {
-                    const double *src = cv.data() + (size_t)comps * nc * n * k;
-                    for (size_t e = 0; e < (size_t)nc * n; ++e) part[e] = src[e * comps + q];
-                    write_dataset(g.id, (q == 0 ? "real" : "imag") + name7(k + 1), cvDims, cvChunk,
-                                  part.data());
-                }
+            write_core_vale(top.id, cv.data(), n, nc, comps, nk);
         }
     } catch (const H5Error &e) {
         std::fprintf(stderr, "olcao_write_scf_integrals_h5: %s failed for %s\n", e.what.c_str(), path);
@@ -285,6 +381,7 @@ extern "C" olcao_status olcao_write_scf_eigen_h5(olcao_ctx *ctx, int numStates, 
     const int n = (int)dims[0], nk = (int)dims[2];
     const bool gamma = flags[0] != 0;
     const int spin = flags[2], comps = gamma ? 1 : 2;
+    if (numStates <= 0 || numStates > n) return OLCAO_ERR_INPUT;
     std::vector<std::vector<double>> w(spin), v(spin);
     for (int q = 0; q < spin; ++q) {
         w[q].resize((size_t)nk * numStates);
@@ -293,53 +390,73 @@ extern "C" olcao_status olcao_write_scf_eigen_h5(olcao_ctx *ctx, int numStates, 
     }
     try {
         H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
-        // an existing HDF5 file gets the groups added; a path that exists
-        // but is not one is left alone
-        std::FILE *probe = std::fopen(path, "rb");
-        hid_t fid;
-        if (probe) {
-            std::fclose(probe);
-            if (H5Fis_hdf5(path) <= 0) throw H5Error{"opening a file that is not HDF5"};
-            fid = check(H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT), "file open");
-        } else {
-            fid = check(H5Fcreate(path, H5F_ACC_EXCL, H5P_DEFAULT, H5P_DEFAULT), "file create");
-        }
-        Handle file{fid, H5Fclose};
-        for (const char *g : {"eigenValues", "eigenVectors"})
-            if (H5Lexists(file.id, g, H5P_DEFAULT) > 0) check(H5Ldelete(file.id, g, H5P_DEFAULT), "old group", 0);
-        Handle gv{check(H5Gcreate2(file.id, "eigenValues", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
-                        "eigenValues"),
-                  H5Gclose};
-        Handle ge{check(H5Gcreate2(file.id, "eigenVectors", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
-                        "eigenVectors"),
-                  H5Gclose};
-        // (valeDim, numStates) in Fortran order; chunks as
-        // hdf5SCFEigVec.F90:82-89
-        const hsize_t vDims[2] = {(hsize_t)numStates, (hsize_t)n};
-        hsize_t vChunk[2] = {(hsize_t)numStates, (hsize_t)n};
-        if ((double)n * numStates > 250000000.0) {
-            vChunk[0] = std::max<hsize_t>(1, (hsize_t)(std::sqrt(250000000.0 / ((double)n * numStates)) * numStates));
-            vChunk[1] = std::max<hsize_t>(1, std::min<hsize_t>(n, 250000000 / vChunk[0]));
-        }
-        const hsize_t sDims[2] = {(hsize_t)numStates, 1}, sChunk[2] = {(hsize_t)numStates, 1};
-        std::vector<double> part((size_t)numStates * n);
-        for (int q = 0; q < spin; ++q)
-            for (int k = 0; k < nk; ++k) {
-                const std::string kq = name7(k + 1) + name7(q + 1);
-                write_dataset(gv.id, kq, sDims, sChunk, w[q].data() + (size_t)k * numStates, 1);
-                const double *src = v[q].data() + (size_t)k * numStates * n * comps;
-                for (int c = 0; c < comps; ++c) {
-                    for (size_t e = 0; e < part.size(); ++e) part[e] = src[e * comps + c];
-                    const std::string name = gamma ? kq : (c == 0 ? "real" : "imag") + kq;
-                    const hid_t ds = write_dataset(ge.id, name, vDims, vChunk, part.data(), 2, c == 0);
-                    if (c == 0) {
-                        Handle keep{ds, H5Dclose};
-                        set_status(keep.id, 0);
-                    }
-                }
-            }
+        Handle file{open_or_create(path), H5Fclose};
+        write_eigen_groups(file.id, w, v, n, nk, numStates, gamma);
     } catch (const H5Error &e) {
         std::fprintf(stderr, "olcao_write_scf_eigen_h5: %s failed for %s\n", e.what.c_str(), path);
+        return OLCAO_ERR_INPUT;
+    }
+    return OLCAO_OK;
+}
+
+extern "C" olcao_status olcao_write_pscf_h5(olcao_ctx *ctx, const double *potCoeffs, int numStates,
+                                            const char *kPointName, const char *path)
+{
+    if (!ctx || !potCoeffs || !kPointName || !*kPointName || !path) return OLCAO_ERR_INPUT;
+    long long dims[8];
+    int flags[4];
+    olcao_status st = olcao_get_dims(ctx, dims);
+    if (st == OLCAO_OK) st = olcao_get_flags(ctx, flags);
+    if (st != OLCAO_OK) return st;
+    const int n = (int)dims[0], nc = (int)dims[1], nk = (int)dims[2];
+    const bool gamma = flags[0] != 0;
+    const int comps = gamma ? 1 : 2;
+    // the reference creates one atomHamOverlap dataset per k and spin under
+    // the same name and stops at the second spin
+    // (hdf5PSCFIntg.F90:338-344, 'Failed to create hamiltonian overlap')
+    if (flags[2] != 1) return OLCAO_ERR_UNSUPPORTED;
+    if (numStates <= 0 || numStates > n) return OLCAO_ERR_INPUT;
+    // the eigenpairs of the last solve first: the sweeps below rebuild the
+    // matrices the solve left
+    std::vector<std::vector<double>> w(1), v(1);
+    w[0].resize((size_t)nk * numStates);
+    v[0].resize((size_t)nk * numStates * n * comps);
+    if ((st = olcao_get_eigen(ctx, 0, numStates, w[0].data(), v[0].data())) != OLCAO_OK) return st;
+    const size_t full = (size_t)comps * n * n * nk;
+    std::vector<double> S(full), H(full);
+    std::vector<double> cv(std::max<size_t>((size_t)comps * nc * n * nk, 1)),
+        cc(std::max<size_t>((size_t)comps * nc * nc * nk, 1)), cvH(cv.size()), ccH(cc.size());
+    if ((st = olcao_gauss_overlap(ctx, 0, S.data(), cv.data(), cc.data())) != OLCAO_OK) return st;
+    if ((st = olcao_gauss_overlap_ham_pscf(ctx, potCoeffs, H.data(), cvH.data(), ccH.data())) != OLCAO_OK)
+        return st;
+    try {
+        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+        Handle file{open_or_create(path), H5Fclose};
+        if (H5Lexists(file.id, kPointName, H5P_DEFAULT) > 0)
+            check(H5Ldelete(file.id, kPointName, H5P_DEFAULT), "old k-point group", 0);
+        Handle kg{check(H5Gcreate2(file.id, kPointName, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                        "k-point group"),
+                  H5Gclose};
+        Handle top{check(H5Gcreate2(kg.id, "atomIntgGroup", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                         "atomIntgGroup"),
+                   H5Gclose};
+        {
+            Handle g{check(H5Gcreate2(top.id, "atomOverlap", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                           "atomOverlap"),
+                     H5Gclose};
+            write_packed_term(g.id, S.data(), n, comps, nk);
+        }
+        {
+            Handle g{check(H5Gcreate2(top.id, "atomHamOverlap", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                           "atomHamOverlap"),
+                     H5Gclose};
+            write_packed_term(g.id, H.data(), n, comps, nk);
+        }
+        if (nc > 0) write_core_vale(top.id, cv.data(), n, nc, comps, nk);
+        empty_operator_groups(top.id, n, comps, nk);
+        write_eigen_groups(kg.id, w, v, n, nk, numStates, gamma);
+    } catch (const H5Error &e) {
+        std::fprintf(stderr, "olcao_write_pscf_h5: %s failed for %s\n", e.what.c_str(), path);
         return OLCAO_ERR_INPUT;
     }
     return OLCAO_OK;

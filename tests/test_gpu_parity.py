@@ -547,6 +547,85 @@ def test_scf_integral_file(gpu, tmp_path, case):
     g.close()
 
 
+@pytest.mark.parametrize("case", [dict(core=True, kmesh=(2, 2, 1)),
+                                  dict(core=True, gamma=True, rel=1, two_types=True)])
+def test_pscf_file(gpu, tmp_path, case):
+    """The PSCF file of a Mode-B solve in the reference's layout, under its
+    k-point group: every group, dataset and status attribute the reference
+    reader opens (accessPSCFIntegralHDF5, hdf5PSCFIntg.F90:455-672); S(k)
+    and H(k) packed and orthogonalised as the oracle's to 1e-10, coreVale
+    raw, the eigenpairs equal to what the solve returned; a second write
+    replaces the group; spin 2 refused as the reference stops there."""
+    import ctypes as C
+    import os
+
+    from tests import h5read
+    from tests.conftest import ROOT
+    if not h5read.available():
+        pytest.skip("no HDF5 C library on this host")
+    s = configs.small_test(**case)
+    g, o = _pair(gpu, s)
+    n, nk, comps, ns = g.vale_dim, g.nk, 1 if s.gamma else 2, g.vale_dim - 1
+    coef = np.ascontiguousarray(s.pot_coeffs, np.float64)
+    g.gaussOverlapOL()
+    g.gaussOverlapHamPSCF(coef)
+    w, v = g.secularEqnPSCF(ns, vectors=True)
+    w = np.asarray(w).reshape(nk, ns)
+    v = np.asarray(v).reshape(nk, n, ns)
+    lib = C.CDLL(os.path.join(ROOT, "olcao_amd", "libolcao_h5.so"))
+    lib.olcao_write_pscf_h5.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_char_p, C.c_char_p]
+    path = str(tmp_path / "gs_pscf-fb.hdf5")
+    kg = "00002_00002_00001"
+    for _ in range(2):
+        assert lib.olcao_write_pscf_h5(g.ctx, coef.ctypes.data, ns, kg.encode(), path.encode()) == 0
+    f = h5read.File(path)
+    top = f"/{kg}/atomIntgGroup/"
+    So = o.integral(oracle.OL)
+    Ho = o.integral(oracle.HAM, coef=coef)
+    refs = {"atomOverlap": oracle.ortho_dataset(*So, So[1], overlap=True),
+            "atomHamOverlap": oracle.ortho_dataset(*Ho, So[1], overlap=False)}
+    for group, ref in refs.items():
+        assert f.status(top + group) == 1, group
+        for k in range(nk):
+            data, chunked, nfilt = f.read(f"{top}{group}/{k + 1:07d}")
+            p = oracle.pack_upper(ref if s.gamma else ref[k])
+            exp = p[:, None] if s.gamma else np.stack([p.real, p.imag], axis=1)
+            assert data.shape == exp.shape and chunked and nfilt >= 1
+            assert _ortho_close(data, exp, 1e-10), (group, k)
+    for k in range(nk):
+        kn = f"{k + 1:07d}"
+        paths = [f"{top}atomOverlap/{kn}", f"{top}atomHamOverlap/{kn}"]
+        for x in range(1, 4):
+            paths += [f"{top}{gr}/{x:07d}/{kn}" for gr in ("atomDMOverlap", "atomMMOverlap")]
+            paths += [f"{top}{gr}/{x:07d}/{part}{kn}" for gr in ("atomKOverlap", "atomKOverlapPlusG")
+                      for part in ("real", "imag")]
+        paths += [f"{top}atomOverlapCV/{part}{kn}" for part in ("real", "imag")[:comps]]
+        for name in paths:
+            assert f.exists(name), name
+        cvk = So[1] if s.gamma else So[1][k]
+        for part, val in (("real", np.real(cvk)), ("imag", np.imag(cvk)))[:comps]:
+            data, _, _ = f.read(f"{top}atomOverlapCV/{part}{kn}")
+            assert block_rel_err(data, val.T) < 1e-10
+        ev, _, _ = f.read(f"/{kg}/eigenValues/{kn}0000001")
+        assert np.array_equal(ev, w[k])
+        parts = [("", np.real(v[k]))] if s.gamma else [("real", np.real(v[k])), ("imag", np.imag(v[k]))]
+        for part, val in parts:
+            data, _, _ = f.read(f"/{kg}/eigenVectors/{part}{kn}0000001")
+            assert data.shape == (ns, n) and np.array_equal(data, val.T)
+        assert f.dataset_status(f"/{kg}/eigenVectors/{parts[0][0]}{kn}0000001") == 0
+    for group in ("atomDMOverlap", "atomMMOverlap", "atomKOverlap", "atomKOverlapPlusG"):
+        assert f.status(top + group) == 0, group
+    f.close()
+    g.close()
+    s2 = configs.small_test(**SMALL["rel_spin2"])
+    g2 = gpu.OlcaoGPU(s2)
+    c2 = np.ascontiguousarray(s2.pot_coeffs, np.float64)
+    g2.scf_iteration(c2, g2.vale_dim)
+    assert lib.olcao_write_pscf_h5(g2.ctx, c2.ctypes.data, g2.vale_dim, kg.encode(),
+                                   str(tmp_path / "spin2.hdf5").encode()) == 7
+    g2.close()
+
+
 def test_diamond_c8_full_parity(gpu):
     """First workload at full size: H(k), S(k) and all eigenvalues."""
     _log("diamond_c8 full")
