@@ -40,7 +40,7 @@ namespace {
 
 constexpr int kTrdThreads = 256;
 constexpr int kTrdWaves = kTrdThreads / 64;
-constexpr int kTrdK = 10;        // 64-column chunks of a row: n <= 640
+constexpr int kTrdK = 12;        // 64-column chunks of a row: n <= 768
 constexpr int kTrdKR = 2;        // chunks held in registers (columns 0..127)
 // the divide and conquer that follows the reduction keeps per-merge arrays
 // of kDcMaxN entries in LDS: the reduction may not take larger matrices
@@ -372,9 +372,12 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
             // p and column i+1 (entries i+1..n-1) go through LDS, each entry
             // loaded once per workgroup; all loads are issued before any
             // store, so the step pays one memory round trip
+            T part = N::zero();
             {
                 constexpr int kPer = (64 * KC + kTrdThreads - 1) / kTrdThreads;
-                T lp[kPer], lc[kPer];
+                // the wave dots (kTrdWaves G <= 4 * 64 entries) with the rest
+                constexpr int kDots = (kTrdWaves * 64 + 63) / 64;
+                T lp[kPer], lc[kPer], ld[kDots];
 #pragma unroll
                 for (int t = 0; t < kPer; ++t) {
                     const int c = i + 1 + (int)threadIdx.x + kTrdThreads * t;
@@ -384,6 +387,11 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
                     }
                 }
 #pragma unroll
+                for (int t = 0; t < kDots; ++t) {
+                    const int q = lane + 64 * t;
+                    ld[t] = q < kTrdWaves * G ? pldT<CPLX>(pPart, q) : N::zero();
+                }
+#pragma unroll
                 for (int t = 0; t < kPer; ++t) {
                     const int c = i + 1 + (int)threadIdx.x + kTrdThreads * t;
                     if (c < n) {
@@ -391,9 +399,10 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
                         stC[c] = lc[t];
                     }
                 }
+#pragma unroll
+                for (int t = 0; t < kDots; ++t)
+                    if (lane + 64 * t < kTrdWaves * G) part = N::add(part, ld[t]);
             }
-            T part = N::zero();
-            for (int q = lane; q < kTrdWaves * G; q += 64) part = N::add(part, pldT<CPLX>(pPart, q));
             const T dot = wsum<CPLX>(part);
             __syncthreads();
             const T p1 = stP[i + 1];
@@ -593,7 +602,9 @@ const void *trd_fn_kc(int kc)
     case 7: return trd_fn<CPLX, 7>();
     case 8: return trd_fn<CPLX, 8>();
     case 9: return trd_fn<CPLX, 9>();
-    default: return trd_fn<CPLX, 10>();
+    case 10: return trd_fn<CPLX, 10>();
+    case 11: return trd_fn<CPLX, 11>();
+    default: return trd_fn<CPLX, 12>();
     }
 }
 

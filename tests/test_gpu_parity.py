@@ -718,19 +718,11 @@ def _pencil(rng, n, nb, cplx, degenerate):
     return A, B
 
 
-_TRD_PAST_640 = pytest.mark.skip(
-    reason="the tridiagonal path stops at n = 640; raising its limit to 768 faulted the GPU "
-           "at n = 720 because the divide-and-conquer merge arrays in LDS are sized for "
-           "n <= 640 (kDcMaxN), so the extension was taken out and these sizes use rocSOLVER")
-
-
 @pytest.mark.parametrize("n,nb,cplx,degenerate", [(100, 3, True, False), (331, 2, True, True),
                                                   (576, 9, True, False), (200, 2, False, False),
-                                                  (640, 2, False, True),
-                                                  pytest.param(720, 3, True, False,
-                                                               marks=_TRD_PAST_640),
-                                                  pytest.param(768, 2, False, True,
-                                                               marks=_TRD_PAST_640)])
+                                                  (640, 2, False, True), (648, 5, True, False),
+                                                  (720, 3, True, False), (768, 2, False, True),
+                                                  (768, 5, True, True)])
 def test_tridiagonal_path_solvers(gpu, n, nb, cplx, degenerate):
     """Sizes past the one-workgroup LDS solver take the cooperative
     tridiagonal reduction: eigenvalues against LAPACK (scipy eigh) within
