@@ -40,7 +40,7 @@ namespace {
 
 constexpr int kTrdThreads = 256;
 constexpr int kTrdWaves = kTrdThreads / 64;
-constexpr int kTrdK = 12;        // 64-column chunks of a row: n <= 768
+constexpr int kTrdK = 14;        // 64-column chunks of a row: n <= 896
 constexpr int kTrdKR = 2;        // chunks held in registers (columns 0..127)
 // the divide and conquer that follows the reduction keeps per-merge arrays
 // of kDcMaxN entries in LDS: the reduction may not take larger matrices
@@ -604,7 +604,9 @@ const void *trd_fn_kc(int kc)
     case 9: return trd_fn<CPLX, 9>();
     case 10: return trd_fn<CPLX, 10>();
     case 11: return trd_fn<CPLX, 11>();
-    default: return trd_fn<CPLX, 12>();
+    case 12: return trd_fn<CPLX, 12>();
+    case 13: return trd_fn<CPLX, 13>();
+    default: return trd_fn<CPLX, 14>();
     }
 }
 

@@ -339,7 +339,7 @@ void launch_chol_diag(bool cplx, const double *B, double *U, double *Dinv, int n
 // matrices (kdc.hip): D (diagonal in, eigenvalues out, ascending), E
 // (couplings), Q (eigenvectors out, column-major n x n per matrix); Qt, W:
 // n x n work per matrix, ord: n ints per matrix.
-constexpr int kDcMaxN = 64 * 12;
+constexpr int kDcMaxN = 64 * 14;
 struct DcLevel {
     int nodeBase = 0, nNodes = 0, tileBase = 0, nTiles = 0, maxM = 0;
     int rootBase = 0, nRoot = 0, colBase = 0, nCol = 0;

@@ -722,7 +722,8 @@ def _pencil(rng, n, nb, cplx, degenerate):
                                                   (576, 9, True, False), (200, 2, False, False),
                                                   (640, 2, False, True), (648, 5, True, False),
                                                   (720, 3, True, False), (768, 2, False, True),
-                                                  (768, 5, True, True)])
+                                                  (768, 5, True, True), (864, 3, False, False),
+                                                  (896, 2, True, False)])
 def test_tridiagonal_path_solvers(gpu, n, nb, cplx, degenerate):
     """Sizes past the one-workgroup LDS solver take the cooperative
     tridiagonal reduction: eigenvalues against LAPACK (scipy eigh) within
