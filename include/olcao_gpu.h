@@ -97,7 +97,7 @@ typedef struct {
  * (one workgroup per k-point: Cholesky, reduction, Jacobi) where the matrices
  * fit (complex n <= 70, real n <= 99), else the device tridiagonal path
  * (blocked Cholesky, cooperative Householder reduction, divide and conquer,
- * blocked back-transform) up to n = 640, and rocSOLVER beyond;
+ * blocked back-transform) up to n = 896, and rocSOLVER beyond;
  * OLCAO_EIG_LIBRARY always uses rocSOLVER. */
 enum { OLCAO_EIG_AUTO = 0, OLCAO_EIG_LIBRARY = 1 };
 
@@ -114,7 +114,7 @@ typedef struct {
     int bloch;                    /* OLCAO_BLOCH_* */
     int graph;                    /* 1: olcao_scf_iteration replays a captured graph
                                      of its device work (build and solve) after its
-                                     first call; the library eigensolver (n > 640)
+                                     first call; the library eigensolver (n > 896)
                                      is not captured and such contexts stay eager */
     int chunkItems;               /* three-centre work per lane: image-list chunks of
                                      about this many items (16..1048576; 0: 192) */
