@@ -103,10 +103,11 @@ enum { OLCAO_EIG_AUTO = 0, OLCAO_EIG_LIBRARY = 1 };
 
 /* The tile shapes chunkItems, segPairs and bloch below are what the
  * project's autotuner (olcao_amd/autotune.py) chooses per workload.
- * Bloch-sum kernel: OLCAO_BLOCH_AUTO takes the matrix-core kernel (FP64
- * MFMA in the FP64 form, exact split BF16 products in the mixed form);
- * OLCAO_BLOCH_VALU the vector-ALU kernel (FP64 in both forms). */
-enum { OLCAO_BLOCH_AUTO = 0, OLCAO_BLOCH_VALU = 1, OLCAO_BLOCH_MFMA = 2 };
+ * Bloch-sum kernel: OLCAO_BLOCH_AUTO and OLCAO_BLOCH_MFMA take the
+ * matrix-core kernel (FP64 MFMA in the FP64 form, exact split BF16 products
+ * in the mixed form); OLCAO_BLOCH_MFMA64 the FP64 MFMA kernel in both
+ * forms; OLCAO_BLOCH_VALU the vector-ALU kernel (FP64 in both forms). */
+enum { OLCAO_BLOCH_AUTO = 0, OLCAO_BLOCH_VALU = 1, OLCAO_BLOCH_MFMA = 2, OLCAO_BLOCH_MFMA64 = 3 };
 
 typedef struct {
     int precision;                /* OLCAO_PREC_* */

@@ -11,7 +11,8 @@ their best values depend on the system:
                segments spread on-site pair-cells (hundreds of pairs) over
                more wavefronts, long ones need fewer partial sums
   bloch        the Bloch-sum kernel: the matrix-core one (FP64 MFMA, or the
-               split BF16 products in the mixed form) or the vector-ALU one
+               split BF16 products in the mixed form), the FP64 MFMA one in
+               either form, or the vector-ALU one
 
 The tuner measures the build (every stage of a Mode-B iteration before the
 eigensolve, HIP events on the context stream) for each candidate, one shape
@@ -38,7 +39,9 @@ TABLE = os.path.join(HERE, "tuning.json")
 CANDIDATES = {
     "chunk_items": [64, 128, 192, 384, 768],
     "seg_pairs": [8, 16, 32, 64],
-    "bloch": [1, 2],          # OLCAO_BLOCH_VALU, OLCAO_BLOCH_MFMA
+    # OLCAO_BLOCH_VALU, OLCAO_BLOCH_MFMA, OLCAO_BLOCH_MFMA64 (the last the
+    # same kernel as the second in the FP64 form, so tried only for mixed)
+    "bloch": [1, 2, 3],
 }
 DEFAULTS = {"chunk_items": 192, "seg_pairs": 32, "bloch": 2}
 
@@ -126,7 +129,7 @@ def tune(sysin, precision: int, reps: int = 5, passes: int = 2, log=print) -> di
         changed = False
         for name, values in CANDIDATES.items():
             for v in values:
-                if v == shapes[name]:
+                if v == shapes[name] or (name == "bloch" and v == 3 and precision == 0):
                     continue
                 cand = dict(shapes, **{name: v})
                 t = measure(cand)

@@ -173,6 +173,7 @@ struct olcao_ctx {
     unsigned long long graphEpoch = 0;
     std::string graphErr;
     bool blochMfma = true;        // matrix-core Bloch sum (else the vector-ALU one)
+    bool blochSplit = true;       // in the mixed form: split BF16 products (else FP64 MFMA)
     int chunkItems = kChunkItems; // three-centre items per image-list chunk
     int segPairs = kSegPP;        // primitive pairs per contraction segment
     std::vector<unsigned short> hPslice;   // phase slices of the split Bloch sum
@@ -1009,7 +1010,7 @@ void contract_scatter(olcao_ctx &c, double *vv, double *cv, double *cc, int negI
     }
     if (c.blochMfma)
         launch_bloch_mfma(c.dev, c.dX.p, c.dPslice.p, vv, cv, cc, c.gamma ? 1 : 0, negI,
-                          c.precision == OLCAO_PREC_MIXED, s);
+                          c.precision == OLCAO_PREC_MIXED && c.blochSplit, s);
     else
         launch_bloch_scatter(c.dev, c.dX.p, vv, cv, cc, c.gamma ? 1 : 0, negI, s);
     HIPCHK(hipGetLastError());
@@ -1552,6 +1553,7 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
         c->precision = opt ? opt->precision : 0;
         c->eigLibrary = opt && opt->eigensolver == OLCAO_EIG_LIBRARY;
         c->blochMfma = !(opt && opt->bloch == OLCAO_BLOCH_VALU);
+        c->blochSplit = !(opt && opt->bloch == OLCAO_BLOCH_MFMA64);
         c->useGraph = opt && opt->graph != 0;
         if (opt && opt->chunkItems) {
             if (opt->chunkItems < 16 || opt->chunkItems > (1 << 20))
@@ -1563,8 +1565,8 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
                 fail(OLCAO_ERR_INPUT, "segPairs must lie in 1..64 (0: default)");
             c->segPairs = opt->segPairs;
         }
-        if (opt && (opt->bloch < OLCAO_BLOCH_AUTO || opt->bloch > OLCAO_BLOCH_MFMA))
-            fail(OLCAO_ERR_INPUT, "bloch must be OLCAO_BLOCH_AUTO, _VALU or _MFMA");
+        if (opt && (opt->bloch < OLCAO_BLOCH_AUTO || opt->bloch > OLCAO_BLOCH_MFMA64))
+            fail(OLCAO_ERR_INPUT, "bloch must be OLCAO_BLOCH_AUTO, _VALU, _MFMA or _MFMA64");
         if (c->precision != OLCAO_PREC_FP64 && c->precision != OLCAO_PREC_MIXED)
             fail(OLCAO_ERR_INPUT, "precision must be OLCAO_PREC_FP64 or OLCAO_PREC_MIXED");
         // The runtime spreads a process's streams over a few hardware

@@ -64,7 +64,7 @@ class _Options(C.Structure):
 
 
 EIG_AUTO, EIG_LIBRARY = 0, 1
-BLOCH_AUTO, BLOCH_VALU, BLOCH_MFMA = 0, 1, 2
+BLOCH_AUTO, BLOCH_VALU, BLOCH_MFMA, BLOCH_MFMA64 = 0, 1, 2, 3
 
 
 _lib = None

@@ -920,7 +920,7 @@ def test_mixed_precision_form_tolerances(gpu, make):
 def test_bloch_kernels_agree(gpu, case):
     """The Bloch sum on the matrix cores against the vector-ALU kernel on
     the same device blocks, within each form: FP64 MFMA within 1e-14 of the
-    matrix scale (the FP64 MFMA adds its four products per step in cell
+    matrix scale (also when the mixed form takes it) (the FP64 MFMA adds its four products per step in cell
     order, as the vector kernel's FMAs do, so the two agree bit for bit);
     the mixed form's exact BF16 slice products within 1e-11, a tenth of the
     integral parity bar (their truncation is about 2^-48 of each block
@@ -932,11 +932,13 @@ def test_bloch_kernels_agree(gpu, case):
     for name, prec, bloch in (("valu", gpu.PREC_FP64, gpu.BLOCH_VALU),
                               ("mfma", gpu.PREC_FP64, gpu.BLOCH_MFMA),
                               ("valu_mixed", gpu.PREC_MIXED, gpu.BLOCH_VALU),
-                              ("split", gpu.PREC_MIXED, gpu.BLOCH_MFMA)):
+                              ("split", gpu.PREC_MIXED, gpu.BLOCH_MFMA),
+                              ("mfma64_mixed", gpu.PREC_MIXED, gpu.BLOCH_MFMA64)):
         g = gpu.OlcaoGPU(s, precision=prec, bloch=bloch)
         res[name] = (g.gaussOverlapOL(), g.gaussOverlapHamPSCF(s.pot_coeffs))
         g.close()
-    for name, ref, tol in (("mfma", "valu", 1e-14), ("split", "valu_mixed", 1e-11)):
+    for name, ref, tol in (("mfma", "valu", 1e-14), ("split", "valu_mixed", 1e-11),
+                           ("mfma64_mixed", "valu_mixed", 1e-14)):
         for a, b in zip(res[name], res[ref]):
             for x, y in zip(a, b):
                 if y is None:
