@@ -34,9 +34,10 @@ namespace olcao {
 // pair and the potential exponent only), which takes the division and the
 // square root out of the item loop.  MIX: mixed-precision exponential.
 // One lane per chunk of a record's image list (chRec, chBeg, chEnd; n
-// chunks); the chunk's Hermite sums go to W[e][chunk].  The wave takes the
-// chunks i0 .. i0 + 63.
-template <int LA, int LB, bool NUC, bool TAB, bool MIX>
+// chunks); the chunk's Hermite sums go to W[e][chunk].  The workgroup of NT
+// threads (64 or 256) takes the chunks i0 .. i0 + NT - 1; its waves share
+// the staged sites and tables, the term table has a column per thread.
+template <int LA, int LB, bool NUC, bool TAB, bool MIX, int NT = 64>
 __device__ __forceinline__ void three_center_wave(const DevCtx &c, const PPRec *recs, int n, int i0,
                                                   const int *chRec, const long long *chBeg,
                                                   const long long *chEnd,
@@ -54,33 +55,33 @@ __device__ __forceinline__ void three_center_wave(const DevCtx &c, const PPRec *
     double *sitePos = dyn;
     int *siteType = reinterpret_cast<int *>(dyn + 3 * c.nPotSites);
 
-    const int lane = threadIdx.x;
-    for (int q = lane; q < c.nPotSites; q += 64) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    for (int q = tid; q < c.nPotSites; q += NT) {
         sitePos[3 * q] = c.potPos[3 * q];
         sitePos[3 * q + 1] = c.potPos[3 * q + 1];
         sitePos[3 * q + 2] = c.potPos[3 * q + 2];
         siteType[q] = c.potSiteType[q];
     }
-    for (int q = lane; q < c.nPotTypes * kMaxPotAlphas; q += 64) {
+    for (int q = tid; q < c.nPotTypes * kMaxPotAlphas; q += NT) {
         const int t = q / kMaxPotAlphas, a = q - t * kMaxPotAlphas;
         const bool ok = a < c.potNA[t];
         tAlpha[q] = ok ? c.potAlpha[t * c.maxPA + a] : 0.0;
         tCoef[q] = (ok && coef) ? coef[c.potCumul[t] + a] : 1.0;
     }
-    for (int t = lane; t < c.nPotTypes; t += 64) {
+    for (int t = tid; t < c.nPotTypes; t += NT) {
         tZ[t] = c.potZ[t];
         tNuc[t] = c.potNuc[t];
     }
-    for (int j = lane; j < 256; j += 64) sExp2[j] = exp2(j * (1.0 / 256.0));
+    for (int j = tid; j < 256; j += NT) sExp2[j] = exp2(j * (1.0 / 256.0));
     __syncthreads();
 
-    const int i = i0 + lane;
+    const int i = i0 + tid;
     const bool active = i < n;
     const int ri = active ? i : 0;
     const PPRec rec = recs[chRec[ri]];
     const PairConst q = make_pair(rec.a1, rec.a2, rec.A, rec.B);
-    // term tables [type * maxPA + alpha][lane] of mu and pref0, after the sites
-    const int nTab = c.nPotTypes * c.maxPA * 64;
+    // term tables [type * maxPA + alpha][thread] of mu and pref0, after the sites
+    const int nTab = c.nPotTypes * c.maxPA * NT;
     double *tabMu = dyn + ((3 * c.nPotSites + (c.nPotSites + 1) / 2 + 1) & ~1);
     double *tabPref = tabMu + nTab;
     if (TAB && !NUC) {
@@ -96,8 +97,8 @@ __device__ __forceinline__ void three_center_wave(const DevCtx &c, const PPRec *
                     mu = q.zeta * a3 * inv;
                     pref0 = wgt * tt * sqrt(tt);
                 }
-                tabMu[(t * c.maxPA + a) * 64 + lane] = mu;
-                tabPref[(t * c.maxPA + a) * 64 + lane] = pref0;
+                tabMu[(t * c.maxPA + a) * NT + tid] = mu;
+                tabPref[(t * c.maxPA + a) * NT + tid] = pref0;
             }
     }
     double acc[NH];
@@ -144,7 +145,7 @@ __device__ __forceinline__ void three_center_wave(const DevCtx &c, const PPRec *
             } else {
                 const int term = kind == K3_HAM ? nn : termIdx;
                 if (TAB) {
-                    const int ix = (t * c.maxPA + term) * 64 + lane;
+                    const int ix = (t * c.maxPA + term) * NT + tid;
                     acc_e3c_herm_tab<L, MIX>(q, tabMu[ix], tabPref[ix], C, sExp2, acc);
                 } else {
                     const double wgt = kind == K3_HAM ? tCoef[t * kMaxPotAlphas + term] : 1.0;
@@ -177,16 +178,16 @@ __device__ __forceinline__ void three_center_wave(const DevCtx &c, const PPRec *
     }
 }
 
-template <int LA, int LB, bool NUC, bool TAB, bool MIX>
-__global__ __launch_bounds__(64) void k_three_center(DevCtx c, const PPRec *recs, int n,
+template <int LA, int LB, bool NUC, bool TAB, bool MIX, int NT = 64>
+__global__ __launch_bounds__(NT) void k_three_center(DevCtx c, const PPRec *recs, int n,
                                                      const int *chRec, const long long *chBeg,
                                                      const long long *chEnd,
                                                      const unsigned long long *img, int kind,
                                                      int termType, int termIdx,
                                                      const double *coef, int addW, double *W)
 {
-    three_center_wave<LA, LB, NUC, TAB, MIX>(c, recs, n, (int)blockIdx.x * 64, chRec, chBeg, chEnd,
-                                             img, kind, termType, termIdx, coef, addW, W);
+    three_center_wave<LA, LB, NUC, TAB, MIX, NT>(c, recs, n, (int)blockIdx.x * NT, chRec, chBeg,
+                                                 chEnd, img, kind, termType, termIdx, coef, addW, W);
 }
 
 // The fused Hamiltonian sweep of many cells in one launch: wave b takes the
@@ -322,6 +323,19 @@ __global__ __launch_bounds__(64) void k_herm_finish_multi(const Cell3c *cells, c
 static size_t site_bytes(const DevCtx &c) { return k3c_site_bytes(c); }
 static size_t table_bytes(const DevCtx &c) { return k3c_table_bytes(c); }
 
+// Threads per workgroup of the tabulated three-Gaussian pass: 256 when four
+// waves sharing one copy of the sites and constants fit more waves into a
+// CU's LDS than single waves each holding their own (the term table, a
+// column per thread, dominates: 16 KB per wave for 16 terms).
+constexpr size_t kCuLds = 160 * 1024;
+constexpr size_t kStaticLds3c = (2 * kMaxPotTypes * kMaxPotAlphas + 2 * kMaxPotTypes + 256) * 8;
+static int tab_threads(const DevCtx &c)
+{
+    const size_t shared = site_bytes(c) + 16 + kStaticLds3c, perWave = table_bytes(c);
+    const size_t w1 = kCuLds / (shared + perWave), w4 = 4 * (kCuLds / (shared + 4 * perWave));
+    return w4 > w1 ? 256 : 64;
+}
+
 template <int LA, int LB>
 static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, int ti,
                       const double *coef, int accumulate, double *blocks, int mixed,
@@ -339,8 +353,19 @@ static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, i
     const size_t dynTab = site_bytes(c) + table_bytes(c) + 16;
     const bool e3c = kind != K3_NP;
     const bool nuc = kind != K3_EP && c.anyNuc;
+    const bool wide = tab && tab_threads(c) == 256;
+    const dim3 grid4((nch + 255) / 256);
+    const size_t dynTab4 = site_bytes(c) + 4 * table_bytes(c) + 16;
     if (e3c) {
-        if (tab && mixed)
+        if (wide && mixed)
+            hipLaunchKernelGGL((k_three_center<LA, LB, false, true, true, 256>), grid4, dim3(256),
+                               dynTab4, s, c, cl.rec[k], nch, cl.chRec[k], cl.chBeg[k], cl.chEnd[k],
+                               cl.img[k], kind, tt, ti, coef, 0, W);
+        else if (wide)
+            hipLaunchKernelGGL((k_three_center<LA, LB, false, true, false, 256>), grid4, dim3(256),
+                               dynTab4, s, c, cl.rec[k], nch, cl.chRec[k], cl.chBeg[k], cl.chEnd[k],
+                               cl.img[k], kind, tt, ti, coef, 0, W);
+        else if (tab && mixed)
             hipLaunchKernelGGL((k_three_center<LA, LB, false, true, true>), grid, dim3(64), dynTab,
                                s, c, cl.rec[k], nch, cl.chRec[k], cl.chBeg[k], cl.chEnd[k], cl.img[k], kind,
                                tt, ti, coef, 0, W);
