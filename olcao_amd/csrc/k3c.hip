@@ -80,25 +80,16 @@ __device__ __forceinline__ void three_center_wave(const DevCtx &c, const PPRec *
     const int ri = active ? i : 0;
     const PPRec rec = recs[chRec[ri]];
     const PairConst q = make_pair(rec.a1, rec.a2, rec.A, rec.B);
-    // term tables [type * maxPA + alpha][thread] of mu and pref0, after the sites
-    const int nTab = c.nPotTypes * c.maxPA * NT;
-    double *tabMu = dyn + ((3 * c.nPotSites + (c.nPotSites + 1) / 2 + 1) & ~1);
-    double *tabPref = tabMu + nTab;
+    // term table [type * maxPA + alpha][thread] of 1 / (zeta + a3), after
+    // the sites: the item's mu and prefactor follow from it with
+    // multiplications and one square root, in the same operations as when
+    // both were tabulated (one table instead of two: more waves per CU)
+    double *tabInv = dyn + ((3 * c.nPotSites + (c.nPotSites + 1) / 2 + 1) & ~1);
     if (TAB && !NUC) {
         for (int t = 0; t < c.nPotTypes; ++t)
             for (int a = 0; a < c.maxPA; ++a) {
                 const double a3 = tAlpha[t * kMaxPotAlphas + a];
-                const double wgt = kind == K3_HAM ? tCoef[t * kMaxPotAlphas + a] : 1.0;
-                double mu = 0.0, pref0 = 0.0;
-                if (a3 > 0.0) {
-                    const double z3 = q.zeta + a3;
-                    const double inv = 1.0 / z3;
-                    const double tt = kPi * inv;
-                    mu = q.zeta * a3 * inv;
-                    pref0 = wgt * tt * sqrt(tt);
-                }
-                tabMu[(t * c.maxPA + a) * NT + tid] = mu;
-                tabPref[(t * c.maxPA + a) * NT + tid] = pref0;
+                tabInv[(t * c.maxPA + a) * NT + tid] = a3 > 0.0 ? 1.0 / (q.zeta + a3) : 0.0;
             }
     }
     double acc[NH];
@@ -145,8 +136,13 @@ __device__ __forceinline__ void three_center_wave(const DevCtx &c, const PPRec *
             } else {
                 const int term = kind == K3_HAM ? nn : termIdx;
                 if (TAB) {
-                    const int ix = (t * c.maxPA + term) * NT + tid;
-                    acc_e3c_herm_tab<L, MIX>(q, tabMu[ix], tabPref[ix], C, sExp2, acc);
+                    const int tq = t * kMaxPotAlphas + term;
+                    const double inv = tabInv[(t * c.maxPA + term) * NT + tid];
+                    const double wgt = kind == K3_HAM ? tCoef[tq] : 1.0;
+                    const double tt = kPi * inv;
+                    const double mu = q.zeta * tAlpha[tq] * inv;
+                    const double pref0 = wgt * tt * sqrt(tt);
+                    acc_e3c_herm_tab<L, MIX>(q, mu, pref0, C, sExp2, acc);
                 } else {
                     const double wgt = kind == K3_HAM ? tCoef[t * kMaxPotAlphas + term] : 1.0;
                     acc_e3c_herm<L>(q, tAlpha[t * kMaxPotAlphas + term], C, wgt, acc);
@@ -326,7 +322,7 @@ static size_t table_bytes(const DevCtx &c) { return k3c_table_bytes(c); }
 // Threads per workgroup of the tabulated three-Gaussian pass: 256 when four
 // waves sharing one copy of the sites and constants fit more waves into a
 // CU's LDS than single waves each holding their own (the term table, a
-// column per thread, dominates: 16 KB per wave for 16 terms).
+// column per thread, dominates: 8 KB per wave for 16 terms).
 constexpr size_t kCuLds = 160 * 1024;
 constexpr size_t kStaticLds3c = (2 * kMaxPotTypes * kMaxPotAlphas + 2 * kMaxPotTypes + 256) * 8;
 static int tab_threads(const DevCtx &c)

@@ -174,14 +174,14 @@ void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind, int te
                          int mixed, unsigned mask, hipStream_t s);
 // Dynamic LDS of the three-centre kernels: site positions and types, then
 // (the tabulated form, used when the table fits kTableLdsMax) the per-lane
-// term table of mu and the prefactor.
+// term table of 1 / (zeta + a3), per 64-lane wave.
 inline size_t k3c_site_bytes(const DevCtx &c)
 {
     return (size_t)(((3 * c.nPotSites + (c.nPotSites + 1) / 2 + 1) & ~1)) * sizeof(double);
 }
 inline size_t k3c_table_bytes(const DevCtx &c)
 {
-    return (size_t)c.nPotTypes * c.maxPA * 64 * 2 * sizeof(double);
+    return (size_t)c.nPotTypes * c.maxPA * 64 * sizeof(double);
 }
 constexpr size_t kTableLdsMax = 48 * 1024;
 
