@@ -96,10 +96,15 @@ typedef struct {
 /* Generalised eigensolve: OLCAO_EIG_AUTO uses the batched in-LDS solver
  * (one workgroup per k-point: Cholesky, reduction, Jacobi) where the matrices
  * fit (complex n <= 70, real n <= 99), else the device tridiagonal path
- * (blocked Cholesky, cooperative Householder reduction, divide and conquer,
- * blocked back-transform) up to n = 896, and rocSOLVER beyond;
- * OLCAO_EIG_LIBRARY always uses rocSOLVER. */
-enum { OLCAO_EIG_AUTO = 0, OLCAO_EIG_LIBRARY = 1 };
+ * (blocked Cholesky, Householder reduction, divide and conquer, blocked
+ * back-transform): its reduction keeps the matrix in LDS across a group of
+ * workgroups up to n = 896 and works on it in global memory by panels of
+ * 32 columns beyond (real n <= 7168, complex n <= 4096); rocSOLVER past
+ * those sizes.  OLCAO_EIG_LIBRARY always uses rocSOLVER; OLCAO_EIG_BLOCKED
+ * takes the global-memory reduction also where the LDS one fits.  The
+ * drop-in solvers (olcao_solve_zhegv/dsygv) read the same choice from the
+ * environment variable OLCAO_EIGENSOLVER (auto, library or blocked). */
+enum { OLCAO_EIG_AUTO = 0, OLCAO_EIG_LIBRARY = 1, OLCAO_EIG_BLOCKED = 2 };
 
 /* The tile shapes chunkItems, segPairs and bloch below are what the
  * project's autotuner (olcao_amd/autotune.py) chooses per workload.
@@ -115,8 +120,9 @@ typedef struct {
     int bloch;                    /* OLCAO_BLOCH_* */
     int graph;                    /* 1: olcao_scf_iteration replays a captured graph
                                      of its device work (build and solve) after its
-                                     first call; the library eigensolver (n > 896)
-                                     is not captured and such contexts stay eager */
+                                     first call; the library eigensolver (real n >
+                                     7168, complex n > 4096) is not captured and
+                                     such contexts stay eager */
     int chunkItems;               /* three-centre work per lane: image-list chunks of
                                      about this many items (16..1048576; 0: 192) */
     int segPairs;                 /* contraction segment length in primitive pairs

@@ -34,6 +34,7 @@ UNITS = [("kernels.hip", "kernels.o", []),
          ("k3c.hip", "k3c_la3.o", ["-DOLCAO_LA=3"]),
          ("ksolve.hip", "ksolve.o", []),
          ("ktrd.hip", "ktrd.o", []),
+         ("ktrdb.hip", "ktrdb.o", []),
          ("kdc.hip", "kdc.o", []),
          ("kchol.hip", "kchol.o", []),
          ("kbloch.hip", "kbloch.o", []),

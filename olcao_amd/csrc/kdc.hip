@@ -80,9 +80,11 @@ __device__ double block_reduce(double v, double *red)
 template <int NT>
 __device__ void block_scan(const int *flag, int *pre, int m, int *red)
 {
-    constexpr int kSeg = (kDcMax + NT - 1) / NT;
+    const int kSeg = (m + NT - 1) / NT;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int b0 = t * kSeg, b1 = min(m, b0 + kSeg);
+    // a thread's segment holds flags other threads wrote
+    __syncthreads();
     int sum = 0;
     for (int p = b0; p < b1; ++p) sum += flag[p];
     int inc = sum;   // inclusive scan within the wave
@@ -143,14 +145,18 @@ struct MergeLds {
 // 160 KiB, for whatever kDcMaxN is set to.
 constexpr size_t kDcMergeLdsBytes = sizeof(MergeLds) + (size_t)kDcMax * (3 * 8 + 2 * 4);
 static_assert(kDcMergeLdsBytes <= 160 * 1024, "divide-and-conquer merge state exceeds LDS");
+// a large merge's preparation keeps 20 B per entry in LDS beside MergeView
+static_assert((size_t)kDcBigMaxN * 20 + 512 <= 160 * 1024, "large-merge LDS exceeds 160 KiB");
+static_assert(kTrdbMaxReal <= kDcBigMaxN && kTrdbMaxComplex <= kDcBigMaxN,
+              "blocked reduction larger than the divide and conquer");
 
 // Sort the children's poles, form z, deflate (dlaed2's rules: small rho z,
 // then Givens rotations of poles closer than the tolerance, in one ascending
 // scan that runs only when a parallel test finds such a pair), and leave the
 // kept poles dk, z, rho z^2 and the ascending deflated values in L.
-template <int NT>
+template <int NT, class ST>
 __device__ void dc_prepare(int n, int off, int m1, int m, const double *D, const double *E,
-                           const double *Q, int *ord, MergeLds &L)
+                           const double *Q, int *ord, ST &L)
 {
     const int tid = threadIdx.x;
     const double beta = E[off + m1 - 1];
@@ -304,6 +310,7 @@ __device__ void dc_prepare(int n, int off, int m1, int m, const double *D, const
 // Root j of 1 + sum zz / (dk - lambda) = 0 by one wave: bracketed, in
 // coordinates shifted to the nearer pole (org), two-pole rational model
 // steps; lambda = dk(org) + tau.
+template <bool BIG = false>
 __device__ void dc_root(int j, int K, const double *dk, const double *zz, int &orgOut,
                         double &tauOut)
 {
@@ -339,19 +346,24 @@ __device__ void dc_root(int j, int K, const double *dk, const double *zz, int &o
         // left poles (ii <= j) give negative terms, right poles positive;
         // the lane's terms unrolled so their reciprocals overlap
         double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
+        auto term = [&](int ii, bool on) {
+            const int iu = on ? ii : 0;
+            const double r = rcp_nr((dk[iu] - dorg) - tau);
+            const double t = on ? zz[iu] * r : 0.0;
+            const bool left = ii <= j;
+            psi += left ? t : 0.0;
+            dpsi += left ? t * r : 0.0;
+            phi += left ? 0.0 : t;
+            dphi += left ? 0.0 : t * r;
+        };
+        if constexpr (BIG) {
+            // the same terms in the same order per lane, any K
+            for (int ii = lane; ii < K; ii += 64) term(ii, true);
+        } else {
 #pragma unroll
-        for (int q = 0; q < kDcMax / 64; ++q) {
-            const int ii = lane + 64 * q;
-            if (64 * q < K) {
-                const bool on = ii < K;
-                const int iu = on ? ii : 0;
-                const double r = rcp_nr((dk[iu] - dorg) - tau);
-                const double t = on ? zz[iu] * r : 0.0;
-                const bool left = ii <= j;
-                psi += left ? t : 0.0;
-                dpsi += left ? t * r : 0.0;
-                phi += left ? 0.0 : t;
-                dphi += left ? 0.0 : t * r;
+            for (int q = 0; q < kDcMax / 64; ++q) {
+                const int ii = lane + 64 * q;
+                if (64 * q < K) term(ii, ii < K);
             }
         }
         wave_sum4(psi, dpsi, phi, dphi);
@@ -559,6 +571,9 @@ __global__ __launch_bounds__(NT) void k_dc_merge(const int4 *nodes, int n, doubl
 struct DcState {
     double *dk, *zk, *dv, *lam, *tau, *rotC, *rotS;
     int *keep, *dp, *defl, *org, *rowKeep, *rotP, *rotQ;
+    // merges larger than kDcMaxN only: zhat, and the preparation's work arrays
+    double *zh, *sd, *sz, *zz;
+    int *order;
     int4 *cnt;      // per node x matrix: K, Nd, Nr
     double *rho;    // per node x matrix
 };
@@ -687,6 +702,216 @@ __global__ __launch_bounds__(256) void k_dc_vecs(const int2 *tiles, const int4 *
                     Wall + (size_t)n * n * mat);
 }
 
+// ---- Merges larger than kDcMaxN (up to kDcBigMaxN): the same steps and the
+// same arithmetic in the same order, with the merge state in the per-matrix
+// global state arrays at the node's offset instead of LDS.  The sorted poles,
+// their z and the deflation flags stay in (dynamic) LDS, where the sequential
+// rotation scan reads them; the roots and the eigenvector columns stage
+// their poles in LDS; zhat is formed once per merge by its own kernel.
+struct MergeView {
+    double *sd, *sz, *ds, *zs, *dk, *zk, *zz, *dv, *rotC, *rotS;
+    int *order, *keep, *rowKeep, *defl, *scratch, *dp, *rotP, *rotQ;
+    double red[16];
+    int ired[16];
+    int K, Nd, Nr, close, unsorted;
+    double rho;
+};
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_dc_prep_big(const int4 *nodes, int nodeBase, int nNodes,
+                                                    int n, const double *Dall, const double *Eall,
+                                                    const double *Qall, int *ordAll, DcState st)
+{
+    const int node = nodeBase + blockIdx.x, mat = blockIdx.y;
+    const int4 nd = nodes[node];
+    const int off = nd.x, m1 = nd.y, m = nd.z;
+    extern __shared__ double big_lds[];
+    __shared__ MergeView L;
+    const size_t b = (size_t)n * mat + off;
+    if (threadIdx.x == 0) {
+        L.ds = big_lds;
+        L.zs = big_lds + m;
+        L.scratch = reinterpret_cast<int *>(big_lds + 2 * m);
+        L.sd = st.sd + b;
+        L.sz = st.sz + b;
+        L.zz = st.zz + b;
+        L.order = st.order + b;
+        L.dk = st.dk + b;
+        L.zk = st.zk + b;
+        L.dv = st.dv + b;
+        L.rotC = st.rotC + b;
+        L.rotS = st.rotS + b;
+        L.keep = st.keep + b;
+        L.rowKeep = st.rowKeep + b;
+        L.defl = st.defl + b;
+        L.dp = st.dp + b;
+        L.rotP = st.rotP + b;
+        L.rotQ = st.rotQ + b;
+    }
+    __syncthreads();
+    dc_prepare<NT>(n, off, m1, m, Dall + (size_t)n * mat, Eall + (size_t)n * mat,
+                   Qall + (size_t)n * n * mat, ordAll + (size_t)2 * n * mat, L);
+    if (threadIdx.x == 0) {
+        st.cnt[(size_t)mat * nNodes + blockIdx.x] = make_int4(L.K, L.Nd, L.Nr, 0);
+        st.rho[(size_t)mat * nNodes + blockIdx.x] = L.rho;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dc_roots_big(const int2 *tiles, const int4 *nodes,
+                                                      int nodeBase, int nNodes, int n, DcState st)
+{
+    const int2 tl = tiles[blockIdx.x];
+    const int mat = blockIdx.y;
+    const int4 nd = nodes[nodeBase + tl.x];
+    const int off = nd.x;
+    const int4 c = st.cnt[(size_t)mat * nNodes + tl.x];
+    const int K = c.x;
+    const int j0 = tl.y * kRootsPerWg;
+    if (j0 >= K) return;
+    const double rho = st.rho[(size_t)mat * nNodes + tl.x];
+    extern __shared__ double big_lds[];
+    double *dk = big_lds, *zz = big_lds + K;
+    const size_t b = (size_t)n * mat + off;
+    for (int i = threadIdx.x; i < K; i += 256) {
+        dk[i] = st.dk[b + i];
+        const double z = st.zk[b + i];
+        zz[i] = rho * z * z;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int j = j0 + wv; j < min(K, j0 + kRootsPerWg); j += 4) {
+        int org;
+        double tau;
+        dc_root<true>(j, K, dk, zz, org, tau);
+        if (lane == 0) {
+            st.lam[b + j] = dk[org] + tau;
+            st.org[b + j] = org;
+            st.tau[b + j] = tau;
+        }
+    }
+}
+
+// zhat of the kept poles, one per lane (tiles: the level's 64-column tiles)
+__global__ __launch_bounds__(64) void k_dc_zhat_big(const int2 *tiles, const int4 *nodes,
+                                                    int nodeBase, int nNodes, int n, DcState st)
+{
+    const int2 tl = tiles[blockIdx.x];
+    const int mat = blockIdx.y;
+    const int4 nd = nodes[nodeBase + tl.x];
+    const size_t b = (size_t)n * mat + nd.x;
+    const int K = st.cnt[(size_t)mat * nNodes + tl.x].x;
+    const int ii = tl.y * kColsPerWg + (int)threadIdx.x;
+    if (ii >= K) return;
+    st.zh[b + ii] = dc_zhat(ii, K, st.dk + b, st.zk + b, st.rho[(size_t)mat * nNodes + tl.x],
+                            st.org + b, st.tau + b);
+}
+
+// dc_columns for one 64-column tile of a large merge: each kept column's
+// norm in a first pass over the poles, its entries in a second.
+__global__ __launch_bounds__(256) void k_dc_vecs_big(const int2 *tiles, const int4 *nodes,
+                                                     int nodeBase, int nNodes, int n, double *Dall,
+                                                     double *Wall, DcState st)
+{
+    const int2 tl = tiles[blockIdx.x];
+    const int mat = blockIdx.y;
+    const int4 nd = nodes[nodeBase + tl.x];
+    const int off = nd.x, m = nd.z;
+    const int c0 = tl.y * kColsPerWg, c1 = min(m, c0 + kColsPerWg);
+    if (c0 >= m) return;
+    const int4 cn = st.cnt[(size_t)mat * nNodes + tl.x];
+    const int K = cn.x, Nd = cn.y, Nr = cn.z;
+    const size_t b = (size_t)n * mat + off;
+    extern __shared__ double big_lds[];
+    double *dkS = big_lds, *zhS = big_lds + K;
+    __shared__ int colInfo[kColsPerWg];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int i = tid; i < K; i += 256) {
+        dkS[i] = st.dk[b + i];
+        zhS[i] = st.zh[b + i];
+    }
+    // the searches below give every column of the tile its entry; a column
+    // they miss (non-finite input) stays a unit vector rather than reading
+    // an index it was never given
+    if (tid < kColsPerWg) colInfo[tid] = -1 - (c0 + tid);
+    __syncthreads();
+    const double *lam = st.lam + b, *rtau = st.tau + b, *dv = st.dv + b;
+    const int *rorg = st.org + b, *dp = st.dp + b, *keep = st.keep + b, *defl = st.defl + b;
+    double *D = Dall + (size_t)n * mat;
+    double *W = Wall + (size_t)n * n * mat;
+    auto Wat = [&](int row, int col) -> double & { return W[(size_t)(off + row) + (size_t)n * (off + col)]; };
+    for (int j = tid; j < K; j += 256) {
+        const double v = lam[j];
+        int lo = 0, hi = Nd;
+        while (lo < hi) {
+            const int md = (lo + hi) >> 1;
+            if (dv[md] < v) lo = md + 1; else hi = md;
+        }
+        const int s = j + lo;
+        if (s >= c0 && s < c1) {
+            colInfo[s - c0] = j;
+            D[off + s] = v;
+        }
+    }
+    for (int q = tid; q < Nd; q += 256) {
+        const double v = dv[q];
+        int lo = 0, hi = K;
+        while (lo < hi) {
+            const int md = (lo + hi) >> 1;
+            if (lam[md] <= v) lo = md + 1; else hi = md;
+        }
+        const int s = q + lo;
+        if (s >= c0 && s < c1) {
+            colInfo[s - c0] = -1 - dp[q];
+            D[off + s] = v;
+        }
+    }
+    __syncthreads();
+    for (int s = c0 + wv; s < c1; s += 4) {
+        const int info = colInfo[s - c0];
+        if (info >= 0) {
+            const int j = info;
+            const double dj = dkS[rorg[j]], tj = rtau[j];
+            double part = 0.0;
+            for (int ii = lane; ii < K; ii += 64) {
+                const double u = zhS[ii] * rcp_nr((dkS[ii] - dj) - tj);
+                part += u * u;
+            }
+            const double inv = 1.0 / sqrt(wave_sum(part));
+            for (int ii = lane; ii < K; ii += 64)
+                Wat(keep[ii], s) = zhS[ii] * rcp_nr((dkS[ii] - dj) - tj) * inv;
+            for (int q = lane; q < Nd; q += 64) Wat(defl[q], s) = 0.0;
+        } else {
+            const int pd = -1 - info;
+            for (int p = lane; p < m; p += 64) Wat(p, s) = p == pd ? 1.0 : 0.0;
+        }
+    }
+    __syncthreads();
+    if (Nr > 0) {
+        const int *rotP = st.rotP + b, *rotQ = st.rotQ + b;
+        const double *rotC = st.rotC + b, *rotS = st.rotS + b;
+        for (int s = c0 + tid; s < c1; s += 256) {
+            int carryRow = -1;
+            double carry = 0.0;
+            for (int q = Nr - 1; q >= 0; --q) {
+                const int p = rotP[q], r = rotQ[q];
+                const double c = rotC[q], sn = rotS[q];
+                double y;
+                if (r == carryRow) {
+                    y = carry;
+                } else {
+                    if (carryRow >= 0) Wat(carryRow, s) = carry;
+                    y = Wat(r, s);
+                }
+                const double x = Wat(p, s);
+                Wat(r, s) = sn * x + c * y;
+                carry = c * x - sn * y;
+                carryRow = p;
+            }
+            Wat(carryRow, s) = carry;
+        }
+    }
+}
+
 // Qt(node) = Q(node) P W over node blocks; tiles: x node, y row tile, z col
 // tile.  Before the merge Q(node) = diag(Q1, Q2) in natural order, so a row
 // of child c takes k over that child's columns only: a tile's k range is
@@ -791,8 +1016,10 @@ __global__ void k_dc_copy(const int4 *tiles, const int4 *nodes, int n, const dou
 bool dc_plan(int n, DcPlan &p)
 {
     p = DcPlan{};
-    // every merge keeps its poles, z and rotations in LDS arrays of kDcMax
-    if (n < 1 || n > kDcMaxN) return false;
+    // merges up to kDcMaxN keep their poles, z and rotations in LDS arrays of
+    // kDcMax; larger ones (levels with maxM > kDcMaxN) keep them in global
+    // memory, with the sorted poles, z and flags in up to 20 B per entry of LDS
+    if (n < 1 || n > kDcBigMaxN) return false;
     p.n = n;
     if (n < 2) return true;
     // merges by height above the 1 x 1 leaves
@@ -833,14 +1060,14 @@ bool dc_plan(int n, DcPlan &p)
     return true;
 }
 
-size_t dc_state_doubles(int n, int nb) { return (size_t)7 * n * nb; }
-size_t dc_state_ints(int n, int nb) { return (size_t)7 * n * nb; }
+size_t dc_state_doubles(int n, int nb) { return (size_t)11 * n * nb; }
+size_t dc_state_ints(int n, int nb) { return (size_t)8 * n * nb; }
 
 bool launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double *E, double *Q,
                double *Qt, double *W, int *ord, hipStream_t s)
 {
     const int n = p.n;
-    if (n < 1 || n > kDcMaxN) return false;
+    if (n < 1 || n > kDcBigMaxN) return false;
     const size_t nn = (size_t)n * nb;
     DcState st;
     st.dk = dv.stD;
@@ -857,11 +1084,26 @@ bool launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double
     st.rowKeep = st.org + nn;
     st.rotP = st.rowKeep + nn;
     st.rotQ = st.rotP + nn;
+    st.zh = st.rotS + nn;
+    st.sd = st.zh + nn;
+    st.sz = st.sd + nn;
+    st.zz = st.sz + nn;
+    st.order = st.rotQ + nn;
     st.cnt = dv.cnt;
     st.rho = dv.rho;
     hipLaunchKernelGGL(k_dc_init, dim3(64, nb), dim3(256), 0, s, D, E, Q, n);
     for (const DcLevel &L : p.levels) {
-        if (L.maxM <= kDcSmallM) {
+        if (L.maxM > kDcMaxN) {
+            const size_t mm = (size_t)L.maxM;
+            hipLaunchKernelGGL(k_dc_prep_big<kDcBig>, dim3(L.nNodes, nb), dim3(kDcBig), mm * 20, s,
+                               dv.nodes, L.nodeBase, L.nNodes, n, D, E, Q, ord, st);
+            hipLaunchKernelGGL(k_dc_roots_big, dim3(L.nRoot, nb), dim3(256), mm * 16, s,
+                               dv.rootTiles + L.rootBase, dv.nodes, L.nodeBase, L.nNodes, n, st);
+            hipLaunchKernelGGL(k_dc_zhat_big, dim3(L.nCol, nb), dim3(64), 0, s,
+                               dv.colTiles + L.colBase, dv.nodes, L.nodeBase, L.nNodes, n, st);
+            hipLaunchKernelGGL(k_dc_vecs_big, dim3(L.nCol, nb), dim3(256), mm * 16, s,
+                               dv.colTiles + L.colBase, dv.nodes, L.nodeBase, L.nNodes, n, D, W, st);
+        } else if (L.maxM <= kDcSmallM) {
             hipLaunchKernelGGL(k_dc_merge<kDcSmall>, dim3(L.nNodes, nb), dim3(kDcSmall), 0, s,
                                dv.nodes + L.nodeBase, n, D, E, Q, W, ord);
         } else {

@@ -327,6 +327,29 @@ void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double 
                         double *U, hipStream_t s);
 void launch_real_to_field(const double *Z, double *X, long long count, bool cplx, hipStream_t s);
 
+// Blocked reduction with the matrix in global memory (ktrdb.hip), for the
+// sizes the LDS-resident reduction does not take: panels of kPanelB columns
+// (start kernel, cooperative panel kernel), each followed by a her2k/syr2k
+// update of the trailing lower triangle by the caller.  Same outputs as
+// launch_trd; W: n x kPanelB per matrix; pub: pubLen doubles per matrix;
+// cnt: 64 words per matrix; matrices in flight: nGroups.
+constexpr int kPanelB = 32;
+constexpr int kTrdbMaxReal = 7168;     // = kDcBigMaxN (asserted in kdc.hip)
+constexpr int kTrdbMaxComplex = 4096;
+struct TrdbPlan {
+    int G = 0;            // workgroups per matrix (<= 256)
+    int nGroups = 0;      // matrices in flight
+    size_t lds = 0;       // dynamic LDS bytes per workgroup
+    size_t pubLen = 0;    // publication doubles per matrix
+};
+bool trdb_plan(int n, int nb, bool cplx, TrdbPlan &p);
+hipError_t launch_trdb_start(bool cplx, const TrdbPlan &pl, const double *A, double *V, double *W,
+                             double *d, double *e, double *tau, double *pub, unsigned *cnt,
+                             int *info, int n, int nMat, int j0, hipStream_t s);
+hipError_t launch_trdb_panel(bool cplx, const TrdbPlan &pl, const double *A, double *V, double *W,
+                             double *d, double *e, double *tau, double *pub, unsigned *cnt,
+                             int *info, int n, int nMat, int j0, int pb, hipStream_t s);
+
 // Blocked batched Cholesky, diagonal step (kchol.hip): factors the updated
 // diagonal block [j0, j0 + jb) of B (upper, column-major) into U and writes
 // its inverse into Dinv (rows j0.., columns 0..jb-1; n x n per matrix);
@@ -340,6 +363,9 @@ void launch_chol_diag(bool cplx, const double *B, double *U, double *Dinv, int n
 // (couplings), Q (eigenvectors out, column-major n x n per matrix); Qt, W:
 // n x n work per matrix, ord: n ints per matrix.
 constexpr int kDcMaxN = 64 * 14;
+// merges past kDcMaxN keep their state in global memory (sorted poles, z and
+// deflation flags in 20 B of LDS per entry): the tridiagonal solver's limit
+constexpr int kDcBigMaxN = 7168;
 struct DcLevel {
     int nodeBase = 0, nNodes = 0, tileBase = 0, nTiles = 0, maxM = 0;
     int rootBase = 0, nRoot = 0, colBase = 0, nCol = 0;
@@ -361,7 +387,7 @@ struct DcDev {
     int4 *cnt;        // nodes x nb
     double *rho;      // nodes x nb
 };
-// false (nothing planned or launched) when n is outside 1..kDcMaxN
+// false (nothing planned or launched) when n is outside 1..kDcBigMaxN
 bool dc_plan(int n, DcPlan &p);
 size_t dc_state_doubles(int n, int nb);
 size_t dc_state_ints(int n, int nb);

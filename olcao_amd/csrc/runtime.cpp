@@ -133,6 +133,8 @@ struct EigWork {
     DBuf<double> dcStD, dcRho;
     DBuf<int> dcStI;
     int lastPath = -1;     // path of the last gen_eig: 0 LDS solver, 1 tridiagonal, 2 library
+    bool forceBlocked = false;   // the global-memory reduction also where the LDS one fits
+    DBuf<double> Wp;       // blocked reduction: the panel's W (n x kPanelB per matrix)
     // events around the tridiagonal reduction (set by a context; the
     // reduction is timed when they exist and the solve is not captured)
     hipEvent_t trdEv[2] = {nullptr, nullptr};
@@ -1124,8 +1126,69 @@ void ensure_mats(olcao_ctx &c)
 // conquer T = Z diag(w) Z^T (kdc.hip), and Y = Q Z by blocks of reflectors,
 // X <- X - U_b (V_b^H X), with batched GEMMs.  D receives w, A receives Y,
 // dinfo the reduction's status.
-void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, bool cplx, int n,
-             int nb, double *A, double *D, double *E, int *dinfo, bool defer)
+// Trailing update C -= V W^H + W V^H of the lower triangle (zher2k/dsyr2k),
+// batched with the strides given.
+void her2k_lower(rocblas_handle h, bool cplx, int m, int k, const double *Vp, rocblas_stride sv,
+                 const double *Wp, rocblas_stride sw, double *C, rocblas_stride sc, int ld, int nb)
+{
+    rocblas_status r;
+    if (cplx) {
+        const rocblas_double_complex alpha = {-1.0, 0.0};
+        const double beta = 1.0;
+        r = rocblas_zher2k_strided_batched(h, rocblas_fill_lower, rocblas_operation_none, m, k, &alpha,
+                                           (const rocblas_double_complex *)Vp, ld, sv,
+                                           (const rocblas_double_complex *)Wp, ld, sw, &beta,
+                                           (rocblas_double_complex *)C, ld, sc, nb);
+    } else {
+        const double alpha = -1.0, beta = 1.0;
+        r = rocblas_dsyr2k_strided_batched(h, rocblas_fill_lower, rocblas_operation_none, m, k, &alpha,
+                                           Vp, ld, sv, Wp, ld, sw, &beta, C, ld, sc, nb);
+    }
+    if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "her2k status %d", (int)r);
+}
+
+// The blocked reduction (ktrdb.hip) of the batch in A (lower triangle used,
+// overwritten by the trailing updates), nGroups matrices at a time: per
+// panel of kPanelB columns the start kernel, the cooperative panel kernel
+// and the update of the trailing lower triangle; then d(n-1).  Outputs as
+// launch_trd: V, tau (in ew), D, E, dinfo (-2: a group barrier timed out).
+void trdb_reduce(EigWork &ew, const TrdbPlan &pl, rocblas_handle h, hipStream_t s, bool cplx, int n,
+                 int nb, double *A, double *D, double *E, int *dinfo)
+{
+    const int el = cplx ? 2 : 1;
+    const size_t per = (size_t)n * n, perW = (size_t)n * kPanelB;
+    ew.Wp.alloc(perW * el * nb);
+    ew.pub.alloc(pl.pubLen * nb);
+    ew.cnt.alloc((size_t)64 * nb);
+    launch_fill(ew.V.p, (long long)(per * el * nb), 0.0, s);
+    launch_zero_words(dinfo, nb, s);
+    for (int m0 = 0; m0 < nb; m0 += pl.nGroups) {
+        const int nm = std::min(pl.nGroups, nb - m0);
+        const double *Am = A + el * per * m0;
+        double *Aw = A + el * per * m0;
+        double *Vm = ew.V.p + el * per * m0, *Wm = ew.Wp.p + el * perW * m0;
+        double *Dm = D + (size_t)n * m0, *Em = E + (size_t)n * m0;
+        double *taum = ew.tau.p + (size_t)el * n * m0, *pubm = ew.pub.p + pl.pubLen * m0;
+        unsigned *cntm = ew.cnt.p + (size_t)64 * m0;
+        int *im = dinfo + m0;
+        for (int j0 = 0; j0 < n - 1; j0 += kPanelB) {
+            const int pb = std::min(kPanelB, n - 1 - j0);
+            HIPCHK(launch_trdb_start(cplx, pl, Am, Vm, Wm, Dm, Em, taum, pubm, cntm, im, n, nm, j0, s));
+            HIPCHK(launch_trdb_panel(cplx, pl, Am, Vm, Wm, Dm, Em, taum, pubm, cntm, im, n, nm, j0, pb,
+                                     s));
+            const int t0 = j0 + pb;
+            if (t0 < n)
+                her2k_lower(h, cplx, n - t0, pb, Vm + el * ((size_t)t0 + (size_t)n * j0), (rocblas_stride)per,
+                            Wm + el * (size_t)t0, (rocblas_stride)perW,
+                            Aw + el * ((size_t)t0 + (size_t)n * t0), (rocblas_stride)per, n, nm);
+        }
+        HIPCHK(launch_trdb_start(cplx, pl, Am, Vm, Wm, Dm, Em, taum, pubm, cntm, im, n, nm, n - 1, s));
+    }
+}
+
+void trd_eig(EigWork &ew, const TrdPlan *plan, const TrdbPlan *bplan, rocblas_handle h,
+             hipStream_t s, bool cplx, int n, int nb, double *A, double *D, double *E, int *dinfo,
+             bool defer)
 {
     const int el = cplx ? 2 : 1;
     const size_t per = (size_t)n * n;
@@ -1133,24 +1196,29 @@ void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, 
     ew.U.alloc(per * el * nb);
     ew.W.alloc((size_t)kTrdBlock * n * el * nb);
     ew.tau.alloc((size_t)n * el * nb);
-    ew.pub.alloc(plan.pubDoubles);
-    ew.cnt.alloc(plan.counters);
+    if (plan) {
+        ew.pub.alloc(plan->pubDoubles);
+        ew.cnt.alloc(plan->counters);
+    }
     ew.Z.alloc(per * nb);
     ew.Zt.alloc(per * nb);
     ew.Wd.alloc(per * nb);
     ew.ord.alloc((size_t)2 * n * nb);   // per matrix: sort order, then its inverse
     if (ew.dc.n != n) {
         if (!dc_plan(n, ew.dc))
-            fail(OLCAO_ERR_UNSUPPORTED, "divide and conquer takes n <= %d, not %d", kDcMaxN, n);
+            fail(OLCAO_ERR_UNSUPPORTED, "divide and conquer takes n <= %d, not %d", kDcBigMaxN, n);
         ew.dcNodes.upload(ew.dc.nodes, s);
         ew.dcTiles.upload(ew.dc.tiles, s);
         ew.dcRoot.upload(ew.dc.rootTiles, s);
         ew.dcCol.upload(ew.dc.colTiles, s);
     }
-    launch_zero_words(ew.cnt.p, (long long)plan.counters, s);
+    if (plan) launch_zero_words(ew.cnt.p, (long long)plan->counters, s);
     const bool timed = ew.trdEv[0] && !defer;
     if (timed) HIPCHK(hipEventRecord(ew.trdEv[0], s));
-    HIPCHK(launch_trd(cplx, plan, n, nb, A, ew.V.p, D, E, ew.tau.p, ew.pub.p, ew.cnt.p, dinfo, s));
+    if (plan)
+        HIPCHK(launch_trd(cplx, *plan, n, nb, A, ew.V.p, D, E, ew.tau.p, ew.pub.p, ew.cnt.p, dinfo, s));
+    else
+        trdb_reduce(ew, *bplan, h, s, cplx, n, nb, A, D, E, dinfo);
     if (timed) {
         HIPCHK(hipEventRecord(ew.trdEv[1], s));
         ew.trdTimed = true;
@@ -1162,7 +1230,7 @@ void trd_eig(EigWork &ew, const TrdPlan &plan, rocblas_handle h, hipStream_t s, 
     const DcDev dv{ew.dcNodes.p, ew.dcTiles.p, ew.dcRoot.p, ew.dcCol.p, ew.dcStD.p, ew.dcStI.p,
                    ew.dcCnt.p, ew.dcRho.p};
     if (!launch_dc(ew.dc, dv, nb, D, E, ew.Z.p, ew.Zt.p, ew.Wd.p, ew.ord.p, s))
-        fail(OLCAO_ERR_UNSUPPORTED, "divide and conquer takes n <= %d, not %d", kDcMaxN, n);
+        fail(OLCAO_ERR_UNSUPPORTED, "divide and conquer takes n <= %d, not %d", kDcBigMaxN, n);
     HIPCHK(hipGetLastError());
     ew.Tw.alloc((size_t)((n - 1 + kTrdBlock - 1) / kTrdBlock) * kTrdBlock * kTrdBlock * el * nb);
     launch_trd_tfactor(cplx, n, nb, ew.V.p, ew.tau.p, ew.Tw.p, ew.U.p, s);
@@ -1348,7 +1416,9 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
         return;
     }
     TrdPlan plan;
-    if (!library && trd_fits(n, nb, cplx, plan)) {
+    TrdbPlan bplan;
+    const bool ldsTrd = !library && !ew.forceBlocked && trd_fits(n, nb, cplx, plan);
+    if (ldsTrd || (!library && trdb_plan(n, nb, cplx, bplan))) {
         // blocked Cholesky, GEMM triangular solves, cooperative reduction,
         // divide and conquer, blocked back-transforms (all batched)
         ew.lastPath = 1;
@@ -1371,7 +1441,8 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
         launch_herm_fill(A, n, nb, el, s);
         solve_left_h(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
         solve_right(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, ew.Y.p, A);
-        trd_eig(ew, plan, h, s, cplx, n, nb, A, D, E, tinfo, defer);
+        trd_eig(ew, ldsTrd ? &plan : nullptr, ldsTrd ? nullptr : &bplan, h, s, cplx, n, nb, A, D, E,
+                tinfo, defer);
         solve_back(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
         HIPCHK(hipMemcpyAsync(A, ew.Y.p, (size_t)n * n * el * nb * sizeof(double),
                               hipMemcpyDeviceToDevice, s));
@@ -1552,6 +1623,7 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
     olcao_status st = guard(c, [&] {
         c->precision = opt ? opt->precision : 0;
         c->eigLibrary = opt && opt->eigensolver == OLCAO_EIG_LIBRARY;
+        c->eig.forceBlocked = opt && opt->eigensolver == OLCAO_EIG_BLOCKED;
         c->blochMfma = !(opt && opt->bloch == OLCAO_BLOCH_VALU);
         c->blochSplit = !(opt && opt->bloch == OLCAO_BLOCH_MFMA64);
         c->useGraph = opt && opt->graph != 0;
@@ -2634,7 +2706,12 @@ static olcao_status solve_host(int n, int nb, double *A, double *B, double *w, b
         HIPCHK(hipMemcpyAsync(dB.p, B, per * nb * 8, hipMemcpyHostToDevice, s));
         std::vector<int> info;
         EigWork ew;
-        gen_eig(ew, h, s, cplx, n, nb, dA.p, dB.p, dD.p, dE.p, dI.p, dTI.p, info, false);
+        // OLCAO_EIGENSOLVER: auto (default), library or blocked, as the
+        // contexts' olcao_options.eigensolver
+        const char *path = std::getenv("OLCAO_EIGENSOLVER");
+        const bool lib = path && std::strcmp(path, "library") == 0;
+        ew.forceBlocked = path && std::strcmp(path, "blocked") == 0;
+        gen_eig(ew, h, s, cplx, n, nb, dA.p, dB.p, dD.p, dE.p, dI.p, dTI.p, info, lib);
         HIPCHK(hipMemcpyAsync(A, dA.p, per * nb * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(w, dD.p, (size_t)n * nb * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));

@@ -63,7 +63,7 @@ class _Options(C.Structure):
                 ("reserved", C.c_int * 2)]
 
 
-EIG_AUTO, EIG_LIBRARY = 0, 1
+EIG_AUTO, EIG_LIBRARY, EIG_BLOCKED = 0, 1, 2
 BLOCH_AUTO, BLOCH_VALU, BLOCH_MFMA, BLOCH_MFMA64 = 0, 1, 2, 3
 
 

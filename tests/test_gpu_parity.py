@@ -746,6 +746,50 @@ def test_tridiagonal_path_solvers(gpu, n, nb, cplx, degenerate):
         assert np.abs(O - np.eye(n)).max() < 1e-9
 
 
+def _check_pencil_solution(A, B, w, X, n, nb):
+    from scipy.linalg import eigh
+    for k in range(nb):
+        wr = eigh(A[k], B[k], eigvals_only=True)
+        scale = np.abs(wr).max()
+        assert np.abs(w[k] - wr).max() < 1e-10 * scale, (k, np.abs(w[k] - wr).max())
+        R = A[k] @ X[k] - B[k] @ X[k] * w[k]
+        assert np.abs(R).max() < 1e-9 * scale * np.abs(B[k]).max()
+        O = np.conj(X[k].T) @ B[k] @ X[k]
+        assert np.abs(O - np.eye(n)).max() < 1e-9
+
+
+@pytest.mark.parametrize("n,nb,cplx,degenerate", [(100, 2, True, False), (331, 2, True, True),
+                                                  (576, 3, True, False), (200, 2, False, False),
+                                                  (640, 2, False, True), (896, 2, True, False)])
+def test_blocked_reduction_small_sizes(gpu, monkeypatch, n, nb, cplx, degenerate):
+    """The global-memory blocked reduction (panels of 32 columns, trailing
+    her2k/syr2k) forced where the LDS-resident one would run
+    (OLCAO_EIGENSOLVER=blocked), so that it meets the divide and conquer of
+    the smaller sizes: eigenvalues against LAPACK (scipy eigh) within 1e-10
+    of the scale, residuals and B-orthonormality within 1e-9."""
+    monkeypatch.setenv("OLCAO_EIGENSOLVER", "blocked")
+    _log(f"blocked reduction n={n} nb={nb} cplx={cplx} degenerate={degenerate}")
+    A, B = _pencil(np.random.default_rng(7 * n + nb), n, nb, cplx, degenerate)
+    solve = gpu.solveZHEGV if cplx else gpu.solveDSYGV
+    w, X = solve(np.triu(A), np.triu(B))
+    _check_pencil_solution(A, B, w, X, n, nb)
+
+
+@pytest.mark.parametrize("n,nb,cplx,degenerate", [(1000, 2, False, False), (1080, 2, True, False),
+                                                  (1296, 3, True, True), (2000, 1, False, True),
+                                                  (2600, 1, True, False), (3001, 1, False, False)])
+def test_large_device_solvers(gpu, n, nb, cplx, degenerate):
+    """Sizes past the LDS-resident reduction: the blocked reduction and the
+    divide and conquer with its merges past 896 in global memory (the
+    batch's n = 1080 and 1296 cells, and real sizes on the way to the
+    1000-atom cell's 5665): the same bounds against LAPACK."""
+    _log(f"large device solver n={n} nb={nb} cplx={cplx} degenerate={degenerate}")
+    A, B = _pencil(np.random.default_rng(n + 5 * nb), n, nb, cplx, degenerate)
+    solve = gpu.solveZHEGV if cplx else gpu.solveDSYGV
+    w, X = solve(np.triu(A), np.triu(B))
+    _check_pencil_solution(A, B, w, X, n, nb)
+
+
 def _tridiagonal_cases(rng):
     w21 = np.abs(np.arange(21) - 10.0)
     yield "glued Wilkinson", np.tile(w21, 7), np.concatenate([np.ones(20), [1e-8]] * 7)[:-1]
