@@ -1,8 +1,8 @@
 // Blocked reduction of large Hermitian (real symmetric) matrices to real
 // tridiagonal form with the matrix in global memory: the zhetrd/dsytrd
 // (uplo 'L') step inside zhegv/dsygv (interfaces.F90:73-200) as zlatrd-style
-// panels of kPanelB columns, each followed by a zher2k/dsyr2k update of the
-// trailing lower triangle, for the sizes whose rows do not fit the
+// panels of kPanelB columns, each followed by the rank-2pb update of the
+// trailing matrix, A -= [W V] [V W]^H in one GEMM, for the sizes whose rows do not fit the
 // LDS-resident reduction of ktrd.hip.  Its output is that kernel's: V (the
 // explicit reflectors, column i = v_i, v_i(i+1) = 1, zero above), tau, d, e.
 //
@@ -42,6 +42,10 @@ namespace olcao {
 namespace {
 
 constexpr int kBT = 256;                   // threads per workgroup
+// tiles per pass of a workgroup, whose loads are in flight together (fewer
+// for complex matrices, whose tiles take twice the registers)
+template <bool CPLX>
+constexpr int kTiles = CPLX ? 2 : 4;
 constexpr long long kBSpinLimit = 1ll << 22;
 
 __device__ __forceinline__ void bpst(double *p, double v)
@@ -136,7 +140,7 @@ __device__ __forceinline__ bool bgroup_barrier(unsigned *cnt, unsigned target, i
 
 struct TrdbArgs {
     const double *A;      // [nb] n x n, lower triangle current
-    double *V, *W;        // V [nb] n x n; W [nb] n x kPanelB
+    double *V, *W;        // V [nb] n x n; W [nb] n x 3 kPanelB: W, V, W of the panel
     double *d, *e, *tau;  // [nb] n (tau complex for complex matrices)
     double *pub;          // [nb] pubLen doubles
     unsigned *cnt;        // [nb] 64 barrier words
@@ -190,7 +194,9 @@ __global__ __launch_bounds__(kBT) void k_trdb_panel(TrdbArgs p)
     constexpr int E = CPLX ? 2 : 1;
     extern __shared__ double lds_raw[];
     T *vraw = reinterpret_cast<T *>(lds_raw);   // column i as published, rows > i
-    __shared__ T rowbuf[2][16][64];
+    constexpr int kTB = kTiles<CPLX>;
+    __shared__ T rowbuf[kTB][16][64];
+    __shared__ T colsum[kTB][64];   // a pass's column sums, stored out by one wave
     __shared__ T sa[kPanelB], sb[kPanelB], sVs[kPanelB], sWs[kPanelB];
     __shared__ double red[8];
     __shared__ int bad;
@@ -201,7 +207,7 @@ __global__ __launch_bounds__(kBT) void k_trdb_panel(TrdbArgs p)
     const long long nn = (long long)n * n;
     const double *A = p.A + E * nn * mat;
     double *V = p.V + E * nn * mat;
-    double *W = p.W + E * (long long)n * kPanelB * mat;
+    double *W = p.W + E * (long long)n * (3 * kPanelB) * mat;
     double *dd = p.d + (long long)n * mat, *ee = p.e + (long long)n * mat;
     double *tauG = p.tau + (long long)E * n * mat;
     double *pub = p.pub + p.pubLen * mat;
@@ -213,6 +219,11 @@ __global__ __launch_bounds__(kBT) void k_trdb_panel(TrdbArgs p)
     const int tr = tid & 15, tc = tid >> 4;
     const int Ib0 = (j0 + 1) >> 6;
     const T one = N::mk(1.0, 0.0);
+    // loads per thread of a whole column (rows > i) and of a row's tile
+    // partials (at most Tn + 1, over four threads), for the largest n
+    constexpr int kMaxN = CPLX ? kTrdbMaxComplex : kTrdbMaxReal;
+    constexpr int kRL = (kMaxN + kBT - 1) / kBT;
+    constexpr int kOL = ((kMaxN + 63) / 64 + 1 + 3) / 4;
 
     for (int i = j0; i < j0 + p.pb; ++i) {
         const int s = i + 1, c = i - j0;
@@ -220,8 +231,71 @@ __global__ __launch_bounds__(kBT) void k_trdb_panel(TrdbArgs p)
         const int NT = nAct * (nAct + 1) / 2;
         const bool more = i + 1 < j0 + p.pb;
 
+        // tiles of this step: q = w, w + G, ..., kTB per pass; tile q is
+        // (I, J), Ib <= J <= I < Tn, enumerated column by column
+        auto tile_ij = [&](int q, int &I, int &J) {
+            int jj = 0, rem = q;
+            while (rem >= nAct - jj) {
+                rem -= nAct - jj;
+                ++jj;
+            }
+            J = Ib + jj;
+            I = J + rem;
+        };
+        T a[kTB][4][4];
+        int tI[kTB], tJ[kTB];
+        auto issue = [&](int q0) {
+#pragma unroll
+            for (int b = 0; b < kTB; ++b) {
+                const int q = q0 + b * G;
+                tI[b] = -1;
+                tJ[b] = 0;
+                if (q < NT) tile_ij(q, tI[b], tJ[b]);
+                const int r0 = 64 * tI[b], c0 = 64 * tJ[b];
+                const bool diag = tI[b] == tJ[b];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int r = r0 + 4 * tr + u, cc = c0 + 4 * tc + j;
+                        const bool ok = tI[b] >= 0 && r < n && cc < n && cc >= s && r >= s &&
+                                        (!diag || cc <= r);
+                        a[b][j][u] = ok ? N::load(A, r + (long long)cc * n) : N::zero();
+                    }
+            }
+        };
+        int q0 = w;
+        issue(q0);
+        // this workgroup's dot (a_k = V(:, j0+k)^H v, b_k = W(:, k)^H v over
+        // rows >= s; k = G - 1 - w < 2c): its column, loaded with the tiles
+        const int kDot = G - 1 - w;
+        const bool hasDot = kDot < 2 * c;
+        T dcol[kRL];
+        {
+            const double *colp = kDot < c ? V + (long long)E * (j0 + kDot) * n
+                                          : W + (long long)E * (kDot - c) * n;
+#pragma unroll
+            for (int t = 0; t < kRL; ++t) {
+                const int r = s + tid + t * kBT;
+                dcol[t] = hasDot && r < n ? bpldT<CPLX>(colp, r) : N::zero();
+            }
+        }
+
         // ---- R: column i and its reflector (zlarfg), in every workgroup
-        for (int r = s + tid; r < n; r += kBT) vraw[r] = bpldT<CPLX>(cpub, r);
+        // all of a thread's loads issued before any is used
+        {
+            T tmp[kRL];
+#pragma unroll
+            for (int t = 0; t < kRL; ++t) {
+                const int r = s + tid + t * kBT;
+                tmp[t] = r < n ? bpldT<CPLX>(cpub, r) : N::zero();
+            }
+#pragma unroll
+            for (int t = 0; t < kRL; ++t) {
+                const int r = s + tid + t * kBT;
+                if (r < n) vraw[r] = tmp[t];
+            }
+        }
         const double xn2 = wg_sum(tid < G ? bpld(normPub + tid) : 0.0, red);
         const T alpha = vraw[s];
         double ar, ai;
@@ -251,78 +325,86 @@ __global__ __launch_bounds__(kBT) void k_trdb_panel(TrdbArgs p)
 
         // ---- T: lower tiles of the trailing matrix
         double scal = 0.0;
-        int nt = 0;
-        for (int q = w; q < NT; q += G, ++nt) {
-            int jj = 0, rem = q;
-            while (rem >= nAct - jj) {
-                rem -= nAct - jj;
-                ++jj;
-            }
-            const int J = Ib + jj, I = J + rem;
-            const int r0 = 64 * I, c0 = 64 * J;
-            const bool diag = I == J;
-            T a[4][4];
+        while (q0 < NT) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int b = 0; b < kTB; ++b) {
+                if (tI[b] < 0) continue;
+                const int I = tI[b], J = tJ[b], r0 = 64 * I, c0 = 64 * J;
+                const bool diag = I == J;
+                T vc[4], vr[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int r = r0 + 4 * tr + u, cc = c0 + 4 * tc + j;
-                    const bool ok = r < n && cc < n && cc >= s && r >= s && (!diag || cc <= r);
-                    a[j][u] = ok ? N::load(A, r + (long long)cc * n) : N::zero();
-                }
-            T vc[4], vr[4];
+                for (int j = 0; j < 4; ++j) vc[j] = vat(c0 + 4 * tc + j);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) vc[j] = vat(c0 + 4 * tc + j);
+                for (int u = 0; u < 4; ++u) vr[u] = vat(r0 + 4 * tr + u);
+                T racc[4], cacc[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) vr[u] = vat(r0 + 4 * tr + u);
-            T racc[4], cacc[4];
+                for (int u = 0; u < 4; ++u) racc[u] = N::zero();
 #pragma unroll
-            for (int u = 0; u < 4; ++u) racc[u] = N::zero();
+                for (int j = 0; j < 4; ++j) cacc[j] = N::zero();
 #pragma unroll
-            for (int j = 0; j < 4; ++j) cacc[j] = N::zero();
+                for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (diag && 4 * tc + j == 4 * tr + u) {
-                        const double dv = N::re(a[j][u]);
-                        racc[u] = N::add(racc[u], N::scale(vc[j], dv));
-                        scal -= dv * N::abs2(vr[u]);
-                    } else {
-                        racc[u] = N::add(racc[u], N::mul(a[j][u], vc[j]));
-                        cacc[j] = N::add(cacc[j], N::cmul(a[j][u], vr[u]));
+                    for (int u = 0; u < 4; ++u) {
+                        if (diag && 4 * tc + j == 4 * tr + u) {
+                            const double dv = N::re(a[b][j][u]);
+                            racc[u] = N::add(racc[u], N::scale(vc[j], dv));
+                            scal -= dv * N::abs2(vr[u]);
+                        } else {
+                            racc[u] = N::add(racc[u], N::mul(a[b][j][u], vc[j]));
+                            cacc[j] = N::add(cacc[j], N::cmul(a[b][j][u], vr[u]));
+                        }
                     }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cacc[j] = row16_sumT<CPLX>(cacc[j]);
+                if (tr == 0) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) colsum[b][4 * tc + j] = cacc[j];
                 }
-            const long long tix = (long long)I * (I + 1) / 2 + J;
-            double *tRow = tp + tix * 128 * E, *tCol = tRow + 64 * E;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) cacc[j] = row16_sumT<CPLX>(cacc[j]);
-            if (tr == 0) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) bpstT<CPLX>(tCol, 4 * tc + j, cacc[j]);
+                for (int u = 0; u < 4; ++u) rowbuf[b][tc][4 * tr + u] = racc[u];
             }
-            const int buf = nt & 1;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) rowbuf[buf][tc][4 * tr + u] = racc[u];
             __syncthreads();
+            // one wave per tile stores its row and column sums, 64 lanes each
             if (tid < 64) {
-                T sum = N::zero();
 #pragma unroll
-                for (int q2 = 0; q2 < 16; ++q2) sum = N::add(sum, rowbuf[buf][q2][tid]);
-                bpstT<CPLX>(tRow, tid, sum);
-                scal += 2.0 * N::re(N::cmul(vat(r0 + tid), sum));
+                for (int b = 0; b < kTB; ++b) {
+                    if (tI[b] < 0) continue;
+                    T sum = N::zero();
+#pragma unroll
+                    for (int q2 = 0; q2 < 16; ++q2) sum = N::add(sum, rowbuf[b][q2][tid]);
+                    const long long tix = (long long)tI[b] * (tI[b] + 1) / 2 + tJ[b];
+                    bpstT<CPLX>(tp + tix * 128 * E, tid, sum);
+                    bpstT<CPLX>(tp + tix * 128 * E + 64 * E, tid, colsum[b][tid]);
+                    scal += 2.0 * N::re(N::cmul(vat(64 * tI[b] + tid), sum));
+                }
             }
+            __syncthreads();
+            q0 += kTB * G;
+            if (q0 < NT) issue(q0);
         }
-        // the dots a_k = V(:, j0+k)^H v, b_k = W(:, k)^H v over rows >= s
-        for (int k = G - 1 - w; k < 2 * c; k += G) {
+        if (hasDot) {
             T acc = N::zero();
-            for (int r = s + tid; r < n; r += kBT) {
-                const T col = k < c ? bpldT<CPLX>(V, r + (long long)(j0 + k) * n)
-                                    : bpldT<CPLX>(W, r + (long long)(k - c) * n);
-                acc = N::add(acc, N::cmul(col, vat(r)));
-            }
+#pragma unroll
+            for (int t = 0; t < kRL; ++t) acc = N::add(acc, N::cmul(dcol[t], vat(s + tid + t * kBT)));
             acc = wg_sumT<CPLX>(acc, red);
-            if (tid == 0) bpstT<CPLX>(dotPub, k, acc);
+            if (tid == 0) bpstT<CPLX>(dotPub, kDot, acc);
+        }
+        // the first owned block's V and W rows and column i+1, written
+        // before this step, loaded while the group assembles
+        const int Iown = Ib0 + w;
+        const bool ownFirst = Iown < Tn && Iown >= Ib;
+        T pv[kPanelB / 4], pw[kPanelB / 4], pa = N::zero();
+        {
+            const int rr = tid & 63, part = tid >> 6, r = 64 * Iown + rr;
+            const bool live = ownFirst && r >= s && r < n;
+#pragma unroll
+            for (int t = 0; t < kPanelB / 4; ++t) {
+                const int k = part + 4 * t;
+                const bool on = live && k < c;
+                pv[t] = on ? bpldT<CPLX>(V, r + (long long)(j0 + k) * n) : N::zero();
+                pw[t] = on ? bpldT<CPLX>(W, r + (long long)k * n) : N::zero();
+            }
+            if (live && more) pa = N::load(A, r + (long long)(i + 1) * n);
         }
         scal = wg_sum(scal, red);
         if (tid == 0) bpst(scalPub + w, scal);
@@ -371,16 +453,29 @@ __global__ __launch_bounds__(kBT) void k_trdb_panel(TrdbArgs p)
             const int nrow = I - Ib + 1, ntot = nrow + (Tn - I);
             T y = N::zero(), corr = N::zero(), cupd = N::zero();
             if (live) {
-                for (int q = part; q < ntot; q += 4) {
-                    const long long tix = q < nrow ? (long long)I * (I + 1) / 2 + (Ib + q)
-                                                   : (long long)(I + q - nrow) * (I + q - nrow + 1) / 2 + I;
-                    y = N::add(y, bpldT<CPLX>(tp + tix * 128 * E + (q < nrow ? 0 : 64 * E), rr));
+                {
+                    T tmp[kOL];
+#pragma unroll
+                    for (int t = 0; t < kOL; ++t) {
+                        const int q = part + 4 * t;
+                        const long long tix = q < nrow ? (long long)I * (I + 1) / 2 + (Ib + q)
+                                                       : (long long)(I + q - nrow) * (I + q - nrow + 1) / 2 + I;
+                        tmp[t] = q < ntot ? bpldT<CPLX>(tp + tix * 128 * E + (q < nrow ? 0 : 64 * E), rr)
+                                          : N::zero();
+                    }
+#pragma unroll
+                    for (int t = 0; t < kOL; ++t) y = N::add(y, tmp[t]);
                 }
-                for (int k = part; k < c; k += 4) {
-                    const T vr = bpldT<CPLX>(V, r + (long long)(j0 + k) * n);
-                    const T wr = bpldT<CPLX>(W, r + (long long)k * n);
-                    corr = N::add(corr, N::add(N::mul(wr, sa[k]), N::mul(vr, sb[k])));
-                    cupd = N::add(cupd, N::add(N::mul(vr, N::conj(sWs[k])), N::mul(wr, N::conj(sVs[k]))));
+                const bool first = I == Iown;
+#pragma unroll
+                for (int t = 0; t < kPanelB / 4; ++t) {
+                    const int k = part + 4 * t;
+                    if (k < c) {
+                        const T vr = first ? pv[t] : bpldT<CPLX>(V, r + (long long)(j0 + k) * n);
+                        const T wr = first ? pw[t] : bpldT<CPLX>(W, r + (long long)k * n);
+                        corr = N::add(corr, N::add(N::mul(wr, sa[k]), N::mul(vr, sb[k])));
+                        cupd = N::add(cupd, N::add(N::mul(vr, N::conj(sWs[k])), N::mul(wr, N::conj(sVs[k]))));
+                    }
                 }
             }
             red4[(0 * 4 + part) * 64 + rr] = y;
@@ -399,9 +494,12 @@ __global__ __launch_bounds__(kBT) void k_trdb_panel(TrdbArgs p)
                 const T wr = N::add(N::mul(tauv, N::sub(ya, ca)), N::scale(vr, aw));
                 bpstT<CPLX>(W, r + (long long)c * n, wr);
                 bpstT<CPLX>(V, r + (long long)i * n, vr);
+                // [W | V | W] side by side for the trailing update's single GEMM
+                N::store(W, r + (long long)(p.pb + c) * n, vr);
+                N::store(W, r + (long long)(2 * p.pb + c) * n, wr);
                 if (more) {
                     // column i+1, with this step's own term V(r,i) conj(w(s)) + w(r) conj(V(s,i) = 1)
-                    T cn = N::load(A, r + (long long)(i + 1) * n);
+                    T cn = I == Iown ? pa : N::load(A, r + (long long)(i + 1) * n);
                     cn = N::sub(cn, N::add(ua, N::add(N::mul(vr, N::conj(ws)), wr)));
                     bpstT<CPLX>(cpub, r, cn);
                     if (r == s) dd[s] = N::re(cn);
@@ -473,6 +571,40 @@ hipError_t launch_trdb_panel(bool cplx, const TrdbPlan &pl, const double *A, dou
     const void *fn = cplx ? reinterpret_cast<const void *>(&k_trdb_panel<true>)
                           : reinterpret_cast<const void *>(&k_trdb_panel<false>);
     return hipLaunchCooperativeKernel(fn, dim3(nMat * pl.G), dim3(kBT), args, (unsigned)pl.lds, s);
+}
+
+}  // namespace olcao
+
+namespace olcao {
+
+namespace {
+
+// C = alpha sum_p P_p + beta C over an m x ncols block (column-major, leading
+// dimension ldc; P: parts of m x ncols, contiguous), real or complex (E
+// doubles per entry): the sum of a split-K GEMM's partial products.
+__global__ void k_splitk_sum(const double *P, int parts, int m, int ncols, int E, double alpha,
+                             double beta, double *C, int ldc)
+{
+    const long long per = (long long)m * ncols * E;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= per) return;
+    const long long e = t / E, comp = t % E;
+    const long long i = e % m, j = e / m;
+    double acc = 0.0;
+    for (int q = 0; q < parts; ++q) acc += P[per * q + t];
+    double *c = C + E * (i + j * (long long)ldc) + comp;
+    *c = beta == 0.0 ? alpha * acc : alpha * acc + beta * *c;
+}
+
+}  // namespace
+
+void launch_splitk_sum(const double *P, int parts, int m, int ncols, bool cplx, double alpha,
+                       double beta, double *C, int ldc, hipStream_t s)
+{
+    const int E = cplx ? 2 : 1;
+    const long long per = (long long)m * ncols * E;
+    hipLaunchKernelGGL(k_splitk_sum, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, s, P, parts, m,
+                       ncols, E, alpha, beta, C, ldc);
 }
 
 }  // namespace olcao

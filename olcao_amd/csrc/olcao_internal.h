@@ -329,10 +329,15 @@ void launch_real_to_field(const double *Z, double *X, long long count, bool cplx
 
 // Blocked reduction with the matrix in global memory (ktrdb.hip), for the
 // sizes the LDS-resident reduction does not take: panels of kPanelB columns
-// (start kernel, cooperative panel kernel), each followed by a her2k/syr2k
-// update of the trailing lower triangle by the caller.  Same outputs as
-// launch_trd; W: n x kPanelB per matrix; pub: pubLen doubles per matrix;
-// cnt: 64 words per matrix; matrices in flight: nGroups.
+// (start kernel, cooperative panel kernel), each followed by the caller's
+// update of the trailing matrix A -= [W V] [V W]^H (one GEMM, k = 2 pb).
+// Same outputs as launch_trd; W: n x 3 kPanelB per matrix, the panel's W, V
+// and W again in columns [0, pb), [pb, 2 pb), [2 pb, 3 pb); pub: pubLen
+// doubles per matrix; cnt: 64 words per matrix; matrices in flight: nGroups.
+// C = alpha sum_p P_p + beta C (P: parts of m x ncols, contiguous; C
+// column-major with leading dimension ldc): the sum of a split-K GEMM.
+void launch_splitk_sum(const double *P, int parts, int m, int ncols, bool cplx, double alpha,
+                       double beta, double *C, int ldc, hipStream_t s);
 constexpr int kPanelB = 32;
 constexpr int kTrdbMaxReal = 7168;     // = kDcBigMaxN (asserted in kdc.hip)
 constexpr int kTrdbMaxComplex = 4096;

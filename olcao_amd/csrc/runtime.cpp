@@ -135,6 +135,7 @@ struct EigWork {
     int lastPath = -1;     // path of the last gen_eig: 0 LDS solver, 1 tridiagonal, 2 library
     bool forceBlocked = false;   // the global-memory reduction also where the LDS one fits
     DBuf<double> Wp;       // blocked reduction: the panel's W (n x kPanelB per matrix)
+    DBuf<double> Psk;      // split-K GEMM partial products
     // events around the tridiagonal reduction (set by a context; the
     // reduction is timed when they exist and the solve is not captured)
     hipEvent_t trdEv[2] = {nullptr, nullptr};
@@ -1126,37 +1127,42 @@ void ensure_mats(olcao_ctx &c)
 // conquer T = Z diag(w) Z^T (kdc.hip), and Y = Q Z by blocks of reflectors,
 // X <- X - U_b (V_b^H X), with batched GEMMs.  D receives w, A receives Y,
 // dinfo the reduction's status.
-// Trailing update C -= V W^H + W V^H of the lower triangle (zher2k/dsyr2k),
-// batched with the strides given.
-void her2k_lower(rocblas_handle h, bool cplx, int m, int k, const double *Vp, rocblas_stride sv,
-                 const double *Wp, rocblas_stride sw, double *C, rocblas_stride sc, int ld, int nb)
+void gemm_one(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, bool ta, int m, int nc, int k,
+              double alpha, const double *A, int lda, const double *B, int ldb, double beta, double *C,
+              int ldc);
+
+// Trailing update C -= X Y^H (m x m, k columns; X = [W V], Y = [V W] from
+// the panel buffer), batched with the strides given: W V^H + V W^H in one
+// GEMM.  Both triangles are kept current.
+void panel_update(rocblas_handle h, bool cplx, int m, int k, const double *X, const double *Y,
+                  rocblas_stride sxy, double *C, rocblas_stride sc, int ld, int nb)
 {
     rocblas_status r;
     if (cplx) {
-        const rocblas_double_complex alpha = {-1.0, 0.0};
-        const double beta = 1.0;
-        r = rocblas_zher2k_strided_batched(h, rocblas_fill_lower, rocblas_operation_none, m, k, &alpha,
-                                           (const rocblas_double_complex *)Vp, ld, sv,
-                                           (const rocblas_double_complex *)Wp, ld, sw, &beta,
-                                           (rocblas_double_complex *)C, ld, sc, nb);
+        const rocblas_double_complex alpha = {-1.0, 0.0}, beta = {1.0, 0.0};
+        r = rocblas_zgemm_strided_batched(h, rocblas_operation_none,
+                                          rocblas_operation_conjugate_transpose, m, m, k, &alpha,
+                                          (const rocblas_double_complex *)X, ld, sxy,
+                                          (const rocblas_double_complex *)Y, ld, sxy, &beta,
+                                          (rocblas_double_complex *)C, ld, sc, nb);
     } else {
         const double alpha = -1.0, beta = 1.0;
-        r = rocblas_dsyr2k_strided_batched(h, rocblas_fill_lower, rocblas_operation_none, m, k, &alpha,
-                                           Vp, ld, sv, Wp, ld, sw, &beta, C, ld, sc, nb);
+        r = rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, m, m,
+                                          k, &alpha, X, ld, sxy, Y, ld, sxy, &beta, C, ld, sc, nb);
     }
-    if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "her2k status %d", (int)r);
+    if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "panel update gemm status %d", (int)r);
 }
 
 // The blocked reduction (ktrdb.hip) of the batch in A (lower triangle used,
 // overwritten by the trailing updates), nGroups matrices at a time: per
 // panel of kPanelB columns the start kernel, the cooperative panel kernel
-// and the update of the trailing lower triangle; then d(n-1).  Outputs as
+// and the GEMM update of the trailing matrix; then d(n-1).  Outputs as
 // launch_trd: V, tau (in ew), D, E, dinfo (-2: a group barrier timed out).
 void trdb_reduce(EigWork &ew, const TrdbPlan &pl, rocblas_handle h, hipStream_t s, bool cplx, int n,
                  int nb, double *A, double *D, double *E, int *dinfo)
 {
     const int el = cplx ? 2 : 1;
-    const size_t per = (size_t)n * n, perW = (size_t)n * kPanelB;
+    const size_t per = (size_t)n * n, perW = (size_t)n * 3 * kPanelB;
     ew.Wp.alloc(perW * el * nb);
     ew.pub.alloc(pl.pubLen * nb);
     ew.cnt.alloc((size_t)64 * nb);
@@ -1178,9 +1184,9 @@ void trdb_reduce(EigWork &ew, const TrdbPlan &pl, rocblas_handle h, hipStream_t 
                                      s));
             const int t0 = j0 + pb;
             if (t0 < n)
-                her2k_lower(h, cplx, n - t0, pb, Vm + el * ((size_t)t0 + (size_t)n * j0), (rocblas_stride)per,
-                            Wm + el * (size_t)t0, (rocblas_stride)perW,
-                            Aw + el * ((size_t)t0 + (size_t)n * t0), (rocblas_stride)per, n, nm);
+                panel_update(h, cplx, n - t0, 2 * pb, Wm + el * (size_t)t0,
+                             Wm + el * ((size_t)t0 + (size_t)n * pb), (rocblas_stride)perW,
+                             Aw + el * ((size_t)t0 + (size_t)n * t0), (rocblas_stride)per, n, nm);
         }
         HIPCHK(launch_trdb_start(cplx, pl, Am, Vm, Wm, Dm, Em, taum, pubm, cntm, im, n, nm, n - 1, s));
     }
@@ -1243,6 +1249,14 @@ void trd_eig(EigWork &ew, const TrdPlan *plan, const TrdbPlan *bplan, rocblas_ha
         const int mb = std::min(kTrdBlock, nref - j0), m = n - 1 - j0;
         const size_t offV = (size_t)(j0 + 1) + (size_t)j0 * n, offX = (size_t)(j0 + 1);
         rocblas_status r;
+        if (nb == 1) {
+            // W = V_b^H X (split-K over the block's rows), X -= U_b W
+            gemm_one(ew, h, s, cplx, true, mb, n, m, 1.0, ew.V.p + (size_t)el * offV, n,
+                     A + (size_t)el * offX, n, 0.0, ew.W.p, kTrdBlock);
+            gemm_one(ew, h, s, cplx, false, m, n, mb, -1.0, ew.U.p + (size_t)el * offV, n, ew.W.p,
+                     kTrdBlock, 1.0, A + (size_t)el * offX, n);
+            continue;
+        }
         if (cplx) {
             const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0}, mone = {-1.0, 0.0};
             auto *Vz = (const rocblas_double_complex *)ew.V.p, *Uz = (const rocblas_double_complex *)ew.U.p;
@@ -1290,22 +1304,78 @@ void gemm_nn(rocblas_handle h, bool cplx, bool ta, bool tb, int m, int nc, int k
     if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "gemm status %d", (int)r);
 }
 
+// C = alpha op(A) op(B) + beta C for one matrix (op(B) = B), with explicit
+// leading dimensions.  A product with a short side (at most 64 rows or
+// columns of C) and a long inner dimension runs split-K: chunks of 512 of the
+// inner dimension as one strided batch of GEMMs into ew.Psk, then summed
+// into C (launch_splitk_sum), so that the chunks fill the device.
+void gemm_one(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, bool ta, int m, int nc, int k,
+              double alpha, const double *A, int lda, const double *B, int ldb, double beta, double *C,
+              int ldc)
+{
+    const int el = cplx ? 2 : 1;
+    constexpr int kc = 512;
+    auto call = [&](int kk, const double *Ap, rocblas_stride sa, const double *Bp, rocblas_stride sb,
+                    double al, double be, double *Cp, int ldcp, rocblas_stride scp, int batch) {
+        rocblas_status r;
+        if (cplx) {
+            const rocblas_double_complex a = {al, 0.0}, b = {be, 0.0};
+            r = rocblas_zgemm_strided_batched(
+                h, ta ? rocblas_operation_conjugate_transpose : rocblas_operation_none,
+                rocblas_operation_none, m, nc, kk, &a, (const rocblas_double_complex *)Ap, lda, sa,
+                (const rocblas_double_complex *)Bp, ldb, sb, &b, (rocblas_double_complex *)Cp, ldcp, scp,
+                batch);
+        } else {
+            r = rocblas_dgemm_strided_batched(h, ta ? rocblas_operation_transpose : rocblas_operation_none,
+                                              rocblas_operation_none, m, nc, kk, &al, Ap, lda, sa, Bp, ldb,
+                                              sb, &be, Cp, ldcp, scp, batch);
+        }
+        if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "gemm status %d", (int)r);
+    };
+    if (k < 4 * kc || (m > 64 && nc > 64)) {
+        call(k, A, 0, B, 0, alpha, beta, C, ldc, 0, 1);
+        return;
+    }
+    const int full = k / kc, rem = k % kc, parts = full + (rem > 0);
+    const size_t part = (size_t)m * nc * el;
+    ew.Psk.alloc(part * parts);
+    // chunk c of the inner dimension: rows c kc.. of the stored A (op = ^H)
+    // or its columns c kc.. (op = N); rows c kc.. of B
+    const rocblas_stride sa = ta ? (rocblas_stride)kc : (rocblas_stride)kc * lda;
+    call(kc, A, sa, B, (rocblas_stride)kc, 1.0, 0.0, ew.Psk.p, m, (rocblas_stride)(m * nc), full);
+    if (rem > 0)
+        call(rem, A + el * (size_t)sa * full, 0, B + el * (size_t)kc * full, 0, 1.0, 0.0,
+             ew.Psk.p + part * full, m, 0, 1);
+    launch_splitk_sum(ew.Psk.p, parts, m, nc, cplx, alpha, beta, C, ldc, s);
+    HIPCHK(hipGetLastError());
+}
+
+// gemm_nn, or gemm_one for a single matrix
+void gemm_big(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, bool ta, int m, int nc, int k,
+              double alpha, const double *A, const double *B, double beta, double *C, int n, int nb)
+{
+    if (nb == 1)
+        gemm_one(ew, h, s, cplx, ta, m, nc, k, alpha, A, n, B, n, beta, C, n);
+    else
+        gemm_nn(h, cplx, ta, false, m, nc, k, alpha, A, B, beta, C, n, nb);
+}
+
 // S = U^H U for the batch in B (upper triangle read; B is overwritten), the
 // factor into U (upper triangle; the rest is not touched), left-looking by
 // 64-wide block rows: a GEMM applies the finished rows to block row J, the
 // diagonal kernel factors and inverts the diagonal block (Dinv), a GEMM
 // forms the row panel U(J, >J) = Dinv_J^H S(J, >J).  dinfo: first failing
 // column (1-based).
-void chol_blocked(rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *B, double *U,
-                  double *Dinv, int *dinfo)
+void chol_blocked(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *B,
+                  double *U, double *Dinv, int *dinfo)
 {
     const int el = cplx ? 2 : 1;
     launch_zero_words(dinfo, nb, s);
     for (int j0 = 0; j0 < n; j0 += kCholBlock) {
         const int jb = std::min(kCholBlock, n - j0), rest = n - j0 - jb;
         if (j0 > 0)
-            gemm_nn(h, cplx, true, false, jb, n - j0, j0, -1.0, U + el * (size_t)n * j0,
-                    U + el * (size_t)n * j0, 1.0, B + el * ((size_t)j0 + (size_t)n * j0), n, nb);
+            gemm_big(ew, h, s, cplx, true, jb, n - j0, j0, -1.0, U + el * (size_t)n * j0,
+                     U + el * (size_t)n * j0, 1.0, B + el * ((size_t)j0 + (size_t)n * j0), n, nb);
         launch_chol_diag(cplx, B, U, Dinv, n, nb, j0, jb, dinfo, s);
         HIPCHK(hipGetLastError());
         if (rest > 0)
@@ -1322,42 +1392,43 @@ void chol_blocked(rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, dou
 //   left:  Y = U^-H A      Y_J = Dinv_J^H (A_J - U_{<J,J}^H Y_{<J})  (A destroyed)
 //   right: C = Y U^-1      C_J = (Y_J - C_{<J} U_{<J,J}) Dinv_J      (Y destroyed)
 //   back:  X = U^-1 Z      X_J = Dinv_J (Z_J - U_{J,>J} X_{>J})       (Z destroyed)
-void solve_left_h(rocblas_handle h, bool cplx, int n, int nb, const double *U, const double *Dinv,
-                  double *A, double *Y)
+void solve_left_h(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int nb,
+                  const double *U, const double *Dinv, double *A, double *Y)
 {
     const int el = cplx ? 2 : 1;
     for (int j0 = 0; j0 < n; j0 += kCholBlock) {
         const int jb = std::min(kCholBlock, n - j0);
         if (j0 > 0)
-            gemm_nn(h, cplx, true, false, jb, n, j0, -1.0, U + el * (size_t)n * j0, Y, 1.0,
-                    A + el * (size_t)j0, n, nb);
+            gemm_big(ew, h, s, cplx, true, jb, n, j0, -1.0, U + el * (size_t)n * j0, Y, 1.0,
+                     A + el * (size_t)j0, n, nb);
         gemm_nn(h, cplx, true, false, jb, n, jb, 1.0, Dinv + el * (size_t)j0, A + el * (size_t)j0,
                 0.0, Y + el * (size_t)j0, n, nb);
     }
 }
-void solve_right(rocblas_handle h, bool cplx, int n, int nb, const double *U, const double *Dinv,
-                 double *Y, double *C)
+void solve_right(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int nb,
+                 const double *U, const double *Dinv, double *Y, double *C)
 {
     const int el = cplx ? 2 : 1;
     for (int j0 = 0; j0 < n; j0 += kCholBlock) {
         const int jb = std::min(kCholBlock, n - j0);
         if (j0 > 0)
-            gemm_nn(h, cplx, false, false, n, jb, j0, -1.0, C, U + el * (size_t)n * j0, 1.0,
-                    Y + el * (size_t)n * j0, n, nb);
+            gemm_big(ew, h, s, cplx, false, n, jb, j0, -1.0, C, U + el * (size_t)n * j0, 1.0,
+                     Y + el * (size_t)n * j0, n, nb);
         gemm_nn(h, cplx, false, false, n, jb, jb, 1.0, Y + el * (size_t)n * j0, Dinv + el * (size_t)j0,
                 0.0, C + el * (size_t)n * j0, n, nb);
     }
 }
-void solve_back(rocblas_handle h, bool cplx, int n, int nb, const double *U, const double *Dinv,
-                double *Z, double *X)
+void solve_back(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int nb,
+                const double *U, const double *Dinv, double *Z, double *X)
 {
     const int el = cplx ? 2 : 1;
     const int last = ((n - 1) / kCholBlock) * kCholBlock;
     for (int j0 = last; j0 >= 0; j0 -= kCholBlock) {
         const int jb = std::min(kCholBlock, n - j0), rest = n - j0 - jb;
         if (rest > 0)
-            gemm_nn(h, cplx, false, false, jb, n, rest, -1.0, U + el * ((size_t)j0 + (size_t)n * (j0 + jb)),
-                    X + el * (size_t)(j0 + jb), 1.0, Z + el * (size_t)j0, n, nb);
+            gemm_big(ew, h, s, cplx, false, jb, n, rest, -1.0,
+                     U + el * ((size_t)j0 + (size_t)n * (j0 + jb)), X + el * (size_t)(j0 + jb), 1.0,
+                     Z + el * (size_t)j0, n, nb);
         gemm_nn(h, cplx, false, false, jb, n, jb, 1.0, Dinv + el * (size_t)j0, Z + el * (size_t)j0, 0.0,
                 X + el * (size_t)j0, n, nb);
     }
@@ -1426,7 +1497,7 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
         ew.Dinv.alloc((size_t)n * n * el * nb);
         ew.Y.alloc((size_t)n * n * el * nb);
         ew.Uc.alloc((size_t)n * n * el * nb);
-        chol_blocked(h, s, cplx, n, nb, B, ew.Uc.p, ew.Dinv.p, dinfo);
+        chol_blocked(ew, h, s, cplx, n, nb, B, ew.Uc.p, ew.Dinv.p, dinfo);
         if (!defer) {
             // stop before the reduction when some S(k) is not positive
             // definite, as zhegv does
@@ -1439,11 +1510,11 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
                 }
         }
         launch_herm_fill(A, n, nb, el, s);
-        solve_left_h(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
-        solve_right(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, ew.Y.p, A);
+        solve_left_h(ew, h, s, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
+        solve_right(ew, h, s, cplx, n, nb, ew.Uc.p, ew.Dinv.p, ew.Y.p, A);
         trd_eig(ew, ldsTrd ? &plan : nullptr, ldsTrd ? nullptr : &bplan, h, s, cplx, n, nb, A, D, E,
                 tinfo, defer);
-        solve_back(h, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
+        solve_back(ew, h, s, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
         HIPCHK(hipMemcpyAsync(A, ew.Y.p, (size_t)n * n * el * nb * sizeof(double),
                               hipMemcpyDeviceToDevice, s));
         if (!defer) read_info(ew, dinfo, tinfo, n, nb, s, info);
