@@ -83,6 +83,14 @@ struct DBuf {
         o.n = 0;
     }
     DBuf &operator=(DBuf &&) = delete;
+    // at least count elements, kept when already as large (workspaces whose
+    // size varies from call to call: a captured sequence allocates nothing
+    // once an eager run has reached the largest)
+    void reserve(size_t count)
+    {
+        if (p && count <= n) return;
+        alloc(count);
+    }
     void alloc(size_t count)
     {
         if (count == n && p) return;
@@ -1338,7 +1346,7 @@ void gemm_one(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, bool ta, 
     }
     const int full = k / kc, rem = k % kc, parts = full + (rem > 0);
     const size_t part = (size_t)m * nc * el;
-    ew.Psk.alloc(part * parts);
+    ew.Psk.reserve(part * parts);
     // chunk c of the inner dimension: rows c kc.. of the stored A (op = ^H)
     // or its columns c kc.. (op = N); rows c kc.. of B
     const rocblas_stride sa = ta ? (rocblas_stride)kc : (rocblas_stride)kc * lda;

@@ -992,10 +992,12 @@ def test_bloch_kernels_agree(gpu, case):
                 assert _ortho_close(x, y, tol), (case, name, e)
 
 
-@pytest.mark.parametrize("make", ["diamond_c8", "quartz", "core_spin2", "si64"])
+@pytest.mark.parametrize("make", ["diamond_c8", "quartz", "core_spin2", "si64", "asio2_1000"])
 def test_graph_replay_matches_eager(gpu, make):
     """Graph mode: the first iteration runs eagerly and captures the
-    iteration's device work; each later one replays it.  Three replays with
+    iteration's device work; each later one replays it (asio2_1000: the
+    blocked reduction, the large divide-and-conquer merges and the split-K
+    GEMMs of one n = 5665 matrix inside the graph).  Three replays with
     other eager work and synchronisations between them, the last two with
     new coefficients, each against an eager context's iteration on the same
     coefficients (bit for bit: the same kernels in the same order); then a
