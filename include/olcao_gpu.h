@@ -208,6 +208,20 @@ olcao_status olcao_gauss_overlap_dm(olcao_ctx *ctx, const double *center, double
  * (opCode 7) and laid out as olcao_gauss_overlap_dm. */
 olcao_status olcao_gauss_overlap_mm(olcao_ctx *ctx, double *vv, double *cv, double *cc);
 
+/* gaussKOverlap (integrals3Terms.F90:956-1384): the overlap with
+ * exp(-i deltaK . r), deltaK = recipVectors(:, axis) / numAxialKPoints(axis),
+ * for the three axes, orthogonalised against the core as ortho3Terms opcode
+ * 8 does (full matrix, no zeroing of small entries; integrals3Terms.F90
+ * :1388-1608).  numAxial: the k-mesh divisions per axis.  vv [3][nk][n][n],
+ * cv [3][nk][nc][n], cc [3][nk][nc][nc] complex, column-major per k (full
+ * matrices: the pair blocks i <= j as saveCurrentPair places them, then the
+ * orthogonalisation over the whole matrix).  The reference's
+ * KOverlapPlusG group holds the same integrals (its plusG argument selects
+ * only the output group).  Complex contexts only: a Gamma context returns
+ * OLCAO_ERR_UNSUPPORTED. */
+olcao_status olcao_gauss_k_overlap(olcao_ctx *ctx, const int numAxial[3], double *vv, double *cv,
+                                   double *cc);
+
 /* Fused Hamiltonian sweep with the potential coefficients folded in
  * (gaussOverlapHamPSCF, integrals.F90:2860): KE (- MV when rel = 1)
  * + sum_j c_j V_j - sum Z N, orthogonalised when coreDim > 0
@@ -356,7 +370,8 @@ olcao_status olcao_solve_zhegv_noalloc(int n, int nbatch, double *A, double *B, 
  * device functions against the reference subroutines overlap2CIntg
  * (gaussIntegrals.f90:26, op 0), kinetic2CIntg (:4422, op 1),
  * electron3CIntg (:11624, op 2), nuclear3CIntg (:16223, op 3),
- * massvel2CIntg (:39070, op 4).
+ * massvel2CIntg (:39070, op 4), and the real (op 7) and imaginary (op 8)
+ * parts of KOverlap2CIntg (:129408) with deltaK = C.
  * a: [n][3] exponents, A/B/C: [n][3], sw: [n] l1l2switch, sh: [n][16][16]
  * column-major blocks (sh(i,j) at i + 16 j); entries outside the class block
  * are zero. */

@@ -20,9 +20,10 @@
  * Datasets are chunked (at most 250M elements per chunk, hdf5SCFIntg.F90:
  * 144-175) and deflated at level 1.  Every group that holds a term gets the
  * integer attribute "status" = 1 once written, as the reference records a
- * finished calculation (integrals.F90:3728-3731).  Groups of operators
- * outside this path (atomDMOverlap, atomMMOverlap, atomKOverlap...) are not
- * written. */
+ * finished calculation (integrals.F90:3728-3731).  The operator groups
+ * (atomDMOverlap, atomMMOverlap, atomKOverlap, atomKOverlapPlusG) are
+ * created with status 0 and filled by olcao_write_scf_vector_terms_h5 and
+ * olcao_write_k_overlap_h5. */
 #ifndef OLCAO_H5_H
 #define OLCAO_H5_H
 
@@ -90,6 +91,25 @@ olcao_status olcao_write_scf_eigen_h5(olcao_ctx *ctx, int numStates, const char 
  * (hdf5PSCFIntg.F90:338-344). */
 olcao_status olcao_write_pscf_h5(olcao_ctx *ctx, const double *potCoeffs, int numStates,
                                  const char *kPointName, const char *path);
+
+/* Fill the k-overlap groups of a file written by olcao_write_scf_integrals_h5
+ * (kPointName NULL or empty) or of the k-point group kPointName of a file
+ * written by olcao_write_pscf_h5, as gaussKOverlap with ortho3Terms leaves
+ * them when setupSCF or intgPSCF run with doMTOP = 1 (OLCAO.F90:302-310,
+ * :689-698; integrals3Terms.F90:956-1384, :1547-1566):
+ *
+ *   .../atomIntgGroup/atomKOverlap/<x>/real<k>, imag<k>       (valeDim, valeDim)
+ *   .../atomIntgGroup/atomKOverlapPlusG/<x>/real<k>, imag<k>  the same values
+ *
+ * x = 0000001..0000003 is the axis of deltaK = recipVectors(:, x) /
+ * numAxial[x]; each matrix is the full orthogonalised one of
+ * olcao_gauss_k_overlap; both groups' status is set to 1.  The reference's
+ * call of KOverlap2CIntg leaves out its plusG argument (the term is commented
+ * out, integrals3Terms.F90:1286-1296), so the PlusG group repeats the
+ * k-overlap.  OLCAO_ERR_UNSUPPORTED for a Gamma context (the Gamma build has
+ * no k-overlap), OLCAO_ERR_INPUT when the file or group is not there. */
+olcao_status olcao_write_k_overlap_h5(olcao_ctx *ctx, const int numAxial[3], const char *kPointName,
+                                      const char *path);
 
 #ifdef __cplusplus
 }

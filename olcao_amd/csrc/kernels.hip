@@ -47,6 +47,8 @@ __global__ __launch_bounds__(64) void k_two_center(DevCtx c, const int *list, in
     double pc[ncart(LA)][ncart(LB)];
     if (op == OP_KE)
         two_center_cart<LA, LB, true>(q, pc);
+    else if (op >= OP_KO)
+        kov_cart<LA, LB>(q, c.kq, op - OP_KO, pc);
     else if (op == OP_MV)
         massvel_cart<LA, LB>(q, pc);
     else if (op >= OP_MM)
@@ -326,6 +328,13 @@ __global__ void k_fill(double *p, long long n, double v)
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
          t += (long long)gridDim.x * blockDim.x)
         p[t] = v;
+}
+
+__global__ void k_add(double *y, const double *x, long long n)
+{
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x)
+        y[t] += x[t];
 }
 
 // H = NP + KE (+ MV) + sum_j c_j V_j on the upper triangle of every k
@@ -656,6 +665,14 @@ void launch_fill(double *p, long long n, double v, hipStream_t s)
     long long blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
+void launch_add(double *y, const double *x, long long n, hipStream_t s)
+{
+    if (n <= 0) return;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_add, dim3((unsigned)blocks), dim3(256), 0, s, y, x, n);
 }
 
 void launch_assemble(long long total, int n, const double *NP, const double *KE,

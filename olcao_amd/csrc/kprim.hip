@@ -18,6 +18,14 @@ __device__ void prim_one(int op, const double *a, const double *A, const double 
         two_center_cart<LA, LB, true>(q, pc);
     } else if (op == 4) {
         massvel_cart<LA, LB>(q, pc);
+    } else if (op == 7 || op == 8) {
+        // the k-overlap block for kq = C: real part (7), imaginary part (8)
+        kov_cart<LA, LB>(q, C, op - 7, pc);
+        if (op == 8)
+#pragma unroll
+            for (int x = 0; x < ncart(LA); ++x)
+#pragma unroll
+                for (int y = 0; y < ncart(LB); ++y) pc[x][y] = -pc[x][y];
     } else {
         constexpr int L = LA + LB;
         double W[nherm(L)];

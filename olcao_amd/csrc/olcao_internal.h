@@ -19,7 +19,9 @@ enum ThreeKind { K3_HAM = 0, K3_EP = 1, K3_NP = 2 };
 // Two-centre operators.
 // OP_DM + c, OP_MM + c: Cartesian component c of the dipole about
 // DevCtx::dipC and of the gradient (the momentum matrix's blocks).
-enum TwoOp { OP_OL = 0, OP_KE = 1, OP_MV = 2, OP_DM = 3, OP_MM = 6 };
+// OP_KO + p: the k-overlap block for DevCtx::kq, its real part (p = 0) or
+// minus its imaginary part (p = 1).
+enum TwoOp { OP_OL = 0, OP_KE = 1, OP_MV = 2, OP_DM = 3, OP_MM = 6, OP_KO = 9 };
 
 // Device view of the system and the screened work lists.  Passed by value
 // to every kernel; all pointers are device memory.
@@ -78,6 +80,7 @@ struct DevCtx {
     unsigned long long *counters;
     int anyNuc;                 // some potential type has Z >= smallThresh
     double dipC[3];             // dipole centre of the OP_DM sweeps
+    double kq[3];               // deltaK of the OP_KO sweeps
 };
 
 inline __host__ __device__ int pp_a1(int ab) { return ab & 255; }
@@ -274,6 +277,7 @@ void launch_bloch_mfma(const DevCtx &c, const double *X, const unsigned short *p
 void phase_slices_host(const std::vector<double> &phase, int nCells, int nk, int gamma,
                        std::vector<unsigned short> &out);
 void launch_fill(double *p, long long n, double v, hipStream_t s);
+void launch_add(double *y, const double *x, long long n, hipStream_t s);   // y += x
 // Zeroes `words` 4-byte words by a kernel: clears counters and status
 // arrays in sequences that may be captured in a graph, where memset nodes
 // are not used.

@@ -709,6 +709,74 @@ OHD void herm_e(double XPA, double XPB, double h, double (&E)[LA + 1][LB + 1][LA
         }
 }
 
+// k-overlap block (KOverlap2CIntg, gaussIntegrals.f90:129408), the block of
+// exp(-i kq . r) between the Gaussians, in McMurchie-Davidson form: with the
+// pair expanded in Hermite Gaussians about P, each axis integrates to
+//   sum_t E^{ij}_t (d/dP)^t [sqrt(pi/zeta) exp(-q^2/(4 zeta)) exp(-i q P)]
+//   = sqrt(pi/zeta) exp(-q^2/(4 zeta)) exp(-i q P) sum_t E^{ij}_t (-i q)^t,
+// so the block is (pi/zeta)^(3/2) exp(-xi d^2 - |q|^2/(4 zeta)) exp(-i q.P)
+// times the product of the three complex axis sums.  pc receives the real
+// part (part 0) or minus the imaginary part (part 1).
+template <int LA, int LB>
+OHD void kov_cart(const PairConst &q, const double *kq, int part,
+                  double (&pc)[ncart(LA)][ncart(LB)])
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB), L = LA + LB;
+    const double h = 1.0 / (2.0 * q.zeta);
+    double Gr[3][LA + 1][LB + 1], Gi[3][LA + 1][LB + 1];
+    double q2 = 0.0, qP = 0.0;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        double E[LA + 1][LB + 1][L + 1];
+        herm_e<LA, LB>(q.P[x] - q.A[x], q.P[x] - q.B[x], h, E);
+        const double qx = kq[x];
+        q2 += qx * qx;
+        qP += qx * q.P[x];
+        // (-i q)^t: real parts t = 0, 2, 4, 6 (1, -q^2, q^4, -q^6), imaginary
+        // parts t = 1, 3, 5 (-q, q^3, -q^5)
+        double pr[L + 1], pi[L + 1];
+        double pw = 1.0;
+#pragma unroll
+        for (int t = 0; t <= L; ++t) {
+            const int ph = t & 3;
+            pr[t] = ph == 0 ? pw : ph == 2 ? -pw : 0.0;
+            pi[t] = ph == 1 ? -pw : ph == 3 ? pw : 0.0;
+            pw *= qx;
+        }
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+            for (int j = 0; j <= LB; ++j) {
+                double sr = 0.0, si = 0.0;
+#pragma unroll
+                for (int t = 0; t <= i + j; ++t) {
+                    sr += E[i][j][t] * pr[t];
+                    si += E[i][j][t] * pi[t];
+                }
+                Gr[x][i][j] = sr;
+                Gi[x][i][j] = si;
+            }
+    }
+    const double t = kPi / q.zeta;
+    const double amp = t * sqrt(t) * exp(-q.xid2 - q2 * 0.25 / q.zeta);
+    const double fr = amp * cos(qP), fi = -amp * sin(qP);
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            double vr = fr, vi = fi;
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                const int i = cexp(a, x), j = cexp(b, x);
+                const double ur = Gr[x][i][j], ui = Gi[x][i][j];
+                const double nr = vr * ur - vi * ui, ni = vr * ui + vi * ur;
+                vr = nr;
+                vi = ni;
+            }
+            pc[a][b] = part == 0 ? vr : -vi;
+        }
+}
+
 // Cartesian component (a, b) of K_AB sum_tuv E E E W.
 template <int LA, int LB>
 OHD double herm_contract(const double (&E)[3][LA + 1][LB + 1][LA + LB + 1], const double *W,

@@ -254,6 +254,7 @@ struct olcao_ctx {
     // Task3c arrays of all cells), held by the batch's first cell
     DBuf<unsigned char> dBatch;
     DBuf<double> H[2], CV_H[2], CC_H[2];   // per spin
+    DBuf<double> Kvv, Kcv, Kcc;            // the k-overlap's imaginary-part sweep
     DBuf<unsigned long long> dCounters;
     // Mode A term matrices (orthogonalised when coreDim > 0)
     DBuf<double> KEm, NPm, MVm;
@@ -970,7 +971,7 @@ void three_center(olcao_ctx &c, int kind, int tt, int ti, const double *coef, in
 
 // ---------------------------------------------------------------------------
 // One integral sweep into (vv, cv, cc) device matrices.
-enum Sweep { SW_OL, SW_KE, SW_NP, SW_EP, SW_HAM, SW_MV, SW_DM, SW_MM };
+enum Sweep { SW_OL, SW_KE, SW_NP, SW_EP, SW_HAM, SW_MV, SW_DM, SW_MM, SW_KO };
 
 // Primitive-pair blocks of one sweep.  SW_MV stores the negated
 // mass-velocity blocks, the term as it enters H (gaussOverlapMV accumulates
@@ -988,6 +989,7 @@ void sweep_blocks(olcao_ctx &c, Sweep kind, int term, int spinIdx)
     case SW_MV: two_center_forked(c, OP_MV, 0); break;
     case SW_DM: two_center_forked(c, OP_DM + term, 0); break;   // centre in c.dev.dipC
     case SW_MM: two_center_forked(c, OP_MM + term, 0); break;
+    case SW_KO: two_center_forked(c, OP_KO + term, 0); break;   // deltaK in c.dev.kq
     case SW_NP: three_center(c, K3_NP, 0, 0, nullptr, 0); break;
     case SW_EP: {
         int t = 0;
@@ -1030,7 +1032,8 @@ void contract_scatter(olcao_ctx &c, double *vv, double *cv, double *cc, int negI
 void sweep(olcao_ctx &c, Sweep kind, int term, int spinIdx, double *vv, double *cv, double *cc)
 {
     sweep_blocks(c, kind, term, spinIdx);
-    contract_scatter(c, vv, cv, cc, kind == SW_MM ? 1 : 0);
+    // the momentum matrix, and the k-overlap's imaginary part, take -i
+    contract_scatter(c, vv, cv, cc, kind == SW_MM || (kind == SW_KO && term == 1) ? 1 : 0);
 }
 
 // Orthogonalisation of valence against core (intgOrtho.F90, integrals.F90
@@ -1062,7 +1065,8 @@ void gemm_b(olcao_ctx &c, bool conjA, int m, int n, int k, double alpha, const d
     if (r != rocblas_status_success) fail(OLCAO_ERR_SOLVER, "gemm status %d", (int)r);
 }
 
-void ortho(olcao_ctx &c, double *Ovv, const double *Ocv, const double *Occ, bool overlap)
+void ortho(olcao_ctx &c, double *Ovv, const double *Ocv, const double *Occ, bool overlap,
+           bool zeroSmall = true)
 {
     if (!c.coreDim) return;
     const int n = c.valeDim, nc = c.coreDim;
@@ -1078,8 +1082,10 @@ void ortho(olcao_ctx &c, double *Ovv, const double *Ocv, const double *Occ, bool
     // T = O_cc S_cv, then O_vv += S_cv^H T
     gemm_b(c, false, nc, n, nc, 1.0, Occ, nc, scc, Scv, nc, scv, 0.0, c.Tmp.p, nc, scv);
     gemm_b(c, true, n, n, nc, 1.0, Scv, nc, scv, c.Tmp.p, nc, scv, 1.0, Ovv, n, svv);
-    launch_ortho_finish(Ovv, n, c.gamma ? 1 : c.nk, c.gamma ? 1 : 2, c.stream);
-    HIPCHK(hipGetLastError());
+    if (zeroSmall) {
+        launch_ortho_finish(Ovv, n, c.gamma ? 1 : c.nk, c.gamma ? 1 : 2, c.stream);
+        HIPCHK(hipGetLastError());
+    }
 }
 
 // The overlap sweep with S_cv and S_cc kept for the later orthogonalisations.
@@ -1868,6 +1874,52 @@ olcao_status olcao_gauss_overlap_mm(olcao_ctx *c, double *vv, double *cv, double
 {
     if (!c) return OLCAO_ERR_INPUT;
     return guard(c, [&] { vector_sweeps(*c, SW_MM, nullptr, vv, cv, cc); });
+}
+
+// gaussKOverlap (integrals3Terms.F90:956-1384) for the three deltaK =
+// recipVectors(:, axis) / numAxialKPoints(axis): per axis the sweep of the
+// blocks' real parts plus the sweep of minus their imaginary parts under the
+// Bloch sum's -i factor (the complex block's Bloch sum, scatter and core
+// blocks), then ortho3Terms' full-matrix orthogonalisation without zeroing.
+olcao_status olcao_gauss_k_overlap(olcao_ctx *c, const int numAxial[3], double *vv, double *cv,
+                                   double *cc)
+{
+    return guard(c, [&] {
+        if (!numAxial || numAxial[0] < 1 || numAxial[1] < 1 || numAxial[2] < 1)
+            fail(OLCAO_ERR_INPUT, "numAxialKPoints must be positive");
+        if (c->gamma)
+            fail(OLCAO_ERR_UNSUPPORTED, "the k-overlap is complex; a Gamma context has no such sweep");
+        ensure_mats(*c);
+        if (c->coreDim && !c->haveS) overlap_sweep(*c);
+        c->Kvv.alloc(c->matDoubles());
+        if (c->coreDim) {
+            c->Kcv.alloc(c->cvDoubles());
+            c->Kcc.alloc(c->ccDoubles());
+        }
+        // b_axis = 2 pi (a_j x a_k) / V, the rows of the inverse lattice matrix times 2 pi
+        for (int axis = 0; axis < 3; ++axis) {
+            for (int x = 0; x < 3; ++x)
+                c->dev.kq[x] = 2.0 * kPiHost * c->dev.inv[3 * axis + x] / numAxial[axis];
+            sweep(*c, SW_KO, 0, 0, c->H[0].p, c->CV_H[0].p, c->CC_H[0].p);
+            sweep(*c, SW_KO, 1, 0, c->Kvv.p, c->Kcv.p, c->Kcc.p);
+            launch_add(c->H[0].p, c->Kvv.p, (long long)c->matDoubles(), c->stream);
+            if (c->coreDim) {
+                launch_add(c->CV_H[0].p, c->Kcv.p, (long long)c->cvDoubles(), c->stream);
+                launch_add(c->CC_H[0].p, c->Kcc.p, (long long)c->ccDoubles(), c->stream);
+            }
+            HIPCHK(hipGetLastError());
+            ortho(*c, c->H[0].p, c->CV_H[0].p, c->CC_H[0].p, false, false);
+            copy_out(*c, c->H[0].p, vv ? vv + (size_t)axis * c->matDoubles() : nullptr, c->matDoubles());
+            if (c->coreDim) {
+                copy_out(*c, c->CV_H[0].p, cv ? cv + (size_t)axis * c->cvDoubles() : nullptr,
+                         c->cvDoubles());
+                copy_out(*c, c->CC_H[0].p, cc ? cc + (size_t)axis * c->ccDoubles() : nullptr,
+                         c->ccDoubles());
+            }
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        c->haveH = false;
+    });
 }
 
 olcao_status olcao_elec_pot_gauss_overlap(olcao_ctx *c, int term, double *vv, double *cv, double *cc)
@@ -2815,7 +2867,7 @@ olcao_status olcao_prim_batch(int op, int n, const double *a, const double *A, c
                               const double *C, const int *sw, double *sh)
 {
     return guard(nullptr, [&] {
-        if (op < 0 || op > 4 || n < 0) fail(OLCAO_ERR_INPUT, "bad op or n");
+        if (op < 0 || (op > 4 && op != 7 && op != 8) || n < 0) fail(OLCAO_ERR_INPUT, "bad op or n");
         if (n == 0) return;
         for (int i = 0; i < n; ++i) {
             int lo = sw[i] & 15, hi = (sw[i] >> 4) & 15;

@@ -81,6 +81,7 @@ FUNCTIONS = {
     "olcao_elec_pot_gauss_overlap": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_gauss_overlap_dm": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_gauss_overlap_mm": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "olcao_gauss_k_overlap": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_gauss_overlap_ham_pscf": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_secular_eqn_pscf": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
     "olcao_setup_scf": [C.c_void_p],
@@ -336,6 +337,17 @@ class OlcaoGPU:
         (gaussOverlapMM, integrals3Terms.F90:511)."""
         vv, cv, cc = self._bufs(3)
         _check(lib().olcao_gauss_overlap_mm(self.ctx, _p(vv), _p(cv), _p(cc)), self.ctx)
+        return self._shape(vv, cv, cc, 3)
+
+    def gaussKOverlap(self, num_axial):
+        """k-overlap matrices <phi_i(k)| exp(-i dK . r) |phi_j(k)>, dK =
+        b_axis / num_axial[axis]: three (vv, cv, cc) tuples, axis-major
+        (gaussKOverlap, integrals3Terms.F90:956; full matrices, orthogonalised
+        without zeroing as ortho3Terms opcode 8 does).  Complex contexts
+        only."""
+        vv, cv, cc = self._bufs(3)
+        na = np.ascontiguousarray(num_axial, np.int32).reshape(3)
+        _check(lib().olcao_gauss_k_overlap(self.ctx, _p(na), _p(vv), _p(cv), _p(cc)), self.ctx)
         return self._shape(vv, cv, cc, 3)
 
     def gaussOverlapHamPSCF(self, pot_coeffs):

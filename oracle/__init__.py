@@ -33,7 +33,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libolcao_ref_prims.so")
 
-OL, KE, NP, EP, HAM, MV, DM, MM = 0, 1, 2, 3, 4, 5, 6, 7
+OL, KE, NP, EP, HAM, MV, DM, MM, KO = 0, 1, 2, 3, 4, 5, 6, 7, 8
 SMALL_THRESH = 1.0e-8
 
 _lib = None
@@ -57,6 +57,7 @@ def lib():
         _lib.orc_last_counts.argtypes = [C.POINTER(C.c_longlong)] * 3
         _lib.orc_set_pair_sample.argtypes = [C.c_int, C.c_int]
         _lib.orc_set_three_center_range_scale.argtypes = [C.c_double]
+        _lib.orc_primk_batch.argtypes = [C.c_int, dp, dp, dp, dp, C.POINTER(C.c_int), dp, dp]
     return _lib
 
 
@@ -89,6 +90,28 @@ def prim_batch(op, a, A, B, C_, sw):
     out = np.zeros((n, 16, 16))
     lib().orc_prim_batch(op, n, _dp(a), _dp(A), _dp(B), _dp(C_), _ip(sw), _dp(out))
     return out.transpose(0, 2, 1)
+
+
+def primk_batch(a, A, B, dK, sw):
+    """k-overlap blocks (KOverlap2CIntg, gaussIntegrals.f90:129408): complex
+    [n, 16, 16] ([i, row, col]) of exp(-i dK . r) between the Gaussians."""
+    a = np.ascontiguousarray(a, np.float64).reshape(-1, 2)
+    n = a.shape[0]
+    A = np.ascontiguousarray(A, np.float64).reshape(n, 3)
+    B = np.ascontiguousarray(B, np.float64).reshape(n, 3)
+    dK = np.ascontiguousarray(dK, np.float64).reshape(n, 3)
+    sw = np.ascontiguousarray(sw, np.int32).reshape(n)
+    re = np.zeros((n, 16, 16))
+    im = np.zeros((n, 16, 16))
+    lib().orc_primk_batch(n, _dp(a), _dp(A), _dp(B), _dp(dK), _ip(sw), _dp(re), _dp(im))
+    return (re + 1j * im).transpose(0, 2, 1)
+
+
+def k_overlap_deltas(sysin, num_axial):
+    """The three deltaK vectors of gaussKOverlap: recipVectors(:, axis) /
+    numAxialKPoints(axis) (integrals3Terms.F90:1230-1236)."""
+    b = recip_vectors(sysin.cell)
+    return [b[x] / float(num_axial[x]) for x in range(3)]
 
 
 # ---------------------------------------------------------------------------
@@ -266,6 +289,21 @@ class Oracle:
         return shape(vv, n, n), shape(cv, nc, n) if nc else None, \
             shape(cc, nc, nc) if nc else None
 
+    def k_overlap(self, num_axial, nthreads=0):
+        """gaussKOverlap (integrals3Terms.F90:956-1384) before
+        orthogonalisation: per axis (vv, cv, cc) complex [k, row, col], the
+        sweep of the blocks' real parts plus the sweep of minus their
+        imaginary parts under the momentum's -i Bloch factor.  Complex
+        builds only (the reference's sweep is complex throughout)."""
+        if self.sysin.gamma:
+            raise ValueError("the k-overlap sweep is complex; the Gamma build has none")
+        out = []
+        for dk in k_overlap_deltas(self.sysin, num_axial):
+            a = self.integral(KO, 0, coef=dk, nthreads=nthreads)
+            b = self.integral(KO, 1, coef=dk, nthreads=nthreads)
+            out.append(tuple(None if x is None else x + y for x, y in zip(a, b)))
+        return out
+
     @staticmethod
     def set_pair_sample(stride, offset=0):
         """Sample every stride-th atom pair in later sweeps (1 = all)."""
@@ -291,6 +329,18 @@ def ortho_overlap(S_vv, S_cv, S_cc):
     A = np.conj(S_cv).swapaxes(-1, -2) @ S_cc @ S_cv
     out = S_vv - 2.0 * (np.conj(S_cv).swapaxes(-1, -2) @ S_cv) + A
     return np.triu(out) if out.ndim == 2 else np.stack([np.triu(o) for o in out])
+
+
+def ortho_k_overlap(O_vv, O_cv, O_cc, S_cv):
+    """ortho3Terms opcode 8/9 (integrals3Terms.F90:1388-1608): the full
+    matrix O_vv - S_cv^H O_cv - O_cv^H S_cv + S_cv^H O_cc S_cv
+    (valeCoreCoreVale fullFlag 1, coreValeCoreCore, makeValeVale fullFlag 1,
+    intgOrtho.F90:48-76, :78-211), with no packing and no zeroing; O_vv as
+    swept when there are no core states."""
+    if O_cv is None:
+        return O_vv
+    H = lambda x: np.conj(x).swapaxes(-1, -2)
+    return O_vv - H(S_cv) @ O_cv - H(O_cv) @ S_cv + H(S_cv) @ O_cc @ S_cv
 
 
 def ortho_operator(O_vv, O_cv, O_cc, S_cv):

@@ -456,6 +456,84 @@ void orc_prim3(int op, double a1, double a2, const double *A, const double *B, c
     for (int c = 0; c < 3; c++) cart_to_sh(la, lb, pc[c], sh + 256 * c);
 }
 
+/* KOverlap2CIntg (gaussIntegrals.f90:129408): the block of exp(-i dK . r)
+ * between the two Gaussians, complex.  Along each axis, with y = x - P,
+ * (x-A)^i (x-B)^j = sum_{k,l} C(i,k) C(j,l) PA^(i-k) PB^(j-l) y^(k+l), and the
+ * moments of y against exp(-zeta y^2 - i q y) are, as the reference writes
+ * them, m_n = s0 H_n(r) (i / (2 sqrt zeta))^n with r = -q / (2 sqrt zeta), H_n
+ * the Hermite polynomials and s0 = sqrt(pi/zeta) exp(-xi d^2) exp(-q^2 /
+ * (4 zeta)) exp(-i q P) (its preFactorko).  The Cartesian block is the
+ * product of the three axes; real and imaginary parts go through the
+ * solid-harmonic transform separately. */
+void orc_primk(double a1, double a2, const double *A, const double *B, const double *dK, int sw,
+               double *shr, double *shi)
+{
+    init_tables();
+    int la, lb;
+    decode_switch(sw, &la, &lb);
+    double zeta = a1 + a2, xi = a1 * a2 / zeta;
+    /* per axis: I[x][i][j] complex, i <= la, j <= lb */
+    double Ir[3][4][4], Ii[3][4][4];
+    for (int x = 0; x < 3; x++) {
+        double P = (a1 * A[x] + a2 * B[x]) / zeta, PA = P - A[x], PB = P - B[x];
+        double d = A[x] - B[x], q = dK[x];
+        double amp = sqrt(PI / zeta) * exp(-xi * d * d) * exp(-q * q / (4.0 * zeta));
+        double s0r = amp * cos(q * P), s0i = -amp * sin(q * P);
+        double r = -q / (2.0 * sqrt(zeta)), g = 1.0 / (2.0 * sqrt(zeta));
+        double H[7];
+        H[0] = 1.0;
+        H[1] = 2.0 * r;
+        for (int m = 1; m < 6; m++) H[m + 1] = 2.0 * r * H[m] - 2.0 * m * H[m - 1];
+        /* t_n = H_n (i g)^n: i^n cycles 1, i, -1, -i */
+        double tr[7], ti[7];
+        for (int m = 0; m <= 6; m++) {
+            double mag = H[m] * pow(g, m);
+            int ph = m & 3;
+            tr[m] = ph == 0 ? mag : ph == 2 ? -mag : 0.0;
+            ti[m] = ph == 1 ? mag : ph == 3 ? -mag : 0.0;
+        }
+        static const double BIN[4][4] = {{1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 1, 0}, {1, 3, 3, 1}};
+        for (int i = 0; i <= la; i++)
+            for (int j = 0; j <= lb; j++) {
+                double sr = 0.0, si = 0.0;
+                for (int k = 0; k <= i; k++)
+                    for (int l = 0; l <= j; l++) {
+                        double c = BIN[i][k] * BIN[j][l] * pow(PA, i - k) * pow(PB, j - l);
+                        sr += c * tr[k + l];
+                        si += c * ti[k + l];
+                    }
+                Ir[x][i][j] = s0r * sr - s0i * si;
+                Ii[x][i][j] = s0r * si + s0i * sr;
+            }
+    }
+    double pr[20][20], pim[20][20];
+    for (int a = 0; a < NCART[la]; a++)
+        for (int b = 0; b < NCART[lb]; b++) {
+            double vr = 1.0, vi = 0.0;
+            for (int x = 0; x < 3; x++) {
+                double ur = Ir[x][CART[a][x]][CART[b][x]], ui = Ii[x][CART[a][x]][CART[b][x]];
+                double nr = vr * ur - vi * ui, ni = vr * ui + vi * ur;
+                vr = nr;
+                vi = ni;
+            }
+            pr[a][b] = vr;
+            pim[a][b] = vi;
+        }
+    for (int q = 0; q < 256; q++) shr[q] = shi[q] = 0.0;
+    cart_to_sh(la, lb, pr, shr);
+    cart_to_sh(la, lb, pim, shi);
+}
+
+void orc_primk_batch(int n, const double *a, const double *A, const double *B, const double *dK,
+                     const int *sw, double *shr, double *shi)
+{
+    init_tables();
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++)
+        orc_primk(a[2 * i], a[2 * i + 1], A + 3 * i, B + 3 * i, dK + 3 * i, sw[i],
+                  shr + (long)256 * i, shi + (long)256 * i);
+}
+
 void orc_prim3_batch(int op, int n, const double *a, const double *A, const double *B,
                      const double *C, const int *sw, double *sh)
 {
@@ -783,6 +861,17 @@ static void pair_block(const drv_ctx *c, const op_args *o, int ti, int tj,
         double three[768];
         orc_prim3(o->op == ORC_DM ? 5 : 6, a1, a2, A, B, o->op == ORC_DM ? o->coef : A, sw, three);
         for (int q = 0; q < 256; q++) blk[q] = three[256 * o->term + q];
+        return;
+    }
+    if (o->op == ORC_KO) {
+        /* k-overlap (gaussKOverlap, integrals3Terms.F90:956) for the deltaK in
+         * coef: even term the real part of the block, odd term minus its
+         * imaginary part, whose Bloch sum takes the factor -i, so that the
+         * two sweeps add to the complex matrix */
+        double im[256];
+        orc_primk(a1, a2, A, B, o->coef, sw, blk, im);
+        if (o->term & 1)
+            for (int q = 0; q < 256; q++) blk[q] = -im[q];
         return;
     }
     if (o->op == ORC_MV) {   /* accumulated with a minus sign, integrals.F90:1434-1447 */
@@ -1131,7 +1220,7 @@ int orc_integral(const orc_sys *s, int op, int term, const double *coef,
                 if (s->gamma) {
                     const double sg = op == ORC_MM ? -1.0 : 1.0;
                     for (int q = 0; q < nt1 * nt2; q++) cur[q] += sg * X[q];
-                } else if (op == ORC_MM) {
+                } else if (op == ORC_MM || (op == ORC_KO && (term & 1))) {
                     for (int kk = 0; kk < nk; kk++) {
                         double pr = c.phase[2 * (k * nk + kk)], pi = c.phase[2 * (k * nk + kk) + 1];
                         double *ck = cur + (long)kk * nt1 * nt2 * 2;

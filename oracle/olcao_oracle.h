@@ -19,7 +19,8 @@
 extern "C" {
 #endif
 
-enum { ORC_OL = 0, ORC_KE = 1, ORC_NP = 2, ORC_EP = 3, ORC_HAM = 4, ORC_MV = 5, ORC_DM = 6, ORC_MM = 7 };
+enum { ORC_OL = 0, ORC_KE = 1, ORC_NP = 2, ORC_EP = 3, ORC_HAM = 4, ORC_MV = 5, ORC_DM = 6, ORC_MM = 7,
+       ORC_KO = 8 };
 enum { ORC_OK = 0, ORC_ERR_CELLS = 1, ORC_ERR_INPUT = 2 };
 
 typedef struct {
@@ -77,6 +78,12 @@ void orc_prim3(int op, double a1, double a2, const double *A, const double *B, c
                int sw, double *sh);
 void orc_prim3_batch(int op, int n, const double *a, const double *A, const double *B,
                      const double *C, const int *sw, double *sh);
+/* KOverlap2CIntg (gaussIntegrals.f90:129408): real and imaginary parts of the
+ * (16,16) column-major block of exp(-i dK . r) between the two Gaussians. */
+void orc_primk(double a1, double a2, const double *A, const double *B, const double *dK, int sw,
+               double *shr, double *shi);
+void orc_primk_batch(int n, const double *a, const double *A, const double *B, const double *dK,
+                     const int *sw, double *shr, double *shi);
 double orc_neglig_limit(const double *vec, double logThresh,
                         double minAtomicAlpha, double minPotAlpha);
 int orc_make_lattice(const double *vec, double negligLimit, int maxCells,

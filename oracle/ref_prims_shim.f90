@@ -47,6 +47,21 @@ contains
       end select
    end subroutine ref_prim3
 
+   ! KOverlap2CIntg: the overlap with exp(-i deltaK . r) between the two
+   ! Gaussians; the complex (16,16) block split into its real and imaginary
+   ! parts (column-major).
+   subroutine ref_primk(a1, a2, A, B, dK, sw, shr, shi) bind(C, name="ref_primk")
+      integer(c_int), value :: sw
+      real(c_double), value :: a1, a2
+      real(c_double), intent(in) :: A(3), B(3), dK(3)
+      real(c_double), intent(inout) :: shr(16,16), shi(16,16)
+      complex(kind=8) :: sh(16,16)
+      sh(:,:) = (0.0d0, 0.0d0)
+      call KOverlap2CIntg(a1, a2, A, B, dK, sw, sh)
+      shr(:,:) = real(sh(:,:), 8)
+      shi(:,:) = aimag(sh(:,:))
+   end subroutine ref_primk
+
    subroutine ref_boys(T, F) bind(C, name="ref_boys")
       real(c_double), value :: T
       real(c_double), intent(out) :: F(7)

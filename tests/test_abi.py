@@ -67,10 +67,13 @@ def test_hdf5_writer_library_exports_its_symbol():
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     names = set(re.findall(r"\b(olcao_[a-z_0-9]+)\s*\(", src))
     assert names == {"olcao_write_scf_integrals_h5", "olcao_write_scf_eigen_h5",
-                     "olcao_write_scf_vector_terms_h5", "olcao_write_pscf_h5"}
+                     "olcao_write_scf_vector_terms_h5", "olcao_write_pscf_h5",
+                     "olcao_write_k_overlap_h5"}
     for name in names:
         assert hasattr(lib, name), name
     lib.olcao_write_scf_integrals_h5.argtypes = [C.c_void_p, C.c_char_p]
     assert lib.olcao_write_scf_integrals_h5(None, b"/nonexistent") == 3
+    lib.olcao_write_k_overlap_h5.argtypes = [C.c_void_p, C.c_void_p, C.c_char_p, C.c_char_p]
+    assert lib.olcao_write_k_overlap_h5(None, None, None, b"/nonexistent") == 3
     lib.olcao_write_pscf_h5.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_char_p, C.c_char_p]
     assert lib.olcao_write_pscf_h5(None, None, 1, b"k", b"/nonexistent") == 3

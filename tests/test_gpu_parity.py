@@ -6,6 +6,7 @@ to the matrix scale (block_rel_err); eigenvalues <= 1e-6 eV; Boys values
 equal the reference's to the cancellation noise of its closed form
 (F_0..F_4 <= 1e-10, F_5/F_6 <= 2e-6 relative just above T = 0.1).
 """
+import os
 import sys
 
 import numpy as np
@@ -172,6 +173,60 @@ def test_dipole_and_momentum_sweeps_match_oracle(gpu, case):
             if cv is not None:
                 assert block_rel_err(dev[comp][1], cv) < 1e-10
                 assert block_rel_err(dev[comp][2], cc) < 1e-10
+    g.close()
+
+
+def test_device_k_overlap_primitive_matches_reference_golden(gpu):
+    """KOverlap2CIntg (gaussIntegrals.f90:129408) on the device (real part op
+    7, imaginary part op 8) against the reference's own blocks: 10 cases per
+    class in {s,p,d,f}^2, |deltaK| from 0 to 4 inverse bohr, within 1e-10 of
+    each block's scale (the device uses Hermite E coefficients, the reference
+    a binomial expansion about P)."""
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "prims_k_golden.npz"))
+    cls, inp = d["k_cls"], d["k_in"]
+    ref = d["k_re"] + 1j * d["k_im"]
+    a = np.column_stack([inp[:, :2], np.zeros(len(inp))])
+    re = gpu.prim_batch(7, a, inp[:, 2:5], inp[:, 5:8], inp[:, 8:11], cls[:, 2])
+    im = gpu.prim_batch(8, a, inp[:, 2:5], inp[:, 5:8], inp[:, 8:11], cls[:, 2])
+    worst = 0.0
+    for i in range(len(cls)):
+        na, nb = NSH[cls[i, 0]], NSH[cls[i, 1]]
+        e = block_rel_err((re[i] + 1j * im[i])[:na, :nb], ref[i][:na, :nb])
+        worst = max(worst, e)
+        assert e < 1e-10, (i, cls[i], e)
+    _log(f"  k-overlap primitive worst {worst:.2e}")
+
+
+@pytest.mark.parametrize("case", ["cubic_sp", "core", "hex_spd", "two_types"])
+def test_k_overlap_sweep_matches_oracle(gpu, case):
+    """gaussKOverlap (integrals3Terms.F90:956-1384) per axis: valeVale is the
+    full orthogonalised matrix (ortho3Terms opcode 8, no zeroing), coreVale
+    and coreCore raw, against the oracle to 1e-10 of each matrix's scale;
+    numAxialKPoints differs per axis so each axis has its own deltaK."""
+    s = configs.small_test(**SMALL[case])
+    g, o = _pair(gpu, s)
+    num_axial = (2, 3, 1)
+    dev = g.gaussKOverlap(num_axial)
+    ref = o.k_overlap(num_axial)
+    S_cv = o.integral(oracle.OL)[1]
+    for axis in range(3):
+        vv, cv, cc = ref[axis]
+        full = oracle.ortho_k_overlap(vv, cv, cc, S_cv)
+        e = block_rel_err(dev[axis][0], full)
+        _log(f"  {case} KO[{axis}]: {e:.2e}")
+        assert e < 1e-10, (case, axis, e)
+        if cv is not None:
+            assert block_rel_err(dev[axis][1], cv) < 1e-10
+            assert block_rel_err(dev[axis][2], cc) < 1e-10
+    g.close()
+
+
+def test_k_overlap_refused_for_gamma(gpu):
+    """The k-overlap matrices are complex: a Gamma context refuses them with
+    an input error rather than returning real parts."""
+    g = gpu.OlcaoGPU(configs.small_test(**SMALL["gamma_spd"]))
+    with pytest.raises(Exception):
+        g.gaussKOverlap((1, 1, 1))
     g.close()
 
 
@@ -436,6 +491,39 @@ def test_vale_charge_si64_invariants(gpu):
     g.close()
 
 
+def _check_k_overlap_file(lib, g, o, path, kg):
+    """olcao_write_k_overlap_h5 into an integral file (kg None) or a PSCF
+    k-point group: atomKOverlap and atomKOverlapPlusG hold, per axis and
+    k-point, the full orthogonalised k-overlap as real/imag (valeDim,
+    valeDim) datasets equal to the oracle's to 1e-10, status 1; a Gamma
+    context is refused with OLCAO_ERR_UNSUPPORTED."""
+    import ctypes as C
+
+    from tests import h5read
+    num_axial = np.array([2, 3, 1], np.int32)
+    lib.olcao_write_k_overlap_h5.argtypes = [C.c_void_p, C.c_void_p, C.c_char_p, C.c_char_p]
+    rc = lib.olcao_write_k_overlap_h5(g.ctx, num_axial.ctypes.data,
+                                      None if kg is None else kg.encode(), path.encode())
+    if g.gamma:
+        assert rc == 7
+        return
+    assert rc == 0
+    top = "/atomIntgGroup/" if kg is None else f"/{kg}/atomIntgGroup/"
+    S_cv = o.integral(oracle.OL)[1]
+    refs = [oracle.ortho_k_overlap(*t, S_cv) for t in o.k_overlap(num_axial)]
+    f = h5read.File(path)
+    for group in ("atomKOverlap", "atomKOverlapPlusG"):
+        assert f.status(top + group) == 1, group
+        for x in range(3):
+            for k in range(g.nk):
+                for part, val in (("real", refs[x][k].real), ("imag", refs[x][k].imag)):
+                    data, chunked, nfilt = f.read(f"{top}{group}/{x + 1:07d}/{part}{k + 1:07d}")
+                    assert data.shape == (g.vale_dim, g.vale_dim) and chunked and nfilt >= 1
+                    e = block_rel_err(data, val.T)
+                    assert e < 1e-10, (group, x, k, part, e)
+    f.close()
+
+
 @pytest.mark.parametrize("case", [dict(core=True, kmesh=(2, 1, 1), spin=2),
                                   dict(core=True, gamma=True, rel=1, two_types=True)])
 def test_scf_integral_file(gpu, tmp_path, case):
@@ -522,6 +610,8 @@ def test_scf_integral_file(gpu, tmp_path, case):
             assert data.shape == (n, s.core_dim) and chunked and nfilt >= 1
             assert block_rel_err(data, val.T) < 1e-10
     f.close()
+    # the k-overlap groups, as setupSCF fills them with doMTOP = 1
+    _check_k_overlap_file(lib, g, o, path, None)
     ns = n - 1
     w, v = g.secularEqnSCF(s.pot_coeffs, ns, vectors=True)
     w = np.asarray(w).reshape(s.spin, nk, ns)
@@ -616,6 +706,7 @@ def test_pscf_file(gpu, tmp_path, case):
     for group in ("atomDMOverlap", "atomMMOverlap", "atomKOverlap", "atomKOverlapPlusG"):
         assert f.status(top + group) == 0, group
     f.close()
+    _check_k_overlap_file(lib, g, o, path, kg)
     g.close()
     s2 = configs.small_test(**SMALL["rel_spin2"])
     g2 = gpu.OlcaoGPU(s2)
