@@ -33,7 +33,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libolcao_ref_prims.so")
 
-OL, KE, NP, EP, HAM, MV, DM, MM, KO = 0, 1, 2, 3, 4, 5, 6, 7, 8
+OL, KE, NP, EP, HAM, MV, DM, MM, KO, FORCE = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9
 SMALL_THRESH = 1.0e-8
 
 _lib = None
@@ -58,6 +58,7 @@ def lib():
         _lib.orc_set_pair_sample.argtypes = [C.c_int, C.c_int]
         _lib.orc_set_three_center_range_scale.argtypes = [C.c_double]
         _lib.orc_primk_batch.argtypes = [C.c_int, dp, dp, dp, dp, C.POINTER(C.c_int), dp, dp]
+        _lib.orc_primd_batch.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, C.POINTER(C.c_int), dp]
     return _lib
 
 
@@ -105,6 +106,23 @@ def primk_batch(a, A, B, dK, sw):
     im = np.zeros((n, 16, 16))
     lib().orc_primk_batch(n, _dp(a), _dp(A), _dp(B), _dp(dK), _ip(sw), _dp(re), _dp(im))
     return (re + 1j * im).transpose(0, 2, 1)
+
+
+def primd_batch(op, a, A, B, C_, sw):
+    """Force-derivative blocks d/dB (computeForceIntg's primitives): op 10
+    dkinetic2CIntg (gaussIntegrals.f90:65512), 11 dnuclear3CIntgCB (:88076),
+    12 delectron3CIntgCB (:112289); [n, 3, 16, 16] ([i, component, row,
+    col]).  A nuclear block of la + lb = 6 raises (it needs F_7)."""
+    a = np.ascontiguousarray(a, np.float64).reshape(-1, 3)
+    n = a.shape[0]
+    A = np.ascontiguousarray(A, np.float64).reshape(n, 3)
+    B = np.ascontiguousarray(B, np.float64).reshape(n, 3)
+    C_ = np.ascontiguousarray(C_, np.float64).reshape(n, 3)
+    sw = np.ascontiguousarray(sw, np.int32).reshape(n)
+    out = np.zeros((n, 3, 16, 16))
+    if lib().orc_primd_batch(op, n, _dp(a), _dp(A), _dp(B), _dp(C_), _ip(sw), _dp(out)):
+        raise ValueError("nuclear derivative of an f-f block (needs F_7)")
+    return out.transpose(0, 1, 3, 2)
 
 
 def k_overlap_deltas(sysin, num_axial):
@@ -302,6 +320,23 @@ class Oracle:
             a = self.integral(KO, 0, coef=dk, nthreads=nthreads)
             b = self.integral(KO, 1, coef=dk, nthreads=nthreads)
             out.append(tuple(None if x is None else x + y for x, y in zip(a, b)))
+        return out
+
+    def force_integrals(self, total_energy, nthreads=0):
+        """computeForceIntg (forces.F90:60-898): per spin and Cartesian
+        component, (vv, cv, cc) of the ket-derivative matrices with that
+        spin's potential coefficients and -2 totalEnergy times the momentum
+        block; complex builds hold the full valeVale (the block and its
+        dagger, a diagonal atom block as its own dagger), the Gamma build the
+        upper triangle; no core orthogonalisation."""
+        s = self.sysin
+        if (np.asarray(self.lmai) == 16).any():
+            raise ValueError("force integrals of f orbitals: the f-f nuclear derivative needs F_7")
+        coefs = np.asarray(s.pot_coeffs, np.float64).reshape(s.spin, -1)
+        out = []
+        for q in range(s.spin):
+            cf = np.concatenate([coefs[q], [float(total_energy)]])
+            out.append([self.integral(FORCE, x, coef=cf, nthreads=nthreads) for x in range(3)])
         return out
 
     @staticmethod

@@ -276,11 +276,11 @@ static void os_kinetic(int la, int lb, double a1, double a2, double xi,
  *       + h3 [N_i(a)(th^m - th^(m+1))(a-1i|b) + N_i(b)(th^m - th^(m+1))(a|b-1i)]
  * th^m(0|0) = F_m(U) 2 pi/zeta3 exp(-(a1a2 d12^2 + a1a3 d13^2 + a2a3 d23^2)/zeta3),
  * U = zeta3 |G - C|^2 (nuclear3CIntg :16255-16271). */
-static void os_nuclear(int la, int lb, const double pf[7], const double GA[3],
-                       const double GB[3], const double GC[3], double h,
-                       double th[7][20][20])
+static void os_nuclear_m(int la, int lb, int L, const double pf[7], const double GA[3],
+                         const double GB[3], const double GC[3], double h,
+                         double th[7][20][20])
 {
-    int na = NCART[la], nb = NCART[lb], L = la + lb;
+    int na = NCART[la], nb = NCART[lb];
     int tot[20];
     for (int c = 0; c < 20; c++) tot[c] = CART[c][0] + CART[c][1] + CART[c][2];
     for (int b = 0; b < nb; b++)
@@ -316,6 +316,14 @@ static void os_nuclear(int la, int lb, const double pf[7], const double GA[3],
                 th[m][a][b] = v;
             }
         }
+}
+
+/* th^m for m <= la + lb - |a| - |b| (what the integral needs). */
+static void os_nuclear(int la, int lb, const double pf[7], const double GA[3],
+                       const double GB[3], const double GC[3], double h,
+                       double th[7][20][20])
+{
+    os_nuclear_m(la, lb, la + lb, pf, GA, GB, GC, h, th);
 }
 
 /* Mass-velocity block (massvel2CIntg, gaussIntegrals.f90:39070):
@@ -626,6 +634,146 @@ void orc_prim_batch(int op, int n, const double *a, const double *A,
         orc_prim(op, a[3 * i], a[3 * i + 1], a[3 * i + 2], A + 3 * i, B + 3 * i,
                  C + 3 * i, sw[i], o);
     }
+}
+
+/* Force-derivative primitives of computeForceIntg (forces.F90:60): the
+ * derivative with respect to the ket centre B, C held fixed, of the kinetic
+ * block (op 10, dkinetic2CIntg, gaussIntegrals.f90:65512), the
+ * Gaussian-damped nuclear block (op 11, dnuclear3CIntgCB, :88076) and the
+ * three-Gaussian overlap block (op 12, delectron3CIntgCB, :112289);
+ * sh: [3][16][16], one column-major block per Cartesian component of B.
+ * With g_b = (r-B)^b exp(-beta |r-B|^2), d/dB_i g_b = 2 beta g_{b+1i} -
+ * N_i(b) g_{b-1i}.  The ket-raising step of each Obara-Saika recursion
+ * turns this into terms of the (a|b) table itself, without the cancellation
+ * of 2 beta (a|b+1i) against N_i(b) (a|b-1i) when beta >> alpha:
+ *   overlap type (X = P, h = 1/2zeta; or X = G, h = 1/2zeta3):
+ *     d_i(a|b) = 2 beta XB_i (a|b) + 2 beta h N_i(a) (a-1i|b)
+ *                - (1 - 2 beta h) N_i(b) (a|b-1i)
+ *   kinetic: the same recursion on T, plus 2 xi d_i S(a|b);
+ *   nuclear, auxiliary index m:
+ *     d_i(a|b) = 2 beta [GB_i th0(a|b) - GC_i th1(a|b)]
+ *                + 2 beta h3 N_i(a) (th0 - th1)(a-1i|b)
+ *                - (1 - 2 beta h3) N_i(b) th0(a|b-1i) - 2 beta h3 N_i(b) th1(a|b-1i)
+ * with 1 - 2 beta h written as (the other exponents) 2h.  Returns 0, or 1
+ * for a nuclear block with la + lb = 6 (it needs F_7; the reference's Boys
+ * function, restated here, stops at F_6). */
+int orc_primd(int op, double a1, double a2, double a3, const double *A, const double *B,
+              const double *C, int sw, double *sh)
+{
+    init_tables();
+    int la, lb;
+    decode_switch(sw, &la, &lb);
+    memset(sh, 0, 768 * sizeof(double));
+    if (op == 11 && la + lb + 1 > 6) return 1;
+    double zeta = a1 + a2;
+    double xi = a1 * a2 / zeta;
+    double P[3], PA[3], PB[3];
+    double d2 = 0.0;
+    for (int x = 0; x < 3; x++) {
+        P[x] = (a1 * A[x] + a2 * B[x]) / zeta;
+        PA[x] = P[x] - A[x];
+        PB[x] = P[x] - B[x];
+        d2 += (A[x] - B[x]) * (A[x] - B[x]);
+    }
+    static __thread double pc[20][20], pk[20][20], th[7][20][20], dp[20][20];
+    int na = NCART[la], nb = NCART[lb];
+    if (op == 10) {
+        double pref = pow(PI / zeta, 1.5) * exp(-xi * d2);
+        double h = 1.0 / (2.0 * zeta);
+        os_overlap(la, lb, pref, PA, PB, h, pc);
+        os_kinetic(la, lb, a1, a2, xi, d2, PA, PB, h, pc, pk);
+        double bh = 2.0 * a2 * h, rest = 2.0 * a1 * h;
+        for (int X = 0; X < 3; X++) {
+            for (int b = 0; b < nb; b++)
+                for (int a = 0; a < na; a++) {
+                    int Na = CART[a][X], Nb = CART[b][X];
+                    double dS = 2.0 * a2 * PB[X] * pc[a][b];
+                    double dT = 2.0 * a2 * PB[X] * pk[a][b];
+                    if (Na) {
+                        dS += bh * Na * pc[dec[a][X]][b];
+                        dT += bh * Na * pk[dec[a][X]][b];
+                    }
+                    if (Nb) {
+                        dS -= rest * Nb * pc[a][dec[b][X]];
+                        dT -= rest * Nb * pk[a][dec[b][X]];
+                    }
+                    dp[a][b] = dT + 2.0 * xi * dS;
+                }
+            cart_to_sh(la, lb, dp, sh + 256 * X);
+        }
+        return 0;
+    }
+    double z3 = zeta + a3;
+    double h3 = 1.0 / (2.0 * z3);
+    double G[3], GA[3], GB[3], GC[3];
+    double pc2 = 0.0, gc2 = 0.0;
+    for (int x = 0; x < 3; x++) {
+        G[x] = (a1 * A[x] + a2 * B[x] + a3 * C[x]) / z3;
+        GA[x] = G[x] - A[x];
+        GB[x] = G[x] - B[x];
+        GC[x] = G[x] - C[x];
+        pc2 += (P[x] - C[x]) * (P[x] - C[x]);
+        gc2 += GC[x] * GC[x];
+    }
+    double bh = 2.0 * a2 * h3, rest = 2.0 * (a1 + a3) * h3;
+    if (op == 12) {
+        double pref = pow(PI / z3, 1.5) * exp(-xi * d2 - (zeta * a3 / z3) * pc2);
+        os_overlap(la, lb, pref, GA, GB, h3, pc);
+        for (int X = 0; X < 3; X++) {
+            for (int b = 0; b < nb; b++)
+                for (int a = 0; a < na; a++) {
+                    int Na = CART[a][X], Nb = CART[b][X];
+                    double v = 2.0 * a2 * GB[X] * pc[a][b];
+                    if (Na) v += bh * Na * pc[dec[a][X]][b];
+                    if (Nb) v -= rest * Nb * pc[a][dec[b][X]];
+                    dp[a][b] = v;
+                }
+            cart_to_sh(la, lb, dp, sh + 256 * X);
+        }
+        return 0;
+    }
+    /* op 11: nuclear, th^m up to m = la + lb + 1 */
+    double F[7], pf[7];
+    orc_boys(z3 * gc2, F);
+    double e = 0.0;
+    for (int x = 0; x < 3; x++) {
+        double dd1 = A[x] - B[x], dd2 = A[x] - C[x], dd3 = B[x] - C[x];
+        e += a1 * a2 * (dd1 * dd1) + a1 * a3 * (dd2 * dd2) + a2 * a3 * (dd3 * dd3);
+    }
+    double ex = exp(-e / z3);
+    for (int m = 0; m < 7; m++) pf[m] = F[m] * 2.0 * (PI / z3) * ex;
+    os_nuclear_m(la, lb, la + lb + 1, pf, GA, GB, GC, h3, th);
+    for (int X = 0; X < 3; X++) {
+        for (int b = 0; b < nb; b++)
+            for (int a = 0; a < na; a++) {
+                int Na = CART[a][X], Nb = CART[b][X];
+                double v = 2.0 * a2 * (GB[X] * th[0][a][b] - GC[X] * th[1][a][b]);
+                if (Na) {
+                    int a1i = dec[a][X];
+                    v += bh * Na * (th[0][a1i][b] - th[1][a1i][b]);
+                }
+                if (Nb) {
+                    int b1i = dec[b][X];
+                    v -= rest * Nb * th[0][a][b1i] + bh * Nb * th[1][a][b1i];
+                }
+                dp[a][b] = v;
+            }
+        cart_to_sh(la, lb, dp, sh + 256 * X);
+    }
+    return 0;
+}
+
+int orc_primd_batch(int op, int n, const double *a, const double *A, const double *B,
+                    const double *C, const int *sw, double *sh)
+{
+    init_tables();
+    init_boys();
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int i = 0; i < n; i++)
+        bad |= orc_primd(op, a[3 * i], a[3 * i + 1], a[3 * i + 2], A + 3 * i, B + 3 * i, C + 3 * i,
+                         sw[i], sh + 768 * (long)i);
+    return bad;
 }
 
 /* ------------------------------------------------------------------ */
@@ -967,6 +1115,51 @@ static void pair_block(const drv_ctx *c, const op_args *o, int ti, int tj,
                 for (int ii = 0; ii < n1; ii++) blk[ii + 16 * jj] += one[ii + 16 * jj];
         });
         *contrib = any;
+    } else if (o->op == ORC_FORCE) {
+        /* computeForceIntg (forces.F90:419-761), component term of the ket
+         * derivative: minus the potential terms' three-Gaussian derivatives
+         * times this spin's coefficients, plus the nuclear derivatives times
+         * Z, minus the kinetic derivative, minus 2 totalEnergy (coef[potDim])
+         * times the momentum block; the HAM sweep's image screening */
+        int X = o->term;
+        double E2 = 2.0 * o->coef[c->potDim];
+        double three[768];
+        for (int t = 0; t < s->nPotTypes; t++) {
+            int na = s->potNAlpha[t];
+            for (int n = 0; n <= na; n++) {
+                double a3, cf = 0.0, Z = s->potZ[t];
+                if (n < na) {
+                    cf = o->coef[c->potCumul[t] + n];
+                    a3 = s->potAlpha[t * s->maxPotAlphas + n];
+                } else {
+                    if (Z < SMALL_THRESH) continue;
+                    a3 = s->potNucAlpha[t];
+                }
+                double tadv = tau * (1 - s2 / adist) * (a1 + a2 + a3) / (a1 + a2) / a3 *
+                              three_range_scale;
+                if (n < na) {
+                    SITE_LOOP(pt == t, a3, tadv, {
+                        orc_primd(12, a1, a2, a3, A, B, Cp, sw, three);
+                        for (int jj = 0; jj < n2; jj++)
+                            for (int ii = 0; ii < n1; ii++)
+                                blk[ii + 16 * jj] -= three[256 * X + ii + 16 * jj] * cf;
+                    });
+                } else {
+                    SITE_LOOP(pt == t, a3, tadv, {
+                        orc_primd(11, a1, a2, a3, A, B, Cp, sw, three);
+                        for (int jj = 0; jj < n2; jj++)
+                            for (int ii = 0; ii < n1; ii++)
+                                blk[ii + 16 * jj] += three[256 * X + ii + 16 * jj] * Z;
+                    });
+                }
+            }
+        }
+        orc_primd(10, a1, a2, 0.0, A, B, A, sw, three);
+        for (int jj = 0; jj < n2; jj++)
+            for (int ii = 0; ii < n1; ii++) blk[ii + 16 * jj] -= three[256 * X + ii + 16 * jj];
+        orc_prim3(6, a1, a2, A, B, A, sw, three);
+        for (int jj = 0; jj < n2; jj++)
+            for (int ii = 0; ii < n1; ii++) blk[ii + 16 * jj] -= E2 * three[256 * X + ii + 16 * jj];
     } else { /* ORC_HAM: integrals.F90:3211-3490 */
         for (int t = 0; t < s->nPotTypes; t++) {
             int na = s->potNAlpha[t];
@@ -1015,7 +1208,7 @@ static void pair_block(const drv_ctx *c, const op_args *o, int ti, int tj,
  * saveCurrentPair / valeValeSaving / coreValeSaving / coreCoreSaving and
  * their Gamma twins).  cur: [nk][n1][n2] complex (or [n1][n2] real). */
 static void save_pair(const drv_ctx *c, int i, int j, const double *cur,
-                      int nt1, int nt2, double *vv, double *cv, double *cc)
+                      int nt1, int nt2, double *vv, double *cv, double *cc, int full)
 {
     const orc_sys *s = c->s;
     int ti = s->atomType[i], tj = s->atomType[j];
@@ -1030,10 +1223,21 @@ static void save_pair(const drv_ctx *c, int i, int j, const double *cur,
         const double *ck = cur + (long)k * nt1 * nt2 * cplx;
 #define CUR(r, cidx, comp) ck[((r) * nt2 + (cidx)) * cplx + (comp)]
         /* valeVale block, upper triangle kept; strictly lower part of a
-         * diagonal atom block zeroed (intgSaving.F90:484-509) */
+         * diagonal atom block zeroed (intgSaving.F90:484-509).  full
+         * (valeValeSavingFull, :513-545): the block and its dagger in the
+         * mirrored place, the dagger last, so that a diagonal atom block
+         * ends as its own dagger */
         for (int b = 0; b < nv2; b++)
             for (int a = 0; a < nv1; a++) {
                 long idx = (long)(v1 + a) + (long)(v2 + b) * n + (long)k * n * n;
+                if (full) {
+                    long idt = (long)(v2 + b) + (long)(v1 + a) * n + (long)k * n * n;
+                    if (i != j)
+                        for (int q = 0; q < cplx; q++) vv[idx * cplx + q] = CUR(a, b, q);
+                    vv[idt * cplx] = CUR(a, b, 0);
+                    if (cplx == 2) vv[idt * cplx + 1] = -CUR(a, b, 1);
+                    continue;
+                }
                 int zero = (i == j) && (a > b);
                 for (int q = 0; q < cplx; q++)
                     vv[idx * cplx + q] = zero ? 0.0 : CUR(a, b, q);
@@ -1055,12 +1259,15 @@ static void save_pair(const drv_ctx *c, int i, int j, const double *cur,
                     if (cplx == 2) cv[idx * cplx + 1] = -CUR(b, nv2 + a, 1);
                 }
         if (nc1 == 0 || nc2 == 0) continue;
-        for (int b = 0; b < nc2; b++)
-            for (int a = 0; a < nc1; a++) {
-                long idx = (long)(c1 + a) + (long)(c2 + b) * nc + (long)k * nc * nc;
-                for (int q = 0; q < cplx; q++) cc[idx * cplx + q] = CUR(nv1 + a, nv2 + b, q);
-            }
-        if (i != j)
+        /* coreCoreSaving; coreCoreSavingFull (intgSaving.F90) writes the
+         * dagger for i == j too, over the block */
+        if (!full || i != j)
+            for (int b = 0; b < nc2; b++)
+                for (int a = 0; a < nc1; a++) {
+                    long idx = (long)(c1 + a) + (long)(c2 + b) * nc + (long)k * nc * nc;
+                    for (int q = 0; q < cplx; q++) cc[idx * cplx + q] = CUR(nv1 + a, nv2 + b, q);
+                }
+        if (i != j || full)
             for (int b = 0; b < nc1; b++)
                 for (int a = 0; a < nc2; a++) {
                     long idx = (long)(c2 + a) + (long)(c1 + b) * nc + (long)k * nc * nc;
@@ -1256,7 +1463,7 @@ int orc_integral(const orc_sys *s, int op, int term, const double *coef,
                     }
                 }
             }
-            save_pair(&c, i, j, cur, nt1, nt2, vv, cv, cc);
+            save_pair(&c, i, j, cur, nt1, nt2, vv, cv, cc, op == ORC_FORCE && !s->gamma);
         }
         free(cur); free(Y); free(X);
     }

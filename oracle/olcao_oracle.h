@@ -20,7 +20,7 @@ extern "C" {
 #endif
 
 enum { ORC_OL = 0, ORC_KE = 1, ORC_NP = 2, ORC_EP = 3, ORC_HAM = 4, ORC_MV = 5, ORC_DM = 6, ORC_MM = 7,
-       ORC_KO = 8 };
+       ORC_KO = 8, ORC_FORCE = 9 };
 enum { ORC_OK = 0, ORC_ERR_CELLS = 1, ORC_ERR_INPUT = 2 };
 
 typedef struct {
@@ -84,6 +84,15 @@ void orc_primk(double a1, double a2, const double *A, const double *B, const dou
                double *shr, double *shi);
 void orc_primk_batch(int n, const double *a, const double *A, const double *B, const double *dK,
                      const int *sw, double *shr, double *shi);
+/* Force-derivative primitives (computeForceIntg): op 10 d/dB kinetic
+ * (dkinetic2CIntg), 11 d/dB nuclear (dnuclear3CIntgCB), 12 d/dB
+ * three-Gaussian overlap (delectron3CIntgCB); sh: [3][16][16] column-major
+ * per component of B; a: [n][3].  Nonzero return: a nuclear block with
+ * la + lb = 6 (not available). */
+int orc_primd(int op, double a1, double a2, double a3, const double *A, const double *B,
+              const double *C, int sw, double *sh);
+int orc_primd_batch(int op, int n, const double *a, const double *A, const double *B,
+                    const double *C, const int *sw, double *sh);
 double orc_neglig_limit(const double *vec, double logThresh,
                         double minAtomicAlpha, double minPotAlpha);
 int orc_make_lattice(const double *vec, double negligLimit, int maxCells,
@@ -97,7 +106,10 @@ void orc_nearest_lattice_vector(const double *vec, const double *v,
  * potential coefficients [potDim] of one spin for ORC_HAM.  ORC_DM, ORC_MM:
  * Cartesian component `term` (0..2) of the dipole about coef[0..2] and of
  * the momentum (-i times the gradient).  ORC_MV is the
- * negated mass-velocity sweep (gaussOverlapMV).  nthreads <= 0: default. */
+ * negated mass-velocity sweep (gaussOverlapMV).  ORC_FORCE: component term
+ * of computeForceIntg's matrices for one spin, coef = [that spin's
+ * potential coefficients (potDim), totalEnergy]; complex builds save the
+ * full matrix (saveCurrentPair saveFull = 1).  nthreads <= 0: default. */
 int orc_integral(const orc_sys *s, int op, int term, const double *coef,
                  double *vv, double *cv, double *cc, int nthreads);
 

@@ -47,6 +47,25 @@ contains
       end select
    end subroutine ref_prim3
 
+   ! The force-derivative primitives of computeForceIntg: op 10
+   ! dkinetic2CIntg, op 11 dnuclear3CIntgCB, op 12 delectron3CIntgCB (a3 and
+   ! C unused by op 10).  sh is column-major (16,16,3).
+   subroutine ref_primd(op, a1, a2, a3, A, B, C, sw, sh) bind(C, name="ref_primd")
+      integer(c_int), value :: op, sw
+      real(c_double), value :: a1, a2, a3
+      real(c_double), intent(in) :: A(3), B(3), C(3)
+      real(c_double), intent(inout) :: sh(16,16,3)
+      sh(:,:,:) = 0.0d0
+      select case (op)
+      case (10)
+         call dkinetic2CIntg(a1, a2, A, B, sw, sh)
+      case (11)
+         call dnuclear3CIntgCB(a1, a2, a3, A, B, C, sw, sh)
+      case (12)
+         call delectron3CIntgCB(a1, a2, a3, A, B, C, sw, sh)
+      end select
+   end subroutine ref_primd
+
    ! KOverlap2CIntg: the overlap with exp(-i deltaK . r) between the two
    ! Gaussians; the complex (16,16) block split into its real and imaginary
    ! parts (column-major).

@@ -84,12 +84,15 @@ class _Brute:
                 first += len(p.alphas)
         return out
 
-    def matrix(self, kind, term=0):
-        """Full [nk, n, n] matrix of the sweep, all atom pairs (i, j)."""
+    def matrix(self, kind, term=0, comp=0):
+        """Full [nk, n, n] matrix of the sweep, all atom pairs (i, j).  The
+        ket-derivative kinds dke, dep, dnp take Cartesian component comp;
+        grad is the gradient block's sweep without the momentum's -i."""
         s, o = self.s, self.o
         nk = len(self.kc)
         ref = np.zeros((nk, self.n, self.n), complex)
-        terms = self._terms(kind, term) if kind in ("ep", "np") else []
+        tk = kind[1:] if kind in ("dep", "dnp") else kind
+        terms = self._terms(tk, term) if tk in ("ep", "np") else []
         for i in range(s.n_atoms):
             for j in range(s.n_atoms):
                 ti, tj = s.atom_type[i], s.atom_type[j]
@@ -106,7 +109,7 @@ class _Brute:
                         s2 = np.sum((A - B) ** 2, axis=1)
                         keep = xi * s2 <= self.lim
                         B, L, s2 = B[keep], self.Lpair[keep], s2[keep]
-                        if kind in ("ol", "ke", "dm", "mm"):
+                        if kind in ("ol", "ke", "dm", "mm", "dke", "grad"):
                             m = len(B)
                             ins["a"].append(np.tile([a1, a2, 0.0], (m, 1)))
                             ins["A"].append(np.tile(A, (m, 1)))
@@ -140,12 +143,15 @@ class _Brute:
                 if not ins["a"]:
                     continue
                 cat = {k: np.concatenate(v) for k, v in ins.items()}
-                if kind in ("dm", "mm"):
+                if kind in ("dm", "mm", "grad"):
                     from tests.test_oracle import _prim3
                     inp = np.concatenate([cat["a"][:, :2], cat["A"], cat["B"], cat["C"]], axis=1)
                     cls = np.zeros((len(inp), 3), np.int32)
                     cls[:, 2] = cat["sw"]
-                    sh = _prim3(5 if kind == "dm" else 6, inp, cls)[:, term]
+                    sh = _prim3(5 if kind == "dm" else 6, inp, cls)[:, comp if kind == "grad" else term]
+                elif kind in ("dke", "dep", "dnp"):
+                    op = {"dke": 10, "dnp": 11, "dep": 12}[kind]
+                    sh = oracle.primd_batch(op, cat["a"], cat["A"], cat["B"], cat["C"], cat["sw"])[:, comp]
                 else:
                     op = {"ol": 0, "ke": 1, "ep": 2, "np": 3}[kind]
                     sh = oracle.prim_batch(op, cat["a"], cat["A"], cat["B"], cat["C"], cat["sw"])
@@ -231,3 +237,34 @@ def test_dipole_and_momentum_sweeps_equal_unscreened_sums(tiny, kind, op, comp):
     if kind == "mm":
         herm = np.abs(ref - np.conj(ref.swapaxes(-1, -2))).max() / np.abs(ref).max()
         assert herm < 1e-11
+
+
+def _full_save(s, b, M):
+    """saveCurrentPair with saveFull = 1 (intgSaving.F90:513-545) applied to
+    a direct sum over all atom pairs: block (i, j), i < j, in place and its
+    dagger at (j, i); a diagonal block replaced by its own dagger."""
+    out = np.zeros_like(M)
+    off = b.off
+    for i in range(s.n_atoms):
+        for j in range(i, s.n_atoms):
+            blk = M[:, off[i]:off[i + 1], off[j]:off[j + 1]]
+            if i != j:
+                out[:, off[i]:off[i + 1], off[j]:off[j + 1]] = blk
+            out[:, off[j]:off[j + 1], off[i]:off[i + 1]] = np.conj(blk.swapaxes(-1, -2))
+    return out
+
+
+@pytest.mark.parametrize("comp", [0, 1, 2])
+def test_force_sweep_equals_unscreened_sum(tiny, comp):
+    """computeForceIntg (forces.F90:60-898): minus the potential terms'
+    three-Gaussian ket derivatives times the coefficients, plus Z times the
+    nuclear ones, minus the kinetic derivative, minus 2 totalEnergy times
+    the gradient block, summed directly over every image in twice the
+    reference's range, with the full save's placement."""
+    s, o, b = tiny
+    E = -3.7
+    F = o.force_integrals(E)[0][comp][0]
+    ref = -b.matrix("dnp", comp=comp) - b.matrix("dke", comp=comp) - 2 * E * b.matrix("grad", comp=comp)
+    for j, c in enumerate(s.pot_coeffs):
+        ref = ref - c * b.matrix("dep", j, comp=comp)
+    assert block_rel_err(F, _full_save(s, b, ref)) < 1e-11
