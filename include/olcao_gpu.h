@@ -222,6 +222,27 @@ olcao_status olcao_gauss_overlap_mm(olcao_ctx *ctx, double *vv, double *cv, doub
 olcao_status olcao_gauss_k_overlap(olcao_ctx *ctx, const int numAxial[3], double *vv, double *cv,
                                    double *cc);
 
+/* computeForceIntg (forces.F90:60-898): the force-derivative matrices for
+ * the potential coefficients coef [spin][potDim] and totalEnergy.  Per spin
+ * q and Cartesian component x (slot 3 q + x), the Bloch sums over the fused
+ * Hamiltonian sweep's screened images of the derivatives with respect to
+ * the ket centre B_x (the potential and nuclear centres held fixed) of
+ *   - sum_j coef_j (three-Gaussian block j)   (delectron3CIntgCB, :581)
+ *   + Z (Gaussian-damped nuclear block)        (dnuclear3CIntgCB, :611)
+ *   - kinetic block                            (dkinetic2CIntg, :648)
+ * plus -2 totalEnergy times the momentum block (momentum2CIntg, :715).
+ * vv [3 spin][nk][n][n], cv [3 spin][nk][nc][n], cc [3 spin][nk][nc][nc],
+ * complex column-major per k (real at Gamma), not orthogonalised.  A
+ * complex build saves each atom pair in full (saveCurrentPair saveFull = 1:
+ * the block and its dagger, a diagonal atom block replaced by its own
+ * dagger); at Gamma the upper triangle as the other sweeps
+ * (saveCurrentPairGamma).  Bases with f orbitals return
+ * OLCAO_ERR_UNSUPPORTED: the f-f nuclear derivative needs F_7, which the
+ * reference's Boys function does not give (its dnuclear3CIntgCB is off by
+ * about 1e-2 there). */
+olcao_status olcao_force_integrals(olcao_ctx *ctx, const double *coef, double totalEnergy,
+                                   double *vv, double *cv, double *cc);
+
 /* Fused Hamiltonian sweep with the potential coefficients folded in
  * (gaussOverlapHamPSCF, integrals.F90:2860): KE (- MV when rel = 1)
  * + sum_j c_j V_j - sum Z N, orthogonalised when coreDim > 0

@@ -108,4 +108,77 @@ static __device__ __forceinline__ void store_staged_blocks(const double *sOut,
     }
 }
 
+
+// Origin shift and scatter of one output element (k, s1, s2) of atom pair
+// (i, j) (kPointLatticeOriginShift and saveCurrentPair, intgSaving.F90:
+// 241-292, :346-459).  mode bit 0: the Bloch sum times -i (the momentum
+// matrix, applyPhaseFactors runCode 1; at Gamma the kept part is -X).
+// mode bit 1: the full save (saveFull = 1, valeValeSavingFull and
+// coreCoreSavingFull): the block in place and its dagger at the mirrored
+// place, the dagger last, so a diagonal atom block ends as its own dagger;
+// otherwise the upper triangle with the strictly lower part of a diagonal
+// block zeroed.  coreVale is the same in both.
+__device__ __forceinline__ void bloch_store(const DevCtx &c, int i, int j, int k, int s1, int s2,
+                                            int nv1, int nv2, int v1, int v2, int c1, int c2,
+                                            const double *lv, double vre, double vim, int gamma,
+                                            int mode, double *vv, double *cv, double *cc)
+{
+    const long long n = c.valeDim, nc = c.coreDim;
+    const bool full = mode & 2;
+    if (mode & 1) {
+        const double t = vre;
+        vre = gamma ? -t : vim;
+        vim = -t;
+    }
+    if (!gamma) {
+        const double *kp = c.kcart + 3 * k;
+        const double dot = kp[0] * lv[0] + kp[1] * lv[1] + kp[2] * lv[2];
+        if (fabs(dot) >= kSmallThresh) {
+            const double cr = cos(dot), ci = sin(dot);
+            const double r2 = cr * vre - ci * vim;
+            vim = cr * vim + ci * vre;
+            vre = r2;
+        }
+    }
+    const int cp = gamma ? 1 : 2;
+    const bool a_core = s1 >= nv1, b_core = s2 >= nv2;
+    if (!a_core && !b_core) {
+        const long long idx = (v1 + s1) + (long long)(v2 + s2) * n + (long long)k * n * n;
+        if (full) {
+            const long long idt = (v2 + s2) + (long long)(v1 + s1) * n + (long long)k * n * n;
+            vv[idt * cp] = vre;
+            if (!gamma) vv[idt * cp + 1] = -vim;
+            if (i != j) {
+                vv[idx * cp] = vre;
+                if (!gamma) vv[idx * cp + 1] = vim;
+            }
+        } else {
+            const bool zero = (i == j) && (s1 > s2);
+            vv[idx * cp] = zero ? 0.0 : vre;
+            if (!gamma) vv[idx * cp + 1] = zero ? 0.0 : vim;
+        }
+    } else if (a_core && !b_core) {
+        if (i != j) {   // for i == j the dagger assignment below wins
+            const long long idx = (c1 + s1 - nv1) + (long long)(v2 + s2) * nc + (long long)k * nc * n;
+            cv[idx * cp] = vre;
+            if (!gamma) cv[idx * cp + 1] = vim;
+        }
+    } else if (!a_core && b_core) {
+        const long long idx = (c2 + s2 - nv2) + (long long)(v1 + s1) * nc + (long long)k * nc * n;
+        cv[idx * cp] = vre;
+        if (!gamma) cv[idx * cp + 1] = -vim;
+    } else {
+        if (!full || i != j) {
+            const long long idx = (c1 + s1 - nv1) + (long long)(c2 + s2 - nv2) * nc + (long long)k * nc * nc;
+            cc[idx * cp] = vre;
+            if (!gamma) cc[idx * cp + 1] = vim;
+        }
+        if (full || i != j) {
+            const long long id2 = (c2 + s2 - nv2) + (long long)(c1 + s1 - nv1) * nc + (long long)k * nc * nc;
+            cc[id2 * cp] = vre;
+            if (!gamma) cc[id2 * cp + 1] = -vim;
+        }
+    }
+}
+
 }  // namespace olcao

@@ -37,7 +37,8 @@ namespace olcao {
 // chunks); the chunk's Hermite sums go to W[e][chunk].  The workgroup of NT
 // threads (64 or 256) takes the chunks i0 .. i0 + NT - 1; its waves share
 // the staged sites and tables, the term table has a column per thread.
-template <int LA, int LB, bool NUC, bool TAB, bool MIX, int NT = 64>
+// DL = 1: the Hermite sums one order up (the force sweep's ket derivatives).
+template <int LA, int LB, bool NUC, bool TAB, bool MIX, int NT = 64, int DL = 0>
 __device__ __forceinline__ void three_center_wave(const DevCtx &c, const PPRec *recs, int n, int i0,
                                                   const int *chRec, const long long *chBeg,
                                                   const long long *chEnd,
@@ -45,7 +46,7 @@ __device__ __forceinline__ void three_center_wave(const DevCtx &c, const PPRec *
                                                   int termType, int termIdx, const double *coef,
                                                   int addW, double *W)
 {
-    constexpr int L = LA + LB, NH = nherm(L);
+    constexpr int L = LA + LB + DL, NH = nherm(L);
     __shared__ double tAlpha[kMaxPotTypes * kMaxPotAlphas];
     __shared__ double tCoef[kMaxPotTypes * kMaxPotAlphas];
     __shared__ double tZ[kMaxPotTypes], tNuc[kMaxPotTypes];
@@ -316,6 +317,91 @@ __global__ __launch_bounds__(64) void k_herm_finish_multi(const Cell3c *cells, c
                              a.nChunk, a.finAcc, a.finKE, a.blocks);
 }
 
+// ---------------------------------------------------------------------------
+// Force integrals (computeForceIntg, forces.F90:60-898): the ket derivative
+// of the fused sweep's three-centre terms.  The Hermite sums of the spin's
+// coefficients and the nuclei, one order up, then per Cartesian component X
+// K_AB sum_tuv D E E W (the derivative table along X, herm_dket), negated
+// (the force takes minus the potential terms' and plus the nuclear terms'
+// derivatives, the fused sweep's signs reversed) and added onto the store,
+// which holds the two-centre part (k_two_center_fd).
+template <int LA, int LB, bool NUC, bool TAB>
+__global__ __launch_bounds__(64) void k_three_center_d(DevCtx c, const PPRec *recs, int n,
+                                                       const int *chRec, const long long *chBeg,
+                                                       const long long *chEnd,
+                                                       const unsigned long long *img,
+                                                       const double *coef, int addW, double *W)
+{
+    three_center_wave<LA, LB, NUC, TAB, false, 64, 1>(c, recs, n, (int)blockIdx.x * 64, chRec,
+                                                      chBeg, chEnd, img, K3_HAM, 0, 0, coef, addW,
+                                                      W);
+}
+
+template <int LA, int LB, int X>
+__global__ __launch_bounds__(64) void k_herm_dfinish(const PPRec *recs, const double *W, int nrec,
+                                                     const int *finOrder, const int *rcOff,
+                                                     const int *rcList, int nChunk, double *blocks)
+{
+    constexpr int L = LA + LB, NH = nherm(L + 1);
+    constexpr int NA = ncart(LA), SA = nsh(LA), SB = nsh(LB);
+    constexpr bool STAGE = SA * SB <= kFinishStage;
+    __shared__ double sOut[STAGE ? 64 * SA * SB + 1 : 1];
+    __shared__ long long sOff[64];
+    const int lane = threadIdx.x;
+    const int li = (int)blockIdx.x * 64 + lane;
+    const bool active = li < nrec;
+    const int i = finOrder[active ? li : nrec - 1];
+    const PPRec rec = recs[i];
+    const PairConst q = make_pair(rec.a1, rec.a2, rec.A, rec.B);
+    double Wl[NH];
+#pragma unroll
+    for (int e = 0; e < NH; ++e) Wl[e] = 0.0;
+    if (active)
+        for (int r = rcOff[i]; r < rcOff[i + 1]; ++r) {
+            const int col = rcList[r];
+#pragma unroll
+            for (int e = 0; e < NH; ++e) Wl[e] += W[(long long)e * nChunk + col];
+        }
+    double E[3][LA + 1][LB + 1][L + 1], D[LA + 1][LB + 1][L + 2];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) herm_e<LA, LB>(q.P[x] - q.A[x], q.P[x] - q.B[x], 0.5 / q.zeta, E[x]);
+    herm_dket<LA, LB>(E[X], q.P[X] - q.B[X], q.a1, q.a2, q.zeta, D);
+    const double kab = -exp(-q.xid2);
+    double *o = blocks + rec.off;
+    static_for<SB>([&](auto j_) __attribute__((always_inline)) {
+        constexpr int j = decltype(j_)::value;
+        double col[NA];
+        static_for<NA>([&](auto a_) __attribute__((always_inline)) {
+            constexpr int a = decltype(a_)::value;
+            double v = 0.0;
+            static_for<sh_nterm(j)>([&](auto t_) __attribute__((always_inline)) {
+                constexpr int tj = decltype(t_)::value;
+                constexpr int b = sh_cart(j, tj);
+                v += sh_weight(j, tj) *
+                     herm_dcontract_ct<LA, LB, X, cexp(a, 0), cexp(a, 1), cexp(a, 2), cexp(b, 0),
+                                       cexp(b, 1), cexp(b, 2)>(E, D, Wl);
+            });
+            col[a] = v;
+        });
+#pragma unroll
+        for (int ii = 0; ii < SA; ++ii) {
+            double s = 0.0;
+#pragma unroll
+            for (int ti = 0; ti < sh_nterm(ii); ++ti) s += sh_weight(ii, ti) * col[sh_cart(ii, ti)];
+            s *= kab;
+            if (STAGE)
+                sOut[lane * SA * SB + ii * SB + j] = s;
+            else if (active)
+                o[ii * SB + j] += s;
+        }
+    });
+    if (STAGE) {
+        sOff[lane] = active ? rec.off : -1;
+        __syncthreads();
+        store_staged_blocks<SA * SB>(sOut, sOff, min(64, nrec - (int)blockIdx.x * 64), 1, blocks);
+    }
+}
+
 static size_t site_bytes(const DevCtx &c) { return k3c_site_bytes(c); }
 static size_t table_bytes(const DevCtx &c) { return k3c_table_bytes(c); }
 
@@ -388,6 +474,48 @@ static void three_one(const DevCtx &c, const ClassLists &cl, int kind, int tt, i
                        addKE ? 0 : (accumulate & 1), addKE ? 1 : 0, blocks);
 }
 
+// Force sweep of class (LA, LB): comp < 0 the Hermite sums of coef and the
+// nuclei into cl.Wd (one order up); comp = X the finish of component X
+// added onto the store.
+template <int LA, int LB>
+static void three_force(const DevCtx &c, const ClassLists &cl, const double *coef, int comp,
+                        double *blocks, hipStream_t s)
+{
+    const int k = LA * 4 + LB;
+    const int n = cl.count[k];
+    if (n == 0) return;
+    double *W = cl.Wd[k];
+    const int nch = cl.nChunk[k];
+    const dim3 grid((nch + 63) / 64), gridRec((n + 63) / 64);
+    if (comp < 0) {
+        const size_t dyn = site_bytes(c) + 16;
+        const bool tab = table_bytes(c) <= kTableLdsMax;
+        const size_t dynTab = site_bytes(c) + table_bytes(c) + 16;
+        if (tab)
+            hipLaunchKernelGGL((k_three_center_d<LA, LB, false, true>), grid, dim3(64), dynTab, s,
+                               c, cl.rec[k], nch, cl.chRec[k], cl.chBeg[k], cl.chEnd[k], cl.img[k],
+                               coef, 0, W);
+        else
+            hipLaunchKernelGGL((k_three_center_d<LA, LB, false, false>), grid, dim3(64), dyn, s, c,
+                               cl.rec[k], nch, cl.chRec[k], cl.chBeg[k], cl.chEnd[k], cl.img[k],
+                               coef, 0, W);
+        if (c.anyNuc)
+            hipLaunchKernelGGL((k_three_center_d<LA, LB, true, false>), grid, dim3(64), dyn, s, c,
+                               cl.rec[k], nch, cl.chRec[k], cl.nOff[k], cl.nOff[k] + 1, cl.imgN[k],
+                               coef, 1, W);
+        return;
+    }
+    if (comp == 0)
+        hipLaunchKernelGGL((k_herm_dfinish<LA, LB, 0>), gridRec, dim3(64), 0, s, cl.rec[k], W, n,
+                           cl.finOrder[k], cl.rcOff[k], cl.rcList[k], nch, blocks);
+    else if (comp == 1)
+        hipLaunchKernelGGL((k_herm_dfinish<LA, LB, 1>), gridRec, dim3(64), 0, s, cl.rec[k], W, n,
+                           cl.finOrder[k], cl.rcOff[k], cl.rcList[k], nch, blocks);
+    else
+        hipLaunchKernelGGL((k_herm_dfinish<LA, LB, 2>), gridRec, dim3(64), 0, s, cl.rec[k], W, n,
+                           cl.finOrder[k], cl.rcOff[k], cl.rcList[k], nch, blocks);
+}
+
 // The fused Hamiltonian sweep of class (LA, LB) over the cells of a batch:
 // the three-Gaussian pass per kernel form, the nuclear pass, the finish.
 template <int LA, int LB>
@@ -426,6 +554,22 @@ void OLCAO_CAT(launch_3c_la, OLCAO_LA)(const DevCtx &c, const ClassLists &cl, in
     if (mask >> (k0 + 1) & 1u) three_one<OLCAO_LA, 1>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
     if (mask >> (k0 + 2) & 1u) three_one<OLCAO_LA, 2>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
     if (mask >> (k0 + 3) & 1u) three_one<OLCAO_LA, 3>(c, cl, kind, tt, ti, coef, accumulate, blocks, mixed, s);
+}
+
+// The force sweep's classes (OLCAO_LA, lb <= 2) in mask; bases with f
+// orbitals are refused before this is reached.
+void OLCAO_CAT(launch_3c_force_la, OLCAO_LA)(const DevCtx &c, const ClassLists &cl,
+                                              const double *coef, int comp, double *blocks,
+                                              unsigned mask, hipStream_t s)
+{
+    constexpr int k0 = 4 * OLCAO_LA;
+    if constexpr (OLCAO_LA <= 2) {
+        if (mask >> (k0 + 0) & 1u) three_force<OLCAO_LA, 0>(c, cl, coef, comp, blocks, s);
+        if (mask >> (k0 + 1) & 1u) three_force<OLCAO_LA, 1>(c, cl, coef, comp, blocks, s);
+        if (mask >> (k0 + 2) & 1u) three_force<OLCAO_LA, 2>(c, cl, coef, comp, blocks, s);
+    } else {
+        (void)c; (void)cl; (void)coef; (void)comp; (void)blocks; (void)mask; (void)s;
+    }
 }
 
 // The batched class m.k = 4 OLCAO_LA + lb.

@@ -40,57 +40,6 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-// origin shift and scatter of one output element (k, s1, s2) of pair pr
-__device__ __forceinline__ void bloch_store(const DevCtx &c, int i, int j, int k, int s1, int s2,
-                                            int nv1, int nv2, int v1, int v2, int c1, int c2,
-                                            const double *lv, double vre, double vim, int gamma,
-                                            int negI, double *vv, double *cv, double *cc)
-{
-    const long long n = c.valeDim, nc = c.coreDim;
-    if (negI) {   // times -i (at Gamma the kept part is -X)
-        const double t = vre;
-        vre = gamma ? -t : vim;
-        vim = -t;
-    }
-    if (!gamma) {
-        const double *kp = c.kcart + 3 * k;
-        const double dot = kp[0] * lv[0] + kp[1] * lv[1] + kp[2] * lv[2];
-        if (fabs(dot) >= kSmallThresh) {
-            const double cr = cos(dot), ci = sin(dot);
-            const double r2 = cr * vre - ci * vim;
-            vim = cr * vim + ci * vre;
-            vre = r2;
-        }
-    }
-    const int cp = gamma ? 1 : 2;
-    const bool a_core = s1 >= nv1, b_core = s2 >= nv2;
-    if (!a_core && !b_core) {
-        const long long idx = (v1 + s1) + (long long)(v2 + s2) * n + (long long)k * n * n;
-        const bool zero = (i == j) && (s1 > s2);
-        vv[idx * cp] = zero ? 0.0 : vre;
-        if (!gamma) vv[idx * cp + 1] = zero ? 0.0 : vim;
-    } else if (a_core && !b_core) {
-        if (i != j) {   // for i == j the dagger assignment below wins
-            const long long idx = (c1 + s1 - nv1) + (long long)(v2 + s2) * nc + (long long)k * nc * n;
-            cv[idx * cp] = vre;
-            if (!gamma) cv[idx * cp + 1] = vim;
-        }
-    } else if (!a_core && b_core) {
-        const long long idx = (c2 + s2 - nv2) + (long long)(v1 + s1) * nc + (long long)k * nc * n;
-        cv[idx * cp] = vre;
-        if (!gamma) cv[idx * cp + 1] = -vim;
-    } else {
-        const long long idx = (c1 + s1 - nv1) + (long long)(c2 + s2 - nv2) * nc + (long long)k * nc * nc;
-        cc[idx * cp] = vre;
-        if (!gamma) cc[idx * cp + 1] = vim;
-        if (i != j) {
-            const long long id2 = (c2 + s2 - nv2) + (long long)(c1 + s1 - nv1) * nc + (long long)k * nc * nc;
-            cc[id2 * cp] = vre;
-            if (!gamma) cc[id2 * cp + 1] = -vim;
-        }
-    }
-}
-
 constexpr int kFlushChunks = 4;   // 4 chunks x 32 cells x 6 products x 2^14 < 2^24
 static_assert(kFlushChunks * 32 * kBlochSlices * (1 << 14) < (1 << 24),
               "an order's FP32 accumulator must stay exact between flushes");
@@ -105,7 +54,7 @@ __device__ __forceinline__ unsigned short bf16_bits(float v)
 template <bool SPLIT>
 __global__ __launch_bounds__(64) void k_bloch_mfma(DevCtx c, const double *X,
                                                    const unsigned short *pslice, double *vv,
-                                                   double *cv, double *cc, int gamma, int negI,
+                                                   double *cv, double *cc, int gamma, int mode,
                                                    int nTiles)
 {
     const int lane = threadIdx.x;
@@ -244,7 +193,7 @@ __global__ __launch_bounds__(64) void k_bloch_mfma(DevCtx c, const double *X,
             // result rows: FP64 MFMA 4 r + grp, BF16 MFMA 4 grp + r
             const int k = k0 + (SPLIT ? 4 * grp + r : 4 * r + grp);
             if (k >= nk) continue;
-            bloch_store(c, i, j, k, s1, s2, nv1, nv2, v1, v2, c1, c2, lv, re[r], im[r], gamma, negI,
+            bloch_store(c, i, j, k, s1, s2, nv1, nv2, v1, v2, c1, c2, lv, re[r], im[r], gamma, mode,
                         vv, cv, cc);
         }
     }
@@ -253,17 +202,17 @@ __global__ __launch_bounds__(64) void k_bloch_mfma(DevCtx c, const double *X,
 }  // namespace
 
 void launch_bloch_mfma(const DevCtx &c, const double *X, const unsigned short *pslice, double *vv,
-                       double *cv, double *cc, int gamma, int negI, bool split, hipStream_t s)
+                       double *cv, double *cc, int gamma, int mode, bool split, hipStream_t s)
 {
     if (c.nPairs == 0) return;
     const int nk = gamma ? 1 : c.nk;
     const int nTiles = (nk + 15) / 16;
     if (split)
         hipLaunchKernelGGL(k_bloch_mfma<true>, dim3(c.nPairs * nTiles), dim3(64), 0, s, c, X, pslice,
-                           vv, cv, cc, gamma, negI, nTiles);
+                           vv, cv, cc, gamma, mode, nTiles);
     else
         hipLaunchKernelGGL(k_bloch_mfma<false>, dim3(c.nPairs * nTiles), dim3(64), 0, s, c, X, pslice,
-                           vv, cv, cc, gamma, negI, nTiles);
+                           vv, cv, cc, gamma, mode, nTiles);
 }
 
 void phase_slices_host(const std::vector<double> &phase, int nCells, int nk, int gamma,

@@ -80,6 +80,38 @@ __global__ __launch_bounds__(64) void k_two_center(DevCtx c, const int *list, in
     }
 }
 
+// The force integrals' two-centre part along component comp (classes up to
+// d-d): minus the ket derivative of the kinetic block plus c.fE2 (2
+// totalEnergy) times that of the overlap block, i.e. -2 totalEnergy times
+// the momentum block (computeForceIntg, forces.F90:648-730).  Its own kernel
+// so that k_two_center, on every iteration's path, stays as it is.
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void k_two_center_fd(DevCtx c, const int *list, int n, int comp,
+                                                      double *blocks)
+{
+    constexpr int SA = nsh(LA), SB = nsh(LB);
+    __shared__ double sOut[64 * SA * SB + 1];
+    __shared__ long long sOff[64];
+    const int lane = threadIdx.x;
+    const int t = blockIdx.x * 64 + lane;
+    const bool active = t < n;
+    const int pp = list[active ? t : n - 1];
+    const PPInfo p = load_pp(c, pp);
+    const PairConst q = make_pair(c.alpha[p.ti * c.maxA + p.ia],
+                                  c.alpha[p.tj * c.maxA + p.ib], p.A, p.B);
+    double pc[ncart(LA)][ncart(LB)];
+    dket_cart<LA, LB>(q, comp, -1.0, c.fE2, pc);
+    double sh[SA][SB];
+    cart_to_sh<LA, LB>(pc, sh);
+#pragma unroll
+    for (int i = 0; i < SA; ++i)
+#pragma unroll
+        for (int j = 0; j < SB; ++j) sOut[lane * SA * SB + i * SB + j] = sh[i][j];
+    sOff[lane] = c.ppOff[pp];
+    __syncthreads();
+    store_staged_blocks<SA * SB>(sOut, sOff, min(64, n - (int)blockIdx.x * 64), 0, blocks);
+}
+
 // ---------------------------------------------------------------------------
 // contraction: X(s1,s2) = sum over the pair-cell's primitive pairs of
 //   C1(a1,s1) * block(lm(s1), lm(s2)) * C2(a2,s2)
@@ -195,7 +227,7 @@ constexpr int kBlochCells = 16;
 constexpr int kBlochOut = 8;
 
 __global__ __launch_bounds__(64) void k_bloch(DevCtx c, const double *X, double *vv,
-                                              double *cv, double *cc, int gamma, int negI,
+                                              double *cv, double *cc, int gamma, int mode,
                                               int KT, int nTiles)
 {
     extern __shared__ double lds[];
@@ -264,54 +296,8 @@ __global__ __launch_bounds__(64) void k_bloch(DevCtx c, const double *X, double 
             if (o >= nout) continue;
             const int kk = o / per, e = o - kk * per;
             const int s2 = e / nt1, s1 = e - s2 * nt1;
-            const int k = k0 + kk;
-            double vre = re[r], vim = im[r];
-            if (negI) {   // times -i (at Gamma the kept part is -X)
-                const double t = vre;
-                vre = gamma ? -t : vim;
-                vim = -t;
-            }
-            if (!gamma) {
-                const double *kp = c.kcart + 3 * k;
-                const double dot = kp[0] * lv[0] + kp[1] * lv[1] + kp[2] * lv[2];
-                if (fabs(dot) >= kSmallThresh) {
-                    const double cr = cos(dot), ci = sin(dot);
-                    const double r2 = cr * vre - ci * vim;
-                    vim = cr * vim + ci * vre;
-                    vre = r2;
-                }
-            }
-            const int cp = gamma ? 1 : 2;
-            const bool a_core = s1 >= nv1, b_core = s2 >= nv2;
-            if (!a_core && !b_core) {
-                const long long idx = (v1 + s1) + (long long)(v2 + s2) * n + (long long)k * n * n;
-                const bool zero = (i == j) && (s1 > s2);
-                vv[idx * cp] = zero ? 0.0 : vre;
-                if (!gamma) vv[idx * cp + 1] = zero ? 0.0 : vim;
-            } else if (a_core && !b_core) {
-                if (i != j) {   // for i == j the dagger assignment below wins
-                    const long long idx = (c1 + s1 - nv1) + (long long)(v2 + s2) * nc +
-                                          (long long)k * nc * n;
-                    cv[idx * cp] = vre;
-                    if (!gamma) cv[idx * cp + 1] = vim;
-                }
-            } else if (!a_core && b_core) {
-                const long long idx = (c2 + s2 - nv2) + (long long)(v1 + s1) * nc +
-                                      (long long)k * nc * n;
-                cv[idx * cp] = vre;
-                if (!gamma) cv[idx * cp + 1] = -vim;
-            } else {
-                const long long idx = (c1 + s1 - nv1) + (long long)(c2 + s2 - nv2) * nc +
-                                      (long long)k * nc * nc;
-                cc[idx * cp] = vre;
-                if (!gamma) cc[idx * cp + 1] = vim;
-                if (i != j) {
-                    const long long id2 = (c2 + s2 - nv2) + (long long)(c1 + s1 - nv1) * nc +
-                                          (long long)k * nc * nc;
-                    cc[id2 * cp] = vre;
-                    if (!gamma) cc[id2 * cp + 1] = -vim;
-                }
-            }
+            bloch_store(c, i, j, k0 + kk, s1, s2, nv1, nv2, v1, v2, c1, c2, lv, re[r], im[r], gamma,
+                        mode, vv, cv, cc);
         }
     }
 }
@@ -420,6 +406,20 @@ void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int acc, d
 #undef OLCAO_TWO
 }
 
+void launch_two_center_fd(const DevCtx &c, const ClassLists &cl, int comp, double *blocks,
+                          unsigned mask, hipStream_t s)
+{
+#define OLCAO_FD(la, lb)                                                                       \
+    if ((mask >> (4 * la + lb) & 1u) && cl.count[4 * la + lb] > 0)                             \
+        hipLaunchKernelGGL((k_two_center_fd<la, lb>), dim3((cl.count[4 * la + lb] + 63) / 64), \
+                           dim3(64), 0, s, c, cl.idx[4 * la + lb], cl.count[4 * la + lb], comp,  \
+                           blocks);
+    OLCAO_FD(0, 0) OLCAO_FD(0, 1) OLCAO_FD(0, 2)
+    OLCAO_FD(1, 0) OLCAO_FD(1, 1) OLCAO_FD(1, 2)
+    OLCAO_FD(2, 0) OLCAO_FD(2, 1) OLCAO_FD(2, 2)
+#undef OLCAO_FD
+}
+
 void launch_3c_la0(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
                    double *, int, unsigned, hipStream_t);
 void launch_3c_la1(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
@@ -428,6 +428,21 @@ void launch_3c_la2(const DevCtx &, const ClassLists &, int, int, int, const doub
                    double *, int, unsigned, hipStream_t);
 void launch_3c_la3(const DevCtx &, const ClassLists &, int, int, int, const double *, int,
                    double *, int, unsigned, hipStream_t);
+
+void launch_3c_force_la0(const DevCtx &, const ClassLists &, const double *, int, double *,
+                         unsigned, hipStream_t);
+void launch_3c_force_la1(const DevCtx &, const ClassLists &, const double *, int, double *,
+                         unsigned, hipStream_t);
+void launch_3c_force_la2(const DevCtx &, const ClassLists &, const double *, int, double *,
+                         unsigned, hipStream_t);
+
+void launch_three_center_force(const DevCtx &c, const ClassLists &cl, const double *coef, int comp,
+                               double *blocks, unsigned mask, hipStream_t s)
+{
+    if (mask & 0x000fu) launch_3c_force_la0(c, cl, coef, comp, blocks, mask, s);
+    if (mask & 0x00f0u) launch_3c_force_la1(c, cl, coef, comp, blocks, mask, s);
+    if (mask & 0x0f00u) launch_3c_force_la2(c, cl, coef, comp, blocks, mask, s);
+}
 
 void launch_3c_multi_la0(const Multi3c &, hipStream_t);
 void launch_3c_multi_la1(const Multi3c &, hipStream_t);
@@ -636,7 +651,7 @@ void launch_contract(const DevCtx &c, const ContractPlan &cp, const double *bloc
 }
 
 void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv, double *cv,
-                          double *cc, int gamma, int negI, hipStream_t s)
+                          double *cc, int gamma, int mode, hipStream_t s)
 {
     if (c.nPairs == 0) return;
     // k tile: about 512 outputs per wavefront, and at least ~1024 wavefronts
@@ -647,7 +662,7 @@ void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv, double *
     const int nTiles = (nk + KT - 1) / KT;
     const size_t lds = ((size_t)kBlochCells * per + 2 * (size_t)kBlochCells * KT) * sizeof(double);
     hipLaunchKernelGGL(k_bloch, dim3(c.nPairs * nTiles), dim3(64), lds, s, c, X, vv, cv, cc,
-                       gamma, negI, KT, nTiles);
+                       gamma, mode, KT, nTiles);
 }
 
 void launch_zero_words(void *p, long long words, hipStream_t s)

@@ -81,6 +81,7 @@ struct DevCtx {
     int anyNuc;                 // some potential type has Z >= smallThresh
     double dipC[3];             // dipole centre of the OP_DM sweeps
     double kq[3];               // deltaK of the OP_KO sweeps
+    double fE2;                 // 2 totalEnergy of the force sweeps
 };
 
 inline __host__ __device__ int pp_a1(int ab) { return ab & 255; }
@@ -139,6 +140,7 @@ struct ClassLists {
     const long long *ioff[16];  // [count + 1] image-list offsets per record
     const unsigned long long *img[16];   // image entries
     double *W[16];              // Hermite sums [nherm(la+lb)][nChunk] per class
+    double *Wd[16];             // the force sweep's, one order up [nherm(la+lb+1)][nChunk]
     int count[16];
     // chunks, heaviest first: record, entry range
     const int *chRec[16];
@@ -175,6 +177,17 @@ constexpr unsigned kFusedKEClasses = 0x37ffu;
 void launch_three_center(const DevCtx &c, const ClassLists &cl, int kind, int termType,
                          int termIdx, const double *coef, int accumulate, double *blocks,
                          int mixed, unsigned mask, hipStream_t s);
+// The force integrals' two-centre part along comp (classes la, lb <= 2 in
+// mask): minus the ket derivative of the kinetic block plus c.fE2 times that
+// of the overlap block, written to the primitive-pair store.
+void launch_two_center_fd(const DevCtx &c, const ClassLists &cl, int comp, double *blocks,
+                          unsigned mask, hipStream_t s);
+// The force sweep's three-centre part (classes la, lb <= 2 in mask): comp < 0
+// the Hermite sums of coef (one spin's coefficients) and the nuclei, one
+// order up, into cl.Wd; comp = 0..2 minus the ket derivative along comp
+// added onto the primitive-pair store.
+void launch_three_center_force(const DevCtx &c, const ClassLists &cl, const double *coef, int comp,
+                               double *blocks, unsigned mask, hipStream_t s);
 // Dynamic LDS of the three-centre kernels: site positions and types, then
 // (the tabulated form, used when the table fits kTableLdsMax) the per-lane
 // term table of 1 / (zeta + a3), per 64-lane wave.
@@ -262,10 +275,12 @@ struct ContractPlan {
 };
 void launch_contract(const DevCtx &c, const ContractPlan &cp, const double *blocks, double *X,
                      hipStream_t s);
-// negI: multiply the Bloch sum by -i (the momentum matrix, applyPhaseFactors
-// runCode 1, intgSaving.F90:178-187; -X at Gamma, :727-731).
+// mode bit 0: multiply the Bloch sum by -i (the momentum matrix,
+// applyPhaseFactors runCode 1, intgSaving.F90:178-187; -X at Gamma,
+// :727-731); bit 1: the full save (saveCurrentPair saveFull = 1, the force
+// matrices), see bloch_store.
 void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv,
-                          double *cv, double *cc, int gamma, int negI, hipStream_t s);
+                          double *cv, double *cc, int gamma, int mode, hipStream_t s);
 // Bloch sum, origin shift and scatter on the matrix cores (kbloch.hip):
 // FP64 MFMA, or (split) exact BF16 slice products with FP32 accumulation
 // over kBlochSlices fixed-point slices per operand.  pslice: the phase
@@ -273,7 +288,7 @@ void launch_bloch_scatter(const DevCtx &c, const double *X, double *vv,
 // phase_slices_host.
 constexpr int kBlochSlices = 6;
 void launch_bloch_mfma(const DevCtx &c, const double *X, const unsigned short *pslice, double *vv,
-                       double *cv, double *cc, int gamma, int negI, bool split, hipStream_t s);
+                       double *cv, double *cc, int gamma, int mode, bool split, hipStream_t s);
 void phase_slices_host(const std::vector<double> &phase, int nCells, int nk, int gamma,
                        std::vector<unsigned short> &out);
 void launch_fill(double *p, long long n, double v, hipStream_t s);

@@ -461,6 +461,75 @@ OHD void vec_cart(const PairConst &q, int comp, const double *C,
         }
 }
 
+// Ket-centre derivative d/dB_X of the kinetic and overlap blocks (the
+// force integrals' dkinetic2CIntg, gaussIntegrals.f90:65512, and the
+// momentum term of computeForceIntg).  With g_j = (x-B)^j e^{-b (x-B)^2},
+// d/dB g_j = 2 b g_{j+1} - j g_{j-1}; the ket step of the 1-D recursions
+// turns 2 b E(i,j+1) - j E(i,j-1) into terms of the (i, j) tables, free of
+// the cancellation between the two when b >> a:
+//   dE(i,j) = 2 b XPB E(i,j) + (b/zeta) i E(i-1,j) - (a/zeta) j E(i,j-1)
+//   dT(i,j) = 2 b XPB T(i,j) + (b/zeta) i T(i-1,j) - (a/zeta) j T(i,j-1)
+//             + 2 xi dE(i,j)
+// pc = cKE d/dB_X <a| -nabla^2/2 |b> + cS d/dB_X <a|b>.
+template <int LA, int LB>
+OHD void dket_cart(const PairConst &q, int X, double cKE, double cS,
+                   double (&pc)[ncart(LA)][ncart(LB)])
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB);
+    const double t = kPi / q.zeta;
+    const double pref = t * sqrt(t) * exp(-q.xid2);
+    const double h = 1.0 / (2.0 * q.zeta);
+    const double bz = q.a2 / q.zeta, az = q.a1 / q.zeta;
+    double E[3][LA + 1][LB + 1], Tk[3][LA + 1][LB + 1];
+    double dE[3][LA + 1][LB + 1], dT[3][LA + 1][LB + 1];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        const double xpb = q.P[x] - q.B[x];
+        os1d<LA, LB>(q.P[x] - q.A[x], xpb, h, E[x]);
+        ke1d<LA, LB>(q.P[x] - q.A[x], xpb, h, q.xi, q.A[x] - q.B[x], q.a1, q.a2, E[x], Tk[x]);
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+            for (int j = 0; j <= LB; ++j) {
+                double de = 2.0 * q.a2 * xpb * E[x][i][j];
+                double dt = 2.0 * q.a2 * xpb * Tk[x][i][j];
+                if (i > 0) {
+                    de += bz * i * E[x][i > 0 ? i - 1 : 0][j];
+                    dt += bz * i * Tk[x][i > 0 ? i - 1 : 0][j];
+                }
+                if (j > 0) {
+                    de -= az * j * E[x][i][j > 0 ? j - 1 : 0];
+                    dt -= az * j * Tk[x][i][j > 0 ? j - 1 : 0];
+                }
+                dE[x][i][j] = de;
+                dT[x][i][j] = dt + 2.0 * q.xi * de;
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            double s = 1.0, k = 0.0;
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                const int i = cexp(a, x), j = cexp(b, x);
+                s *= x == X ? dE[x][i][j] : E[x][i][j];
+            }
+#pragma unroll
+            for (int y = 0; y < 3; ++y) {
+                double p = 1.0;
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    const int i = cexp(a, x), j = cexp(b, x);
+                    p *= x == y ? (x == X ? dT[x][i][j] : Tk[x][i][j])
+                                : (x == X ? dE[x][i][j] : E[x][i][j]);
+                }
+                k += p;
+            }
+            pc[a][b] = pref * (cKE * k + cS * s);
+        }
+}
+
 // ---------------------------------------------------------------------
 // Three-centre integrals in McMurchie-Davidson form about the pair centre P.
 //
@@ -820,6 +889,61 @@ OHD double herm_contract_ct(const double (&E)[3][LA + 1][LB + 1][LA + LB + 1],
             static_for<AZ + BZ + 1>([&](auto v_) __attribute__((always_inline)) {
                 constexpr int v = decltype(v_)::value;
                 z += E[2][AZ][BZ][v] * W[herm_index(t, u, v)];
+            });
+            s += exy * z;
+        });
+    });
+    return s;
+}
+
+// Ket-centre derivative of the Hermite expansion along one axis:
+//   d/dB [K_AB sum_t E^{ij}_t Lambda_t] = K_AB sum_t D^{ij}_t Lambda_t,
+//   D^{ij}_t = 2b E^{i,j+1}_t - j E^{i,j-1}_t
+//            = 2b XPB E^{ij}_t + (b/zeta) E^{ij}_{t-1} + (b/zeta) i E^{i-1,j}_t
+//              - (a/zeta) j E^{i,j-1}_t,
+// by the ket step E^{i,j+1}_t = XPB E_t + E_{t-1}/(2 zeta) + (t+1) E_{t+1}
+// and E^{ij}_{t+1} = (i E^{i-1,j}_t + j E^{i,j-1}_t) / (2 zeta (t+1)); t runs
+// to la + lb + 1, so the Hermite sums W are needed one order higher.
+template <int LA, int LB>
+OHD void herm_dket(const double (&E)[LA + 1][LB + 1][LA + LB + 1], double XPB, double a,
+                   double b, double zeta, double (&D)[LA + 1][LB + 1][LA + LB + 2])
+{
+    constexpr int L = LA + LB;
+    const double bz = b / zeta, az = a / zeta;
+#pragma unroll
+    for (int i = 0; i <= LA; ++i)
+#pragma unroll
+        for (int j = 0; j <= LB; ++j)
+#pragma unroll
+            for (int t = 0; t <= L + 1; ++t) {
+                double v = t <= L ? 2.0 * b * XPB * E[i][j][t <= L ? t : 0] : 0.0;
+                if (t > 0) v += bz * E[i][j][t > 0 ? t - 1 : 0];
+                if (i > 0 && t <= L) v += bz * i * E[i > 0 ? i - 1 : 0][j][t <= L ? t : 0];
+                if (j > 0 && t <= L) v -= az * j * E[i][j > 0 ? j - 1 : 0][t <= L ? t : 0];
+                D[i][j][t] = v;
+            }
+}
+
+// herm_contract_ct with axis X's coefficients replaced by its derivative
+// table DX; W holds the sums up to order la + lb + 1.
+template <int LA, int LB, int X, int AX, int AY, int AZ, int BX, int BY, int BZ>
+OHD double herm_dcontract_ct(const double (&E)[3][LA + 1][LB + 1][LA + LB + 1],
+                             const double (&DX)[LA + 1][LB + 1][LA + LB + 2],
+                             const double (&W)[nherm(LA + LB + 1)])
+{
+    constexpr int NT = AX + BX + 1 + (X == 0), NU = AY + BY + 1 + (X == 1),
+                  NV = AZ + BZ + 1 + (X == 2);
+    double s = 0.0;
+    static_for<NT>([&](auto t_) __attribute__((always_inline)) {
+        constexpr int t = decltype(t_)::value;
+        const double ex = X == 0 ? DX[AX][BX][t] : E[0][AX][BX][X == 0 ? 0 : t];
+        static_for<NU>([&](auto u_) __attribute__((always_inline)) {
+            constexpr int u = decltype(u_)::value;
+            const double exy = ex * (X == 1 ? DX[AY][BY][u] : E[1][AY][BY][X == 1 ? 0 : u]);
+            double z = 0.0;
+            static_for<NV>([&](auto v_) __attribute__((always_inline)) {
+                constexpr int v = decltype(v_)::value;
+                z += (X == 2 ? DX[AZ][BZ][v] : E[2][AZ][BZ][X == 2 ? 0 : v]) * W[herm_index(t, u, v)];
             });
             s += exy * z;
         });

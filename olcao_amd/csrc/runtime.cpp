@@ -230,7 +230,7 @@ struct olcao_ctx {
     DBuf<long long> dChBeg[16], dChEnd[16];
     DBuf<unsigned long long> dImg[16], dImgN[16];
     DBuf<long long> dNOff[16];
-    DBuf<double> dW;
+    DBuf<double> dW, dWd;
     long long imageEntries = 0;
     double listMs = 0.0;
     DBuf<double> dBlocks, dX, dCoef;
@@ -969,6 +969,22 @@ void three_center(olcao_ctx &c, int kind, int tt, int ti, const double *coef, in
     three_center_forked(c, kind, tt, ti, coef, accumulate);
 }
 
+// The force sweep's three-centre part (classes with la, lb <= 2), weighted
+// like the three-centre sweep one Hermite order up.
+void three_center_force_forked(olcao_ctx &c, const double *coef, int comp)
+{
+    double w[16];
+    for (int k = 0; k < 16; ++k) {
+        const int la = k / 4, lb = k % 4, L = la + lb + 1;
+        w[k] = (la <= 2 && lb <= 2 && c.cls.count[k])
+                   ? (double)c.cls.nChunk[k] * ((L + 1) * (L + 2) * (L + 3) / 6)
+                   : 0.0;
+    }
+    fork_classes(c, w, [&](unsigned mask, hipStream_t s) {
+        launch_three_center_force(c.dev, c.cls, coef, comp, c.dBlocks.p, mask, s);
+    });
+}
+
 // ---------------------------------------------------------------------------
 // One integral sweep into (vv, cv, cc) device matrices.
 enum Sweep { SW_OL, SW_KE, SW_NP, SW_EP, SW_HAM, SW_MV, SW_DM, SW_MM, SW_KO };
@@ -1010,9 +1026,9 @@ void sweep_blocks(olcao_ctx &c, Sweep kind, int term, int spinIdx)
     }
 }
 
-// Contraction, Bloch sum and scatter of the current blocks; negI: the
-// Bloch sum times -i (the momentum matrix).
-void contract_scatter(olcao_ctx &c, double *vv, double *cv, double *cc, int negI = 0)
+// Contraction, Bloch sum and scatter of the current blocks; mode bit 0:
+// the Bloch sum times -i (the momentum matrix); bit 1: the full save.
+void contract_scatter(olcao_ctx &c, double *vv, double *cv, double *cc, int mode = 0)
 {
     hipStream_t s = c.stream;
     launch_contract(c.dev, c.cplan, c.dBlocks.p, c.dX.p, s);
@@ -1022,10 +1038,10 @@ void contract_scatter(olcao_ctx &c, double *vv, double *cv, double *cc, int negI
         launch_fill(cc, (long long)c.ccDoubles(), 0.0, s);
     }
     if (c.blochMfma)
-        launch_bloch_mfma(c.dev, c.dX.p, c.dPslice.p, vv, cv, cc, c.gamma ? 1 : 0, negI,
+        launch_bloch_mfma(c.dev, c.dX.p, c.dPslice.p, vv, cv, cc, c.gamma ? 1 : 0, mode,
                           c.precision == OLCAO_PREC_MIXED && c.blochSplit, s);
     else
-        launch_bloch_scatter(c.dev, c.dX.p, vv, cv, cc, c.gamma ? 1 : 0, negI, s);
+        launch_bloch_scatter(c.dev, c.dX.p, vv, cv, cc, c.gamma ? 1 : 0, mode, s);
     HIPCHK(hipGetLastError());
 }
 
@@ -1917,6 +1933,62 @@ olcao_status olcao_gauss_k_overlap(olcao_ctx *c, const int numAxial[3], double *
                          c->ccDoubles());
             }
             HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        c->haveH = false;
+    });
+}
+
+// computeForceIntg (forces.F90:60-898) for the coefficients coef
+// [spin][potDim] and totalEnergy: per spin q and Cartesian component x
+// (output slot 3 q + x), the Bloch sums of the ket derivatives d/dB_x of
+// minus the potential terms' three-Gaussian blocks times the coefficients,
+// plus the nuclear blocks times Z, minus the kinetic block, and
+// -2 totalEnergy times the momentum block (+2 totalEnergy d/dB_x of the
+// overlap), over the fused sweep's screened images; saved in full in a
+// complex build (saveCurrentPair saveFull = 1: the block and its dagger, a
+// diagonal atom block as its own dagger), as the upper triangle at Gamma
+// (saveCurrentPairGamma); no orthogonalisation.  Bases with f orbitals are
+// refused: the f-f nuclear derivative needs F_7, which the reference's Boys
+// function does not give (its dnuclear3CIntgCB is 1e-2 off there).
+olcao_status olcao_force_integrals(olcao_ctx *c, const double *coef, double totalEnergy,
+                                   double *vv, double *cv, double *cc)
+{
+    if (!c || !coef) return OLCAO_ERR_INPUT;
+    return guard(c, [&] {
+        for (int k = 0; k < 16; ++k)
+            if ((k / 4 == 3 || k % 4 == 3) && c->cls.count[k] > 0)
+                fail(OLCAO_ERR_UNSUPPORTED,
+                     "force integrals of f orbitals (the f-f nuclear derivative needs F_7)");
+        ensure_mats(*c);
+        HIPCHK(hipMemcpyAsync(c->dCoef.p, coef, (size_t)c->potDim * c->spin * 8,
+                              hipMemcpyHostToDevice, c->stream));
+        size_t tot = 0, off[16] = {};
+        for (int k = 0; k < 16; ++k) {
+            const int la = k / 4, lb = k % 4, L = la + lb + 1;
+            off[k] = tot;
+            if (la <= 2 && lb <= 2)
+                tot += (size_t)((L + 1) * (L + 2) * (L + 3) / 6) * c->cls.nChunk[k];
+        }
+        c->dWd.reserve(std::max<size_t>(tot, 1));
+        for (int k = 0; k < 16; ++k) c->cls.Wd[k] = c->dWd.p + off[k];
+        c->dev.fE2 = 2.0 * totalEnergy;
+        const int mode = c->gamma ? 0 : 2;
+        for (int q = 0; q < c->spin; ++q) {
+            three_center_force_forked(*c, c->dCoef.p + (size_t)q * c->potDim, -1);
+            for (int x = 0; x < 3; ++x) {
+                launch_two_center_fd(c->dev, c->cls, x, c->dBlocks.p, 0x0777u, c->stream);
+                three_center_force_forked(*c, nullptr, x);
+                contract_scatter(*c, c->H[0].p, c->CV_H[0].p, c->CC_H[0].p, mode);
+                const size_t slot = (size_t)q * 3 + x;
+                copy_out(*c, c->H[0].p, vv ? vv + slot * c->matDoubles() : nullptr, c->matDoubles());
+                if (c->coreDim) {
+                    copy_out(*c, c->CV_H[0].p, cv ? cv + slot * c->cvDoubles() : nullptr,
+                             c->cvDoubles());
+                    copy_out(*c, c->CC_H[0].p, cc ? cc + slot * c->ccDoubles() : nullptr,
+                             c->ccDoubles());
+                }
+                HIPCHK(hipStreamSynchronize(c->stream));
+            }
         }
         c->haveH = false;
     });

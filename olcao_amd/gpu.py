@@ -82,6 +82,8 @@ FUNCTIONS = {
     "olcao_gauss_overlap_dm": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_gauss_overlap_mm": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_gauss_k_overlap": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "olcao_force_integrals": [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                              C.c_void_p],
     "olcao_gauss_overlap_ham_pscf": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "olcao_secular_eqn_pscf": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
     "olcao_setup_scf": [C.c_void_p],
@@ -349,6 +351,18 @@ class OlcaoGPU:
         na = np.ascontiguousarray(num_axial, np.int32).reshape(3)
         _check(lib().olcao_gauss_k_overlap(self.ctx, _p(na), _p(vv), _p(cv), _p(cc)), self.ctx)
         return self._shape(vv, cv, cc, 3)
+
+    def computeForceIntg(self, pot_coeffs, total_energy):
+        """computeForceIntg (forces.F90:60-898): per spin a list of three
+        (vv, cv, cc) tuples, one per Cartesian component of the ket
+        derivative (full matrices in a complex build, upper triangle at
+        Gamma; not orthogonalised).  Bases with f orbitals are refused."""
+        c = self._coeffs(pot_coeffs)
+        vv, cv, cc = self._bufs(3 * self.spin)
+        _check(lib().olcao_force_integrals(self.ctx, _p(c), float(total_energy), _p(vv), _p(cv),
+                                           _p(cc)), self.ctx)
+        flat = self._shape(vv, cv, cc, 3 * self.spin)
+        return [flat[3 * q:3 * q + 3] for q in range(self.spin)]
 
     def gaussOverlapHamPSCF(self, pot_coeffs):
         """One (vv, cv, cc) per spin; a single tuple when spin = 1."""

@@ -230,6 +230,70 @@ def test_k_overlap_refused_for_gamma(gpu):
     g.close()
 
 
+@pytest.mark.skip(reason="the first version of this test (derivative ops in the parity-primitive "
+                  "kernel, whose f-f instance it pushed to a new scratch maximum) went silent on the "
+                  "GPU with no cause shown on the CPU; those ops are withdrawn and the derivative "
+                  "primitives are checked through the force sweep against the oracle")
+def test_device_force_primitives_match_reference_golden(gpu):
+    """The force integrals' primitives on the device: d/dB of the kinetic
+    (ops 10-12), nuclear (13-15) and three-Gaussian (16-18) blocks per
+    Cartesian component, against the reference's dkinetic2CIntg,
+    dnuclear3CIntgCB and delectron3CIntgCB (gaussIntegrals.f90:65512,
+    :88076, :112289) on 8 cases per class, within 1e-10 of each block's
+    scale; the three-centre f-f class is refused on the device (it needs
+    F_7) and left out."""
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "prims_d_golden.npz"))
+    cls, inp, ref = d["d_cls"], d["d_in"], d["d_sh"]
+    worst = 0.0
+    for op, base in ((10, 10), (11, 13), (12, 16)):
+        m = (cls[:, 0] == op) & ~((op != 10) & (cls[:, 1] == 3) & (cls[:, 2] == 3))
+        for x in range(3):
+            out = gpu.prim_batch(base + x, inp[m, :3], inp[m, 3:6], inp[m, 6:9], inp[m, 9:12],
+                                 cls[m, 3])
+            for o_, r, c in zip(out, ref[m][:, x], cls[m]):
+                na, nb = NSH[c[1]], NSH[c[2]]
+                e = block_rel_err(o_[:na, :nb], r[:na, :nb])
+                worst = max(worst, e)
+                assert e < 1e-10, (op, x, c, e)
+    _log(f"  force primitives worst {worst:.2e}")
+
+
+FORCE_CASES = ["two_types", "core", "gamma_spd", "rel_spin2", "core_gamma_rel"]
+
+
+@pytest.mark.parametrize("case", FORCE_CASES)
+def test_force_integrals_match_oracle(gpu, case):
+    """computeForceIntg (forces.F90:60-898) on the device against the
+    oracle, every spin and Cartesian component: valeVale saved in full in
+    complex builds (the upper triangle at Gamma), coreVale and coreCore,
+    within 1e-10 of each matrix's scale."""
+    _log(f"force integrals {case}")
+    s = configs.small_test(**SMALL[case])
+    g, o = _pair(gpu, s)
+    E = -7.25
+    dev = g.computeForceIntg(s.pot_coeffs, E)
+    ref = o.force_integrals(E)
+    for q in range(s.spin):
+        for x in range(3):
+            for part in range(3):
+                if ref[q][x][part] is None:
+                    continue
+                e = block_rel_err(dev[q][x][part], ref[q][x][part])
+                _log(f"  {case} F[{q}][{x}][{part}]: {e:.2e}")
+                assert e < 1e-10, (case, q, x, part, e)
+    g.close()
+
+
+def test_force_integrals_refuse_f_orbitals(gpu):
+    """A basis with f orbitals is refused (the f-f nuclear derivative needs
+    F_7)."""
+    s = configs.small_test(**SMALL["spdf"])
+    g = gpu.OlcaoGPU(s)
+    with pytest.raises(gpu.OlcaoError):
+        g.computeForceIntg(s.pot_coeffs, -1.0)
+    g.close()
+
+
 @pytest.mark.parametrize("case", ["core", "rel_spin2", "core_gamma_rel"])
 def test_solve_core_rel_spin_matches_oracle(gpu, case):
     """Eigenvalues of every spin (Mode B) and of Mode A after the core
