@@ -29,13 +29,16 @@ namespace olcao {
 // stores are contiguous runs.
 // op OP_MV stores the negated mass-velocity block (the sign with which the
 // term enters H); accumulate adds to the stored block instead of writing it.
-template <int LA, int LB>
+// KIND 0: the overlap, 1: the kinetic block, 2: the other operators by op.
+// One kernel per kind, so that the overlap and kinetic sweeps carry only
+// their own registers (with all operators behind a run-time switch the d-d
+// instance took 512 registers and spilled, at one wave per SIMD).
+template <int LA, int LB, int KIND>
 __global__ __launch_bounds__(64) void k_two_center(DevCtx c, const int *list, int n,
                                                    int op, int accumulate, double *blocks)
 {
     constexpr int SA = nsh(LA), SB = nsh(LB);
     constexpr bool STAGE = SA * SB <= 81;
-    __shared__ double sOut[STAGE ? 64 * SA * SB + 1 : 1];
     __shared__ long long sOff[64];
     const int lane = threadIdx.x;
     const int t = blockIdx.x * 64 + lane;
@@ -44,39 +47,65 @@ __global__ __launch_bounds__(64) void k_two_center(DevCtx c, const int *list, in
     const PPInfo p = load_pp(c, pp);
     const PairConst q = make_pair(c.alpha[p.ti * c.maxA + p.ia],
                                   c.alpha[p.tj * c.maxA + p.ib], p.A, p.B);
-    double pc[ncart(LA)][ncart(LB)];
-    if (op == OP_KE)
-        two_center_cart<LA, LB, true>(q, pc);
-    else if (op >= OP_KO)
-        kov_cart<LA, LB>(q, c.kq, op - OP_KO, pc);
-    else if (op == OP_MV)
-        massvel_cart<LA, LB>(q, pc);
-    else if (op >= OP_MM)
-        vec_cart<LA, LB, false>(q, op - OP_MM, c.dipC, pc);
-    else if (op >= OP_DM)
-        vec_cart<LA, LB, true>(q, op - OP_DM, c.dipC, pc);
-    else
-        two_center_cart<LA, LB, false>(q, pc);
-    double sh[SA][SB];
-    cart_to_sh<LA, LB>(pc, sh);
-    const double sg = op == OP_MV ? -1.0 : 1.0;
     const long long off = c.ppOff[pp];
-    if (STAGE) {
-#pragma unroll
-        for (int i = 0; i < SA; ++i)
-#pragma unroll
-            for (int j = 0; j < SB; ++j) sOut[lane * SA * SB + i * SB + j] = sg * sh[i][j];
+    const int nr = min(64, n - (int)blockIdx.x * 64);
+    if constexpr (KIND <= 1 && SA * SB > 45 && STAGE) {
+        // d-d overlap and kinetic straight in the harmonics, staged by halves
+        // of the harmonic rows: no Cartesian block in registers and half the
+        // LDS of a whole block, so two waves per SIMD instead of one
+        TwoCenterTables<LA, LB, KIND == 1> tb;
+        two_center_tables<LA, LB, KIND == 1>(q, tb);
+        constexpr int RA = (SA + 1) / 2, NE1 = RA * SB, NE2 = (SA - RA) * SB;
+        __shared__ double sHalf[64 * NE1 + 1];
+        two_center_sh_rows<LA, LB, KIND == 1, 0, RA>(
+            tb, [&](int i, int j, double v) { sHalf[lane * NE1 + i * SB + j] = v; });
         sOff[lane] = off;
         __syncthreads();
-        store_staged_blocks<SA * SB>(sOut, sOff, min(64, n - (int)blockIdx.x * 64), accumulate,
-                                     blocks);
-    } else if (active) {
-        double *o = blocks + off;
+        store_staged_blocks<NE1>(sHalf, sOff, nr, accumulate, blocks);
+        __syncthreads();
+        two_center_sh_rows<LA, LB, KIND == 1, RA, SA>(
+            tb, [&](int i, int j, double v) { sHalf[lane * NE2 + (i - RA) * SB + j] = v; });
+        sOff[lane] = off + NE1;
+        __syncthreads();
+        store_staged_blocks<NE2>(sHalf, sOff, nr, accumulate, blocks);
+    } else {
+        __shared__ double sOut[STAGE ? 64 * SA * SB + 1 : 1];
+        double pc[ncart(LA)][ncart(LB)];
+        if constexpr (KIND == 0) {
+            two_center_cart<LA, LB, false>(q, pc);
+        } else if constexpr (KIND == 1) {
+            two_center_cart<LA, LB, true>(q, pc);
+        } else {
+            if (op >= OP_KO)
+                kov_cart<LA, LB>(q, c.kq, op - OP_KO, pc);
+            else if (op == OP_MV)
+                massvel_cart<LA, LB>(q, pc);
+            else if (op >= OP_MM)
+                vec_cart<LA, LB, false>(q, op - OP_MM, c.dipC, pc);
+            else if (op >= OP_DM)
+                vec_cart<LA, LB, true>(q, op - OP_DM, c.dipC, pc);
+            else
+                two_center_cart<LA, LB, false>(q, pc);   // not reached: OL and KE have their kinds
+        }
+        double sh[SA][SB];
+        cart_to_sh<LA, LB>(pc, sh);
+        const double sg = op == OP_MV ? -1.0 : 1.0;
+        if (STAGE) {
 #pragma unroll
-        for (int i = 0; i < SA; ++i)
+            for (int i = 0; i < SA; ++i)
 #pragma unroll
-            for (int j = 0; j < SB; ++j)
-                o[i * SB + j] = accumulate ? o[i * SB + j] + sg * sh[i][j] : sg * sh[i][j];
+                for (int j = 0; j < SB; ++j) sOut[lane * SA * SB + i * SB + j] = sg * sh[i][j];
+            sOff[lane] = off;
+            __syncthreads();
+            store_staged_blocks<SA * SB>(sOut, sOff, nr, accumulate, blocks);
+        } else if (active) {
+            double *o = blocks + off;
+#pragma unroll
+            for (int i = 0; i < SA; ++i)
+#pragma unroll
+                for (int j = 0; j < SB; ++j)
+                    o[i * SB + j] = accumulate ? o[i * SB + j] + sg * sh[i][j] : sg * sh[i][j];
+        }
     }
 }
 
@@ -84,7 +113,7 @@ __global__ __launch_bounds__(64) void k_two_center(DevCtx c, const int *list, in
 // d-d): minus the ket derivative of the kinetic block plus c.fE2 (2
 // totalEnergy) times that of the overlap block, i.e. -2 totalEnergy times
 // the momentum block (computeForceIntg, forces.F90:648-730).  Its own kernel
-// so that k_two_center, on every iteration's path, stays as it is.
+// so that k_two_center, on every iteration's path, keeps its registers.
 template <int LA, int LB>
 __global__ __launch_bounds__(64) void k_two_center_fd(DevCtx c, const int *list, int n, int comp,
                                                       double *blocks)
@@ -390,8 +419,16 @@ static void two_one(const DevCtx &c, const ClassLists &cl, int op, int acc, doub
     const int k = LA * 4 + LB;
     const int n = cl.count[k];
     if (n == 0) return;
-    hipLaunchKernelGGL((k_two_center<LA, LB>), dim3((n + 63) / 64), dim3(64), 0, s, c,
-                       cl.idx[k], n, op, acc, blocks);
+    const dim3 grid((n + 63) / 64);
+    if (op == OP_OL)
+        hipLaunchKernelGGL((k_two_center<LA, LB, 0>), grid, dim3(64), 0, s, c, cl.idx[k], n, op, acc,
+                           blocks);
+    else if (op == OP_KE)
+        hipLaunchKernelGGL((k_two_center<LA, LB, 1>), grid, dim3(64), 0, s, c, cl.idx[k], n, op, acc,
+                           blocks);
+    else
+        hipLaunchKernelGGL((k_two_center<LA, LB, 2>), grid, dim3(64), 0, s, c, cl.idx[k], n, op, acc,
+                           blocks);
 }
 
 void launch_two_center(const DevCtx &c, const ClassLists &cl, int op, int acc, double *blocks,

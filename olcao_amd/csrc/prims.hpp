@@ -316,6 +316,64 @@ OHD void two_center_cart(const PairConst &q, double (&pc)[ncart(LA)][ncart(LB)])
         }
 }
 
+// The overlap (KIN = false) or kinetic (KIN = true) block straight in the
+// solid harmonics: sh(i, j) = pref sum over the Cartesian terms of harmonics
+// i and j of their weights times Ex Ey Ez (or the kinetic sum).  No
+// Cartesian block is held, so the sweep kernels stay small.  The 1-D tables
+// first, then the harmonic rows [I0, I1) handed to emit(i, j, value).
+template <int LA, int LB, bool KIN>
+struct TwoCenterTables {
+    double pref;
+    double E[3][LA + 1][LB + 1];
+    double Tk[3][LA + 1][LB + 1];
+};
+
+template <int LA, int LB, bool KIN>
+OHD void two_center_tables(const PairConst &q, TwoCenterTables<LA, LB, KIN> &tb)
+{
+    const double t = kPi / q.zeta;
+    tb.pref = t * sqrt(t) * exp(-q.xid2);
+    const double h = 1.0 / (2.0 * q.zeta);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        os1d<LA, LB>(q.P[x] - q.A[x], q.P[x] - q.B[x], h, tb.E[x]);
+        if (KIN)
+            ke1d<LA, LB>(q.P[x] - q.A[x], q.P[x] - q.B[x], h, q.xi, q.A[x] - q.B[x], q.a1, q.a2,
+                         tb.E[x], tb.Tk[x]);
+    }
+}
+
+template <int LA, int LB, bool KIN, int I0, int I1, class F>
+OHD void two_center_sh_rows(const TwoCenterTables<LA, LB, KIN> &tb, F &&emit)
+{
+    static_for<I1 - I0>([&](auto i_) __attribute__((always_inline)) {
+        constexpr int i = I0 + decltype(i_)::value;
+        static_for<nsh(LB)>([&](auto j_) __attribute__((always_inline)) {
+            constexpr int j = decltype(j_)::value;
+            double s = 0.0;
+            static_for<sh_nterm(i)>([&](auto ti_) __attribute__((always_inline)) {
+                constexpr int a = sh_cart(i, decltype(ti_)::value);
+                constexpr double wa = sh_weight(i, decltype(ti_)::value);
+                constexpr int ax = cexp(a, 0), ay = cexp(a, 1), az = cexp(a, 2);
+                static_for<sh_nterm(j)>([&](auto tj_) __attribute__((always_inline)) {
+                    constexpr int b = sh_cart(j, decltype(tj_)::value);
+                    constexpr double wb = sh_weight(j, decltype(tj_)::value);
+                    constexpr int bx = cexp(b, 0), by = cexp(b, 1), bz = cexp(b, 2);
+                    double v;
+                    if constexpr (KIN)
+                        v = tb.Tk[0][ax][bx] * tb.E[1][ay][by] * tb.E[2][az][bz] +
+                            tb.E[0][ax][bx] * tb.Tk[1][ay][by] * tb.E[2][az][bz] +
+                            tb.E[0][ax][bx] * tb.E[1][ay][by] * tb.Tk[2][az][bz];
+                    else
+                        v = tb.E[0][ax][bx] * tb.E[1][ay][by] * tb.E[2][az][bz];
+                    s += (wa * wb) * v;
+                });
+            });
+            emit(i, j, tb.pref * s);
+        });
+    });
+}
+
 // Mass-velocity block (massvel2CIntg, gaussIntegrals.f90:39070):
 //   (alpha_fs^2 / 8) <a| nabla^4 |b>,  alpha_fs = fineStructure * 1e-3,
 // with nabla^4 = sum_i d_i^4 + 2 sum_{i<j} d_i^2 d_j^2 applied to the more
