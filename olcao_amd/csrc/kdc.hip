@@ -126,6 +126,100 @@ __global__ void k_dc_init(double *Dall, const double *Eall, double *Qall, int n)
     }
 }
 
+// A leaf block (off, m <= kDcLeaf) solved directly: its tridiagonal (the
+// diagonal as k_dc_init tore it, plus the couplings inside the block, so the
+// block's outer edges stay torn as the merges above expect) by cyclic Jacobi
+// in one wave, the matrix in LDS, every rotation of a sweep in turn; then the
+// eigenvalues ascending into D and their vectors into the block of Q.  The
+// sweeps stop when the off-diagonal part is below 2^-53 of the diagonal
+// (norms squared compared) or after kDcLeafSweeps.
+constexpr int kDcLeafSweeps = 24;
+__global__ __launch_bounds__(64) void k_dc_leaf(const int2 *leaves, int n, double *Dall,
+                                                const double *Eall, double *Qall)
+{
+    constexpr int ML = kDcLeaf;
+    const int2 lf = leaves[blockIdx.x];
+    const int off = lf.x, m = lf.y, mat = blockIdx.y;
+    double *D = Dall + (size_t)n * mat;
+    const double *E = Eall + (size_t)n * mat;
+    double *Q = Qall + (size_t)n * n * mat;
+    __shared__ double A[ML][ML + 1], V[ML][ML + 1];
+    const int lane = threadIdx.x;
+    for (int e = lane; e < ML * ML; e += 64) {
+        const int r = e / ML, c = e - r * ML;
+        double v = 0.0;
+        if (r < m && c < m) {
+            if (r == c) {
+                v = D[off + r];
+                if (r > 0) v += fabs(E[off + r - 1]);
+                if (r < m - 1) v += fabs(E[off + r]);
+            } else if (c == r + 1) {
+                v = E[off + r];
+            } else if (r == c + 1) {
+                v = E[off + c];
+            }
+        }
+        A[r][c] = v;
+        V[r][c] = r == c ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < kDcLeafSweeps; ++sweep) {
+        double offn = 0.0, dn = 0.0;
+        for (int e = lane; e < m * m; e += 64) {
+            const int r = e / m, c = e - r * m;
+            const double a = A[r][c];
+            if (r == c)
+                dn += a * a;
+            else
+                offn += a * a;
+        }
+        offn = wave_sum(offn);
+        dn = wave_sum(dn);
+        if (!(offn > kEps * kEps * dn)) break;
+        for (int p = 0; p < m - 1; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                const double apq = A[p][q];
+                if (apq == 0.0) continue;   // the same on every lane
+                const double app = A[p][p], aqq = A[q][q];
+                const double th = (aqq - app) / (2.0 * apq);
+                const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+                __syncthreads();   // every lane has read A(p, q), A(p, p), A(q, q)
+                if (lane < m && lane != p && lane != q) {
+                    const double arp = A[lane][p], arq = A[lane][q];
+                    const double nrp = c * arp - s * arq, nrq = s * arp + c * arq;
+                    A[lane][p] = nrp;
+                    A[p][lane] = nrp;
+                    A[lane][q] = nrq;
+                    A[q][lane] = nrq;
+                }
+                if (lane == 0) {
+                    A[p][p] = app - t * apq;
+                    A[q][q] = aqq + t * apq;
+                    A[p][q] = 0.0;
+                    A[q][p] = 0.0;
+                }
+                if (lane < m) {
+                    const double vp = V[lane][p], vq = V[lane][q];
+                    V[lane][p] = c * vp - s * vq;
+                    V[lane][q] = s * vp + c * vq;
+                }
+                __syncthreads();
+            }
+    }
+    // eigenvalues ascending (ties by index), vectors with them
+    if (lane < m) {
+        const double lam = A[lane][lane];
+        int rank = 0;
+        for (int k = 0; k < m; ++k) {
+            const double lk = A[k][k];
+            rank += (lk < lam || (lk == lam && k < lane)) ? 1 : 0;
+        }
+        D[off + rank] = lam;
+        for (int i = 0; i < m; ++i) Q[(size_t)(off + i) + (size_t)n * (off + rank)] = V[i][lane];
+    }
+}
+
 // LDS state of one merge.
 struct MergeLds {
     double sd[kDcMax], sz[kDcMax], ds[kDcMax], zs[kDcMax];
@@ -1022,20 +1116,24 @@ bool dc_plan(int n, DcPlan &p)
     if (n < 1 || n > kDcBigMaxN) return false;
     p.n = n;
     if (n < 2) return true;
-    // merges by height above the 1 x 1 leaves
+    // merges by height above the leaves: the blocks of at most kDcLeaf
+    // entries that halving reaches first
     std::vector<std::vector<int4>> byH;
     struct Rec {
-        static int go(int off, int m, std::vector<std::vector<int4>> &byH)
+        static int go(int off, int m, std::vector<std::vector<int4>> &byH, std::vector<int2> &lv)
         {
-            if (m == 1) return 0;
+            if (m <= kDcLeaf) {
+                lv.push_back(make_int2(off, m));
+                return 0;
+            }
             const int m1 = m / 2;
-            const int h = 1 + std::max(go(off, m1, byH), go(off + m1, m - m1, byH));
+            const int h = 1 + std::max(go(off, m1, byH, lv), go(off + m1, m - m1, byH, lv));
             if ((int)byH.size() < h) byH.resize(h);
             byH[h - 1].push_back(make_int4(off, m1, m, 0));
             return h;
         }
     };
-    Rec::go(0, n, byH);
+    Rec::go(0, n, byH, p.leaves);
     for (auto &lv : byH) {
         DcLevel L;
         L.nodeBase = (int)p.nodes.size();
@@ -1092,6 +1190,9 @@ bool launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double
     st.cnt = dv.cnt;
     st.rho = dv.rho;
     hipLaunchKernelGGL(k_dc_init, dim3(64, nb), dim3(256), 0, s, D, E, Q, n);
+    if (!p.leaves.empty())
+        hipLaunchKernelGGL(k_dc_leaf, dim3((unsigned)p.leaves.size(), nb), dim3(64), 0, s,
+                           dv.leaves, n, D, E, Q);
     for (const DcLevel &L : p.levels) {
         if (L.maxM > kDcMaxN) {
             const size_t mm = (size_t)L.maxM;

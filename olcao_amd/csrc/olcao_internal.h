@@ -394,8 +394,12 @@ struct DcLevel {
     int nodeBase = 0, nNodes = 0, tileBase = 0, nTiles = 0, maxM = 0;
     int rootBase = 0, nRoot = 0, colBase = 0, nCol = 0;
 };
+// Blocks of at most kDcLeaf entries are the tree's leaves, solved directly
+// (k_dc_leaf) instead of merged up from 1 x 1 pieces.
+constexpr int kDcLeaf = 12;
 struct DcPlan {
     int n = 0;
+    std::vector<int2> leaves;      // (offset, size) of the directly solved blocks
     std::vector<int4> nodes;       // (offset, first child size, size, -) by height
     std::vector<int4> tiles;       // GEMM tiles: (node, row tile, column tile, -)
     std::vector<int2> rootTiles;   // (node within level, root chunk)
@@ -410,6 +414,7 @@ struct DcDev {
     int *stI;         // dc_state_ints
     int4 *cnt;        // nodes x nb
     double *rho;      // nodes x nb
+    const int2 *leaves;
 };
 // false (nothing planned or launched) when n is outside 1..kDcBigMaxN
 bool dc_plan(int n, DcPlan &p);

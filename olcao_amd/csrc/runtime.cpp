@@ -137,7 +137,7 @@ struct EigWork {
     DBuf<int> ord;
     DcPlan dc;
     DBuf<int4> dcNodes, dcTiles, dcCnt;
-    DBuf<int2> dcRoot, dcCol;
+    DBuf<int2> dcRoot, dcCol, dcLeaf;
     DBuf<double> dcStD, dcRho;
     DBuf<int> dcStI;
     int lastPath = -1;     // path of the last gen_eig: 0 LDS solver, 1 tridiagonal, 2 library
@@ -1247,6 +1247,7 @@ void trd_eig(EigWork &ew, const TrdPlan *plan, const TrdbPlan *bplan, rocblas_ha
         ew.dcTiles.upload(ew.dc.tiles, s);
         ew.dcRoot.upload(ew.dc.rootTiles, s);
         ew.dcCol.upload(ew.dc.colTiles, s);
+        ew.dcLeaf.upload(ew.dc.leaves, s);
     }
     if (plan) launch_zero_words(ew.cnt.p, (long long)plan->counters, s);
     const bool timed = ew.trdEv[0] && !defer;
@@ -1264,7 +1265,7 @@ void trd_eig(EigWork &ew, const TrdPlan *plan, const TrdbPlan *bplan, rocblas_ha
     ew.dcCnt.alloc(ew.dc.nodes.size() * nb);
     ew.dcRho.alloc(ew.dc.nodes.size() * nb);
     const DcDev dv{ew.dcNodes.p, ew.dcTiles.p, ew.dcRoot.p, ew.dcCol.p, ew.dcStD.p, ew.dcStI.p,
-                   ew.dcCnt.p, ew.dcRho.p};
+                   ew.dcCnt.p, ew.dcRho.p, ew.dcLeaf.p};
     if (!launch_dc(ew.dc, dv, nb, D, E, ew.Z.p, ew.Zt.p, ew.Wd.p, ew.ord.p, s))
         fail(OLCAO_ERR_UNSUPPORTED, "divide and conquer takes n <= %d, not %d", kDcBigMaxN, n);
     HIPCHK(hipGetLastError());
