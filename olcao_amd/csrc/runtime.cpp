@@ -199,6 +199,16 @@ struct olcao_ctx {
     static constexpr int kSide = 3;
     hipStream_t side[kSide] = {};
     hipEvent_t evFork = nullptr, evJoin[kSide] = {};
+    // the solve's Cholesky of S, started on the last side stream beside the
+    // Hamiltonian sweep (one Mode-B iteration): its rocBLAS handle, the
+    // events around it and its status per k
+    rocblas_handle blasSide = nullptr;
+    hipEvent_t evS = nullptr, evChol = nullptr;
+    DBuf<int> dCholInfo;
+    // streams the per-class forks use: the context stream and the first
+    // forkStreams - 1 side streams (the last one is left to that Cholesky
+    // while it runs)
+    int forkStreams = 1 + kSide;
 
     // host-side derived setup
     int nAtoms = 0, nT = 0, maxA = 0, maxS = 0, nPT = 0, maxPA = 0, nPS = 0;
@@ -907,6 +917,7 @@ template <class F>
 void fork_classes(olcao_ctx &c, const double (&w)[16], F launch)
 {
     constexpr int nS = 1 + olcao_ctx::kSide;
+    const int use = std::max(1, std::min(nS, c.forkStreams));
     hipStream_t st[nS] = {c.stream};
     for (int i = 1; i < nS; ++i) st[i] = c.side[i - 1];
     int order[16];
@@ -916,7 +927,7 @@ void fork_classes(olcao_ctx &c, const double (&w)[16], F launch)
     unsigned mask[nS] = {};
     for (int k : order) {
         if (w[k] == 0.0) continue;
-        const int i = (int)(std::min_element(load, load + nS) - load);
+        const int i = (int)(std::min_element(load, load + use) - load);
         mask[i] |= 1u << k;
         load[i] += w[k];
     }
@@ -1468,6 +1479,18 @@ void solve_back(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, 
 // Whether the tridiagonal path takes (n, nb), and its launch plan.
 bool trd_fits(int n, int nb, bool cplx, TrdPlan &plan) { return trd_plan(n, nb, cplx, plan); }
 
+// The path gen_eig takes for (n, nb): 0 the LDS solver, 1 the tridiagonal
+// device path (blocked Cholesky first), 2 rocSOLVER.
+int eig_path(const EigWork &ew, int n, int nb, bool cplx, bool library)
+{
+    if (library) return 2;
+    if (small_hegv_fits(n, cplx)) return 0;
+    TrdPlan plan;
+    TrdbPlan bplan;
+    if ((!ew.forceBlocked && trd_fits(n, nb, cplx, plan)) || trdb_plan(n, nb, cplx, bplan)) return 1;
+    return 2;
+}
+
 // zhegv-style status of the last gen_eig from its device info arrays
 // (dinfo: the LDS solver's or the Cholesky's or rocSOLVER's; tinfo: the
 // tridiagonal reduction's): 0 ok, > n not positive definite, 1..n no
@@ -1503,12 +1526,17 @@ void read_info(const EigWork &ew, const int *dinfo, const int *tinfo, int n, int
 // defer: enqueue only, with no host synchronisation (graph capture); the
 // caller reads the status with read_info once the work has run.  The
 // library path synchronises inside and is refused then.
+// preInfo: the tridiagonal path's Cholesky already ran (chol_blocked into
+// ew.Uc and ew.Dinv, ordered before this call on s) and left its status
+// there; B is not read.
 void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *A,
              double *B, double *D, double *E, int *dinfo, int *tinfo, std::vector<int> &info,
-             bool library, bool defer = false)
+             bool library, bool defer = false, const int *preInfo = nullptr)
 {
     const rocblas_stride st = (rocblas_stride)n * n;
     info.assign(nb, 0);
+    if (preInfo && eig_path(ew, n, nb, cplx, library) != 1)
+        fail(OLCAO_ERR_STATE, "a Cholesky factor given to a solve that does not take one");
     if (!library && small_hegv_fits(n, cplx)) {
         // one workgroup per matrix, everything in LDS (ksolve.hip)
         ew.lastPath = 0;
@@ -1528,7 +1556,10 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
         ew.Dinv.alloc((size_t)n * n * el * nb);
         ew.Y.alloc((size_t)n * n * el * nb);
         ew.Uc.alloc((size_t)n * n * el * nb);
-        chol_blocked(ew, h, s, cplx, n, nb, B, ew.Uc.p, ew.Dinv.p, dinfo);
+        if (preInfo)
+            HIPCHK(hipMemcpyAsync(dinfo, preInfo, nb * sizeof(int), hipMemcpyDeviceToDevice, s));
+        else
+            chol_blocked(ew, h, s, cplx, n, nb, B, ew.Uc.p, ew.Dinv.p, dinfo);
         if (!defer) {
             // stop before the reduction when some S(k) is not positive
             // definite, as zhegv does
@@ -1615,8 +1646,10 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
 // solve_enqueue queues the copies and the solve (defer: with no host
 // synchronisation, for graph capture; info is then read by the caller with
 // read_info), solve_finish checks the status and returns the eigenpairs.
+// pre: prefactor_s ran after S was last written, and the solve uses its
+// factor (the stream waits for it) instead of factoring S again.
 void solve_enqueue(olcao_ctx &c, int q, const double *Hsrc, const double *Ssrc, bool defer,
-                   std::vector<int> &info)
+                   std::vector<int> &info, bool pre = false)
 {
     hipStream_t s = c.stream;
     const int n = c.valeDim, nk = c.nk;
@@ -1630,9 +1663,39 @@ void solve_enqueue(olcao_ctx &c, int q, const double *Hsrc, const double *Ssrc, 
     c.dInfo[q].alloc(nk);
     c.tInfo[q].alloc(nk);
     HIPCHK(hipMemcpyAsync(Aq.p, Hsrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
+    if (pre)
+        HIPCHK(hipStreamWaitEvent(s, c.evChol, 0));
+    else
+        HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
     gen_eig(c.eig, c.blas, s, !c.gamma, n, nk, Aq.p, c.Bw.p, c.D[q].p, c.E[q].p, c.dInfo[q].p,
-            c.tInfo[q].p, info, c.eigLibrary, defer);
+            c.tInfo[q].p, info, c.eigLibrary, defer, pre ? c.dCholInfo.p : nullptr);
+}
+
+// Where the solve takes the tridiagonal path, its Cholesky of S (which does
+// not depend on H) is started on the last side stream as soon as S is
+// complete, so that it runs beside the Hamiltonian sweep; every spin's
+// solve then uses the one factor (solve_enqueue with pre).  The side
+// stream's own rocBLAS handle keeps its GEMMs off the context handle.  The
+// factor is the same computation on the same data as inside gen_eig, so the
+// results are unchanged.  Returns whether it was started.
+bool prefactor_s(olcao_ctx &c)
+{
+    const int n = c.valeDim, nk = c.nk;
+    const bool cplx = !c.gamma;
+    if (eig_path(c.eig, n, nk, cplx, c.eigLibrary) != 1) return false;
+    const size_t m = (size_t)n * n * (cplx ? 2 : 1) * nk;
+    c.Bw.alloc(c.matDoubles());
+    c.eig.Dinv.alloc(m);
+    c.eig.Y.alloc(m);
+    c.eig.Uc.alloc(m);
+    c.dCholInfo.alloc(nk);
+    hipStream_t t = c.side[olcao_ctx::kSide - 1];
+    HIPCHK(hipEventRecord(c.evS, c.stream));
+    HIPCHK(hipStreamWaitEvent(t, c.evS, 0));
+    HIPCHK(hipMemcpyAsync(c.Bw.p, c.S.p, c.matDoubles() * 8, hipMemcpyDeviceToDevice, t));
+    chol_blocked(c.eig, c.blasSide, t, cplx, n, nk, c.Bw.p, c.eig.Uc.p, c.eig.Dinv.p, c.dCholInfo.p);
+    HIPCHK(hipEventRecord(c.evChol, t));
+    return true;
 }
 
 void solve_finish(olcao_ctx &c, int q, int numStates, const std::vector<int> &info,
@@ -1670,12 +1733,12 @@ void solve_finish(olcao_ctx &c, int q, int numStates, const std::vector<int> &in
 }
 
 void solve_device(olcao_ctx &c, int q, int numStates, const double *Hsrc, const double *Ssrc,
-                  double *evals, double *evecs)
+                  double *evals, double *evecs, bool pre = false)
 {
     if (numStates <= 0 || numStates > c.valeDim)
         fail(OLCAO_ERR_INPUT, "numStates must lie in 1..%d", c.valeDim);
     std::vector<int> info;
-    solve_enqueue(c, q, Hsrc, Ssrc, false, info);
+    solve_enqueue(c, q, Hsrc, Ssrc, false, info, pre);
     solve_finish(c, q, numStates, info, evals, evecs);
 }
 
@@ -1763,6 +1826,12 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
         // no atomics in the library's GEMMs and in rocSOLVER's BLAS calls:
         // the same inputs give the same bits whatever else shares the device
         rocblas_set_atomics_mode(c->blas, rocblas_atomics_not_allowed);
+        if (rocblas_create_handle(&c->blasSide) != rocblas_status_success)
+            fail(OLCAO_ERR_HIP, "rocblas_create_handle failed");
+        rocblas_set_stream(c->blasSide, c->side[olcao_ctx::kSide - 1]);
+        rocblas_set_atomics_mode(c->blasSide, rocblas_atomics_not_allowed);
+        HIPCHK(hipEventCreateWithFlags(&c->evS, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->evChol, hipEventDisableTiming));
         setup_system(*c, *sys);
         build_lists(*c);
         upload(*c);
@@ -1786,6 +1855,9 @@ void olcao_ctx_destroy(olcao_ctx *c)
     if (c->graphExec) (void)hipGraphExecDestroy(c->graphExec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->blas) rocblas_destroy_handle(c->blas);
+    if (c->blasSide) rocblas_destroy_handle(c->blasSide);
+    if (c->evS) (void)hipEventDestroy(c->evS);
+    if (c->evChol) (void)hipEventDestroy(c->evChol);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->evFork) (void)hipEventDestroy(c->evFork);
@@ -2025,14 +2097,15 @@ olcao_status olcao_gauss_overlap_ham_pscf(olcao_ctx *c, const double *coef, doub
 }
 
 // Solve every spin's H against S: evals [spin][nk][numStates], evecs
-// [spin][nk][numStates][valeDim].
-static void solve_spins(olcao_ctx *c, int numStates, double *evals, double *evecs)
+// [spin][nk][numStates][valeDim].  pre: S's factor from prefactor_s.
+static void solve_spins(olcao_ctx *c, int numStates, double *evals, double *evecs,
+                        bool pre = false)
 {
     const size_t ev = (size_t)c->nk * numStates;
     const size_t vec = ev * c->valeDim * (c->gamma ? 1 : 2);
     for (int q = 0; q < c->spin; ++q)
         solve_device(*c, q, numStates, c->H[q].p, c->S.p, evals ? evals + q * ev : nullptr,
-                     evecs ? evecs + q * vec : nullptr);
+                     evecs ? evecs + q * vec : nullptr, pre);
 }
 
 olcao_status olcao_secular_eqn_pscf(olcao_ctx *c, int numStates, double *evals, double *evecs)
@@ -2236,7 +2309,9 @@ olcao_status olcao_vale_charge(olcao_ctx *c, int numStates, const double *occupa
 // measured: the sweep slowed by as much as the solve gained.)
 // The device work of the build, everything after the coefficient upload;
 // events: record the stage events (not inside a graph capture).
-static void iteration_device(olcao_ctx &c, bool events)
+// prefactor: start the solve's Cholesky of S beside the Hamiltonian sweep
+// (prefactor_s); returns whether it was started.
+static bool iteration_device(olcao_ctx &c, bool events, bool prefactor)
 {
     hipStream_t s = c.stream;
     auto mark = [&](int e) {
@@ -2244,6 +2319,12 @@ static void iteration_device(olcao_ctx &c, bool events)
     };
     overlap_sweep(c);
     mark(1);
+    const bool pre = prefactor && prefactor_s(c);
+    struct Restore {
+        olcao_ctx &c;
+        ~Restore() { c.forkStreams = 1 + olcao_ctx::kSide; }
+    } restore{c};
+    if (pre) c.forkStreams = olcao_ctx::kSide;
     // events: 0 start, 1 overlap done; per spin q: 2+3q before the
     // three-centre sweep, 3+3q after it, 4+3q after contraction, Bloch
     // sum and orthogonalisation; 8 after the solve
@@ -2262,6 +2343,7 @@ static void iteration_device(olcao_ctx &c, bool events)
         mark(4 + 3 * q);
     }
     c.haveH = true;
+    return pre;
 }
 
 static void upload_coef(olcao_ctx &c, const double *coef)
@@ -2271,12 +2353,12 @@ static void upload_coef(olcao_ctx &c, const double *coef)
                           c.stream));
 }
 
-static void iteration_build(olcao_ctx &c, const double *coef)
+static bool iteration_build(olcao_ctx &c, const double *coef, bool prefactor = false)
 {
     c.trdMs = 0.0;
     upload_coef(c, coef);
     HIPCHK(hipEventRecord(c.ev[0], c.stream));
-    iteration_device(c, true);
+    return iteration_device(c, true, prefactor);
 }
 
 // Second half: the solve for all k and spins, then the stage times.
@@ -2305,10 +2387,10 @@ static void record_timings(olcao_ctx &c, double solveMs)
     c.timings[7] = c.trdMs;
 }
 
-static void iteration_solve(olcao_ctx &c, int numStates, double *evals)
+static void iteration_solve(olcao_ctx &c, int numStates, double *evals, bool pre = false)
 {
     hipStream_t s = c.stream;
-    solve_spins(&c, numStates, evals, nullptr);
+    solve_spins(&c, numStates, evals, nullptr, pre);
     HIPCHK(hipEventRecord(c.ev[8], s));
     HIPCHK(hipEventSynchronize(c.ev[8]));
     record_timings(c, -1.0);
@@ -2330,9 +2412,9 @@ static void capture_iteration(olcao_ctx &c)
         HIPCHK(hipStreamSynchronize(s));
         HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
-            iteration_device(c, false);
+            const bool pre = iteration_device(c, false, true);
             std::vector<int> info;
-            for (int q = 0; q < c.spin; ++q) solve_enqueue(c, q, c.H[q].p, c.S.p, true, info);
+            for (int q = 0; q < c.spin; ++q) solve_enqueue(c, q, c.H[q].p, c.S.p, true, info, pre);
         } catch (...) {
             hipGraph_t junk = nullptr;
             (void)hipStreamEndCapture(s, &junk);
@@ -2686,8 +2768,8 @@ olcao_status olcao_scf_iteration(olcao_ctx *c, const double *coef, int numStates
             iteration_replay(*c, coef, numStates, evals);
             return;
         }
-        iteration_build(*c, coef);
-        iteration_solve(*c, numStates, evals);
+        const bool pre = iteration_build(*c, coef, true);
+        iteration_solve(*c, numStates, evals, pre);
         if (c->useGraph && !c->graphFailed) capture_iteration(*c);
     });
 }
