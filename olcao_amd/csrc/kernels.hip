@@ -759,6 +759,26 @@ void launch_ortho_finish(double *M, int n, int nb, int cplx, hipStream_t s)
     hipLaunchKernelGGL(k_ortho_finish, dim3((unsigned)blocks), dim3(256), 0, s, M, n, nb, cplx);
 }
 
+// nb identity matrices of order n (cplx = 2: complex entries)
+__global__ void k_identity(double *M, int n, int nb, int cplx)
+{
+    const long long per = (long long)n * n;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < per * nb * cplx;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long e = (t / cplx) % per;
+        const bool re = t % cplx == 0;
+        M[t] = (re && e % n == e / n) ? 1.0 : 0.0;
+    }
+}
+
+void launch_identity(double *M, int n, int nb, int cplx, hipStream_t s)
+{
+    long long total = (long long)n * n * nb * cplx;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_identity, dim3((unsigned)blocks), dim3(256), 0, s, M, n, nb, cplx);
+}
+
 void launch_herm_fill(double *A, int n, int nb, int cplx, hipStream_t s)
 {
     long long total = (long long)n * n * nb;

@@ -307,6 +307,7 @@ void launch_prim_batch(int op, int n, const double *a, const double *A,
                        const int *classCount, double *sh, hipStream_t s);
 void launch_boys_batch(int n, const double *T, double *F, hipStream_t s);
 void launch_herm_fill(double *A, int n, int nb, int cplx, hipStream_t s);
+void launch_identity(double *M, int n, int nb, int cplx, hipStream_t s);
 // Batched small-matrix generalised eigensolve in LDS (ksolve.hip): A (H upper
 // in, eigenvectors out), B (S upper in, U out), w eigenvalues, info per k.
 bool small_hegv_fits(int n, bool cplx);

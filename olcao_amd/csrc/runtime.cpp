@@ -133,6 +133,7 @@ struct EigWork {
     DBuf<double> V, U, W, Tw, tau, pub;    // reflectors, block factors, GEMM work
     DBuf<double> Z, Zt, Wd;                 // divide and conquer: vectors, work
     DBuf<double> Uc, Dinv, Y;               // Cholesky factor, inverted blocks, solve work
+    DBuf<double> Ui;                        // U^-1 (upper triangular), when formed ahead
     DBuf<unsigned> cnt;
     DBuf<int> ord;
     DcPlan dc;
@@ -205,6 +206,7 @@ struct olcao_ctx {
     rocblas_handle blasSide = nullptr;
     hipEvent_t evS = nullptr, evChol = nullptr;
     DBuf<int> dCholInfo;
+    bool preInv = false;   // that Cholesky also formed U^-1 (EigWork::Ui)
     // streams the per-class forks use: the context stream and the first
     // forkStreams - 1 side streams (the last one is left to that Cholesky
     // while it runs)
@@ -1528,10 +1530,13 @@ void read_info(const EigWork &ew, const int *dinfo, const int *tinfo, int n, int
 // library path synchronises inside and is refused then.
 // preInfo: the tridiagonal path's Cholesky already ran (chol_blocked into
 // ew.Uc and ew.Dinv, ordered before this call on s) and left its status
-// there; B is not read.
+// there; B is not read.  preInv (with preInfo): U^-1 was formed too, and the
+// transformations C = U^-H A U^-1 and X = U^-1 Y are products with it
+// instead of blocked triangular solves.
 void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int nb, double *A,
              double *B, double *D, double *E, int *dinfo, int *tinfo, std::vector<int> &info,
-             bool library, bool defer = false, const int *preInfo = nullptr)
+             bool library, bool defer = false, const int *preInfo = nullptr,
+             const double *preInv = nullptr)
 {
     const rocblas_stride st = (rocblas_stride)n * n;
     info.assign(nb, 0);
@@ -1572,11 +1577,36 @@ void gen_eig(EigWork &ew, rocblas_handle h, hipStream_t s, bool cplx, int n, int
                 }
         }
         launch_herm_fill(A, n, nb, el, s);
-        solve_left_h(ew, h, s, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
-        solve_right(ew, h, s, cplx, n, nb, ew.Uc.p, ew.Dinv.p, ew.Y.p, A);
+        // with U^-1 = [[Ui_LL, Ui_LR], [0, Ui_RR]] (L the first m1 rows and
+        // columns), each product is two GEMMs that skip the zero block:
+        //   T = A U^-1:    T(:, L) = A(:, L) Ui_LL,  T(:, R) = A Ui(:, R)
+        //   C = U^-H T:    C(L, :) = Ui_LL^H T(L, :), C(R, :) = Ui(:, R)^H T
+        //   X = U^-1 Y:    X(L, :) = Ui(L, :) Y,      X(R, :) = Ui_RR Y(R, :)
+        const int m1 = n / 2, m2 = n - m1;
+        auto at = [&](const double *M, int r, int col) {
+            return const_cast<double *>(M) + el * ((size_t)r + (size_t)n * col);
+        };
+        if (preInv) {
+            double *Tm = ew.Y.p;
+            gemm_nn(h, cplx, false, false, n, m1, m1, 1.0, A, preInv, 0.0, Tm, n, nb);
+            gemm_nn(h, cplx, false, false, n, m2, n, 1.0, A, at(preInv, 0, m1), 0.0, at(Tm, 0, m1), n,
+                    nb);
+            gemm_nn(h, cplx, true, false, m1, n, m1, 1.0, preInv, Tm, 0.0, A, n, nb);
+            gemm_nn(h, cplx, true, false, m2, n, n, 1.0, at(preInv, 0, m1), Tm, 0.0, at(A, m1, 0), n,
+                    nb);
+        } else {
+            solve_left_h(ew, h, s, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
+            solve_right(ew, h, s, cplx, n, nb, ew.Uc.p, ew.Dinv.p, ew.Y.p, A);
+        }
         trd_eig(ew, ldsTrd ? &plan : nullptr, ldsTrd ? nullptr : &bplan, h, s, cplx, n, nb, A, D, E,
                 tinfo, defer);
-        solve_back(ew, h, s, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
+        if (preInv) {
+            gemm_nn(h, cplx, false, false, m1, n, n, 1.0, preInv, A, 0.0, ew.Y.p, n, nb);
+            gemm_nn(h, cplx, false, false, m2, n, m2, 1.0, at(preInv, m1, m1), at(A, m1, 0), 0.0,
+                    at(ew.Y.p, m1, 0), n, nb);
+        } else {
+            solve_back(ew, h, s, cplx, n, nb, ew.Uc.p, ew.Dinv.p, A, ew.Y.p);
+        }
         HIPCHK(hipMemcpyAsync(A, ew.Y.p, (size_t)n * n * el * nb * sizeof(double),
                               hipMemcpyDeviceToDevice, s));
         if (!defer) read_info(ew, dinfo, tinfo, n, nb, s, info);
@@ -1668,7 +1698,8 @@ void solve_enqueue(olcao_ctx &c, int q, const double *Hsrc, const double *Ssrc, 
     else
         HIPCHK(hipMemcpyAsync(c.Bw.p, Ssrc, c.matDoubles() * 8, hipMemcpyDeviceToDevice, s));
     gen_eig(c.eig, c.blas, s, !c.gamma, n, nk, Aq.p, c.Bw.p, c.D[q].p, c.E[q].p, c.dInfo[q].p,
-            c.tInfo[q].p, info, c.eigLibrary, defer, pre ? c.dCholInfo.p : nullptr);
+            c.tInfo[q].p, info, c.eigLibrary, defer, pre ? c.dCholInfo.p : nullptr,
+            pre && c.preInv ? c.eig.Ui.p : nullptr);
 }
 
 // Where the solve takes the tridiagonal path, its Cholesky of S (which does
@@ -1694,6 +1725,17 @@ bool prefactor_s(olcao_ctx &c)
     HIPCHK(hipStreamWaitEvent(t, c.evS, 0));
     HIPCHK(hipMemcpyAsync(c.Bw.p, c.S.p, c.matDoubles() * 8, hipMemcpyDeviceToDevice, t));
     chol_blocked(c.eig, c.blasSide, t, cplx, n, nk, c.Bw.p, c.eig.Uc.p, c.eig.Dinv.p, c.dCholInfo.p);
+    // where the reduction is the LDS-resident one (n <= 896), also U^-1 =
+    // U^-1 I by the blocked back substitution (Bw, free now, holds I), so
+    // that the solve's three triangular transformations become GEMMs
+    TrdPlan plan;
+    c.preInv = !c.eig.forceBlocked && trd_fits(n, nk, cplx, plan);
+    if (c.preInv) {
+        c.eig.Ui.alloc(m);
+        launch_identity(c.Bw.p, n, nk, cplx ? 2 : 1, t);
+        HIPCHK(hipGetLastError());
+        solve_back(c.eig, c.blasSide, t, cplx, n, nk, c.eig.Uc.p, c.eig.Dinv.p, c.Bw.p, c.eig.Ui.p);
+    }
     HIPCHK(hipEventRecord(c.evChol, t));
     return true;
 }
