@@ -449,12 +449,89 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
     }
 }
 
+// Partial Gram matrices of the back-transform blocks: workgroup (block,
+// matrix, split) sums G(a, j) = v_a^H v_j over its share of the block's rows
+// (rows j0+1..n-1 cut into kTrdGramSplit contiguous ranges), so that a block's
+// Gram is formed by kTrdGramSplit workgroups instead of one.  Thread t < 136
+// owns the 4 x 4 tile (ta, tj), ta <= tj, of the 64 x 64 upper triangle: per
+// row it reads 8 entries from LDS for 16 products.  The partial sums go to
+// P[((block * nb + mat) * kTrdGramSplit + split) * 64 * 64], column-major;
+// entries below the diagonal tiles are not written and not read.
+constexpr int kTrdGramTiles = 16 * 17 / 2;
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_trd_gram(const double *__restrict__ Vall, double *Pall,
+                                                  int n)
+{
+    using N = Num<CPLX>;
+    using T = typename N::T;
+    constexpr int E = CPLX ? 2 : 1;
+    constexpr int NB = kTrdBlock;
+    static_assert(NB == 64, "4 x 4 tiles of a 64 x 64 Gram matrix");
+    __shared__ T Vs[64][NB + 1];
+    const int tid = threadIdx.x;
+    const int j0 = blockIdx.x * NB, mat = blockIdx.y, split = blockIdx.z;
+    const int nb = gridDim.y;
+    const int mb = min(NB, n - 1 - j0);
+    const long long per = (long long)n * n;
+    const double *V = Vall + E * per * mat;
+    const int rows = n - 1 - j0, len = (rows + kTrdGramSplit - 1) / kTrdGramSplit;
+    const int rs = j0 + 1 + split * len, re = min(n, rs + len);
+    // tile of this thread: tiles in row-major order of the upper triangle
+    int ta = 0, tj = 0;
+    {
+        int q = tid;
+        while (ta < 16 && q >= 16 - ta) {
+            q -= 16 - ta;
+            ++ta;
+        }
+        tj = ta + q;
+    }
+    const bool owner = tid < kTrdGramTiles;
+    T acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = N::zero();
+    for (int rb = rs; rb < re; rb += 64) {
+        for (int e = tid; e < 64 * NB; e += 256) {
+            const int rr = e % 64, a = e / 64, r = rb + rr;
+            Vs[rr][a] = (r < re && a < mb) ? N::load(V, r + (long long)(j0 + a) * n) : N::zero();
+        }
+        __syncthreads();
+        if (owner) {
+#pragma unroll 2
+            for (int rr = 0; rr < 64; ++rr) {
+                T va[4], vj[4];
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    va[x] = Vs[rr][4 * ta + x];
+                    vj[x] = Vs[rr][4 * tj + x];
+                }
+#pragma unroll
+                for (int x = 0; x < 4; ++x)
+#pragma unroll
+                    for (int y = 0; y < 4; ++y) acc[x][y] = N::add(acc[x][y], N::cmul(va[x], vj[y]));
+            }
+        }
+        __syncthreads();
+    }
+    if (owner) {
+        double *P = Pall + E * (long long)NB * NB *
+                               (((long long)blockIdx.x * nb + mat) * kTrdGramSplit + split);
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) N::store(P, (4 * ta + x) + NB * (4 * tj + y), acc[x][y]);
+    }
+}
+
 // U_b = V_b T_b for one block of kTrdBlock reflectors of one matrix.
 // V: explicit reflectors (column j = v_j, zero above row j+1, 1 at j+1).
-// The Gram matrix G(a, j) = v_a^H v_j (a < j) is overwritten column by column
-// with T (zlarft: T(j, j) = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) G(0:j, j)).
+// The Gram matrix G(a, j) = v_a^H v_j (a < j), summed from k_trd_gram's
+// partials in a fixed order, is overwritten column by column with T (zlarft:
+// T(j, j) = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) G(0:j, j)).
 template <bool CPLX>
-__global__ __launch_bounds__(256) void k_trd_tfactor(const double *__restrict__ Vall,
+__global__ __launch_bounds__(256) void k_trd_tfactor(const double *__restrict__ Pall,
                                                      const double *__restrict__ tauAll,
                                                      double *Tall, int n)
 {
@@ -462,51 +539,26 @@ __global__ __launch_bounds__(256) void k_trd_tfactor(const double *__restrict__ 
     using T = typename N::T;
     constexpr int E = CPLX ? 2 : 1;
     constexpr int NB = kTrdBlock;
-    constexpr int NP = NB * (NB - 1) / 2;
-    constexpr int PPT = (NP + 255) / 256;
-    __shared__ T Vs[64][NB + 1];
     __shared__ T Gm[NB][NB + 1];
     __shared__ T tcol[NB];
     const int tid = threadIdx.x;
     const int j0 = blockIdx.x * NB, mat = blockIdx.y;
     const int nref = n - 1;
     const int mb = min(NB, nref - j0);
-    const long long per = (long long)n * n;
-    const double *V = Vall + E * per * mat;
     const double *tau = tauAll + (long long)E * n * mat;
-    const int r0 = j0 + 1;   // first row of the block's reflectors
 
     // Gram matrix of the block: Gm(a, j) = v_a^H v_j, a < j
-    T acc[PPT];
-    int pa[PPT], pj[PPT];
+    const double *P = Pall + E * (long long)NB * NB *
+                                 (((long long)blockIdx.x * gridDim.y + mat) * kTrdGramSplit);
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int a = e % NB, j = e / NB;
+        if (a >= j) continue;
+        T g = N::zero();
 #pragma unroll
-    for (int t = 0; t < PPT; ++t) {
-        acc[t] = N::zero();
-        int q = tid + 256 * t, j = 1;
-        while (q >= j && j < NB) {
-            q -= j;
-            ++j;
-        }
-        pa[t] = q;
-        pj[t] = j;
+        for (int sp = 0; sp < kTrdGramSplit; ++sp)
+            g = N::add(g, N::load(P + E * (long long)NB * NB * sp, e));
+        Gm[a][j] = g;
     }
-    for (int rb = r0; rb < n; rb += 64) {
-        for (int e = tid; e < 64 * NB; e += 256) {
-            const int rr = e % 64, a = e / 64, r = rb + rr;
-            Vs[rr][a] = (r < n && a < mb) ? N::load(V, r + (long long)(j0 + a) * n) : N::zero();
-        }
-        __syncthreads();
-#pragma unroll 1
-        for (int rr = 0; rr < 64; ++rr) {
-#pragma unroll
-            for (int t = 0; t < PPT; ++t)
-                acc[t] = N::add(acc[t], N::cmul(Vs[rr][pa[t] < NB ? pa[t] : 0], Vs[rr][pj[t] < NB ? pj[t] : 0]));
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int t = 0; t < PPT; ++t)
-        if (tid + 256 * t < NP) Gm[pa[t]][pj[t]] = acc[t];
     __syncthreads();
     // T over G, column by column (columns < j already hold T):
     // T(a, j) = -tau_j sum_{a <= q < j} T(a, q) G(q, j); row a's sum is split
@@ -664,16 +716,27 @@ hipError_t launch_trd(bool cplx, const TrdPlan &p, int n, int nb, const double *
                                       dim3(kTrdThreads), args, (unsigned)p.lds, s);
 }
 
+size_t trd_tfactor_work(int n, int nb, bool cplx)
+{
+    const size_t blocks = (n - 1 + kTrdBlock - 1) / kTrdBlock;
+    return blocks * nb * (1 + kTrdGramSplit) * kTrdBlock * kTrdBlock * (cplx ? 2 : 1);
+}
+
 void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double *tau, double *Tw,
                         double *U, hipStream_t s)
 {
     const int blocks = (n - 1 + kTrdBlock - 1) / kTrdBlock;
     const dim3 g2(blocks, nb, (n + 63) / 64);
+    const dim3 gg(blocks, nb, kTrdGramSplit);
+    // T for every block first, then the Gram partial sums
+    double *P = Tw + (size_t)blocks * nb * kTrdBlock * kTrdBlock * (cplx ? 2 : 1);
     if (cplx) {
-        hipLaunchKernelGGL(k_trd_tfactor<true>, dim3(blocks, nb), dim3(256), 0, s, V, tau, Tw, n);
+        hipLaunchKernelGGL(k_trd_gram<true>, gg, dim3(256), 0, s, V, P, n);
+        hipLaunchKernelGGL(k_trd_tfactor<true>, dim3(blocks, nb), dim3(256), 0, s, P, tau, Tw, n);
         hipLaunchKernelGGL(k_trd_vt<true>, g2, dim3(256), 0, s, V, Tw, U, n);
     } else {
-        hipLaunchKernelGGL(k_trd_tfactor<false>, dim3(blocks, nb), dim3(256), 0, s, V, tau, Tw, n);
+        hipLaunchKernelGGL(k_trd_gram<false>, gg, dim3(256), 0, s, V, P, n);
+        hipLaunchKernelGGL(k_trd_tfactor<false>, dim3(blocks, nb), dim3(256), 0, s, P, tau, Tw, n);
         hipLaunchKernelGGL(k_trd_vt<false>, g2, dim3(256), 0, s, V, Tw, U, n);
     }
 }

@@ -342,7 +342,10 @@ hipError_t launch_trd(bool cplx, const TrdPlan &p, int n, int nb, const double *
                       double *d, double *e, double *tau, double *pub, unsigned *cnt, int *info,
                       hipStream_t s);
 // U_b = V_b T_b per block of kTrdBlock reflectors (rows j0+1..n-1 of columns
-// j0..j0+kTrdBlock-1 of U); Tw: kTrdBlock^2 per block and matrix (work)
+// j0..j0+kTrdBlock-1 of U); Tw: work of trd_tfactor_work doubles (T, then
+// kTrdGramSplit partial Gram matrices per block and matrix)
+constexpr int kTrdGramSplit = 8;
+size_t trd_tfactor_work(int n, int nb, bool cplx);
 void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double *tau, double *Tw,
                         double *U, hipStream_t s);
 void launch_real_to_field(const double *Z, double *X, long long count, bool cplx, hipStream_t s);

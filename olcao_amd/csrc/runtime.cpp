@@ -1282,7 +1282,7 @@ void trd_eig(EigWork &ew, const TrdPlan *plan, const TrdbPlan *bplan, rocblas_ha
     if (!launch_dc(ew.dc, dv, nb, D, E, ew.Z.p, ew.Zt.p, ew.Wd.p, ew.ord.p, s))
         fail(OLCAO_ERR_UNSUPPORTED, "divide and conquer takes n <= %d, not %d", kDcBigMaxN, n);
     HIPCHK(hipGetLastError());
-    ew.Tw.alloc((size_t)((n - 1 + kTrdBlock - 1) / kTrdBlock) * kTrdBlock * kTrdBlock * el * nb);
+    ew.Tw.alloc(trd_tfactor_work(n, nb, cplx));
     launch_trd_tfactor(cplx, n, nb, ew.V.p, ew.tau.p, ew.Tw.p, ew.U.p, s);
     HIPCHK(hipGetLastError());
     launch_real_to_field(ew.Z.p, A, (long long)per * nb, cplx, s);
