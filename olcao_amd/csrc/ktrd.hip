@@ -451,12 +451,12 @@ __global__ __launch_bounds__(kTrdThreads) void k_trd(const double *__restrict__ 
 
 // Partial Gram matrices of the back-transform blocks: workgroup (block,
 // matrix, split) sums G(a, j) = v_a^H v_j over its share of the block's rows
-// (rows j0+1..n-1 cut into kTrdGramSplit contiguous ranges), so that a block's
-// Gram is formed by kTrdGramSplit workgroups instead of one.  Thread t < 136
+// (rows j0+1..n-1 cut into S = gridDim.z contiguous ranges), so that a
+// block's Gram is formed by S workgroups instead of one.  Thread t < 136
 // owns the 4 x 4 tile (ta, tj), ta <= tj, of the 64 x 64 upper triangle: per
 // row it reads 8 entries from LDS for 16 products.  The partial sums go to
-// P[((block * nb + mat) * kTrdGramSplit + split) * 64 * 64], column-major;
-// entries below the diagonal tiles are not written and not read.
+// P[((block * nb + mat) * S + split) * 64 * 64], column-major; entries below
+// the diagonal tiles are not written and not read.
 constexpr int kTrdGramTiles = 16 * 17 / 2;
 template <bool CPLX>
 __global__ __launch_bounds__(256) void k_trd_gram(const double *__restrict__ Vall, double *Pall,
@@ -470,11 +470,11 @@ __global__ __launch_bounds__(256) void k_trd_gram(const double *__restrict__ Val
     __shared__ T Vs[64][NB + 1];
     const int tid = threadIdx.x;
     const int j0 = blockIdx.x * NB, mat = blockIdx.y, split = blockIdx.z;
-    const int nb = gridDim.y;
+    const int nb = gridDim.y, S = gridDim.z;
     const int mb = min(NB, n - 1 - j0);
     const long long per = (long long)n * n;
     const double *V = Vall + E * per * mat;
-    const int rows = n - 1 - j0, len = (rows + kTrdGramSplit - 1) / kTrdGramSplit;
+    const int rows = n - 1 - j0, len = (rows + S - 1) / S;
     const int rs = j0 + 1 + split * len, re = min(n, rs + len);
     // tile of this thread: tiles in row-major order of the upper triangle
     int ta = 0, tj = 0;
@@ -516,8 +516,7 @@ __global__ __launch_bounds__(256) void k_trd_gram(const double *__restrict__ Val
         __syncthreads();
     }
     if (owner) {
-        double *P = Pall + E * (long long)NB * NB *
-                               (((long long)blockIdx.x * nb + mat) * kTrdGramSplit + split);
+        double *P = Pall + E * (long long)NB * NB * (((long long)blockIdx.x * nb + mat) * S + split);
 #pragma unroll
         for (int x = 0; x < 4; ++x)
 #pragma unroll
@@ -533,7 +532,7 @@ __global__ __launch_bounds__(256) void k_trd_gram(const double *__restrict__ Val
 template <bool CPLX>
 __global__ __launch_bounds__(256) void k_trd_tfactor(const double *__restrict__ Pall,
                                                      const double *__restrict__ tauAll,
-                                                     double *Tall, int n)
+                                                     double *Tall, int n, int S)
 {
     using N = Num<CPLX>;
     using T = typename N::T;
@@ -548,14 +547,12 @@ __global__ __launch_bounds__(256) void k_trd_tfactor(const double *__restrict__ 
     const double *tau = tauAll + (long long)E * n * mat;
 
     // Gram matrix of the block: Gm(a, j) = v_a^H v_j, a < j
-    const double *P = Pall + E * (long long)NB * NB *
-                                 (((long long)blockIdx.x * gridDim.y + mat) * kTrdGramSplit);
+    const double *P = Pall + E * (long long)NB * NB * (((long long)blockIdx.x * gridDim.y + mat) * S);
     for (int e = tid; e < NB * NB; e += 256) {
         const int a = e % NB, j = e / NB;
         if (a >= j) continue;
         T g = N::zero();
-#pragma unroll
-        for (int sp = 0; sp < kTrdGramSplit; ++sp)
+        for (int sp = 0; sp < S; ++sp)
             g = N::add(g, N::load(P + E * (long long)NB * NB * sp, e));
         Gm[a][j] = g;
     }
@@ -716,10 +713,19 @@ hipError_t launch_trd(bool cplx, const TrdPlan &p, int n, int nb, const double *
                                       dim3(kTrdThreads), args, (unsigned)p.lds, s);
 }
 
+// Row splits of each block's Gram matrix: enough workgroups to fill the
+// device (about kTrdGramWg), at most kTrdGramSplit, one when the blocks of
+// the batch already fill it (this also bounds the partial sums' memory)
+static int trd_gram_splits(int n, int nb)
+{
+    const long long units = (long long)((n - 1 + kTrdBlock - 1) / kTrdBlock) * nb;
+    return (int)std::max(1LL, std::min<long long>(kTrdGramSplit, kTrdGramWg / std::max(units, 1LL)));
+}
+
 size_t trd_tfactor_work(int n, int nb, bool cplx)
 {
     const size_t blocks = (n - 1 + kTrdBlock - 1) / kTrdBlock;
-    return blocks * nb * (1 + kTrdGramSplit) * kTrdBlock * kTrdBlock * (cplx ? 2 : 1);
+    return blocks * nb * (1 + trd_gram_splits(n, nb)) * kTrdBlock * kTrdBlock * (cplx ? 2 : 1);
 }
 
 void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double *tau, double *Tw,
@@ -727,16 +733,17 @@ void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double 
 {
     const int blocks = (n - 1 + kTrdBlock - 1) / kTrdBlock;
     const dim3 g2(blocks, nb, (n + 63) / 64);
-    const dim3 gg(blocks, nb, kTrdGramSplit);
+    const int S = trd_gram_splits(n, nb);
+    const dim3 gg(blocks, nb, S);
     // T for every block first, then the Gram partial sums
     double *P = Tw + (size_t)blocks * nb * kTrdBlock * kTrdBlock * (cplx ? 2 : 1);
     if (cplx) {
         hipLaunchKernelGGL(k_trd_gram<true>, gg, dim3(256), 0, s, V, P, n);
-        hipLaunchKernelGGL(k_trd_tfactor<true>, dim3(blocks, nb), dim3(256), 0, s, P, tau, Tw, n);
+        hipLaunchKernelGGL(k_trd_tfactor<true>, dim3(blocks, nb), dim3(256), 0, s, P, tau, Tw, n, S);
         hipLaunchKernelGGL(k_trd_vt<true>, g2, dim3(256), 0, s, V, Tw, U, n);
     } else {
         hipLaunchKernelGGL(k_trd_gram<false>, gg, dim3(256), 0, s, V, P, n);
-        hipLaunchKernelGGL(k_trd_tfactor<false>, dim3(blocks, nb), dim3(256), 0, s, P, tau, Tw, n);
+        hipLaunchKernelGGL(k_trd_tfactor<false>, dim3(blocks, nb), dim3(256), 0, s, P, tau, Tw, n, S);
         hipLaunchKernelGGL(k_trd_vt<false>, g2, dim3(256), 0, s, V, Tw, U, n);
     }
 }

@@ -343,8 +343,10 @@ hipError_t launch_trd(bool cplx, const TrdPlan &p, int n, int nb, const double *
                       hipStream_t s);
 // U_b = V_b T_b per block of kTrdBlock reflectors (rows j0+1..n-1 of columns
 // j0..j0+kTrdBlock-1 of U); Tw: work of trd_tfactor_work doubles (T, then
-// kTrdGramSplit partial Gram matrices per block and matrix)
+// up to kTrdGramSplit partial Gram matrices per block and matrix: as many as
+// bring the Gram workgroups to about kTrdGramWg)
 constexpr int kTrdGramSplit = 8;
+constexpr int kTrdGramWg = 576;
 size_t trd_tfactor_work(int n, int nb, bool cplx);
 void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double *tau, double *Tw,
                         double *U, hipStream_t s);

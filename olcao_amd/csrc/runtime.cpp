@@ -201,8 +201,10 @@ struct olcao_ctx {
     hipStream_t side[kSide] = {};
     hipEvent_t evFork = nullptr, evJoin[kSide] = {};
     // the solve's Cholesky of S, started on the last side stream beside the
-    // Hamiltonian sweep (one Mode-B iteration): its rocBLAS handle, the
-    // events around it and its status per k
+    // Hamiltonian sweep (one Mode-B iteration): its rocBLAS handle (created
+    // on first use: each handle holds device workspace, and contexts that
+    // only take part in batched calls never need it), the events around it
+    // and its status per k
     rocblas_handle blasSide = nullptr;
     hipEvent_t evS = nullptr, evChol = nullptr;
     DBuf<int> dCholInfo;
@@ -1715,6 +1717,14 @@ bool prefactor_s(olcao_ctx &c)
     const bool cplx = !c.gamma;
     if (eig_path(c.eig, n, nk, cplx, c.eigLibrary) != 1) return false;
     const size_t m = (size_t)n * n * (cplx ? 2 : 1) * nk;
+    if (!c.blasSide) {
+        if (rocblas_create_handle(&c.blasSide) != rocblas_status_success) {
+            c.blasSide = nullptr;
+            fail(OLCAO_ERR_HIP, "rocblas_create_handle failed");
+        }
+        rocblas_set_stream(c.blasSide, c.side[olcao_ctx::kSide - 1]);
+        rocblas_set_atomics_mode(c.blasSide, rocblas_atomics_not_allowed);
+    }
     c.Bw.alloc(c.matDoubles());
     c.eig.Dinv.alloc(m);
     c.eig.Y.alloc(m);
@@ -1868,10 +1878,6 @@ olcao_status olcao_ctx_create(const olcao_system *sys, const olcao_options *opt,
         // no atomics in the library's GEMMs and in rocSOLVER's BLAS calls:
         // the same inputs give the same bits whatever else shares the device
         rocblas_set_atomics_mode(c->blas, rocblas_atomics_not_allowed);
-        if (rocblas_create_handle(&c->blasSide) != rocblas_status_success)
-            fail(OLCAO_ERR_HIP, "rocblas_create_handle failed");
-        rocblas_set_stream(c->blasSide, c->side[olcao_ctx::kSide - 1]);
-        rocblas_set_atomics_mode(c->blasSide, rocblas_atomics_not_allowed);
         HIPCHK(hipEventCreateWithFlags(&c->evS, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&c->evChol, hipEventDisableTiming));
         setup_system(*c, *sys);
