@@ -37,7 +37,10 @@ constexpr int kDcMax = kDcMaxN;
 constexpr int kSecularIters = 64;
 constexpr int kTile = 64;
 constexpr int kDcBig = 1024;     // threads of a merge of more than kDcSmallM
-constexpr int kDcSmall = 256;
+// threads of a whole small merge in one workgroup: its LDS state holds the
+// workgroup to one per CU, so sixteen waves cost no occupancy and solve the
+// secular roots (one per wave at a time) four times as wide as four would
+constexpr int kDcSmall = 1024;
 constexpr int kDcSmallM = 128;
 
 // Four wave sums side by side.

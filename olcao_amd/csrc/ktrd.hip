@@ -529,11 +529,13 @@ __global__ __launch_bounds__(256) void k_trd_gram(const double *__restrict__ Val
 // The Gram matrix G(a, j) = v_a^H v_j (a < j), summed from k_trd_gram's
 // partials in a fixed order, is overwritten column by column with T (zlarft:
 // T(j, j) = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) G(0:j, j)).
-template <bool CPLX>
+// SC > 0: the number of partials at compile time (S = SC); 0: S at run time
+template <bool CPLX, int SC>
 __global__ __launch_bounds__(256) void k_trd_tfactor(const double *__restrict__ Pall,
                                                      const double *__restrict__ tauAll,
-                                                     double *Tall, int n, int S)
+                                                     double *Tall, int n, int Srt)
 {
+    const int S = SC > 0 ? SC : Srt;
     using N = Num<CPLX>;
     using T = typename N::T;
     constexpr int E = CPLX ? 2 : 1;
@@ -552,8 +554,12 @@ __global__ __launch_bounds__(256) void k_trd_tfactor(const double *__restrict__ 
         const int a = e % NB, j = e / NB;
         if (a >= j) continue;
         T g = N::zero();
-        for (int sp = 0; sp < S; ++sp)
-            g = N::add(g, N::load(P + E * (long long)NB * NB * sp, e));
+        if constexpr (SC > 0) {
+#pragma unroll
+            for (int sp = 0; sp < SC; ++sp) g = N::add(g, N::load(P + E * (long long)NB * NB * sp, e));
+        } else {
+            for (int sp = 0; sp < S; ++sp) g = N::add(g, N::load(P + E * (long long)NB * NB * sp, e));
+        }
         Gm[a][j] = g;
     }
     __syncthreads();
@@ -739,11 +745,21 @@ void launch_trd_tfactor(bool cplx, int n, int nb, const double *V, const double 
     double *P = Tw + (size_t)blocks * nb * kTrdBlock * kTrdBlock * (cplx ? 2 : 1);
     if (cplx) {
         hipLaunchKernelGGL(k_trd_gram<true>, gg, dim3(256), 0, s, V, P, n);
-        hipLaunchKernelGGL(k_trd_tfactor<true>, dim3(blocks, nb), dim3(256), 0, s, P, tau, Tw, n, S);
+        if (S == kTrdGramSplit)
+            hipLaunchKernelGGL((k_trd_tfactor<true, kTrdGramSplit>), dim3(blocks, nb), dim3(256), 0, s,
+                               P, tau, Tw, n, S);
+        else
+            hipLaunchKernelGGL((k_trd_tfactor<true, 0>), dim3(blocks, nb), dim3(256), 0, s, P, tau,
+                               Tw, n, S);
         hipLaunchKernelGGL(k_trd_vt<true>, g2, dim3(256), 0, s, V, Tw, U, n);
     } else {
         hipLaunchKernelGGL(k_trd_gram<false>, gg, dim3(256), 0, s, V, P, n);
-        hipLaunchKernelGGL(k_trd_tfactor<false>, dim3(blocks, nb), dim3(256), 0, s, P, tau, Tw, n, S);
+        if (S == kTrdGramSplit)
+            hipLaunchKernelGGL((k_trd_tfactor<false, kTrdGramSplit>), dim3(blocks, nb), dim3(256), 0,
+                               s, P, tau, Tw, n, S);
+        else
+            hipLaunchKernelGGL((k_trd_tfactor<false, 0>), dim3(blocks, nb), dim3(256), 0, s, P, tau,
+                               Tw, n, S);
         hipLaunchKernelGGL(k_trd_vt<false>, g2, dim3(256), 0, s, V, Tw, U, n);
     }
 }
