@@ -713,11 +713,15 @@ __global__ __launch_bounds__(NT) void k_dc_prep(const int4 *nodes, int nodeBase,
 }
 
 constexpr int kRootsPerWg = 16;
+// one wave per root: a root's secular iteration is a chain of dependent
+// wave reductions, so the workgroup's roots run side by side
+constexpr int kRootThreads = 64 * kRootsPerWg;
 constexpr int kColsPerWg = 64;
 
 // tiles: (node index within the level, chunk)
-__global__ __launch_bounds__(256) void k_dc_roots(const int2 *tiles, const int4 *nodes, int nodeBase,
-                                                  int nNodes, int n, DcState st)
+__global__ __launch_bounds__(kRootThreads) void k_dc_roots(const int2 *tiles, const int4 *nodes,
+                                                           int nodeBase, int nNodes, int n,
+                                                           DcState st)
 {
     const int2 tl = tiles[blockIdx.x];
     const int mat = blockIdx.y;
@@ -730,14 +734,14 @@ __global__ __launch_bounds__(256) void k_dc_roots(const int2 *tiles, const int4 
     const double rho = st.rho[(size_t)mat * nNodes + tl.x];
     __shared__ double dk[kDcMax], zz[kDcMax];
     const size_t b = (size_t)n * mat + off;
-    for (int i = threadIdx.x; i < K; i += 256) {
+    for (int i = threadIdx.x; i < K; i += kRootThreads) {
         dk[i] = st.dk[b + i];
         const double z = st.zk[b + i];
         zz[i] = rho * z * z;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int j = j0 + wv; j < min(K, j0 + kRootsPerWg); j += 4) {
+    for (int j = j0 + wv; j < min(K, j0 + kRootsPerWg); j += kRootsPerWg) {
         int org;
         double tau;
         dc_root(j, K, dk, zz, org, tau);
@@ -854,8 +858,9 @@ __global__ __launch_bounds__(NT) void k_dc_prep_big(const int4 *nodes, int nodeB
     }
 }
 
-__global__ __launch_bounds__(256) void k_dc_roots_big(const int2 *tiles, const int4 *nodes,
-                                                      int nodeBase, int nNodes, int n, DcState st)
+__global__ __launch_bounds__(kRootThreads) void k_dc_roots_big(const int2 *tiles, const int4 *nodes,
+                                                               int nodeBase, int nNodes, int n,
+                                                               DcState st)
 {
     const int2 tl = tiles[blockIdx.x];
     const int mat = blockIdx.y;
@@ -869,14 +874,14 @@ __global__ __launch_bounds__(256) void k_dc_roots_big(const int2 *tiles, const i
     extern __shared__ double big_lds[];
     double *dk = big_lds, *zz = big_lds + K;
     const size_t b = (size_t)n * mat + off;
-    for (int i = threadIdx.x; i < K; i += 256) {
+    for (int i = threadIdx.x; i < K; i += kRootThreads) {
         dk[i] = st.dk[b + i];
         const double z = st.zk[b + i];
         zz[i] = rho * z * z;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int j = j0 + wv; j < min(K, j0 + kRootsPerWg); j += 4) {
+    for (int j = j0 + wv; j < min(K, j0 + kRootsPerWg); j += kRootsPerWg) {
         int org;
         double tau;
         dc_root<true>(j, K, dk, zz, org, tau);
@@ -1201,7 +1206,7 @@ bool launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double
             const size_t mm = (size_t)L.maxM;
             hipLaunchKernelGGL(k_dc_prep_big<kDcBig>, dim3(L.nNodes, nb), dim3(kDcBig), mm * 20, s,
                                dv.nodes, L.nodeBase, L.nNodes, n, D, E, Q, ord, st);
-            hipLaunchKernelGGL(k_dc_roots_big, dim3(L.nRoot, nb), dim3(256), mm * 16, s,
+            hipLaunchKernelGGL(k_dc_roots_big, dim3(L.nRoot, nb), dim3(kRootThreads), mm * 16, s,
                                dv.rootTiles + L.rootBase, dv.nodes, L.nodeBase, L.nNodes, n, st);
             hipLaunchKernelGGL(k_dc_zhat_big, dim3(L.nCol, nb), dim3(64), 0, s,
                                dv.colTiles + L.colBase, dv.nodes, L.nodeBase, L.nNodes, n, st);
@@ -1213,7 +1218,7 @@ bool launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double
         } else {
             hipLaunchKernelGGL(k_dc_prep<kDcBig>, dim3(L.nNodes, nb), dim3(kDcBig), 0, s, dv.nodes,
                                L.nodeBase, L.nNodes, n, D, E, Q, ord, st);
-            hipLaunchKernelGGL(k_dc_roots, dim3(L.nRoot, nb), dim3(256), 0, s, dv.rootTiles + L.rootBase,
+            hipLaunchKernelGGL(k_dc_roots, dim3(L.nRoot, nb), dim3(kRootThreads), 0, s, dv.rootTiles + L.rootBase,
                                dv.nodes, L.nodeBase, L.nNodes, n, st);
             hipLaunchKernelGGL(k_dc_vecs, dim3(L.nCol, nb), dim3(256), 0, s, dv.colTiles + L.colBase,
                                dv.nodes, L.nodeBase, L.nNodes, n, D, W, st);
