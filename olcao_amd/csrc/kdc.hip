@@ -753,9 +753,12 @@ __global__ __launch_bounds__(kRootThreads) void k_dc_roots(const int2 *tiles, co
     }
 }
 
-__global__ __launch_bounds__(256) void k_dc_vecs(const int2 *tiles, const int4 *nodes, int nodeBase,
-                                                 int nNodes, int n, double *Dall, double *Wall,
-                                                 DcState st)
+// 512 threads: the tile's columns eight per wave, and the zhat products
+// (K of K terms each) over twice as many lanes (1024 threads would spill)
+constexpr int kVecThreads = 512;
+__global__ __launch_bounds__(kVecThreads) void k_dc_vecs(const int2 *tiles, const int4 *nodes,
+                                                         int nodeBase, int nNodes, int n,
+                                                         double *Dall, double *Wall, DcState st)
 {
     const int2 tl = tiles[blockIdx.x];
     const int mat = blockIdx.y;
@@ -775,7 +778,7 @@ __global__ __launch_bounds__(256) void k_dc_vecs(const int2 *tiles, const int4 *
         L.Nr = Nr;
         L.rho = st.rho[(size_t)mat * nNodes + tl.x];
     }
-    for (int i = threadIdx.x; i < m; i += 256) {
+    for (int i = threadIdx.x; i < m; i += kVecThreads) {
         if (i < K) {
             L.dk[i] = st.dk[b + i];
             L.zk[i] = st.zk[b + i];
@@ -797,10 +800,11 @@ __global__ __launch_bounds__(256) void k_dc_vecs(const int2 *tiles, const int4 *
         }
     }
     __syncthreads();
-    for (int ii = threadIdx.x; ii < K; ii += 256) zh[ii] = dc_zhat(ii, K, L.dk, L.zk, L.rho, rorg, rtau);
+    for (int ii = threadIdx.x; ii < K; ii += kVecThreads)
+        zh[ii] = dc_zhat(ii, K, L.dk, L.zk, L.rho, rorg, rtau);
     __syncthreads();
-    dc_columns<256>(n, off, m, c0, c1, L, lam, rorg, rtau, zh, colInfo, Dall + (size_t)n * mat,
-                    Wall + (size_t)n * n * mat);
+    dc_columns<kVecThreads>(n, off, m, c0, c1, L, lam, rorg, rtau, zh, colInfo,
+                            Dall + (size_t)n * mat, Wall + (size_t)n * n * mat);
 }
 
 // ---- Merges larger than kDcMaxN (up to kDcBigMaxN): the same steps and the
@@ -1220,7 +1224,7 @@ bool launch_dc(const DcPlan &p, const DcDev &dv, int nb, double *D, const double
                                L.nodeBase, L.nNodes, n, D, E, Q, ord, st);
             hipLaunchKernelGGL(k_dc_roots, dim3(L.nRoot, nb), dim3(kRootThreads), 0, s, dv.rootTiles + L.rootBase,
                                dv.nodes, L.nodeBase, L.nNodes, n, st);
-            hipLaunchKernelGGL(k_dc_vecs, dim3(L.nCol, nb), dim3(256), 0, s, dv.colTiles + L.colBase,
+            hipLaunchKernelGGL(k_dc_vecs, dim3(L.nCol, nb), dim3(kVecThreads), 0, s, dv.colTiles + L.colBase,
                                dv.nodes, L.nodeBase, L.nNodes, n, D, W, st);
         }
         hipLaunchKernelGGL(k_dc_gemm, dim3(L.nTiles, nb), dim3(256), 0, s, dv.tiles + L.tileBase,
