@@ -314,13 +314,18 @@ __device__ void dc_prepare(int n, int off, int m1, int m, const double *D, const
     for (int p = tid; p < m; p += NT)
         if (big[p]) L.keep[pre[p]] = p;
     __syncthreads();
+    // each pair of consecutive kept poles, close or not with the values as
+    // they are now (L.dp, free until the deflated values are ranked below,
+    // holds the verdict of the pair ending at the q-th kept pole)
     for (int q = tid + 1; q < nBig; q += NT) {
         const int pj = L.keep[q - 1], nj = L.keep[q];
         double s = L.zs[pj], c = L.zs[nj];
         const double tau = sqrt(c * c + s * s);
         c /= tau;
         s = -s / tau;
-        if (fabs((L.ds[nj] - L.ds[pj]) * c * s) <= tol) L.close = 1;
+        const int cl = fabs((L.ds[nj] - L.ds[pj]) * c * s) <= tol;
+        L.dp[q] = cl;
+        if (cl) L.close = 1;
     }
     __syncthreads();
     if (!L.close) {
@@ -332,14 +337,25 @@ __device__ void dc_prepare(int n, int off, int m1, int m, const double *D, const
             L.Nr = 0;
         }
     } else if (tid == 0) {
-        int K = 0, nd = 0, nr = 0;
+        // the sequential scan: a rotation changes the values of the pair's
+        // second pole, which the next pair reads; a pair after one that did
+        // not rotate sees the values the parallel pass saw, and takes its
+        // verdict (the same arithmetic on the same values)
+        int K = 0, nd = 0, nr = 0, q = 0;
         int pj = -1;
+        bool moved = false;
         for (int nj = 0; nj < m; ++nj) {
             if (!big[nj]) {
                 L.defl[nd++] = nj;
                 continue;
             }
+            const int qn = q++;
             if (pj < 0) {
+                pj = nj;
+                continue;
+            }
+            if (!moved && !L.dp[qn]) {
+                L.keep[K++] = pj;
                 pj = nj;
                 continue;
             }
@@ -360,8 +376,10 @@ __device__ void dc_prepare(int n, int off, int m1, int m, const double *D, const
                 L.ds[nj] = L.ds[pj] * s * s + L.ds[nj] * c * c;
                 L.ds[pj] = t2;
                 L.defl[nd++] = pj;
+                moved = true;
             } else {
                 L.keep[K++] = pj;
+                moved = false;
             }
             pj = nj;
         }
