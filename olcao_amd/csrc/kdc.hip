@@ -147,7 +147,11 @@ __global__ __launch_bounds__(64) void k_dc_leaf(const int2 *leaves, int n, doubl
     const double *E = Eall + (size_t)n * mat;
     double *Q = Qall + (size_t)n * n * mat;
     __shared__ double A[ML][ML + 1], V[ML][ML + 1];
+    __shared__ double rC[ML / 2], rS[ML / 2], rApp[ML / 2], rAqq[ML / 2];
+    __shared__ int rP[ML / 2], rQ[ML / 2], rOn[ML / 2];
+    static_assert(ML % 2 == 0, "round-robin pairs of an even order");
     const int lane = threadIdx.x;
+    const int M = m + (m & 1), H = M / 2;
     for (int e = lane; e < ML * ML; e += 64) {
         const int r = e / ML, c = e - r * ML;
         double v = 0.0;
@@ -179,36 +183,71 @@ __global__ __launch_bounds__(64) void k_dc_leaf(const int2 *leaves, int n, doubl
         offn = wave_sum(offn);
         dn = wave_sum(dn);
         if (!(offn > kEps * kEps * dn)) break;
-        for (int p = 0; p < m - 1; ++p)
-            for (int q = p + 1; q < m; ++q) {
-                const double apq = A[p][q];
-                if (apq == 0.0) continue;   // the same on every lane
-                const double app = A[p][p], aqq = A[q][q];
-                const double th = (aqq - app) / (2.0 * apq);
-                const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
-                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-                __syncthreads();   // every lane has read A(p, q), A(p, p), A(q, q)
-                if (lane < m && lane != p && lane != q) {
-                    const double arp = A[lane][p], arq = A[lane][q];
-                    const double nrp = c * arp - s * arq, nrq = s * arp + c * arq;
-                    A[lane][p] = nrp;
-                    A[p][lane] = nrp;
-                    A[lane][q] = nrq;
-                    A[q][lane] = nrq;
+        // round-robin order: M - 1 rounds of M / 2 disjoint pairs (index 0
+        // fixed, the others rotating; a pair with the pad index M - 1 >= m of
+        // an odd m is idle), the round's rotations computed from the same A
+        // and applied together: columns (with V), rows, the lower triangle
+        // copied from the upper, then each pair's 2 x 2 block set exactly
+        for (int r = 0; r < M - 1; ++r) {
+            if (lane < H) {
+                const int a = lane == 0 ? 0 : (r + lane - 1) % (M - 1) + 1;
+                const int b = (r + M - 2 - lane) % (M - 1) + 1;
+                const int p = min(a, b), q = max(a, b);
+                const double apq = q < m ? A[p][q] : 0.0;
+                double c = 1.0, s = 0.0, tt = 0.0;
+                if (apq != 0.0) {
+                    const double app = A[p][p], aqq = A[q][q];
+                    const double th = (aqq - app) / (2.0 * apq);
+                    tt = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
+                    c = 1.0 / sqrt(1.0 + tt * tt);
+                    s = tt * c;
+                    rApp[lane] = app - tt * apq;
+                    rAqq[lane] = aqq + tt * apq;
                 }
-                if (lane == 0) {
-                    A[p][p] = app - t * apq;
-                    A[q][q] = aqq + t * apq;
-                    A[p][q] = 0.0;
-                    A[q][p] = 0.0;
-                }
-                if (lane < m) {
-                    const double vp = V[lane][p], vq = V[lane][q];
-                    V[lane][p] = c * vp - s * vq;
-                    V[lane][q] = s * vp + c * vq;
-                }
-                __syncthreads();
+                rP[lane] = p;
+                rQ[lane] = q;
+                rOn[lane] = apq != 0.0;
+                rC[lane] = c;
+                rS[lane] = s;
             }
+            __syncthreads();
+            for (int e = lane; e < m * H; e += 64) {
+                const int i = e / H, k = e - i * H;
+                if (!rOn[k]) continue;
+                const int p = rP[k], q = rQ[k];
+                const double c = rC[k], s = rS[k];
+                const double ap = A[i][p], aq = A[i][q];
+                A[i][p] = c * ap - s * aq;
+                A[i][q] = s * ap + c * aq;
+                const double vp = V[i][p], vq = V[i][q];
+                V[i][p] = c * vp - s * vq;
+                V[i][q] = s * vp + c * vq;
+            }
+            __syncthreads();
+            for (int e = lane; e < H * m; e += 64) {
+                const int k = e / m, j = e - k * m;
+                if (!rOn[k]) continue;
+                const int p = rP[k], q = rQ[k];
+                const double c = rC[k], s = rS[k];
+                const double ap = A[p][j], aq = A[q][j];
+                A[p][j] = c * ap - s * aq;
+                A[q][j] = s * ap + c * aq;
+            }
+            __syncthreads();
+            for (int e = lane; e < m * m; e += 64) {
+                const int i = e / m, j = e - i * m;
+                if (i > j) A[i][j] = A[j][i];
+            }
+            __syncthreads();
+            if (lane < H && rOn[lane]) {
+                const int p = rP[lane], q = rQ[lane];
+                A[p][p] = rApp[lane];
+                A[q][q] = rAqq[lane];
+                A[p][q] = 0.0;
+                A[q][p] = 0.0;
+            }
+            __syncthreads();
+        }
     }
     // eigenvalues ascending (ties by index), vectors with them
     if (lane < m) {
